@@ -1,0 +1,63 @@
+// Shared helpers of the gfx950 ERT library: status handling, launch helpers and
+// wavefront reductions.  Wavefronts are 64 lanes wide on CDNA4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/adt_hip.h"
+
+namespace adt {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;  // four wavefronts per workgroup
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups are dealt round-robin across them
+
+int set_error(int code, const char* fmt, ...);
+
+#define ADT_HIP_CHECK(expr)                                                                    \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return ::adt::set_error(ADT_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                \
+                                    hipGetErrorString(_e), __FILE__, __LINE__);                \
+    } while (0)
+
+#define ADT_REQUIRE(cond, ...)                                                                 \
+    do {                                                                                       \
+        if (!(cond)) return ::adt::set_error(ADT_ERR_INVALID, __VA_ARGS__);                    \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Workgroup b runs on XCD b % 8.  Remap so that each XCD receives a contiguous range of
+// tiles (neighbouring row ranges of one system share its L2).  Bijective for any count.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n / kXcds, rmd = n % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * q + (x < rmd ? x : rmd) + i;
+}
+
+// Butterfly sum over the lanes of a wavefront that share (lane % width): lanes whose ids
+// differ in bits >= log2(width).  Fixed order -> bitwise repeatable.
+template <int W>
+__device__ __forceinline__ double wave_sum_strided(double v) {
+#pragma unroll
+    for (int off = W; off < kWave; off <<= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Full-wave sum (every lane receives the total), fixed butterfly order.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+inline int blocks_for(int64_t n, int per) { return static_cast<int>((n + per - 1) / per); }
+
+}  // namespace adt

@@ -1,0 +1,422 @@
+// (c) fused normal sensitivity, (d) adjoint cell gradient, and the small measurement kernels.
+#include "adt_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int kMaxWaves = 64;
+
+struct SensArgs {
+    const double* phi;
+    const int32_t* a;
+    const int32_t* b;
+    const int32_t* m;
+    const int32_t* n;
+    const int32_t* dofs;
+    const double* stiff;
+    const double* mass;
+    double* out;
+    int64_t n_dofs;
+    int W, S, D, C, dch;
+    double ws[kMaxWaves];
+    double wm[kMaxWaves];
+};
+
+template <typename T>
+__device__ __forceinline__ double pot(const double* f, int row, int S, int64_t N, int node) {
+    return row < S ? f[static_cast<int64_t>(row) * N + node] : 0.0;
+}
+
+// One thread owns one (measurement, cell) entry: lanes walk 64 consecutive cells (coalesced
+// output rows and neighbouring potential gathers), the four wavefronts of a workgroup take
+// interleaved measurements of the block's measurement chunk.  Cell dofs and unit templates
+// stay in registers while the thread sweeps its measurements; the sum runs in the fixed order
+// (w, i, j).  P = float gives FP32 products with FP64 accumulation over wavenumbers.
+template <int K, typename P>
+__global__ __launch_bounds__(256) void sensitivity_kernel(SensArgs A) {
+    const int c = blockIdx.x * 64 + static_cast<int>(threadIdx.x & 63);
+    if (c >= A.C) return;
+    const int sub = static_cast<int>(threadIdx.x >> 6);
+    const int d0 = blockIdx.y * A.dch;
+    const int d1 = min(d0 + A.dch, A.D);
+    int dof[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) dof[k] = A.dofs[static_cast<int64_t>(c) * K + k];
+    constexpr bool kRegs = K <= 4;
+    P kt[kRegs ? K * K : 1], mt[kRegs ? K * K : 1];
+    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+    if constexpr (kRegs) {
+#pragma unroll
+        for (int e = 0; e < K * K; ++e) {
+            kt[e] = static_cast<P>(Kc[e]);
+            mt[e] = Mc ? static_cast<P>(Mc[e]) : P(0);
+        }
+    }
+    const int64_t N = A.n_dofs;
+    for (int d = d0 + sub; d < d1; d += 4) {
+        const int ia = A.a[d], ib = A.b[d], im = A.m[d], in = A.n[d];
+        double acc = 0.0;
+        for (int w = 0; w < A.W; ++w) {
+            const double* f = A.phi + static_cast<int64_t>(w) * A.S * N;
+            P cur[K], rec[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                cur[k] = static_cast<P>(pot<P>(f, ia, A.S, N, dof[k]) - pot<P>(f, ib, A.S, N, dof[k]));
+                rec[k] = static_cast<P>(pot<P>(f, im, A.S, N, dof[k]) - pot<P>(f, in, A.S, N, dof[k]));
+            }
+            P sk = 0, sm = 0;
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                P yk = 0, ym = 0;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    if constexpr (kRegs) {
+                        yk = fma(kt[i * K + j], cur[j], yk);
+                        ym = fma(mt[i * K + j], cur[j], ym);
+                    } else {
+                        yk = fma(static_cast<P>(Kc[i * K + j]), cur[j], yk);
+                        if (Mc) ym = fma(static_cast<P>(Mc[i * K + j]), cur[j], ym);
+                    }
+                }
+                sk = fma(rec[i], yk, sk);
+                sm = fma(rec[i], ym, sm);
+            }
+            acc = fma(A.ws[w], static_cast<double>(sk), acc);
+            acc = fma(A.wm[w], static_cast<double>(sm), acc);
+        }
+        A.out[static_cast<int64_t>(d) * A.C + c] = -acc;
+    }
+}
+
+template <int K>
+int launch_sensitivity(const SensArgs& a, int precision, hipStream_t st) {
+    dim3 grid(adt::blocks_for(a.C, 64), adt::blocks_for(a.D, a.dch));
+    if (precision == 1)
+        hipLaunchKernelGGL((sensitivity_kernel<K, float>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((sensitivity_kernel<K, double>), grid, dim3(256), 0, st, a);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// (d) adjoint cell gradient on interleaved (N, Rp) fields: one wavefront per cell, lanes over
+// right-hand sides, every row access a contiguous Rp*8-byte run.
+// ------------------------------------------------------------------------------------------
+
+struct GradArgs {
+    const double* lam;
+    const double* phi;
+    const int32_t* dofs;
+    const double* stiff;
+    const double* mass;
+    const int32_t* bnd_ptr;   // per cell: its boundary edges (CSR), fixed order
+    const int32_t* bnd_list;
+    const int32_t* bnd_dofs;  // (n_bnd, kb)
+    const double* bmass;      // (n_bnd, kb, kb)
+    const double* balpha;     // (W, n_bnd)
+    double* out;
+    int W, rp_lam, rp_phi, r, C, kb, n_bnd;
+    int64_t lam_off[kMaxWaves];
+    int64_t phi_off[kMaxWaves];
+    double ws[kMaxWaves];
+    double wm[kMaxWaves];
+    double bws[kMaxWaves];
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void cell_gradient_kernel(GradArgs A) {
+    const int c = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    if (c >= A.C) return;
+    int dof[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) dof[k] = A.dofs[static_cast<int64_t>(c) * K + k];
+    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+    double acc = 0.0;
+    for (int w = 0; w < A.W; ++w) {
+        const double* L = A.lam + A.lam_off[w];
+        const double* F = A.phi + A.phi_off[w];
+        double s_k = 0.0, s_m = 0.0;
+        for (int s = lane; s < A.r; s += 64) {
+            double lv[K], pv[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                lv[k] = L[static_cast<int64_t>(dof[k]) * A.rp_lam + s];
+                pv[k] = F[static_cast<int64_t>(dof[k]) * A.rp_phi + s];
+            }
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                double yk = 0.0, ym = 0.0;
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    yk = fma(Kc[i * K + j], pv[j], yk);
+                    if (Mc) ym = fma(Mc[i * K + j], pv[j], ym);
+                }
+                s_k = fma(lv[i], yk, s_k);
+                s_m = fma(lv[i], ym, s_m);
+            }
+        }
+        acc = fma(A.ws[w], s_k, acc);
+        acc = fma(A.wm[w], s_m, acc);
+    }
+    if (A.bnd_ptr) {
+        for (int e = A.bnd_ptr[c]; e < A.bnd_ptr[c + 1]; ++e) {
+            const int b = A.bnd_list[e];
+            const int32_t* bd = A.bnd_dofs + static_cast<int64_t>(b) * A.kb;
+            const double* Mb = A.bmass + static_cast<int64_t>(b) * A.kb * A.kb;
+            for (int w = 0; w < A.W; ++w) {
+                const double* L = A.lam + A.lam_off[w];
+                const double* F = A.phi + A.phi_off[w];
+                const double coef = A.bws[w] * A.balpha[static_cast<int64_t>(w) * A.n_bnd + b];
+                double sb = 0.0;
+                for (int s = lane; s < A.r; s += 64)
+                    for (int i = 0; i < A.kb; ++i) {
+                        const double li = L[static_cast<int64_t>(bd[i]) * A.rp_lam + s];
+                        for (int j = 0; j < A.kb; ++j)
+                            sb = fma(li * Mb[i * A.kb + j], F[static_cast<int64_t>(bd[j]) * A.rp_phi + s], sb);
+                    }
+                acc = fma(coef, sb, acc);
+            }
+        }
+    }
+    acc = adt::wave_sum(acc);
+    if (lane == 0) A.out[c] = -acc;
+}
+
+template <int K>
+int launch_gradient(const GradArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(cell_gradient_kernel<K>, dim3(adt::blocks_for(a.C, 4)), dim3(256), 0, st, a);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// electrode potentials, deinterleave, column aggregation
+// ------------------------------------------------------------------------------------------
+
+__global__ void potentials_kernel(const double* __restrict__ x, const int64_t* __restrict__ vec_off,
+                                  const int32_t* __restrict__ enode, const double* __restrict__ eweight,
+                                  const double* __restrict__ q, double* __restrict__ p, int n_models, int W,
+                                  int rp, int r, int n_recv, int kmax) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    const int t = blockIdx.z;
+    if (s >= r) return;
+    double acc = 0.0;
+    for (int w = 0; w < W; ++w) {
+        const double* X = x + vec_off[static_cast<int64_t>(t) * W + w];
+        double v = 0.0;
+        for (int k = 0; k < kmax; ++k) {
+            const double wt = eweight[static_cast<int64_t>(f) * kmax + k];
+            if (wt != 0.0) v = fma(wt, X[static_cast<int64_t>(enode[static_cast<int64_t>(f) * kmax + k]) * rp + s], v);
+        }
+        acc = fma(q[w], v, acc);
+    }
+    p[(static_cast<int64_t>(t) * r + s) * n_recv + f] = acc;
+}
+
+__global__ void deinterleave_kernel(const double* __restrict__ x, const int64_t* __restrict__ vec_off,
+                                    double* __restrict__ out, int rp, int r, int64_t n, int64_t stride) {
+    __shared__ double tile[64][65];
+    const int sys = blockIdx.z;
+    const int64_t n0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int s0 = blockIdx.y * 64;
+    const double* X = x + vec_off[sys];
+    double* O = out + static_cast<int64_t>(sys) * stride;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t node = n0 + i;
+        const int s = s0 + tx;
+        tile[i][tx] = (node < n && s < r) ? X[node * rp + s] : 0.0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int s = s0 + i;
+        const int64_t node = n0 + tx;
+        if (s < r && node < n) O[static_cast<int64_t>(s) * n + node] = tile[tx][i];
+    }
+}
+
+__global__ void aggregate_kernel(const double* __restrict__ in, int64_t in_cols, const int32_t* __restrict__ ptr,
+                                 const int32_t* __restrict__ idx, double* __restrict__ out, int n_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int d = blockIdx.y;
+    if (p >= n_out) return;
+    const double* row = in + static_cast<int64_t>(d) * in_cols;
+    double acc = 0.0;
+    for (int i = ptr[p]; i < ptr[p + 1]; ++i) acc += row[idx[i]];
+    out[static_cast<int64_t>(d) * n_out + p] = acc;
+}
+
+// small device scratch for per-call offset tables (grow-only, per host thread)
+struct Scratch {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+int scratch_upload(Scratch& s, const void* host, size_t bytes, hipStream_t st) {
+    if (s.cap < bytes) {
+        if (s.ptr) ADT_HIP_CHECK(hipFree(s.ptr));
+        ADT_HIP_CHECK(hipMalloc(&s.ptr, bytes));
+        s.cap = bytes;
+    }
+    ADT_HIP_CHECK(hipMemcpyAsync(s.ptr, host, bytes, hipMemcpyHostToDevice, st));
+    return ADT_OK;
+}
+
+}  // namespace
+
+extern "C" int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_cells, int k_local,
+                               const double* d_phi, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
+                               const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff,
+                               const double* d_mass, const double* h_ws, const double* h_wm, int precision,
+                               double* d_out, void* stream) {
+    ADT_REQUIRE(n_waves >= 1 && n_waves <= kMaxWaves, "adt_sensitivity: 1 <= n_waves <= %d", kMaxWaves);
+    ADT_REQUIRE(precision == 0 || precision == 1, "adt_sensitivity: precision must be 0 or 1");
+    ADT_REQUIRE(h_ws && (d_mass == nullptr || h_wm), "adt_sensitivity: missing wavenumber weights");
+    if (n_meas == 0 || n_cells == 0) return ADT_OK;
+    ADT_REQUIRE(d_phi && d_a && d_b && d_m && d_n && d_cell_dofs && d_stiff && d_out, "adt_sensitivity: null pointer");
+    SensArgs a{};
+    a.phi = d_phi;
+    a.a = d_a;
+    a.b = d_b;
+    a.m = d_m;
+    a.n = d_n;
+    a.dofs = d_cell_dofs;
+    a.stiff = d_stiff;
+    a.mass = d_mass;
+    a.out = d_out;
+    a.n_dofs = n_dofs;
+    a.W = n_waves;
+    a.S = s_rows;
+    a.D = n_meas;
+    a.C = n_cells;
+    for (int w = 0; w < n_waves; ++w) {
+        a.ws[w] = h_ws[w];
+        a.wm[w] = d_mass ? h_wm[w] : 0.0;
+    }
+    // enough workgroups to fill 256 CUs several times; a multiple of 4 measurements each
+    const int cell_blocks = adt::blocks_for(n_cells, 64);
+    int chunks = std::max(1, (4096 + cell_blocks - 1) / cell_blocks);
+    int dch = (n_meas + chunks - 1) / chunks;
+    dch = std::max(4, (dch + 3) / 4 * 4);
+    a.dch = dch;
+    ADT_REQUIRE(adt::blocks_for(n_meas, dch) <= 65535, "adt_sensitivity: too many measurement chunks");
+    hipStream_t st = adt::as_stream(stream);
+    switch (k_local) {
+        case 3: return launch_sensitivity<3>(a, precision, st);
+        case 4: return launch_sensitivity<4>(a, precision, st);
+        case 6: return launch_sensitivity<6>(a, precision, st);
+        case 10: return launch_sensitivity<10>(a, precision, st);
+        default: return adt::set_error(ADT_ERR_INVALID, "adt_sensitivity: unsupported local size %d", k_local);
+    }
+}
+
+extern "C" int adt_cell_gradient(int n_waves, int r, int n_cells, int k_local, const double* d_lam, int rp_lam,
+                                 const int64_t* h_lam_off, const double* d_phi, int rp_phi,
+                                 const int64_t* h_phi_off, const int32_t* d_cell_dofs, const double* d_stiff,
+                                 const double* d_mass, const double* h_ws, const double* h_wm, int n_bnd, int k_bnd,
+                                 const int32_t* d_bnd_dofs, const int32_t* d_cell_bnd, const double* d_bmass,
+                                 const double* d_balpha, const double* h_bws, double* d_out, void* stream) {
+    ADT_REQUIRE(n_waves >= 1 && n_waves <= kMaxWaves, "adt_cell_gradient: 1 <= n_waves <= %d", kMaxWaves);
+    ADT_REQUIRE(r >= 1 && rp_lam >= r && rp_phi >= r, "adt_cell_gradient: invalid r/rp");
+    if (n_cells == 0) return ADT_OK;
+    ADT_REQUIRE(d_lam && d_phi && d_cell_dofs && d_stiff && d_out && h_ws && h_lam_off && h_phi_off,
+                "adt_cell_gradient: null pointer");
+    ADT_REQUIRE(d_mass == nullptr || h_wm, "adt_cell_gradient: mass needs wm");
+    GradArgs a{};
+    a.lam = d_lam;
+    a.phi = d_phi;
+    a.dofs = d_cell_dofs;
+    a.stiff = d_stiff;
+    a.mass = d_mass;
+    a.out = d_out;
+    a.W = n_waves;
+    a.rp_lam = rp_lam;
+    a.rp_phi = rp_phi;
+    a.r = r;
+    a.C = n_cells;
+    a.kb = k_bnd;
+    a.n_bnd = n_bnd;
+    if (n_bnd > 0) {
+        ADT_REQUIRE(d_bnd_dofs && d_cell_bnd && d_bmass && d_balpha && h_bws && k_bnd >= 1,
+                    "adt_cell_gradient: boundary arrays missing");
+        a.bnd_ptr = d_cell_bnd;
+        a.bnd_list = d_cell_bnd + n_cells + 1;
+        a.bnd_dofs = d_bnd_dofs;
+        a.bmass = d_bmass;
+        a.balpha = d_balpha;
+    }
+    for (int w = 0; w < n_waves; ++w) {
+        a.lam_off[w] = h_lam_off[w];
+        a.phi_off[w] = h_phi_off[w];
+        a.ws[w] = h_ws[w];
+        a.wm[w] = d_mass ? h_wm[w] : 0.0;
+        a.bws[w] = n_bnd > 0 ? h_bws[w] : 0.0;
+    }
+    hipStream_t st = adt::as_stream(stream);
+    switch (k_local) {
+        case 3: return launch_gradient<3>(a, st);
+        case 4: return launch_gradient<4>(a, st);
+        case 6: return launch_gradient<6>(a, st);
+        case 10: return launch_gradient<10>(a, st);
+        default: return adt::set_error(ADT_ERR_INVALID, "adt_cell_gradient: unsupported local size %d", k_local);
+    }
+}
+
+extern "C" int adt_electrode_potentials(int n_models, int n_waves, int rp, int r, int n_recv, int kmax,
+                                        const int64_t* h_vec_off, const double* d_x, const int32_t* d_enode,
+                                        const double* d_eweight, const double* h_q, double* d_p, void* stream) {
+    ADT_REQUIRE(n_models >= 0 && n_waves >= 1 && r >= 1 && rp >= r && n_recv >= 0 && kmax >= 1,
+                "adt_electrode_potentials: invalid sizes");
+    if (n_models == 0 || n_recv == 0) return ADT_OK;
+    ADT_REQUIRE(n_recv <= 65535 && n_models <= 65535, "adt_electrode_potentials: too many rows");
+    hipStream_t st = adt::as_stream(stream);
+    static thread_local Scratch offs, qs;
+    const size_t nsys = static_cast<size_t>(n_models) * n_waves;
+    int rc = scratch_upload(offs, h_vec_off, sizeof(int64_t) * nsys, st);
+    if (rc) return rc;
+    if ((rc = scratch_upload(qs, h_q, sizeof(double) * n_waves, st))) return rc;
+    dim3 grid(adt::blocks_for(r, 64), n_recv, n_models);
+    hipLaunchKernelGGL(potentials_kernel, grid, dim3(64), 0, st, d_x, static_cast<const int64_t*>(offs.ptr),
+                       d_enode, d_eweight, static_cast<const double*>(qs.ptr), d_p, n_models, n_waves, rp, r,
+                       n_recv, kmax);
+    ADT_HIP_CHECK(hipGetLastError());
+    ADT_HIP_CHECK(hipStreamSynchronize(st));  // scratch reuse
+    return ADT_OK;
+}
+
+extern "C" int adt_deinterleave(int n_sys, int rp, int r, int64_t n, const int64_t* h_vec_off, const double* d_x,
+                                double* d_out, int64_t out_stride, void* stream) {
+    ADT_REQUIRE(n_sys >= 0 && r >= 1 && rp >= r && n >= 0, "adt_deinterleave: invalid sizes");
+    if (n_sys == 0 || n == 0) return ADT_OK;
+    ADT_REQUIRE(n_sys <= 65535, "adt_deinterleave: too many systems");
+    hipStream_t st = adt::as_stream(stream);
+    static thread_local Scratch offs;
+    int rc = scratch_upload(offs, h_vec_off, sizeof(int64_t) * n_sys, st);
+    if (rc) return rc;
+    dim3 grid(static_cast<unsigned>((n + 63) / 64), adt::blocks_for(r, 64), n_sys);
+    hipLaunchKernelGGL(deinterleave_kernel, grid, dim3(256), 0, st, d_x, static_cast<const int64_t*>(offs.ptr),
+                       d_out, rp, r, n, out_stride);
+    ADT_HIP_CHECK(hipGetLastError());
+    ADT_HIP_CHECK(hipStreamSynchronize(st));
+    return ADT_OK;
+}
+
+extern "C" int adt_aggregate_columns(int n_rows, int64_t in_cols, int n_out, const double* d_in,
+                                     const int32_t* d_col_ptr, const int32_t* d_col_idx, double* d_out,
+                                     void* stream) {
+    ADT_REQUIRE(n_rows >= 0 && n_out >= 0 && in_cols >= 0, "adt_aggregate_columns: invalid sizes");
+    if (n_rows == 0 || n_out == 0) return ADT_OK;
+    ADT_REQUIRE(n_rows <= 65535, "adt_aggregate_columns: at most 65535 rows per call");
+    dim3 grid(adt::blocks_for(n_out, 256), n_rows);
+    hipLaunchKernelGGL(aggregate_kernel, grid, dim3(256), 0, adt::as_stream(stream), d_in, in_cols, d_col_ptr,
+                       d_col_idx, d_out, n_out);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}

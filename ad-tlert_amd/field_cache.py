@@ -1,0 +1,85 @@
+"""LRU cache of solved fields keyed by a hash of the conductivity model.
+
+Same semantics as ``adtlert/forward/field_cache.py:17-110`` (keys, hit/miss counters,
+``step_batches`` splitting, newest-batches-first adjoint order).
+"""
+
+from __future__ import annotations
+
+import hashlib
+from collections import OrderedDict
+from collections.abc import Callable, Sequence
+from typing import Generic, TypeVar
+
+import numpy as np
+import torch
+
+Fields = TypeVar("Fields")
+ModelKey = bytes | None
+
+
+def step_batches(steps: Sequence[int], batch_steps: int) -> list[list[int]]:
+    """Near-equal consecutive batches of at most ``batch_steps``."""
+
+    if not len(steps):
+        return []
+    count = -(-len(steps) // batch_steps)
+    return [batch.tolist() for batch in np.array_split(np.asarray(steps), count)]
+
+
+def newest_batches_first(step_count: int, batch_steps: int) -> list[list[int]]:
+    return step_batches(range(step_count), batch_steps)[::-1]
+
+
+class FieldCache(Generic[Fields]):
+    def __init__(self, max_entries: int) -> None:
+        if int(max_entries) < 0:
+            raise ValueError("normal_field_cache_max_entries must be non-negative")
+        self.max_entries = int(max_entries)
+        self._entries: OrderedDict[bytes, Fields] = OrderedDict()
+        self.hits = self.misses = 0
+
+    def key(self, model: torch.Tensor) -> ModelKey:
+        if self.max_entries < 1:
+            return None
+        host = np.ascontiguousarray(model.detach().cpu().numpy())
+        return hashlib.blake2b(host.view(np.uint8), digest_size=16).digest() + host.dtype.str.encode()
+
+    def _lookup(self, key: ModelKey):
+        if key is None:
+            return None
+        found = self._entries.get(key)
+        if found is None:
+            self.misses += 1
+        else:
+            self.hits += 1
+            self._entries.move_to_end(key)
+        return found
+
+    def _store(self, key: ModelKey, fields) -> None:
+        if key is None:
+            return
+        self._entries[key] = fields
+        self._entries.move_to_end(key)
+        while len(self._entries) > self.max_entries:
+            self._entries.popitem(last=False)
+
+    def solve_series(self, models: torch.Tensor, batch_steps: int,
+                     solve: Callable[[torch.Tensor], Sequence[Fields]]):
+        """Fields and keys of every row of ``models``; misses are solved ``batch_steps`` at a time."""
+
+        keys = [self.key(model) for model in models]
+        fields = {step: found for step, key in enumerate(keys) if (found := self._lookup(key)) is not None}
+        missing = [step for step in range(len(keys)) if step not in fields]
+        for batch in step_batches(missing, batch_steps):
+            for step, solved in zip(batch, solve(models[batch]), strict=True):
+                fields[step] = solved
+                self._store(keys[step], solved)
+        return [fields[step] for step in range(len(keys))], keys
+
+    def clear(self) -> None:
+        self._entries.clear()
+
+    def info(self) -> dict[str, int]:
+        return {"entries": len(self._entries), "max_entries": self.max_entries, "hits": self.hits,
+                "misses": self.misses}

@@ -1,0 +1,227 @@
+"""ctypes binding of ``_lib/libadt_hip.so`` (the C ABI declared in ``include/adt_hip.h``).
+
+Arguments are torch tensors on the current HIP device; every call enqueues on
+``torch.cuda.current_stream()``.  There is no CPU fallback: when the library is missing or
+no GPU is visible the calls raise ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+import torch
+
+ABI_VERSION = 3
+LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().parent / "_lib" / "libadt_hip.so"))
+
+ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
+PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+
+c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+P = c_void_p
+
+_SIGNATURES = {
+    "adt_last_error": (ctypes.c_char_p, []),
+    "adt_abi_version": (c_int, []),
+    "adt_device_info": (c_int, [c_int, P, P, c_int]),
+    "adt_assemble": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_i64, P, P, P, P, P, P, P, P, P,
+                             P, c_int, P, P]),
+    "adt_spmm": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P]),
+    "adt_pcg_create": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P, P, P, P]),
+    "adt_pcg_layout": (c_int, [P, P, P, P]),
+    "adt_pcg_solve": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, P, c_int, P, P, P]),
+    "adt_pcg_destroy": (c_int, [P]),
+    "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
+    "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
+                                  P, P, P, P, P, P, P]),
+    "adt_electrode_potentials": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P]),
+    "adt_deinterleave": (c_int, [c_int, c_int, c_int, c_i64, P, P, P, c_i64, P]),
+    "adt_aggregate_columns": (c_int, [c_int, c_i64, c_int, P, P, P, P, P]),
+}
+
+_lib = None
+
+
+def library():
+    """Load the shared library once (raises ``RuntimeError`` when it is not built)."""
+
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python ad-tlert_amd/build.py` (hipcc, gfx950)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (restype, argtypes) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = restype, argtypes
+    if lib.adt_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI {lib.adt_abi_version()}, expected {ABI_VERSION}; rebuild it")
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def check(rc: int) -> None:
+    if rc == ADT_OK:
+        return
+    message = library().adt_last_error().decode(errors="replace")
+    if rc == ADT_ERR_INVALID:
+        raise ValueError(message)
+    if rc == ADT_ERR_NONFINITE:
+        raise FloatingPointError(message)
+    raise RuntimeError(f"adt_hip error {rc}: {message}")
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("ADTLERT-MI355X requires an AMD GPU (HIP); none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream() -> c_void_p:
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t: torch.Tensor | None) -> c_void_p:
+    if t is None:
+        return c_void_p(0)
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return c_void_p(t.data_ptr())
+
+
+def host(a: np.ndarray) -> c_void_p:
+    return a.ctypes.data_as(c_void_p)
+
+
+def device_info(index: int = 0) -> tuple[int, str]:
+    cus = c_int(0)
+    name = ctypes.create_string_buffer(64)
+    check(library().adt_device_info(index, ctypes.byref(cus), name, 64))
+    return cus.value, name.value.decode()
+
+
+# ----------------------------------------------------------------------------------------------
+# thin wrappers
+# ----------------------------------------------------------------------------------------------
+
+
+def assemble(sigma, cell_col, stiff, mass, k2, slot_ptr, slot_entry, values, *, n_waves, bnd_col=None, bmass=None,
+             balpha=None, fixed=None):
+    """values[(t*W + w), :] for conductivity rows sigma[t] (see ``adt_assemble``)."""
+
+    lib = library()
+    n_models = int(sigma.shape[0])
+    n_cells, k = int(stiff.shape[0]), int(stiff.shape[1])
+    n_bnd = 0 if bmass is None else int(bmass.shape[0])
+    k_bnd = 0 if bmass is None else int(bmass.shape[1])
+    nnz = int(slot_ptr.shape[0]) - 1
+    check(lib.adt_assemble(n_models, n_waves, nnz, n_cells, k, n_bnd, k_bnd, ptr(sigma), int(sigma.stride(0)),
+                           ptr(cell_col), ptr(bnd_col), ptr(stiff), ptr(mass), ptr(k2), ptr(bmass), ptr(balpha),
+                           ptr(slot_ptr), ptr(slot_entry), ptr(fixed), 0 if fixed is None else int(fixed.numel()),
+                           ptr(values), stream()))
+
+
+def spmm(layout, indptr, indices, values, x, y, mode=0):
+    rows, ptr_off, idx_off, val_off, vec_off, lanes, rp = layout
+    check(library().adt_spmm(len(rows), lanes, rp, host(rows), host(ptr_off), host(idx_off), host(val_off),
+                             host(vec_off), ptr(indptr), ptr(indices), ptr(values), ptr(x), ptr(y), mode, stream()))
+
+
+class PcgPlan:
+    """Owner of one ``adt_pcg_plan`` (batch layout + Krylov workspace)."""
+
+    def __init__(self, rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off=None):
+        self._lib = library()
+        self.rows = np.ascontiguousarray(rows, dtype=np.int32)
+        self.n_sys = int(self.rows.size)
+        self.r, self.lanes = int(r), int(lanes)
+        self.ptr_off = np.ascontiguousarray(ptr_off, dtype=np.int64)
+        self.idx_off = np.ascontiguousarray(idx_off, dtype=np.int64)
+        self.val_off = np.ascontiguousarray(val_off, dtype=np.int64)
+        self.b_off = None if b_off is None else np.ascontiguousarray(b_off, dtype=np.int64)
+        handle = c_void_p()
+        check(self._lib.adt_pcg_create(ctypes.byref(handle), self.n_sys, self.r, self.lanes, int(rows_per_tile),
+                                       host(self.rows), host(self.ptr_off), host(self.idx_off), host(self.val_off),
+                                       c_void_p(0) if self.b_off is None else host(self.b_off)))
+        self._handle = handle
+        rp, vec_len = c_int(0), c_i64(0)
+        self.vec_off = np.zeros(self.n_sys, dtype=np.int64)
+        check(self._lib.adt_pcg_layout(handle, ctypes.byref(rp), ctypes.byref(vec_len), host(self.vec_off)))
+        self.rp, self.vec_len = rp.value, vec_len.value
+
+    def solve(self, indptr, indices, values, b, x, tol, max_iter, *, precond=PRECOND_JACOBI, gnn=None, check_every=16):
+        iters = np.zeros((self.n_sys, self.r), dtype=np.int32)
+        relres = np.zeros((self.n_sys, self.r), dtype=np.float64)
+        rc = self._lib.adt_pcg_solve(self._handle, ptr(indptr), ptr(indices), ptr(values), ptr(b), ptr(x), float(tol),
+                                     int(max_iter), int(precond), c_void_p(0) if gnn is None else gnn,
+                                     int(check_every), host(iters), host(relres), stream())
+        return rc, iters, relres
+
+    def close(self):
+        if getattr(self, "_handle", None):
+            self._lib.adt_pcg_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sensitivity(phi, a, b, m, n, cell_dofs, stiff, mass, ws, wm, out, precision=0):
+    W, S, N = phi.shape
+    D, (C, K) = int(a.numel()), cell_dofs.shape
+    ws = np.ascontiguousarray(ws, dtype=np.float64)
+    wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
+    check(library().adt_sensitivity(W, S, N, D, C, K, ptr(phi), ptr(a), ptr(b), ptr(m), ptr(n), ptr(cell_dofs),
+                                    ptr(stiff), ptr(mass), host(ws), host(wm), int(precision), ptr(out), stream()))
+
+
+def cell_gradient(lam, rp_lam, lam_off, phi, rp_phi, phi_off, r, cell_dofs, stiff, mass, ws, wm, out, *,
+                  bnd_dofs=None, cell_bnd=None, bmass=None, balpha=None, bws=None):
+    """``out[c] = -sum_w sum_s lam_w[:,s]^T (ws_w K_c + wm_w M_c) phi_w[:,s]`` (+ Robin edges)."""
+
+    W = len(lam_off)
+    C, K = cell_dofs.shape
+    n_bnd = 0 if bmass is None else int(bmass.shape[0])
+    k_bnd = 0 if bmass is None else int(bmass.shape[1])
+    lam_off = np.ascontiguousarray(lam_off, dtype=np.int64)
+    phi_off = np.ascontiguousarray(phi_off, dtype=np.int64)
+    ws = np.ascontiguousarray(ws, dtype=np.float64)
+    wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
+    bws = np.ascontiguousarray(bws if bws is not None else np.zeros(W), dtype=np.float64)
+    check(library().adt_cell_gradient(W, int(r), C, K, ptr(lam), int(rp_lam), host(lam_off), ptr(phi), int(rp_phi),
+                                      host(phi_off), ptr(cell_dofs), ptr(stiff), ptr(mass), host(ws), host(wm), n_bnd,
+                                      k_bnd, ptr(bnd_dofs), ptr(cell_bnd), ptr(bmass), ptr(balpha), host(bws), ptr(out),
+                                      stream()))
+
+
+def electrode_potentials(x, vec_off, n_models, n_waves, rp, r, enode, eweight, q, out):
+    vec_off = np.ascontiguousarray(vec_off, dtype=np.int64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n_recv, kmax = enode.shape
+    check(library().adt_electrode_potentials(n_models, n_waves, rp, r, n_recv, kmax, host(vec_off), ptr(x), ptr(enode),
+                                             ptr(eweight), host(q), ptr(out), stream()))
+
+
+def deinterleave(x, vec_off, rp, r, n, out):
+    vec_off = np.ascontiguousarray(vec_off, dtype=np.int64)
+    check(library().adt_deinterleave(len(vec_off), rp, r, n, host(vec_off), ptr(x), ptr(out), r * n, stream()))
+
+
+def aggregate_columns(inp, col_ptr, col_idx, out):
+    rows = int(inp.shape[0])
+    for r0 in range(0, rows, 65535):
+        r1 = min(rows, r0 + 65535)
+        check(library().adt_aggregate_columns(r1 - r0, int(inp.shape[1]), int(out.shape[1]), ptr(inp[r0:r1]),
+                                              ptr(col_ptr), ptr(col_idx), ptr(out[r0:r1]), stream()))

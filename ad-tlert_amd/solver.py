@@ -1,0 +1,125 @@
+"""Batched iterative SPD solves on the GPU (replaces the cuDSS ``BatchedSolver``).
+
+The reference factorizes every matrix with cuDSS and back-substitutes all right-hand sides
+(``forward/cudss.py:182-247``).  Here each system is solved by FP64 preconditioned conjugate
+gradients in the native library; a plan per batch layout (number of systems, rows, right-hand
+sides) owns the Krylov workspace and the captured iteration graph and is reused across calls.
+
+Vectors are interleaved ``(N, Rp)`` blocks (node-major, sources contiguous).  ``Rp`` pads the
+source count to a multiple of the lane-group width ``L`` chosen by :func:`choose_lanes`;
+padding columns are zero and are never iterated.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import native
+
+LANE_CHOICES = (64, 32, 16, 8)
+
+
+def choose_lanes(r: int) -> int:
+    """Lane-group width minimising the padded source count (ties: wider groups)."""
+
+    best = None
+    for lanes in LANE_CHOICES:
+        rp = -(-r // lanes) * lanes
+        if best is None or rp < best[0]:
+            best = (rp, lanes)
+    return best[1]
+
+
+def padded(r: int, lanes: int) -> int:
+    return -(-r // lanes) * lanes
+
+
+def choose_rows_per_tile(n_sys: int, rows: int, r: int, lanes: int, target_tiles: int = 4096) -> int:
+    """Rows per workgroup tile: about ``target_tiles`` tiles over the batch, at least one row per
+    thread group and at most 4096 rows (bounded per-tile partial reductions)."""
+
+    groups = padded(r, lanes) // lanes
+    per_pass = 256 // lanes
+    rb = -(-(n_sys * groups * rows) // target_tiles)
+    rb = max(per_pass, min(4096, rb))
+    return -(-rb // per_pass) * per_pass
+
+
+@dataclass
+class SolveStats:
+    iterations: np.ndarray
+    relative_residual: np.ndarray
+    seconds: float
+
+
+@dataclass
+class BatchedSolver:
+    """PCG plans keyed by ``(key, n_sys, rows, r)``; ``close()`` releases their workspaces."""
+
+    tolerance: float = 1e-10
+    max_iterations: int = 50000
+    check_every: int = 16
+    _plans: dict = field(default_factory=dict)
+    last: SolveStats | None = None
+
+    def plan(self, key, rows, r: int, *, pattern_offsets=None, b_off=None, lanes=None, rows_per_tile=None):
+        """Plan for systems with ``rows[s]`` rows sharing one CSR pattern unless ``pattern_offsets``
+        ``(ptr_off, idx_off, val_off)`` says otherwise."""
+
+        rows = np.asarray(rows, dtype=np.int32)
+        n_sys = int(rows.size)
+        lanes = choose_lanes(r) if lanes is None else int(lanes)
+        if rows_per_tile is None:
+            rows_per_tile = choose_rows_per_tile(n_sys, int(rows.max()), r, lanes)
+        bkey = None if b_off is None else tuple(np.asarray(b_off).tolist())
+        full_key = (key, tuple(rows.tolist()), int(r), lanes, int(rows_per_tile), bkey)
+        plan = self._plans.get(full_key)
+        if plan is None:
+            if pattern_offsets is None:
+                ptr_off = np.zeros(n_sys, dtype=np.int64)
+                idx_off = np.zeros(n_sys, dtype=np.int64)
+                val_off = None
+            else:
+                ptr_off, idx_off, val_off = pattern_offsets
+            if val_off is None:
+                raise ValueError("val_off is required")
+            plan = native.PcgPlan(rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off)
+            self._plans[full_key] = plan
+        return plan
+
+    def solve(self, plan: native.PcgPlan, indptr, indices, values, b, x, *, tol=None, max_iter=None,
+              precond=native.PRECOND_JACOBI, gnn=None, raise_on_failure=True):
+        """Solve in place: ``x`` holds the initial guess on entry and the solution on return."""
+
+        tol = self.tolerance if tol is None else tol
+        max_iter = self.max_iterations if max_iter is None else max_iter
+        t0 = time.perf_counter()
+        rc, iters, relres = plan.solve(indptr, indices, values, b, x, tol, max_iter, precond=precond, gnn=gnn,
+                                       check_every=self.check_every)
+        self.last = SolveStats(iters, relres, time.perf_counter() - t0)
+        if raise_on_failure and rc == native.ADT_ERR_NONFINITE:
+            raise FloatingPointError("conjugate gradients produced non-finite values")
+        if raise_on_failure and rc == native.ADT_ERR_NOT_CONVERGED:
+            raise RuntimeError(f"conjugate gradients did not reach tol={tol} in {max_iter} iterations "
+                               f"(worst relative residual {float(relres.max()):.3e})")
+        if rc not in (native.ADT_OK, native.ADT_ERR_NONFINITE, native.ADT_ERR_NOT_CONVERGED):
+            native.check(rc)
+        return self.last
+
+    def close(self) -> None:
+        for plan in self._plans.values():
+            plan.close()
+        self._plans.clear()
+
+
+def interleave(fields: torch.Tensor, rp: int) -> torch.Tensor:
+    """``(S, R, N)`` field rows -> ``(S, N, Rp)`` interleaved blocks (zero padding columns)."""
+
+    s, r, n = fields.shape
+    out = torch.zeros((s, n, rp), dtype=torch.float64, device=fields.device)
+    out[:, :, :r] = fields.transpose(1, 2)
+    return out

@@ -1,0 +1,192 @@
+/*
+ * adt_hip.h -- C ABI of the MI355X (gfx950) ERT forward-and-sensitivity library.
+ *
+ * Every pointer argument named d_* is a device pointer owned by the caller (in practice
+ * a torch tensor's data_ptr()); h_* arguments are host pointers.  `stream` is a
+ * hipStream_t passed as void* (0 = legacy default stream); every call only enqueues
+ * work on that stream unless documented otherwise.  Functions return ADT_OK (0) or a
+ * status code; adt_last_error() gives a message for the calling thread.
+ *
+ * Field layout ("interleaved"): the solution/right-hand-side block of one linear system
+ * is an (N, Rp) row-major array -- node-major, with the Rp right-hand sides (sources)
+ * contiguous -- so one multi-RHS sparse product streams each matrix row once and reads
+ * Rp contiguous doubles per nonzero.  Rp is R rounded up to a multiple of the lane
+ * group width L (8, 16, 32 or 64).  Several systems are concatenated; system s starts
+ * at element offset vec_off[s].
+ *
+ * The reference entry points each call replaces are cited per function
+ * (paths relative to the AD-TLERT 0.3.0 source tree).
+ */
+#ifndef ADT_HIP_H
+#define ADT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ADT_OK = 0,
+    ADT_ERR_INVALID = 1,       /* argument validation failed (maps to ValueError)           */
+    ADT_ERR_HIP = 2,           /* HIP runtime error (RuntimeError)                            */
+    ADT_ERR_NONFINITE = 3,     /* non-finite values in a Krylov recurrence (FloatingPointError) */
+    ADT_ERR_NOT_CONVERGED = 4, /* iteration limit reached before the tolerance                */
+    ADT_ERR_NOMEM = 5          /* device allocation of a workspace failed                     */
+};
+
+/* Message of the last failing call on this host thread (static storage, never NULL). */
+const char* adt_last_error(void);
+
+/* ABI revision of this header; the Python loader refuses a library with another value. */
+int adt_abi_version(void);
+
+/* Number of device-side compute units and the name of device `device` (for reports). */
+int adt_device_info(int device, int* h_cus, char* h_name, int name_len);
+
+/* ------------------------------------------------------------------------------------------
+ * (a) Deterministic finite-element assembly.
+ *
+ * Replaces the gather-levels GroupSum of ERTForward2p5D._assemble
+ * (forward/ert2p5d.py:275-314, forward/kernels.py:79-134) and ERTForward3D._assemble_batch
+ * (forward/ert3d.py:436-461).  For system s = model*n_waves + w and CSR slot j:
+ *
+ *   values[s, j] = sum_{e in slot_entry[slot_ptr[j] : slot_ptr[j+1]]} coef(s, e)
+ *   coef(s, e) = sigma[model, cell_col[c]] * (stiff[e] + k2[w] * mass[e])      e <  C*K*K, c = e/(K*K)
+ *   coef(s, e) = sigma[model, bnd_col[b]] * balpha[w, b] * bmass[e - C*K*K]       e >= C*K*K, b = .../(Kb*Kb)
+ *
+ * summed in the fixed order of slot_entry, so results are bitwise repeatable.  Slots
+ * listed in fixed_slots are then set to 1 (Dirichlet rows).  mass/k2 may be NULL
+ * (3D: no wavenumber term); n_bnd may be 0.
+ * ---------------------------------------------------------------------------------------- */
+int adt_assemble(int n_models, int n_waves, int nnz, int n_cells, int k_local, int n_bnd, int k_bnd,
+                 const double* d_sigma, int64_t sigma_stride, const int32_t* d_cell_col,
+                 const int32_t* d_bnd_col, const double* d_stiff, const double* d_mass, const double* d_k2,
+                 const double* d_bmass, const double* d_balpha, const int32_t* d_slot_ptr,
+                 const int32_t* d_slot_entry, const int32_t* d_fixed_slots, int n_fixed,
+                 double* d_values, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched multi-RHS sparse product  y_s = A_s x_s  (+ optional  y = beta_in * y_in - A x).
+ *
+ * Replaces SparsePattern.matvec (forward/discretization.py:60-92).  Systems share or
+ * concatenate CSR patterns: system s uses indptr[ptr_off[s] ...], indices[idx_off[s] ...],
+ * values[val_off[s] ...] and vectors at vec_off[s], with rows[s] rows.  All offset arrays
+ * are host arrays of n_sys entries.  mode 0: y = A x;  mode 1: y = b - A x (b = d_y on
+ * entry).  Rows are summed in CSR order: deterministic.
+ * ---------------------------------------------------------------------------------------- */
+int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t* h_ptr_off,
+             const int64_t* h_idx_off, const int64_t* h_val_off, const int64_t* h_vec_off,
+             const int32_t* d_indptr, const int32_t* d_indices, const double* d_values, const double* d_x,
+             double* d_y, int mode, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * (b) Batched FP64 preconditioned conjugate gradients on SPD systems.
+ *
+ * Replaces BatchedSolver.solve (forward/cudss.py:182-247), the cuDSS direct solve that
+ * ERTForward2p5D._solve (forward/ert2p5d.py:328-347) and ERTForward3D._solve
+ * (forward/ert3d.py:463-481) call.  A plan fixes the batch layout (systems, rows, CSR
+ * offsets, lane width) and owns the Krylov workspace; solves then take the matrix values,
+ * right-hand sides and the initial guess (overwritten by the solution) by pointer.
+ *
+ * precond: ADT_PRECOND_JACOBI (plain PCG) or ADT_PRECOND_LEARNED (flexible CG whose
+ * preconditioner is the learned graph network of adt_gnn_*; Jacobi-PCG restarts from the
+ * current iterate for any right-hand side whose residual stalls).
+ * h_b_off (optional) places system s's right-hand side block at d_b + h_b_off[s] (systems of
+ * different models that share one right-hand side block); by default it mirrors the x layout
+ * reported by adt_pcg_layout.
+ * Convergence per right-hand side: ||b - A x||_2 <= tol * ||b||_2 (b != 0); a zero b gives x = 0.
+ * h_iters (R per system, optional) receives iteration counts, h_relres the final
+ * relative residuals.  The call synchronizes the stream once per `check_every` iterations.
+ * ---------------------------------------------------------------------------------------- */
+enum { ADT_PRECOND_JACOBI = 0, ADT_PRECOND_LEARNED = 1 };
+
+typedef struct adt_pcg_plan adt_pcg_plan;
+
+int adt_pcg_create(adt_pcg_plan** plan, int n_sys, int r, int lanes, int rows_per_tile,
+                   const int32_t* h_rows, const int64_t* h_ptr_off, const int64_t* h_idx_off,
+                   const int64_t* h_val_off, const int64_t* h_b_off);
+int adt_pcg_layout(const adt_pcg_plan* plan, int* h_rp, int64_t* h_vec_len, int64_t* h_vec_off);
+int adt_pcg_solve(adt_pcg_plan* plan, const int32_t* d_indptr, const int32_t* d_indices,
+                  const double* d_values, const double* d_b, double* d_x, double tol, int max_iter,
+                  int precond, const void* gnn, int check_every, int32_t* h_iters, double* h_relres,
+                  void* stream);
+int adt_pcg_destroy(adt_pcg_plan* plan);
+
+/* ------------------------------------------------------------------------------------------
+ * (c) Fused normal sensitivity (Jacobian rows).
+ *
+ * Replaces the NVRTC kernel normal_sensitivity (forward/kernels.py:132-178, launched by
+ * kernels.py:222-268):
+ *
+ *   out[d, c] = - sum_w sum_ij (phi_w[m_d] - phi_w[n_d])[cell_dofs[c,i]]
+ *                       * (ws[w] * stiff[c,i,j] + wm[w] * mass[c,i,j])
+ *                       * (phi_w[a_d] - phi_w[b_d])[cell_dofs[c,j]]
+ *
+ * phi is (W, S, N) (field row major); a row index equal to S is a remote electrode with
+ * zero potential.  mass/wm may be NULL.  precision 0: FP64 products; 1: FP32 products
+ * with FP64 accumulation.  Each output is accumulated by one thread in the fixed order
+ * (w, i, j): bitwise repeatable.  out is (D, C) row-major.
+ * ---------------------------------------------------------------------------------------- */
+int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_cells, int k_local,
+                    const double* d_phi, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
+                    const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff,
+                    const double* d_mass, const double* h_ws, const double* h_wm, int precision,
+                    double* d_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * (d) Adjoint cell gradient:  g[c] = - sum_w sum_s lam_w[:, s]^T (ws[w] K_c + wm[w] M_c) phi_w[:, s]
+ *     (+ Robin edges:  - sum_w sum_s lam_w[e]^T (balpha[w,b] * bws[w] * Mb_b) phi_w[e]).
+ *
+ * Replaces ERTForward2p5D._cell_gradient / ERTForward3D._cell_gradient
+ * (forward/ert2p5d.py:765-825, forward/ert3d.py:713-734) including their two SDDMMs
+ * (forward/kernels.py:116-127).  lam and phi are interleaved blocks of n_waves systems:
+ * lam_w = d_lam + h_lam_off[w] with row stride rp_lam, phi_w likewise; the first r columns
+ * are used.  The SDDMM q_w(i,j) = sum_s lam_w[i,s] phi_w[j,s] is formed on the fly per local
+ * entry.  Robin edges (n_bnd > 0): d_cell_bnd is a per-cell CSR of boundary-edge ids
+ * ((C+1) pointers followed by the edge list), d_bnd_dofs (n_bnd, k_bnd), d_bmass
+ * (n_bnd, k_bnd, k_bnd), d_balpha (n_waves, n_bnd).  One wavefront per cell; every reduction
+ * is in a fixed order.
+ * ---------------------------------------------------------------------------------------- */
+int adt_cell_gradient(int n_waves, int r, int n_cells, int k_local, const double* d_lam, int rp_lam,
+                      const int64_t* h_lam_off, const double* d_phi, int rp_phi, const int64_t* h_phi_off,
+                      const int32_t* d_cell_dofs, const double* d_stiff, const double* d_mass,
+                      const double* h_ws, const double* h_wm, int n_bnd, int k_bnd, const int32_t* d_bnd_dofs,
+                      const int32_t* d_cell_bnd, const double* d_bmass, const double* d_balpha,
+                      const double* h_bws, double* d_out, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Electrode potentials of interleaved fields:  P[t, s, f] = sum_w q[w] * sum_k ew[f,k] * x_{t,w}[en[f,k], s]
+ *
+ * Replaces the integration + E x E transfer map of ERTForward2p5D._integrate /
+ * _normal_reciprocal (forward/ert2p5d.py:560-589) and ERTForward3D._measurement_resistances
+ * (forward/ert3d.py:559-567).  The electrode interpolation is given as up to kmax
+ * (node, weight) pairs per receiver row f (weight 0 pads).  Systems are (t, w) with
+ * vec_off[t*n_waves + w]; P is (n_models, r, n_recv) row-major.
+ * ---------------------------------------------------------------------------------------- */
+int adt_electrode_potentials(int n_models, int n_waves, int rp, int r, int n_recv, int kmax,
+                             const int64_t* h_vec_off, const double* d_x, const int32_t* d_enode,
+                             const double* d_eweight, const double* h_q, double* d_p, void* stream);
+
+/* Interleaved (N, Rp) block of each system -> (R, N) field rows:  out[s][i, n] = x_s[n, i]. */
+int adt_deinterleave(int n_sys, int rp, int r, int64_t n, const int64_t* h_vec_off, const double* d_x,
+                     double* d_out, int64_t out_stride, void* stream);
+
+/* Deterministic column aggregation (CellAggregation, forward/kernels.py:137-162):
+ * out[d, p] = sum_{i in col_ptr[p] .. col_ptr[p+1]} in[d, col_idx[i]] in CSR order. */
+int adt_aggregate_columns(int n_rows, int64_t in_cols, int n_out, const double* d_in,
+                          const int32_t* d_col_ptr, const int32_t* d_col_idx, double* d_out,
+                          void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Learned preconditioner: a 3-layer graph message-passing network on the matrix graph
+ * (hidden width 64, BF16 weights, FP32 accumulation).  See adt_gnn.h-level description in
+ * the package documentation; the opaque handle is created from host weights.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct adt_gnn adt_gnn;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADT_HIP_H */

@@ -1,0 +1,150 @@
+"""Native gfx950 kernels through the C ABI against plain references (GPU).
+
+Mirrors the reference kernel suite (``tests/test_kernels_gpu.py:15-270``): the sensitivity
+kernel against the einsum formula for K in {3, 4, 6, 10} at FP64 (1e-12) and FP32-product
+(1e-4) precision, bitwise repeatability, the current stream, ragged tiles and empty
+measurement sets; assembly and the multi-RHS product against host scatters and dense products.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from tests.conftest import adt, relmax
+
+pytestmark = pytest.mark.gpu
+native = adt.native
+
+
+def dev():
+    return torch.device("cuda", 0)
+
+
+def random_problem(k, W=5, S=9, N=60, C=40, D=17, seed=0):
+    gen = torch.Generator().manual_seed(seed)
+    draw = lambda *shape: torch.randn(*shape, generator=gen, dtype=torch.float64)  # noqa: E731
+    phi = draw(W, S, N)
+    stiff, mass = draw(C, k, k), draw(C, k, k)
+    ws, wm = draw(W).abs(), draw(W).abs()
+    cells = torch.randint(0, N, (C, k), generator=gen)
+    a, b, m, n = (torch.randint(0, S + 1, (D,), generator=gen) for _ in range(4))  # S = remote
+    return phi, a, b, m, n, cells, stiff, mass, ws, wm
+
+
+def einsum_reference(phi, a, b, m, n, cells, stiff, mass, ws, wm):
+    padded = torch.cat((phi, torch.zeros_like(phi[:, :1])), dim=1)
+    cur = (padded[:, a] - padded[:, b])[..., cells]
+    rec = (padded[:, m] - padded[:, n])[..., cells]
+    T = ws[:, None, None, None] * stiff[None] + wm[:, None, None, None] * mass[None]
+    return -torch.einsum("wdci,wcij,wdcj->dc", rec, T, cur)
+
+
+def run_sensitivity(problem, precision=0):
+    phi, a, b, m, n, cells, stiff, mass, ws, wm = problem
+    g = lambda t, dt=None: t.to(dev(), dtype=dt or t.dtype).contiguous()  # noqa: E731
+    out = torch.empty((a.numel(), cells.shape[0]), dtype=torch.float64, device=dev())
+    native.sensitivity(g(phi), *(g(x, torch.int32) for x in (a, b, m, n)), g(cells, torch.int32), g(stiff), g(mass),
+                       ws.numpy(), wm.numpy(), out, precision)
+    return out
+
+
+@pytest.mark.parametrize("k", [3, 4, 6, 10])
+@pytest.mark.parametrize("precision, tol", [(0, 1e-12), (1, 1e-4)])
+def test_sensitivity_kernel_matches_the_einsum_reference(k, precision, tol):
+    problem = random_problem(k)
+    expected = einsum_reference(*problem)
+    actual = run_sensitivity(problem, precision).cpu()
+    assert actual.shape == expected.shape
+    assert relmax(actual, expected) < tol
+
+
+def test_sensitivity_kernel_is_repeatable_follows_the_stream_and_handles_empty_sets():
+    problem = random_problem(10, C=301, D=33)
+    first = run_sensitivity(problem)
+    assert all(torch.equal(first, run_sensitivity(problem)) for _ in range(5))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        on_stream = run_sensitivity(problem)
+    stream.synchronize()
+    assert torch.equal(on_stream, first)
+    phi, a, b, m, n, cells, stiff, mass, ws, wm = problem
+    empty = run_sensitivity((phi, a[:0], b[:0], m[:0], n[:0], cells, stiff, mass, ws, wm))
+    assert empty.shape == (0, cells.shape[0])
+
+
+def test_sensitivity_kernel_handles_ragged_tiles():
+    problem = random_problem(3, C=37, D=5)
+    assert relmax(run_sensitivity(problem).cpu(), einsum_reference(*problem)) < 1e-12
+
+
+def test_multi_rhs_product_matches_dense_products_for_every_lane_width():
+    rng = np.random.default_rng(1)
+    n, nnz_row = 300, 7
+    rows = np.repeat(np.arange(n), nnz_row)
+    cols = rng.integers(0, n, rows.size)
+    keys = np.unique(rows * n + cols)
+    indptr = np.concatenate(([0], np.cumsum(np.bincount(keys // n, minlength=n)))).astype(np.int32)
+    indices = (keys % n).astype(np.int32)
+    S = 3
+    values = rng.standard_normal((S, keys.size))
+    for lanes, rp in ((1, 5), (8, 8), (16, 48), (32, 96), (64, 128)):
+        x = rng.standard_normal((S, n, rp))
+        y = torch.zeros((S, n, rp), dtype=torch.float64, device=dev())
+        layout = (np.full(S, n, np.int32), np.zeros(S, np.int64), np.zeros(S, np.int64),
+                  np.arange(S, dtype=np.int64) * keys.size, np.arange(S, dtype=np.int64) * n * rp, lanes, rp)
+        native.spmm(layout, torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()),
+                    torch.as_tensor(values, device=dev()), torch.as_tensor(x, device=dev()), y, 0)
+        dense = np.zeros((S, n, n))
+        r = keys // n
+        for s in range(S):
+            dense[s, r, indices] = values[s]
+        assert relmax(y.cpu().numpy(), dense @ x) < 1e-13, lanes
+
+
+def test_pcg_solves_random_spd_batches_to_tolerance():
+    rng = np.random.default_rng(2)
+    n = 400
+    # 2D Laplacian-like SPD matrices with random positive coefficients, two systems
+    side = 20
+    idx = np.arange(n).reshape(side, side)
+    rows, cols = [], []
+    for di, dj in ((0, 0), (0, 1), (1, 0), (0, -1), (-1, 0)):
+        i0, j0 = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+        i1, j1 = i0 + di, j0 + dj
+        ok = (i1 >= 0) & (i1 < side) & (j1 >= 0) & (j1 < side)
+        rows.append(idx[i0[ok], j0[ok]])
+        cols.append(idx[i1[ok], j1[ok]])
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    keys = np.unique(rows * n + cols)
+    r, c = keys // n, keys % n
+    indptr = np.concatenate(([0], np.cumsum(np.bincount(r, minlength=n)))).astype(np.int32)
+    S, R = 2, 13
+    values = np.zeros((S, keys.size))
+    dense = np.zeros((S, n, n))
+    for s in range(S):
+        w = rng.uniform(0.5, 2.0, keys.size)
+        off = r != c
+        sym = {}
+        for k in np.flatnonzero(off):
+            a_, b_ = min(r[k], c[k]), max(r[k], c[k])
+            sym.setdefault((a_, b_), -w[k])
+        for k in np.flatnonzero(off):
+            values[s, k] = sym[(min(r[k], c[k]), max(r[k], c[k]))]
+        dense[s, r[off], c[off]] = values[s, off]
+        diag = -dense[s].sum(axis=1) + 0.01
+        values[s, ~off] = diag[r[~off]]
+        dense[s, r[~off], c[~off]] = values[s, ~off]
+    plan = native.PcgPlan(np.full(S, n), R, 16, 32, np.zeros(S), np.zeros(S), np.arange(S) * keys.size)
+    b = rng.standard_normal((S, n, plan.rp))
+    b[:, :, R:] = 0.0
+    b[1, :, 3] = 0.0  # a zero right-hand side -> zero solution
+    x = torch.zeros((S, n, plan.rp), dtype=torch.float64, device=dev())
+    x[0, :, 5] = 1.0
+    rc, iters, relres = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(c.astype(np.int32), device=dev()),
+                                   torch.as_tensor(values, device=dev()), torch.as_tensor(b, device=dev()), x, 1e-10, 5000)
+    assert rc == 0, native.library().adt_last_error()
+    assert relres.max() <= 1e-10 * 1.5
+    assert iters[1, 3] == 0 and float(x[1, :, 3].abs().max()) == 0.0
+    sol = np.linalg.solve(dense, b[:, :, :R])
+    assert relmax(x.cpu().numpy()[:, :, :R], sol) < 1e-7
+    plan.close()
