@@ -30,7 +30,7 @@ _SIGNATURES = {
     "adt_assemble": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_i64, P, P, P, P, P, P, P, P, P,
                              P, c_int, P, P]),
     "adt_spmm": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P]),
-    "adt_pcg_create": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P, P, P, P]),
+    "adt_pcg_create": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P, P, P]),
     "adt_pcg_layout": (c_int, [P, P, P, P]),
     "adt_pcg_solve": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, P, c_int, P, P, P]),
     "adt_pcg_destroy": (c_int, [P]),
