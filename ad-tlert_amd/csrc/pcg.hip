@@ -58,6 +58,7 @@ struct PcgArgs {
     int* flags;  // 1 active, 0 converged, 2 zero right-hand side, -1 breakdown / non-finite
     int* stall;
     int* grp_active;
+    int* grp_iters;
     int* active_total;
     double tol2;
     int rp, groups, r_real, n_tiles;
@@ -270,6 +271,7 @@ __global__ __launch_bounds__(256) void cg_init_reduce(const PcgArgs* __restrict_
         act = flag == 1;
         if (act) atomicAdd(a.active_total, 1);
     }
+    if (threadIdx.x == 0) a.grp_iters[gid] = 0;
     refresh_group<L>(a, gid, act);
 }
 
@@ -393,6 +395,7 @@ __global__ __launch_bounds__(256) void cg_beta(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
     if (!a.grp_active[gid]) return;
+    if (threadIdx.x == 0) a.grp_iters[gid] += 1;
     const int t0 = a.grp_tile0[gid], nt = a.grp_ntiles[gid];
     const double rz = group_partials<L>(a.part1, t0, nt);
     const double rr = group_partials<L>(a.part2, t0, nt);
@@ -461,6 +464,8 @@ __global__ __launch_bounds__(256) void cg_true_reduce(const PcgArgs* __restrict_
 // plan
 // ------------------------------------------------------------------------------------------
 
+constexpr int kMaxChunk = 64;
+
 struct adt_pcg_plan {
     int n_sys = 0, r = 0, L = 0, rp = 0, rb = 0, groups = 0, n_tiles = 0, n_groups = 0;
     int64_t vec_len = 0, diag_len = 0;
@@ -479,9 +484,15 @@ struct adt_pcg_plan {
     PcgArgs args_h{};
     PcgArgs* args_d = nullptr;
     hipStream_t cap = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // Jacobi, flexible
-    int graph_iters[2] = {0, 0};
+    // two identical iteration graphs replayed alternately, so a replay in flight never touches
+    // the profiling events of the replay being read
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_iters = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    int profile = 0;
+    hipEvent_t pev[2][kMaxChunk][4] = {};
+    double prof_ms[2] = {0.0, 0.0};
+    int64_t prof_launches = 0;
     ~adt_pcg_plan();
 };
 
@@ -490,6 +501,10 @@ adt_pcg_plan::~adt_pcg_plan() {
         if (g) (void)hipGraphExecDestroy(g);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& set : pev)
+        for (auto& it : set)
+            for (auto& e : it)
+                if (e) (void)hipEventDestroy(e);
     if (cap) (void)hipStreamDestroy(cap);
     void* bufs[] = {sys_d, tiles_d, grp_tile0_d, grp_ntiles_d, vec, dinv, part, scal, ints, args_d};
     for (void* p : bufs)
@@ -509,30 +524,62 @@ int dalloc(T** p, int64_t count) {
 }
 
 template <int L>
-void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity) {
+void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, hipEvent_t* events) {
     const PcgArgs* pa = p->args_d;
+    if (events) (void)hipEventRecord(events[0], st);
     hipLaunchKernelGGL(cg_spmv<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+    if (events) (void)hipEventRecord(events[1], st);
     hipLaunchKernelGGL(cg_alpha<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    if (events) (void)hipEventRecord(events[2], st);
     hipLaunchKernelGGL(cg_update<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+    if (events) (void)hipEventRecord(events[3], st);
     hipLaunchKernelGGL(cg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
 }
 
-// Capture `chunk` (even) iterations once; replays keep the p ping-pong parity at 0.
+// Capture `chunk` (even) iterations; replays keep the p ping-pong parity at 0.
 template <int L>
-int ensure_graph(adt_pcg_plan* p, int chunk) {
-    if (p->graph[0] && p->graph_iters[0] == chunk) return ADT_OK;
-    if (p->graph[0]) {
-        (void)hipGraphExecDestroy(p->graph[0]);
-        p->graph[0] = nullptr;
+int ensure_graphs(adt_pcg_plan* p, int chunk) {
+    if (p->graph[0] && p->graph_iters == chunk) return ADT_OK;
+    for (auto& g : p->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    for (int set = 0; set < 2; ++set) {
+        hipGraph_t g = nullptr;
+        ADT_HIP_CHECK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeRelaxed));
+        for (int i = 0; i < chunk; ++i) launch_iteration<L>(p, p->cap, i & 1, nullptr);
+        ADT_HIP_CHECK(hipStreamEndCapture(p->cap, &g));
+        hipError_t e = hipGraphInstantiate(&p->graph[set], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return adt::set_error(ADT_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    hipGraph_t g = nullptr;
-    ADT_HIP_CHECK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeRelaxed));
-    for (int i = 0; i < chunk; ++i) launch_iteration<L>(p, p->cap, i & 1);
-    ADT_HIP_CHECK(hipStreamEndCapture(p->cap, &g));
-    hipError_t e = hipGraphInstantiate(&p->graph[0], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return adt::set_error(ADT_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    p->graph_iters[0] = chunk;
+    p->graph_iters = chunk;
+    return ADT_OK;
+}
+
+// Profiling mode: the same kernels launched directly on the caller's stream with timing events
+// around every spmv and update launch (HIP event nodes inside graphs cannot be timed).
+template <int L>
+int launch_profiled_chunk(adt_pcg_plan* p, hipStream_t st, int chunk, int set) {
+    for (int i = 0; i < chunk; ++i) {
+        for (auto& e : p->pev[set][i])
+            if (!e) ADT_HIP_CHECK(hipEventCreate(&e));
+        launch_iteration<L>(p, st, i & 1, p->pev[set][i]);
+    }
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
+int accumulate_profile(adt_pcg_plan* p, int set, int chunk) {
+    for (int i = 0; i < chunk; ++i) {
+        float a = 0.f, b = 0.f;
+        ADT_HIP_CHECK(hipEventElapsedTime(&a, p->pev[set][i][0], p->pev[set][i][1]));
+        ADT_HIP_CHECK(hipEventElapsedTime(&b, p->pev[set][i][2], p->pev[set][i][3]));
+        p->prof_ms[0] += a;
+        p->prof_ms[1] += b;
+        p->prof_launches += 1;
+    }
     return ADT_OK;
 }
 
@@ -545,23 +592,30 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk) {
     hipLaunchKernelGGL(cg_zero_solution<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     ADT_HIP_CHECK(hipGetLastError());
     if (max_iter > 0) {
-        int rc = ensure_graph<L>(p, chunk);
+        int rc = p->profile ? ADT_OK : ensure_graphs<L>(p, chunk);
         if (rc) return rc;
-        int k = 0, done = 0;
-        for (int it = 0; it < max_iter; it += chunk, ++k) {
-            ADT_HIP_CHECK(hipGraphLaunch(p->graph[0], st));
-            ADT_HIP_CHECK(hipMemcpyAsync(&p->h_active[k & 1], p->args_h.active_total, sizeof(int),
+        int k = 0;
+        bool stop = false;
+        for (int it = 0; it < max_iter && !stop; it += chunk, ++k) {
+            const int set = k & 1;
+            if (p->profile) {
+                if ((rc = launch_profiled_chunk<L>(p, st, chunk, set))) return rc;
+            } else {
+                ADT_HIP_CHECK(hipGraphLaunch(p->graph[set], st));
+            }
+            ADT_HIP_CHECK(hipMemcpyAsync(&p->h_active[set], p->args_h.active_total, sizeof(int),
                                          hipMemcpyDeviceToHost, st));
-            ADT_HIP_CHECK(hipEventRecord(p->ev[k & 1], st));
+            ADT_HIP_CHECK(hipEventRecord(p->ev[set], st));
             if (k >= 1) {
-                ADT_HIP_CHECK(hipEventSynchronize(p->ev[(k - 1) & 1]));
-                if (p->h_active[(k - 1) & 1] == 0) {
-                    done = 1;
-                    break;
-                }
+                ADT_HIP_CHECK(hipEventSynchronize(p->ev[set ^ 1]));
+                if (p->profile && (rc = accumulate_profile(p, set ^ 1, chunk))) return rc;
+                stop = p->h_active[set ^ 1] == 0;
             }
         }
-        (void)done;
+        // the last replay launched is still in flight (or never synchronized)
+        const int last = (k - 1) & 1;
+        ADT_HIP_CHECK(hipEventSynchronize(p->ev[last]));
+        if (p->profile && (rc = accumulate_profile(p, last, chunk))) return rc;
     }
     hipLaunchKernelGGL(cg_true_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     hipLaunchKernelGGL(cg_true_reduce<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
@@ -616,7 +670,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
         (rc = dalloc(&p->grp_tile0_d, p->n_groups)) || (rc = dalloc(&p->grp_ntiles_d, p->n_groups)) ||
         (rc = dalloc(&p->vec, 5 * vo)) || (rc = dalloc(&p->dinv, dvo)) ||
         (rc = dalloc(&p->part, 3 * static_cast<int64_t>(p->n_tiles) * lanes)) ||
-        (rc = dalloc(&p->scal, 7 * nsc)) || (rc = dalloc(&p->ints, 3 * nsc + p->n_groups + 1)) ||
+        (rc = dalloc(&p->scal, 7 * nsc)) || (rc = dalloc(&p->ints, 3 * nsc + 2 * p->n_groups + 1)) ||
         (rc = dalloc(&p->args_d, 1))) {
         delete p;
         return rc;
@@ -660,7 +714,8 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     a.flags = p->ints + nsc;
     a.stall = p->ints + 2 * nsc;
     a.grp_active = p->ints + 3 * nsc;
-    a.active_total = p->ints + 3 * nsc + p->n_groups;
+    a.grp_iters = p->ints + 3 * nsc + p->n_groups;
+    a.active_total = p->ints + 3 * nsc + 2 * p->n_groups;
     a.rp = p->rp;
     a.groups = p->groups;
     a.r_real = r;
@@ -692,7 +747,7 @@ extern "C" int adt_pcg_solve(adt_pcg_plan* p, const int32_t* d_indptr, const int
     ADT_REQUIRE(max_iter >= 0, "adt_pcg_solve: max_iter must be >= 0");
     ADT_REQUIRE(precond == ADT_PRECOND_JACOBI, "adt_pcg_solve: unsupported preconditioner %d", precond);
     (void)gnn;
-    const int chunk = std::max(2, (check_every + 1) / 2 * 2);
+    const int chunk = std::min(kMaxChunk, std::max(2, (check_every + 1) / 2 * 2));
     hipStream_t st = adt::as_stream(stream);
     PcgArgs& a = p->args_h;
     a.indptr = d_indptr;
@@ -774,5 +829,28 @@ extern "C" int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, con
     if (rc) return rc;
     // the metadata buffer is reused by the next call: keep the host copy alive until uploaded
     ADT_HIP_CHECK(hipStreamSynchronize(st));
+    return ADT_OK;
+}
+
+extern "C" int adt_pcg_profile(adt_pcg_plan* p, int enable) {
+    ADT_REQUIRE(p, "adt_pcg_profile: null plan");
+    p->profile = enable ? 1 : 0;
+    p->prof_ms[0] = p->prof_ms[1] = 0.0;
+    p->prof_launches = 0;
+    return ADT_OK;
+}
+
+extern "C" int adt_pcg_profile_read(const adt_pcg_plan* p, double* h_out) {
+    ADT_REQUIRE(p && h_out, "adt_pcg_profile_read: null pointer");
+    std::vector<int> gi(p->n_groups);
+    ADT_HIP_CHECK(hipMemcpy(gi.data(), p->args_h.grp_iters, sizeof(int) * p->n_groups, hipMemcpyDeviceToHost));
+    int64_t group_iters = 0;
+    for (int v : gi) group_iters += v;
+    h_out[0] = p->prof_ms[0];
+    h_out[1] = p->prof_ms[1];
+    h_out[2] = static_cast<double>(p->prof_launches);
+    h_out[3] = static_cast<double>(group_iters);
+    h_out[4] = static_cast<double>(p->L);
+    h_out[5] = static_cast<double>(p->rp);
     return ADT_OK;
 }

@@ -34,6 +34,8 @@ _SIGNATURES = {
     "adt_pcg_layout": (c_int, [P, P, P, P]),
     "adt_pcg_solve": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, P, c_int, P, P, P]),
     "adt_pcg_destroy": (c_int, [P]),
+    "adt_pcg_profile": (c_int, [P, c_int]),
+    "adt_pcg_profile_read": (c_int, [P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
@@ -165,6 +167,15 @@ class PcgPlan:
                                      int(max_iter), int(precond), c_void_p(0) if gnn is None else gnn,
                                      int(check_every), host(iters), host(relres), stream())
         return rc, iters, relres
+
+    def profile(self, enable: bool = True) -> None:
+        check(self._lib.adt_pcg_profile(self._handle, int(bool(enable))))
+
+    def profile_read(self) -> dict:
+        out = np.zeros(6, dtype=np.float64)
+        check(self._lib.adt_pcg_profile_read(self._handle, host(out)))
+        return dict(spmv_ms=out[0], update_ms=out[1], iterations=int(out[2]), group_iterations=int(out[3]),
+                    lanes=int(out[4]), rp=int(out[5]))
 
     def close(self):
         if getattr(self, "_handle", None):

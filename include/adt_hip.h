@@ -113,6 +113,15 @@ int adt_pcg_solve(adt_pcg_plan* plan, const int32_t* d_indptr, const int32_t* d_
                   void* stream);
 int adt_pcg_destroy(adt_pcg_plan* plan);
 
+/* Profiling of the captured iterations (no reference counterpart; used by the benchmark to
+ * report the live roofline of the dominant kernels).  adt_pcg_profile(plan, 1) switches the
+ * iterations from graph replay to direct launches of the same kernels on the caller's stream,
+ * with HIP timing events around every spmv and update launch, and zeroes the accumulators;
+ * adt_pcg_profile(plan, 0) returns to graph replay.  adt_pcg_profile_read fills h_out[6] = {total spmv ms, total update ms, number
+ * of profiled iterations, summed lane-group iterations of the last solve, L, Rp}. */
+int adt_pcg_profile(adt_pcg_plan* plan, int enable);
+int adt_pcg_profile_read(const adt_pcg_plan* plan, double* h_out);
+
 /* ------------------------------------------------------------------------------------------
  * (c) Fused normal sensitivity (Jacobian rows).
  *
