@@ -18,19 +18,25 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <vector>
 
 namespace {
 
 struct Sys {
-    int64_t rows, ptr_off, idx_off, val_off, vec_off, diag_off, b_off;
+    int64_t rows, ptr_off, idx_off, val_off, vec_off, diag_off, b_off, lidx_off;
 };
 
+// ptile: index of the tile's row range in the halo tables of its CSR pattern (-1: none)
 struct Tile {
-    int sys, grp, row0, row1;
+    int sys, grp, row0, row1, ptile, pad;
 };
 
 struct PcgArgs {
+    const uint16_t* lidx;   // per nonzero: LDS slot of its column within the row's tile
+    const int* halo_ptr;    // per pattern tile: range in halo_rows
+    const int* halo_rows;   // rows outside a tile that its rows reference
     const Sys* sys;
     const Tile* tiles;
     const int* grp_tile0;
@@ -62,9 +68,11 @@ struct PcgArgs {
     int* active_total;
     double tol2;
     int rp, groups, r_real, n_tiles;
+    int lds_rows, lds_nnz, lds_own, lds_halo;  // LDS-staged product capacities
 };
 
 constexpr int kRowsPerPass(int L) { return 256 / L; }
+constexpr int kSpmvUnroll = 8;
 
 // Sum over the threads of a 256-wide block that share lane = t % L; thread t < L gets the
 // lane total.  Two independent values are reduced together.  Fixed order.
@@ -313,10 +321,23 @@ __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, i
         for (int row = t.row0 + static_cast<int>(threadIdx.x / L); row < t.row1; row += kRowsPerPass(L)) {
             double acc = 0.0;
             const int k1 = ptr[row + 1];
-            for (int k = ptr[row]; k < k1; ++k) {
-                const int64_t j = idx[k];
-                const double zj = D[j] * R[j * rp];
-                acc = fma(val[k], fma(beta, Pold[j * rp], zj), acc);
+            // kSpmvUnroll entries at a time: all index loads, then all gathers, so one thread keeps
+            // several independent memory requests in flight (padding entries re-read the own row
+            // with a zero coefficient).
+            for (int kb = ptr[row]; kb < k1; kb += kSpmvUnroll) {
+                int64_t jj[kSpmvUnroll];
+                double vv[kSpmvUnroll];
+#pragma unroll
+                for (int u = 0; u < kSpmvUnroll; ++u) {
+                    const bool ok = kb + u < k1;
+                    jj[u] = ok ? idx[kb + u] : row;
+                    vv[u] = ok ? val[kb + u] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kSpmvUnroll; ++u) {
+                    const double zj = D[jj[u]] * R[jj[u] * rp];
+                    acc = fma(vv[u], fma(beta, Pold[jj[u] * rp], zj), acc);
+                }
             }
             const int64_t o = static_cast<int64_t>(row) * rp;
             const double zi = D[row] * R[o];
@@ -324,6 +345,97 @@ __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, i
             Pnew[o] = pi;
             Q[o] = acc;
             pq = fma(pi, acc, pq);
+        }
+    }
+    pq = block_sum1<L>(pq);
+    if (threadIdx.x < L) a.part0[static_cast<int64_t>(tix) * L + threadIdx.x] = pq;
+}
+
+// LDS-staged variant of cg_spmv, three phases per tile:
+//   1. coalesced bulk copy of the tile's CSR slice (values, tile-local column slots, row
+//      pointers) and halo row list into LDS;
+//   2. stage p_new = D^-1 r + beta p_old for the tile's rows and halo rows (independent
+//      loads, unrolled) into LDS, writing p_new of the own rows;
+//   3. q = A p_new entirely from LDS (slot-major with L lanes contiguous: conflict-free).
+// Every vector row is read from memory once per tile and the inner product loop has no
+// global-memory latency in it.
+template <int L>
+__global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ pa, int parity) {
+    extern __shared__ double smem[];
+    const PcgArgs& a = *pa;
+    const int tix = tile_index(a);
+    const Tile t = a.tiles[tix];
+    if (!a.grp_active[t.grp]) return;
+    const Sys S = a.sys[t.sys];
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    const int col = (t.grp % a.groups) * L + lane;
+    const int64_t sc = static_cast<int64_t>(t.sys) * a.rp + col;
+    const bool act = a.flags[sc] == 1;
+    const double beta = act ? a.beta[sc] : 0.0;
+    const int32_t* ptr = a.indptr + S.ptr_off;
+    const int rp = a.rp;
+    const double* R = a.r + S.vec_off + col;
+    const double* Pold = a.p[parity] + S.vec_off + col;
+    double* Pnew = a.p[parity ^ 1] + S.vec_off + col;
+    double* Q = a.q + S.vec_off + col;
+    const double* D = a.dinv + S.diag_off;
+    const int own = t.row1 - t.row0;
+    const int h0 = a.halo_ptr[t.ptile], nh = a.halo_ptr[t.ptile + 1] - h0;
+    const int staged = own + nh;
+    const int e0 = ptr[t.row0], ne = ptr[t.row1] - e0;
+    double* stage = smem;                                      // lds_rows x L
+    double* sval = smem + static_cast<int64_t>(a.lds_rows) * L;  // lds_nnz
+    int* sptr = reinterpret_cast<int*>(sval + a.lds_nnz);        // lds_own + 1
+    int* shalo = sptr + a.lds_own + 1;                           // lds_halo
+    uint16_t* sidx = reinterpret_cast<uint16_t*>(shalo + a.lds_halo);  // lds_nnz
+    // phase 1
+    const double* val = a.values + S.val_off + e0;
+    const uint16_t* lidx = a.lidx + S.lidx_off + e0;
+    for (int e = static_cast<int>(threadIdx.x); e < ne; e += 256) {
+        sval[e] = val[e];
+        sidx[e] = lidx[e];
+    }
+    for (int i = static_cast<int>(threadIdx.x); i <= own; i += 256) sptr[i] = ptr[t.row0 + i] - e0;
+    for (int i = static_cast<int>(threadIdx.x); i < nh; i += 256) shalo[i] = a.halo_rows[h0 + i];
+    __syncthreads();
+    // phase 2
+    if (act) {
+        constexpr int U = 4;
+        for (int s0 = rsub; s0 < staged; s0 += U * kRowsPerPass(L)) {
+            int64_t jj[U];
+            double dj[U], rj[U], pj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int s = s0 + u * kRowsPerPass(L);
+                jj[u] = s < own ? t.row0 + s : (s < staged ? shalo[s - own] : t.row0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                dj[u] = D[jj[u]];
+                rj[u] = R[jj[u] * rp];
+                pj[u] = Pold[jj[u] * rp];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int s = s0 + u * kRowsPerPass(L);
+                if (s < staged) {
+                    const double v = fma(beta, pj[u], dj[u] * rj[u]);
+                    stage[s * L + lane] = v;
+                    if (s < own) Pnew[jj[u] * rp] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // phase 3
+    double pq = 0.0;
+    if (act) {
+        for (int s = rsub; s < own; s += kRowsPerPass(L)) {
+            const int k1 = sptr[s + 1];
+            double acc = 0.0;
+            for (int k = sptr[s]; k < k1; ++k) acc = fma(sval[k], stage[static_cast<int>(sidx[k]) * L + lane], acc);
+            Q[static_cast<int64_t>(t.row0 + s) * rp] = acc;
+            pq = fma(stage[s * L + lane], acc, pq);
         }
     }
     pq = block_sum1<L>(pq);
@@ -490,6 +602,11 @@ struct adt_pcg_plan {
     int graph_iters = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int profile = 0;
+    int lds_rows = 0;  // staged rows of the LDS spmv (0: gather kernel)
+    int lds_nnz = 0, lds_halo = 0;
+    uint16_t* lidx_d = nullptr;
+    int* halo_ptr_d = nullptr;
+    int* halo_rows_d = nullptr;
     hipEvent_t pev[2][kMaxChunk][4] = {};
     double prof_ms[2] = {0.0, 0.0};
     int64_t prof_launches = 0;
@@ -506,7 +623,8 @@ adt_pcg_plan::~adt_pcg_plan() {
             for (auto& e : it)
                 if (e) (void)hipEventDestroy(e);
     if (cap) (void)hipStreamDestroy(cap);
-    void* bufs[] = {sys_d, tiles_d, grp_tile0_d, grp_ntiles_d, vec, dinv, part, scal, ints, args_d};
+    void* bufs[] = {sys_d, tiles_d, grp_tile0_d, grp_ntiles_d, vec, dinv, part, scal, ints, args_d, lidx_d, halo_ptr_d,
+                    halo_rows_d};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h_active) (void)hipHostFree(h_active);
@@ -523,11 +641,18 @@ int dalloc(T** p, int64_t count) {
     return ADT_OK;
 }
 
+int64_t lds_bytes_for(int staged, int nnz, int own, int halo, int lanes);
+
 template <int L>
 void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, hipEvent_t* events) {
     const PcgArgs* pa = p->args_d;
     if (events) (void)hipEventRecord(events[0], st);
-    hipLaunchKernelGGL(cg_spmv<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+    if (p->lds_rows > 0)
+        hipLaunchKernelGGL(cg_spmv_lds<L>, dim3(p->n_tiles), dim3(256),
+                           static_cast<size_t>(lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->lds_halo, L)), st,
+                           pa, parity);
+    else
+        hipLaunchKernelGGL(cg_spmv<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
     if (events) (void)hipEventRecord(events[1], st);
     hipLaunchKernelGGL(cg_alpha<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
     if (events) (void)hipEventRecord(events[2], st);
@@ -625,9 +750,55 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk) {
 
 }  // namespace
 
+namespace {
+
+// Halo tables of one CSR pattern cut into row tiles of rb rows.
+struct PatternTiles {
+    std::vector<uint16_t> lidx;
+    std::vector<int> halo_ptr;  // one entry per tile (start), closed by the caller
+    std::vector<int> halo_rows;
+    int max_staged = 0, max_nnz = 0, max_halo = 0;
+};
+
+int64_t lds_bytes_for(int staged, int nnz, int own, int halo, int lanes) {
+    return static_cast<int64_t>(staged) * lanes * 8 + static_cast<int64_t>(nnz) * 8 + (own + 1 + halo) * 4 +
+           static_cast<int64_t>(nnz) * 2;
+}
+
+PatternTiles build_pattern_tiles(const int32_t* ptr, const int32_t* idx, int rows, int rb) {
+    PatternTiles pt;
+    pt.lidx.resize(static_cast<size_t>(ptr[rows]));
+    std::vector<int> halo;
+    for (int r0 = 0; r0 < rows; r0 += rb) {
+        const int r1 = std::min(r0 + rb, rows);
+        halo.clear();
+        for (int k = ptr[r0]; k < ptr[r1]; ++k)
+            if (idx[k] < r0 || idx[k] >= r1) halo.push_back(idx[k]);
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        const int own = r1 - r0;
+        pt.max_staged = std::max(pt.max_staged, own + static_cast<int>(halo.size()));
+        pt.max_nnz = std::max(pt.max_nnz, ptr[r1] - ptr[r0]);
+        pt.max_halo = std::max(pt.max_halo, static_cast<int>(halo.size()));
+        for (int k = ptr[r0]; k < ptr[r1]; ++k) {
+            const int c = idx[k];
+            const int slot = (c >= r0 && c < r1)
+                                 ? c - r0
+                                 : own + static_cast<int>(std::lower_bound(halo.begin(), halo.end(), c) - halo.begin());
+            pt.lidx[k] = static_cast<uint16_t>(slot);
+        }
+        pt.halo_ptr.push_back(static_cast<int>(pt.halo_rows.size()));
+        pt.halo_rows.insert(pt.halo_rows.end(), halo.begin(), halo.end());
+    }
+    return pt;
+}
+
+}  // namespace
+
 extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, int rows_per_tile,
                               const int32_t* h_rows, const int64_t* h_ptr_off, const int64_t* h_idx_off,
-                              const int64_t* h_val_off, const int64_t* h_b_off) {
+                              const int64_t* h_val_off, const int64_t* h_b_off, const int32_t* h_indptr,
+                              const int32_t* h_indices, int lds_bytes) {
     ADT_REQUIRE(out && h_rows && h_ptr_off && h_idx_off && h_val_off, "adt_pcg_create: null pointer");
     ADT_REQUIRE(n_sys >= 1 && r >= 1, "adt_pcg_create: need at least one system and one right-hand side");
     ADT_REQUIRE(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "adt_pcg_create: lanes must be 8/16/32/64");
@@ -640,15 +811,69 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     p->groups = p->rp / lanes;
     const int rpp = 256 / lanes;
     p->rb = (rows_per_tile + rpp - 1) / rpp * rpp;
-    std::vector<Tile> tiles;
-    std::vector<int> t0, nt;
-    int64_t vo = 0, dvo = 0;
-    for (int s = 0; s < n_sys; ++s) {
+    for (int s = 0; s < n_sys; ++s)
         if (h_rows[s] < 1) {
             delete p;
             return adt::set_error(ADT_ERR_INVALID, "adt_pcg_create: system %d has no rows", s);
         }
-        Sys S{h_rows[s], h_ptr_off[s], h_idx_off[s], h_val_off[s], vo, dvo, h_b_off ? h_b_off[s] : vo};
+    // Halo tables per distinct pattern; shrink the tile height until the staged rows fit in LDS.
+    typedef std::tuple<int64_t, int64_t, int> PatKey;
+    std::map<PatKey, int> pattern_of;
+    std::vector<PatKey> patterns;
+    for (int s = 0; s < n_sys; ++s) {
+        PatKey key{h_ptr_off[s], h_idx_off[s], h_rows[s]};
+        if (!pattern_of.count(key)) {
+            pattern_of[key] = static_cast<int>(patterns.size());
+            patterns.push_back(key);
+        }
+    }
+    std::vector<PatternTiles> ptiles;
+    if (h_indptr && h_indices && lds_bytes > 0) {
+        int device = 0, limit = 0;
+        if (hipGetDevice(&device) == hipSuccess &&
+            hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+            limit > 0)
+            lds_bytes = std::min(lds_bytes, limit - 8192);  // the block reductions keep static LDS
+        for (int rb = p->rb; rb >= rpp; rb = (rb / 2 + rpp - 1) / rpp * rpp) {
+            ptiles.clear();
+            int max_staged = 0, max_nnz = 0, max_halo = 0;
+            for (const auto& key : patterns) {
+                ptiles.push_back(build_pattern_tiles(h_indptr + std::get<0>(key), h_indices + std::get<1>(key),
+                                                     std::get<2>(key), rb));
+                max_staged = std::max(max_staged, ptiles.back().max_staged);
+                max_nnz = std::max(max_nnz, ptiles.back().max_nnz);
+                max_halo = std::max(max_halo, ptiles.back().max_halo);
+            }
+            if (lds_bytes_for(max_staged, max_nnz, rb, max_halo, lanes) <= lds_bytes && max_staged < 65536) {
+                p->rb = rb;
+                p->lds_rows = max_staged;
+                p->lds_nnz = max_nnz;
+                p->lds_halo = max_halo;
+                break;
+            }
+            if (rb == rpp) break;
+        }
+        if (!p->lds_rows) ptiles.clear();
+    }
+    std::vector<int64_t> lidx_base(patterns.size(), 0), ptile_base(patterns.size(), 0);
+    std::vector<uint16_t> lidx_all;
+    std::vector<int> halo_ptr_all, halo_rows_all;
+    for (size_t q = 0; q < ptiles.size(); ++q) {
+        lidx_base[q] = static_cast<int64_t>(lidx_all.size());
+        ptile_base[q] = static_cast<int64_t>(halo_ptr_all.size());
+        lidx_all.insert(lidx_all.end(), ptiles[q].lidx.begin(), ptiles[q].lidx.end());
+        const int shift = static_cast<int>(halo_rows_all.size());
+        for (int h : ptiles[q].halo_ptr) halo_ptr_all.push_back(h + shift);
+        halo_rows_all.insert(halo_rows_all.end(), ptiles[q].halo_rows.begin(), ptiles[q].halo_rows.end());
+    }
+    halo_ptr_all.push_back(static_cast<int>(halo_rows_all.size()));
+    std::vector<Tile> tiles;
+    std::vector<int> t0, nt;
+    int64_t vo = 0, dvo = 0;
+    for (int s = 0; s < n_sys; ++s) {
+        const int q = pattern_of[PatKey{h_ptr_off[s], h_idx_off[s], h_rows[s]}];
+        Sys S{h_rows[s], h_ptr_off[s], h_idx_off[s], h_val_off[s], vo, dvo, h_b_off ? h_b_off[s] : vo,
+              p->lds_rows ? lidx_base[q] : 0};
         p->sys_h.push_back(S);
         vo += static_cast<int64_t>(h_rows[s]) * p->rp;
         dvo += h_rows[s];
@@ -656,7 +881,8 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
             t0.push_back(static_cast<int>(tiles.size()));
             int count = 0;
             for (int r0 = 0; r0 < h_rows[s]; r0 += p->rb, ++count)
-                tiles.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + p->rb, h_rows[s])});
+                tiles.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + p->rb, h_rows[s]),
+                                     p->lds_rows ? static_cast<int>(ptile_base[q]) + count : -1, 0});
             nt.push_back(count);
         }
     }
@@ -671,9 +897,20 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
         (rc = dalloc(&p->vec, 5 * vo)) || (rc = dalloc(&p->dinv, dvo)) ||
         (rc = dalloc(&p->part, 3 * static_cast<int64_t>(p->n_tiles) * lanes)) ||
         (rc = dalloc(&p->scal, 7 * nsc)) || (rc = dalloc(&p->ints, 3 * nsc + 2 * p->n_groups + 1)) ||
-        (rc = dalloc(&p->args_d, 1))) {
+        (rc = dalloc(&p->args_d, 1)) || (rc = dalloc(&p->lidx_d, static_cast<int64_t>(lidx_all.size()))) ||
+        (rc = dalloc(&p->halo_ptr_d, static_cast<int64_t>(halo_ptr_all.size()))) ||
+        (rc = dalloc(&p->halo_rows_d, static_cast<int64_t>(halo_rows_all.size())))) {
         delete p;
         return rc;
+    }
+    if ((!lidx_all.empty() &&
+         hipMemcpy(p->lidx_d, lidx_all.data(), sizeof(uint16_t) * lidx_all.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(p->halo_ptr_d, halo_ptr_all.data(), sizeof(int) * halo_ptr_all.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        (!halo_rows_all.empty() && hipMemcpy(p->halo_rows_d, halo_rows_all.data(), sizeof(int) * halo_rows_all.size(),
+                                             hipMemcpyHostToDevice) != hipSuccess)) {
+        delete p;
+        return adt::set_error(ADT_ERR_HIP, "adt_pcg_create: halo table upload failed");
     }
     if (hipHostMalloc(reinterpret_cast<void**>(&p->h_active), 2 * sizeof(int)) != hipSuccess ||
         hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess ||
@@ -690,6 +927,9 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
         return adt::set_error(ADT_ERR_HIP, "adt_pcg_create: metadata upload failed");
     }
     PcgArgs& a = p->args_h;
+    a.lidx = p->lidx_d;
+    a.halo_ptr = p->halo_ptr_d;
+    a.halo_rows = p->halo_rows_d;
     a.sys = p->sys_d;
     a.tiles = p->tiles_d;
     a.grp_tile0 = p->grp_tile0_d;
@@ -717,10 +957,22 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     a.grp_iters = p->ints + 3 * nsc + p->n_groups;
     a.active_total = p->ints + 3 * nsc + 2 * p->n_groups;
     a.rp = p->rp;
+    a.lds_rows = p->lds_rows;
+    a.lds_nnz = p->lds_nnz;
+    a.lds_own = p->rb;
+    a.lds_halo = p->lds_halo;
     a.groups = p->groups;
     a.r_real = r;
     a.n_tiles = p->n_tiles;
     *out = p;
+    return ADT_OK;
+}
+
+extern "C" int adt_pcg_tiling(const adt_pcg_plan* p, int* h_rows_per_tile, int* h_tiles, int* h_lds_rows) {
+    ADT_REQUIRE(p, "adt_pcg_tiling: null plan");
+    if (h_rows_per_tile) *h_rows_per_tile = p->rb;
+    if (h_tiles) *h_tiles = p->n_tiles;
+    if (h_lds_rows) *h_lds_rows = p->lds_rows;
     return ADT_OK;
 }
 
@@ -805,7 +1057,7 @@ extern "C" int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, con
     std::vector<Sys> sys(n_sys);
     int max_rows = 0;
     for (int s = 0; s < n_sys; ++s) {
-        sys[s] = Sys{h_rows[s], h_ptr_off[s], h_idx_off[s], h_val_off[s], h_vec_off[s], 0, h_vec_off[s]};
+        sys[s] = Sys{h_rows[s], h_ptr_off[s], h_idx_off[s], h_val_off[s], h_vec_off[s], 0, h_vec_off[s], 0};
         max_rows = std::max(max_rows, h_rows[s]);
     }
     if (max_rows == 0) return ADT_OK;

@@ -23,6 +23,7 @@ import torch
 from scipy.special import k0 as bessel_k0
 
 from . import fem, native
+from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
 from .field_cache import FieldCache, newest_batches_first
 from .inputs import (cell_vector, check_finite, conductivity_model, conductivity_series, cotangent_series,
@@ -200,17 +201,25 @@ def pair_scatter(slots: np.ndarray, count: int, normal: np.ndarray, reciprocal: 
 
 @dataclass
 class _Fields:
-    """Assembled values ``(W, nnz)`` and interleaved total fields ``(W, N, Rp)`` of one model."""
+    """Assembled values ``(W, nnz)`` and interleaved total fields ``(W, N, Rp)`` of one model,
+    in the device DOF numbering; ``inv`` maps mesh nodes to device rows."""
 
     values: Tensor
     blocks: Tensor
     count: int
+    inv: Tensor
+
+    @property
+    def rows(self) -> Tensor:
+        """Field rows ``(W, E, N)`` in the device numbering (kernel input)."""
+
+        return self.blocks[..., : self.count].transpose(1, 2).contiguous()
 
     @property
     def total(self) -> Tensor:
-        """Field rows ``(W, E, N)`` as the reference stores them."""
+        """Field rows ``(W, E, N)`` in mesh node order, as the reference stores them."""
 
-        return self.blocks[..., : self.count].transpose(1, 2).contiguous()
+        return self.blocks[:, self.inv, : self.count].transpose(1, 2).contiguous()
 
     def on(self, device) -> Tensor:
         return self.total.to(device)
@@ -245,7 +254,8 @@ class _Columns:
 
 
 class _Device:
-    """Device-resident operator data of one discretization."""
+    """Device-resident operator data of one discretization, in a Hilbert-curve DOF numbering
+    (``perm`` maps device rows to mesh nodes, ``inv`` mesh nodes to device rows)."""
 
     def __init__(self, op: "ERTForward2p5D"):
         d = op.discretization
@@ -253,15 +263,23 @@ class _Device:
         self.device = dev
         f64 = dict(dtype=torch.float64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
-        self.indptr = torch.as_tensor(d.pattern.indptr, **i32)
-        self.indices = torch.as_tensor(d.pattern.indices, **i32)
-        nnz = d.pattern.nnz
+        perm = dof_order(np.asarray(d.dof_nodes, dtype=float))
+        inv = inverse(perm)
+        self.perm_host, self.inv_host = perm, inv
+        self.inv = torch.as_tensor(inv, dtype=torch.long, device=dev)
+        cells = inv[d.cell_dofs.numpy()]
+        bdofs = inv[d.boundary_dofs.numpy()]
+        pattern = sparse_pattern(cells, bdofs, d.dof_count)
+        self.pattern = pattern
+        self.indptr = torch.as_tensor(pattern.indptr, **i32)
+        self.indices = torch.as_tensor(pattern.indices, **i32)
+        nnz = pattern.nnz
         self.nnz = nnz
-        inverse = np.concatenate((d.pattern.volume_inverse, d.pattern.boundary_inverse))
-        sptr, sent = slot_map(inverse, nnz)
+        inverse_map = np.concatenate((pattern.volume_inverse, pattern.boundary_inverse))
+        sptr, sent = slot_map(inverse_map, nnz)
         self.slot_ptr = torch.as_tensor(sptr, **i32)
         self.slot_entry = torch.as_tensor(sent, **i32)
-        vptr, vent = slot_map(d.pattern.volume_inverse, nnz)
+        vptr, vent = slot_map(pattern.volume_inverse, nnz)
         self.vslot_ptr = torch.as_tensor(vptr, **i32)
         self.vslot_entry = torch.as_tensor(vent, **i32)
         self.stiff = torch.as_tensor(d.stiffness, **f64).contiguous()
@@ -275,13 +293,15 @@ class _Device:
         parents = d.parent_cell_ids.numpy()
         self.cell_col = torch.as_tensor(parents, **i32)
         self.bnd_col = torch.as_tensor(parents[d.boundary_cells.numpy()], **i32)
-        self.cell_dofs = torch.as_tensor(d.cell_dofs.numpy(), **i32).contiguous()
-        self.bnd_dofs = torch.as_tensor(d.boundary_dofs.numpy(), **i32).contiguous()
+        self.cell_dofs = torch.as_tensor(cells, **i32).contiguous()
+        # the Jacobian kernel walks cells in mesh order and reads fields in mesh node order
+        self.cell_dofs_mesh = torch.as_tensor(d.cell_dofs.numpy(), **i32).contiguous()
+        self.bnd_dofs = torch.as_tensor(bdofs, **i32).contiguous()
         bcells = d.boundary_cells.numpy()
         order = np.argsort(bcells, kind="stable")
         cptr = np.concatenate(([0], np.cumsum(np.bincount(bcells, minlength=d.cell_dofs.shape[0]))))
         self.cell_bnd = torch.as_tensor(np.concatenate((cptr, order)).astype(np.int32), **i32)
-        emat = d.electrode_matrix.double().numpy()
+        emat = d.electrode_matrix.double().numpy()[:, perm]
         self.electrodes_t = torch.as_tensor(emat.T.copy(), **f64)  # (N, E)
         en, ew = electrode_stencil(emat)
         self.enode = torch.as_tensor(en, **i32)
@@ -296,7 +316,7 @@ class _Device:
         # unit primary (W, N, Rp) and the reference right-hand side A(1) u_p
         primary = op._analytic_primary()
         self.primary = torch.zeros((self.W, self.N, self.rp), **f64)
-        self.primary[:, :, : self.E] = torch.as_tensor(primary, **f64).transpose(1, 2)
+        self.primary[:, :, : self.E] = torch.as_tensor(primary[:, :, perm], **f64).transpose(1, 2)
         ones = torch.ones((1, d.cell_dofs.shape[0]), **f64)
         unit_values = op._assemble_device(self, ones)
         self.rhs = torch.empty_like(self.primary)
@@ -456,7 +476,8 @@ class ERTForward2p5D:
         val_off = np.arange(n_sys, dtype=np.int64) * dev.nnz
         b_off = (np.arange(n_sys) % dev.W).astype(np.int64) * dev.N * x.shape[-1] if shared_rhs else None
         plan = self._solver.plan((key, n_sys), rows, r, pattern_offsets=(np.zeros(n_sys, np.int64),
-                                 np.zeros(n_sys, np.int64), val_off), b_off=b_off, lanes=None)
+                                 np.zeros(n_sys, np.int64), val_off), b_off=b_off, lanes=None,
+                                 host_pattern=(dev.pattern.indptr, dev.pattern.indices))
         if plan.rp != x.shape[-1]:
             raise RuntimeError(f"interleaved width {x.shape[-1]} does not match the plan's {plan.rp}")
         expected = np.arange(n_sys, dtype=np.int64) * dev.N * plan.rp
@@ -484,7 +505,7 @@ class ERTForward2p5D:
         x = x.reshape(T, dev.W, dev.N, dev.rp)
         fields = []
         for t in range(T):
-            f = _Fields(values[t * dev.W:(t + 1) * dev.W], x[t], dev.E)
+            f = _Fields(values[t * dev.W:(t + 1) * dev.W], x[t], dev.E, dev.inv)
             fields.append(f)
         return fields
 
@@ -550,7 +571,7 @@ class ERTForward2p5D:
     def _integrated(self, fields: _Fields) -> Tensor:
         dev = self._dev
         q = torch.as_tensor(dev.q_host, dtype=torch.float64, device=dev.device)
-        return torch.tensordot(q, fields.blocks[..., : dev.E], dims=([0], [0])).T.contiguous()
+        return torch.tensordot(q, fields.blocks[:, dev.inv, : dev.E], dims=([0], [0])).T.contiguous()
 
     def _response(self, fields, potentials, normal, reciprocal, currents) -> ForwardResponse:
         resistance = self._combine(normal, reciprocal).cpu().to(FLOAT_DTYPE)
@@ -740,10 +761,10 @@ class ERTForward2p5D:
         """``-sum_w w_w (u_M - u_N)^T (K + k_w^2 M) (u_A - u_B)`` by the fused kernel (``ert2p5d.py:984-1010``)."""
 
         dev = self._dev
-        phi = fields.total  # (W, E, N)
+        phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
         stiff = columns.select(dev.stiff, 0).contiguous()
         mass = columns.select(dev.mass, 0).contiguous()
-        dofs = columns.select(dev.cell_dofs, 0).contiguous()
+        dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
         ws, wm = dev.q_host, dev.q_host * dev.k2_host
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         D = self.survey.measurement_count

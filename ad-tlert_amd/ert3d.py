@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from . import fem, native
+from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
 from .ert2p5d import _Columns, electrode_stencil, slot_map
 from .field_cache import FieldCache, newest_batches_first, step_batches
@@ -131,7 +132,6 @@ class ERTForward3D:
                                 np.asarray(mesh.boundary_face_cells), np.asarray(mesh.boundary_face_centers),
                                 np.asarray(mesh.boundary_face_areas), np.asarray(mesh.boundary_face_normals),
                                 np.asarray(mesh.surface_face_mask), electrodes.mean(axis=0), self.element_order)
-        self._build_assembly_routing()
         self._shared_source_receiver_rhs = bool((not self.singularity_removal or not mesh.is_flat_surface)
                                                 and np.array_equal(self._current_electrode_ids,
                                                                    self._receiver_electrode_ids))
@@ -152,18 +152,18 @@ class ERTForward3D:
         far = self._boundary_connectivity[~np.asarray(self.mesh.surface_face_mask, dtype=bool)]
         return np.unique(far.reshape(-1))
 
-    def _build_assembly_routing(self) -> None:
-        """Local-entry -> CSR slot routing with Dirichlet masking (``ert3d.py:290-351``)."""
+    def _assembly_routing(self, conn: np.ndarray, fixed: np.ndarray):
+        """Local-entry -> CSR slot routing for connectivity ``conn`` with the Dirichlet nodes
+        ``fixed`` masked (``ert3d.py:290-351``): CSR arrays, per-entry slot (-1 masked), unit
+        diagonal slots and the masked flat templates."""
 
-        cells = self._cell_connectivity
-        k = cells.shape[1]
+        k = conn.shape[1]
         n = self.N
-        rows = np.repeat(cells, k, axis=1).reshape(-1)
-        cols = np.tile(cells, (1, k)).reshape(-1)
+        C = conn.shape[0]
+        rows = np.repeat(conn, k, axis=1).reshape(-1)
+        cols = np.tile(conn, (1, k)).reshape(-1)
         keys = rows * n + cols
-        C = cells.shape[0]
-        if self.boundary_mode == "dirichlet":
-            fixed = self._dirichlet_node_ids()
+        if fixed.size:
             mask = np.zeros(n, dtype=bool)
             mask[fixed] = True
             volume_pair = ~mask[rows] & ~mask[cols]
@@ -175,18 +175,14 @@ class ERTForward3D:
             flat = np.where(pm, self._cell_templates.reshape(C, -1), 0.0)
             flat_v = np.where(pm, self._volume_templates.reshape(C, -1), 0.0)
         else:
-            unique, inverse = np.unique(keys, return_inverse=True)
-            groups = inverse.reshape(-1)
+            unique, inv = np.unique(keys, return_inverse=True)
+            groups = inv.reshape(-1)
             fixed_slots = np.zeros(0, dtype=np.int64)
             flat = self._cell_templates.reshape(C, -1)
             flat_v = self._volume_templates.reshape(C, -1)
-        self._csr_indices = (unique % n).astype(np.int32)
-        self._csr_indptr = np.concatenate(([0], np.cumsum(np.bincount(unique // n, minlength=n)))).astype(np.int32)
-        self._assembly_groups = groups
-        self._fixed_diag_slots = fixed_slots
-        self._flat_templates = np.ascontiguousarray(flat)
-        self._flat_volume_templates = np.ascontiguousarray(flat_v)
-        self.nnz = int(unique.size)
+        indices = (unique % n).astype(np.int32)
+        indptr = np.concatenate(([0], np.cumsum(np.bincount(unique // n, minlength=n)))).astype(np.int32)
+        return indptr, indices, groups, fixed_slots, np.ascontiguousarray(flat), np.ascontiguousarray(flat_v)
 
     def _primary(self, positions: np.ndarray) -> np.ndarray:
         """``(n, N)`` half-space primaries ``(1/r + 1/r')/(4 pi)`` with the node fallback radius (``ert3d.py:367-385``)."""
@@ -214,30 +210,42 @@ class ERTForward3D:
         return self._dev_state
 
     def _build_device(self):
+        """Device state in a Hilbert-curve DOF numbering (``perm``: device row -> mesh DOF)."""
+
         dev = native.device()
         f64 = dict(dtype=torch.float64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
         s = type("Dev3D", (), {})()
         s.device = dev
-        s.indptr = torch.as_tensor(self._csr_indptr, **i32)
-        s.indices = torch.as_tensor(self._csr_indices, **i32)
-        sptr, sent = slot_map(self._assembly_groups, self.nnz)
+        perm = dof_order(self._dof_nodes)
+        inv = inverse(perm)
+        s.perm_host, s.inv_host = perm, inv
+        s.inv = torch.as_tensor(inv, dtype=torch.long, device=dev)
+        conn = inv[self._cell_connectivity]
+        fixed_mesh = self._dirichlet_node_ids() if self.boundary_mode == "dirichlet" else np.zeros(0, np.int64)
+        fixed_nodes = np.sort(inv[fixed_mesh])
+        indptr, indices, groups, fixed_slots, flat, flat_v = self._assembly_routing(conn, fixed_nodes)
+        self.nnz = int(indices.size)
+        s.indptr = torch.as_tensor(indptr, **i32)
+        s.indices = torch.as_tensor(indices, **i32)
+        s.host_pattern = (indptr, indices)
+        sptr, sent = slot_map(groups, self.nnz)
         s.slot_ptr, s.slot_entry = torch.as_tensor(sptr, **i32), torch.as_tensor(sent, **i32)
-        s.fixed = torch.as_tensor(self._fixed_diag_slots.astype(np.int32), **i32) if self._fixed_diag_slots.size else None
-        k = self._cell_connectivity.shape[1]
-        C = self._cell_connectivity.shape[0]
-        s.templates = torch.as_tensor(self._flat_templates.reshape(C, k, k), **f64).contiguous()
-        s.volume_templates = torch.as_tensor(self._flat_volume_templates.reshape(C, k, k), **f64).contiguous()
+        s.fixed = torch.as_tensor(fixed_slots.astype(np.int32), **i32) if fixed_slots.size else None
+        k = conn.shape[1]
+        C = conn.shape[0]
+        s.templates = torch.as_tensor(flat.reshape(C, k, k), **f64).contiguous()
+        s.volume_templates = torch.as_tensor(flat_v.reshape(C, k, k), **f64).contiguous()
         s.cell_col = torch.arange(C, **i32)
-        s.cell_dofs = torch.as_tensor(self._cell_connectivity, **i32).contiguous()
+        s.cell_dofs = torch.as_tensor(conn, **i32).contiguous()
+        s.cell_dofs_mesh = torch.as_tensor(self._cell_connectivity, **i32).contiguous()
         s.N, s.nnz = self.N, self.nnz
         nc, nr = self._current_electrode_ids.size, self._receiver_electrode_ids.size
         s.nc, s.nr = nc, nr
         s.lanes_c, s.lanes_r = choose_lanes(nc), choose_lanes(nr)
         s.rp_c, s.rp_r = -(-nc // s.lanes_c) * s.lanes_c, -(-nr // s.lanes_r) * s.lanes_r
-        fixed_nodes = self._dirichlet_node_ids() if self.boundary_mode == "dirichlet" else np.zeros(0, np.int64)
         # receiver rows (Dirichlet-zeroed electrode interpolation rows) as stencils and dense
-        en, ew = self._electrode_nodes.copy(), self._electrode_weights.copy()
+        en, ew = inv[self._electrode_nodes], self._electrode_weights.copy()
         if fixed_nodes.size:
             ew[np.isin(en, fixed_nodes)] = 0.0
         recv = self._receiver_electrode_ids
@@ -246,12 +254,12 @@ class ERTForward3D:
         dense = np.zeros((nr, self.N))
         np.add.at(dense, (np.repeat(np.arange(nr), en.shape[1]), en[recv].reshape(-1)), ew[recv].reshape(-1))
         s.receiver_rows_t = torch.as_tensor(dense.T.copy(), **f64)  # (N, n_recv)
-        # right-hand sides
+        # right-hand sides (device numbering)
         positions = np.asarray(self.survey.electrode_positions, dtype=float)
         unit = self._assemble(s, torch.ones((1, C), **f64), robin=True, fixed=True)
         s.unit_values = unit
-        prim_c = self._primary(positions[self._current_electrode_ids])
-        prim_r = self._primary(positions[recv])
+        prim_c = self._primary(positions[self._current_electrode_ids])[:, perm]
+        prim_r = self._primary(positions[recv])[:, perm]
         s.primary_c = self._interleave(prim_c, s.rp_c, dev)
         s.primary_r = self._interleave(prim_r, s.rp_r, dev)
         if self.singularity_removal and self.mesh.is_flat_surface:
@@ -312,7 +320,7 @@ class ERTForward3D:
         val_off = np.arange(n_sys, dtype=np.int64) * self.nnz
         b_off = np.zeros(n_sys, dtype=np.int64) if shared_rhs else None
         plan = self._solver.plan((key, n_sys), rows, r, pattern_offsets=(np.zeros(n_sys, np.int64),
-                                 np.zeros(n_sys, np.int64), val_off), b_off=b_off)
+                                 np.zeros(n_sys, np.int64), val_off), b_off=b_off, host_pattern=s.host_pattern)
         if plan.rp != x.shape[-1] or not np.array_equal(plan.vec_off, np.arange(n_sys) * self.N * plan.rp):
             raise RuntimeError("unexpected PCG vector layout")
         self._solver.solve(plan, s.indptr, s.indices, values.contiguous(), rhs.contiguous(), x)
@@ -413,7 +421,8 @@ class ERTForward3D:
         apparent = self._geometric_factors() * resistance / torch.as_tensor(currents, dtype=FLOAT_DTYPE)
         if not include_fields:
             return ForwardResponse3D(apparent, resistance)
-        nodes = fields.current[:, : self._dev.nc].T.cpu().numpy()
+        s = self._dev
+        nodes = fields.current[s.inv, : s.nc].T.cpu().numpy()
         en, ew = self._electrode_nodes, self._electrode_weights
         at_electrodes = np.einsum("snk,ek->se", nodes[:, en], ew) if nodes.size else np.zeros((0, 0))
         return ForwardResponse3D(apparent, resistance, electrode_potentials=self._per_electrode(at_electrodes),
@@ -569,16 +578,16 @@ class ERTForward3D:
         fields, _ = self._fields_series(sigma[None])
         (receiver,) = self._receivers(fields, sigma[None])
         S = s.nc + s.nr
-        phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)
-        phi[0, : s.nc] = fields[0].current[:, : s.nc].T
-        phi[0, s.nc:] = receiver[:, : s.nr].T
+        phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)  # mesh DOF order
+        phi[0, : s.nc] = fields[0].current[s.inv, : s.nc].T
+        phi[0, s.nc:] = receiver[s.inv, : s.nr].T
         a, b, m, n = (x.copy() for x in s.abmn_host)
         a = np.where(a == s.nc, S, a)
         b = np.where(b == s.nc, S, b)
         m, n = m + s.nc, n + s.nc
         idx = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in (a, b, m, n)]
         templates = columns.select(s.templates if include_robin_boundary_derivative else s.volume_templates, 0)
-        dofs = columns.select(s.cell_dofs, 0).contiguous()
+        dofs = columns.select(s.cell_dofs_mesh, 0).contiguous()
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         result = torch.empty((count, columns.count), dtype=torch.float64, device=s.device)
         for start in range(0, count, batch_size):

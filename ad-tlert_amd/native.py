@@ -19,6 +19,7 @@ LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().paren
 
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
 PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+LDS_BYTES = int(os.environ.get("ADT_SPMV_LDS_BYTES", 49152))  # 0 disables the LDS-staged product
 
 c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 P = c_void_p
@@ -30,7 +31,8 @@ _SIGNATURES = {
     "adt_assemble": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_i64, P, P, P, P, P, P, P, P, P,
                              P, c_int, P, P]),
     "adt_spmm": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P]),
-    "adt_pcg_create": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P, P, P]),
+    "adt_pcg_create": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P, c_int]),
+    "adt_pcg_tiling": (c_int, [P, P, P, P]),
     "adt_pcg_layout": (c_int, [P, P, P, P]),
     "adt_pcg_solve": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, P, c_int, P, P, P]),
     "adt_pcg_destroy": (c_int, [P]),
@@ -141,7 +143,8 @@ def spmm(layout, indptr, indices, values, x, y, mode=0):
 class PcgPlan:
     """Owner of one ``adt_pcg_plan`` (batch layout + Krylov workspace)."""
 
-    def __init__(self, rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off=None):
+    def __init__(self, rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off=None, host_pattern=None,
+                 lds_bytes=LDS_BYTES):
         self._lib = library()
         self.rows = np.ascontiguousarray(rows, dtype=np.int32)
         self.n_sys = int(self.rows.size)
@@ -150,15 +153,23 @@ class PcgPlan:
         self.idx_off = np.ascontiguousarray(idx_off, dtype=np.int64)
         self.val_off = np.ascontiguousarray(val_off, dtype=np.int64)
         self.b_off = None if b_off is None else np.ascontiguousarray(b_off, dtype=np.int64)
+        if host_pattern is not None:
+            hp = tuple(np.ascontiguousarray(a, dtype=np.int32) for a in host_pattern)
         handle = c_void_p()
         check(self._lib.adt_pcg_create(ctypes.byref(handle), self.n_sys, self.r, self.lanes, int(rows_per_tile),
                                        host(self.rows), host(self.ptr_off), host(self.idx_off), host(self.val_off),
-                                       c_void_p(0) if self.b_off is None else host(self.b_off)))
+                                       c_void_p(0) if self.b_off is None else host(self.b_off),
+                                       c_void_p(0) if host_pattern is None else host(hp[0]),
+                                       c_void_p(0) if host_pattern is None else host(hp[1]),
+                                       int(lds_bytes) if host_pattern is not None else 0))
         self._handle = handle
         rp, vec_len = c_int(0), c_i64(0)
         self.vec_off = np.zeros(self.n_sys, dtype=np.int64)
         check(self._lib.adt_pcg_layout(handle, ctypes.byref(rp), ctypes.byref(vec_len), host(self.vec_off)))
         self.rp, self.vec_len = rp.value, vec_len.value
+        rb, tiles, lds_rows = c_int(0), c_int(0), c_int(0)
+        check(self._lib.adt_pcg_tiling(handle, ctypes.byref(rb), ctypes.byref(tiles), ctypes.byref(lds_rows)))
+        self.rows_per_tile, self.tiles, self.lds_rows = rb.value, tiles.value, lds_rows.value
 
     def solve(self, indptr, indices, values, b, x, tol, max_iter, *, precond=PRECOND_JACOBI, gnn=None, check_every=16):
         iters = np.zeros((self.n_sys, self.r), dtype=np.int32)

@@ -38,14 +38,15 @@ def padded(r: int, lanes: int) -> int:
     return -(-r // lanes) * lanes
 
 
-def choose_rows_per_tile(n_sys: int, rows: int, r: int, lanes: int, target_tiles: int = 4096) -> int:
-    """Rows per workgroup tile: about ``target_tiles`` tiles over the batch, at least one row per
-    thread group and at most 4096 rows (bounded per-tile partial reductions)."""
+def choose_rows_per_tile(n_sys: int, rows: int, r: int, lanes: int, target_tiles: int = 16384) -> int:
+    """Rows per workgroup tile: at least ``target_tiles`` tiles over the batch (many more than
+    the 2048 resident workgroups, so the last round of tiles is short), at least one pass of the
+    256-thread block and at most 256 rows (the LDS-staged product may shrink it further)."""
 
     groups = padded(r, lanes) // lanes
     per_pass = 256 // lanes
     rb = -(-(n_sys * groups * rows) // target_tiles)
-    rb = max(per_pass, min(4096, rb))
+    rb = max(per_pass, min(256, rb))
     return -(-rb // per_pass) * per_pass
 
 
@@ -63,16 +64,20 @@ class BatchedSolver:
     tolerance: float = 1e-10
     max_iterations: int = 50000
     check_every: int = 16
+    rows_per_tile: int | None = None
     _plans: dict = field(default_factory=dict)
     last: SolveStats | None = None
 
-    def plan(self, key, rows, r: int, *, pattern_offsets=None, b_off=None, lanes=None, rows_per_tile=None):
+    def plan(self, key, rows, r: int, *, pattern_offsets=None, b_off=None, lanes=None, rows_per_tile=None,
+             host_pattern=None):
         """Plan for systems with ``rows[s]`` rows sharing one CSR pattern unless ``pattern_offsets``
         ``(ptr_off, idx_off, val_off)`` says otherwise."""
 
         rows = np.asarray(rows, dtype=np.int32)
         n_sys = int(rows.size)
         lanes = choose_lanes(r) if lanes is None else int(lanes)
+        if rows_per_tile is None:
+            rows_per_tile = self.rows_per_tile
         if rows_per_tile is None:
             rows_per_tile = choose_rows_per_tile(n_sys, int(rows.max()), r, lanes)
         bkey = None if b_off is None else tuple(np.asarray(b_off).tolist())
@@ -87,7 +92,7 @@ class BatchedSolver:
                 ptr_off, idx_off, val_off = pattern_offsets
             if val_off is None:
                 raise ValueError("val_off is required")
-            plan = native.PcgPlan(rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off)
+            plan = native.PcgPlan(rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off, host_pattern)
             self._plans[full_key] = plan
         return plan
 

@@ -73,6 +73,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2, help="time steps in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=32, help="Jacobian rows in the CPU-baseline sample")
     ap.add_argument("--no-jacobian", action="store_true")
+    ap.add_argument("--rows-per-tile", type=int, default=None, help="PCG workgroup tile height (default: auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -93,6 +94,7 @@ def main() -> int:
     survey = adt.Survey.from_arrays(case.electrodes, case.measurements)
     op = adt.ERTForward2p5D.from_mesh_survey(mesh, survey, solver_tolerance=args.tol)
     op._solver.check_every = args.check_every
+    op._solver.rows_per_tile = args.rows_per_tile
     d = op._dev
     W, E, N, D = d.W, d.E, d.N, survey.measurement_count
     C = mesh.cell_count
@@ -168,7 +170,8 @@ def main() -> int:
                    "electrodes": E, "quadrupoles": D, "wavenumbers": W, "wavenumber_rule": "reference (integration.py)",
                    "time_steps_per_step": args.batch_steps, "rhs_per_step": args.batch_steps * W * E,
                    "solver": "FP64 Jacobi-PCG, graph-replayed", "tolerance": args.tol,
-                   "lanes": plan.lanes, "rp": plan.rp, "n_tiles": None},
+                   "lanes": plan.lanes, "rp": plan.rp, "n_tiles": plan.tiles, "rows_per_tile": plan.rows_per_tile,
+                   "lds_staged_rows": plan.lds_rows, "dof_order": os.environ.get("ADT_DOF_ORDER", "hilbert")},
         "solver": {"preconditioner": "jacobi", "rhs_solves_per_s_pcg": rhs_per_s, "rhs_solves_per_s_learned_fcg": None,
                    "mean_iterations": float(np.mean(iters_all)), "worst_true_relres": worst_relres},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": dominant[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",

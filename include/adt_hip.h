@@ -92,6 +92,11 @@ int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t*
  * precond: ADT_PRECOND_JACOBI (plain PCG) or ADT_PRECOND_LEARNED (flexible CG whose
  * preconditioner is the learned graph network of adt_gnn_*; Jacobi-PCG restarts from the
  * current iterate for any right-hand side whose residual stalls).
+ * h_indptr/h_indices (optional): host copies of the concatenated CSR patterns.  With them and
+ * lds_bytes > 0 the plan cuts every pattern into row tiles whose rows and halo (referenced rows
+ * outside the tile) fit in lds_bytes of LDS per workgroup, and the sparse product stages each
+ * tile once in LDS instead of gathering every neighbour row from memory; rows_per_tile is then
+ * an upper bound.  adt_pcg_tiling reports the chosen tile height, tile count and staged rows.
  * h_b_off (optional) places system s's right-hand side block at d_b + h_b_off[s] (systems of
  * different models that share one right-hand side block); by default it mirrors the x layout
  * reported by adt_pcg_layout.
@@ -105,7 +110,9 @@ typedef struct adt_pcg_plan adt_pcg_plan;
 
 int adt_pcg_create(adt_pcg_plan** plan, int n_sys, int r, int lanes, int rows_per_tile,
                    const int32_t* h_rows, const int64_t* h_ptr_off, const int64_t* h_idx_off,
-                   const int64_t* h_val_off, const int64_t* h_b_off);
+                   const int64_t* h_val_off, const int64_t* h_b_off, const int32_t* h_indptr,
+                   const int32_t* h_indices, int lds_bytes);
+int adt_pcg_tiling(const adt_pcg_plan* plan, int* h_rows_per_tile, int* h_tiles, int* h_lds_rows);
 int adt_pcg_layout(const adt_pcg_plan* plan, int* h_rp, int64_t* h_vec_len, int64_t* h_vec_off);
 int adt_pcg_solve(adt_pcg_plan* plan, const int32_t* d_indptr, const int32_t* d_indices,
                   const double* d_values, const double* d_b, double* d_x, double tol, int max_iter,

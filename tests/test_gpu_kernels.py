@@ -101,7 +101,8 @@ def test_multi_rhs_product_matches_dense_products_for_every_lane_width():
         assert relmax(y.cpu().numpy(), dense @ x) < 1e-13, lanes
 
 
-def test_pcg_solves_random_spd_batches_to_tolerance():
+@pytest.mark.parametrize("staged", [False, True], ids=["gather", "lds-halo"])
+def test_pcg_solves_random_spd_batches_to_tolerance(staged):
     rng = np.random.default_rng(2)
     n = 400
     # 2D Laplacian-like SPD matrices with random positive coefficients, two systems
@@ -134,7 +135,9 @@ def test_pcg_solves_random_spd_batches_to_tolerance():
         diag = -dense[s].sum(axis=1) + 0.01
         values[s, ~off] = diag[r[~off]]
         dense[s, r[~off], c[~off]] = values[s, ~off]
-    plan = native.PcgPlan(np.full(S, n), R, 16, 32, np.zeros(S), np.zeros(S), np.arange(S) * keys.size)
+    plan = native.PcgPlan(np.full(S, n), R, 16, 32, np.zeros(S), np.zeros(S), np.arange(S) * keys.size,
+                          host_pattern=(indptr, c.astype(np.int32)) if staged else None, lds_bytes=8192)
+    assert (plan.lds_rows > 0) == staged
     b = rng.standard_normal((S, n, plan.rp))
     b[:, :, R:] = 0.0
     b[1, :, 3] = 0.0  # a zero right-hand side -> zero solution
