@@ -291,13 +291,13 @@ def config_2p5d(node_target: int, electrodes: int, spacing: float, seed: int, su
     return Case2D(nodes, cells, positions, quads, sigma), rng
 
 
-def config_c1(seed=1):
+def strip_dd_4k(seed=1):
     """~4k nodes, 48 electrodes at 2 m, ~1000 dipole-dipole rows."""
 
     return config_2p5d(4000, 48, 2.0, seed, "dd", 1000, 1.0)[0]
 
 
-def config_c2(seed=2, steps=365, node_target=20000):
+def timelapse_ws_20k(seed=2, steps=365, node_target=20000):
     """~20k nodes, 96 electrodes at 1 m, ~3000 Wenner-Schlumberger rows, a seeded time-lapse series.
 
     ``log10 sigma_t = log10 sigma_0 + 0.2 * sum_k a_k(t) G_k / |a|`` with four smooth spatial fields
@@ -316,7 +316,7 @@ def config_c2(seed=2, steps=365, node_target=20000):
     return case, series
 
 
-def config_c3(seed=3):
+def strip_dd_50k(seed=3):
     """~50k nodes, 64 electrodes at 1 m, 2000 dipole-dipole rows."""
 
     return config_2p5d(50000, 64, 1.0, seed, "dd", 2000, 0.25, dd_spacings=(1, 2, 3, 4, 5, 6, 7))[0]
@@ -337,7 +337,7 @@ def grf_3d(points, rng, mean=-2.0, std=0.5, length=20.0, grid=2.0):
     return 10.0 ** (mean + std * field[idx[:, 0], idx[:, 1], idx[:, 2]])
 
 
-def config_c4(seed=4, fine=1.25, growth=1.15, surface_step=0.5, depth_growth=1.06, rows=2000):
+def box_p1_200k(seed=4, fine=1.25, growth=1.15, surface_step=0.5, depth_growth=1.06, rows=2000):
     """200 x 200 x 100 m box, 8 x 8 electrodes at 5 m, ~200k nodes / ~1.1M tets, random ABMN rows."""
 
     rng = np.random.default_rng(seed)
@@ -363,7 +363,7 @@ def sinusoid_relief(rng, count=6, amplitude=5.0):
     return surface
 
 
-def config_c5(seed=5, fine=2.5, growth=1.25, surface_step=1.0, depth_growth=1.15, rows=2000):
+def terrain_p2_150k(seed=5, fine=2.5, growth=1.25, surface_step=1.0, depth_growth=1.15, rows=2000):
     """Terrain box for ten-node tets: ~25k hexes -> ~150k tets, surface relief from 6 sinusoids."""
 
     rng = np.random.default_rng(seed)

@@ -244,7 +244,7 @@ def test_dof_numbering_is_a_permutation_with_compact_tiles():
     import importlib
 
     ordering = importlib.import_module("ad-tlert_amd.ordering")
-    case = adt.cases.config_c1()
+    case = adt.cases.strip_dd_4k()
     perm = ordering.hilbert_order(case.nodes)
     assert np.array_equal(np.sort(perm), np.arange(case.nodes.shape[0]))
     inv = ordering.inverse(perm)
