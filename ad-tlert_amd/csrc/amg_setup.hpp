@@ -1,0 +1,52 @@
+// Host-side symbolic setup of the smoothed-aggregation multigrid preconditioner.
+//
+// One hierarchy is built per CSR pattern from a representative matrix of that pattern
+// (strength of connection and the damping weights need values).  Each level l holds the
+// pattern of A_l, its aggregation, the pattern of the smoothed prolongator
+//     P_l = (I - omega_l D_l^-1 A_l) P0_l      (P0_l: piecewise-constant on aggregates)
+// its transpose (restriction), and the gather lists with which the device recomputes, for
+// every system and every solve, the numeric P_l, T_l = A_l P_l and A_{l+1} = P_l^T T_l in
+// a fixed order (so the Galerkin operators are bitwise repeatable).
+//
+// Method: P. Vanek, J. Mandel, M. Brezina, "Algebraic multigrid by smoothed aggregation
+// for second and fourth order elliptic problems", Computing 56 (1996) 179-196.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace adt {
+namespace amg {
+
+struct HostLevel {
+    int n = 0;   // rows of A_l
+    int nc = 0;  // rows of A_{l+1} (aggregates); 0 on the coarsest level
+    std::vector<int> ptr, idx, diag;  // pattern of A_l and the slot of each diagonal entry
+    std::vector<double> rep;          // representative values of A_l
+    double omega = 2.0 / 3.0;         // smoother and prolongation damping 4 / (3 lambda_max(D^-1 A))
+    std::vector<int> agg;             // aggregate of each row, -1: left out (no strong connection)
+    // prolongator P (n x nc)
+    std::vector<int> pptr, pidx;
+    std::vector<int> pmap_ptr, pmap;  // per P slot: the A_l slots of row i whose column lies in aggregate J
+    std::vector<int> pself;           // per P slot: 1 if J is the row's own aggregate
+    // restriction R = P^T (nc x n): per coarse row, (fine row, P slot) entries in fine-row order
+    std::vector<int> rptr, rrow, rslot;
+    // T = A P (n x nc): pattern and (A slot, P slot) product pairs per T slot
+    std::vector<int> tptr, tidx, tpair_ptr, tpair_a, tpair_p;
+    // A_{l+1} = R T: (P slot, T slot) product pairs per slot of the next level's pattern
+    std::vector<int> cpair_ptr, cpair_p, cpair_t;
+};
+
+struct Options {
+    double theta = 0.08;    // strength threshold |a_ij| >= theta sqrt(a_ii a_jj)
+    int coarse_max = 256;   // stop coarsening at this many rows (dense inverse on the coarsest level)
+    int max_levels = 12;
+};
+
+// Builds the levels from a CSR pattern and representative values (rows = n).  The last
+// level is the coarsest (dense inverse).  Returns an empty vector if the input is invalid.
+std::vector<HostLevel> build_hierarchy(int n, const int32_t* ptr, const int32_t* idx, const double* values,
+                                       const Options& opt);
+
+}  // namespace amg
+}  // namespace adt

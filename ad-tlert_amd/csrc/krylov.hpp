@@ -1,0 +1,180 @@
+// Device-side data of the batched Krylov solver shared by the conjugate-gradient kernels
+// (pcg.hip) and the multigrid preconditioner kernels (amg_kernels.hpp).
+//
+// Layout: each system's vectors are an (N, Rp) row-major block (sources contiguous).  A
+// workgroup owns a tile = (system, lane group of L right-hand sides, row range).  Lanes
+// t % L walk the L right-hand sides of one row, so every nonzero reads one contiguous
+// L*8-byte run of the gathered vector; the CSR row itself is read once per lane group.
+#pragma once
+
+#include "adt_common.hpp"
+
+namespace adt {
+namespace krylov {
+
+struct Sys {
+    int64_t rows, ptr_off, idx_off, val_off, vec_off, diag_off, b_off, lidx_off;
+};
+
+// ptile: index of the tile's row range in the halo tables of its CSR pattern (-1: none)
+struct Tile {
+    int sys, grp, row0, row1, ptile, pad;
+};
+
+struct PcgArgs {
+    const uint16_t* lidx;   // per nonzero: LDS slot of its column within the row's tile
+    const int* halo_ptr;    // per pattern tile: range in halo_rows
+    const int* halo_rows;   // rows outside a tile that its rows reference
+    const Sys* sys;
+    const Tile* tiles;
+    const int* grp_tile0;
+    const int* grp_ntiles;
+    const int32_t* indptr;
+    const int32_t* indices;
+    const double* values;
+    const double* b;
+    double* x;
+    double* r;
+    double* p[2];
+    double* q;
+    double* z;  // flexible CG: preconditioned residual (also the fine-level smoothed residual)
+    double* dinv;
+    double* part0;
+    double* part1;
+    double* part2;
+    double* alpha;
+    double* beta;
+    double* rho;
+    double* bb;
+    double* rr;
+    double* rz_old;
+    double* pq_old;   // flexible CG: p.q of the previous iteration
+    double* rr_best;  // stall detection: smallest ||r||^2 seen
+    int* iters;
+    int* flags;  // 1 active, 0 converged, 2 zero right-hand side, -1 breakdown / non-finite, -2 stalled
+    int* stall;
+    int* grp_active;
+    int* grp_iters;
+    int* active_total;
+    double tol2;
+    int rp, groups, r_real, n_tiles;
+    int lds_rows, lds_nnz, lds_own, lds_halo;  // LDS-staged product capacities
+    int stall_limit;                           // iterations without a 10% residual decrease (0: off)
+    int pad_;
+};
+
+// One multigrid level of one system.  Offsets index the level's concatenated arrays.  On the
+// finest level the matrix is the caller's (ptr/idx/val offsets are the system's CSR offsets).
+struct LvSys {
+    int rows, nc;           // rows on this level, rows on the next (0 on the coarsest)
+    int n_p, n_t, n_next;   // slots of P, of T = A P and of the next level's matrix
+    int pad;
+    double omega;           // smoother and prolongation damping
+    int64_t ptr, idx, diag, val;   // matrix pattern, diagonal slots, values
+    int64_t vec, dinv;             // (rows, Rp) vector block, inverse diagonal
+    int64_t pptr, pidx, pmap_ptr, pmap, pval;  // prolongator pattern / gather map / values
+    int64_t rptr, rent;            // restriction rows (nc + 1) and (fine row, P slot) entries
+    int64_t tptr, tpair_ptr, tpair, tval;      // T = A P
+    int64_t cpair_ptr, cpair, next_val;        // next level values from (P slot, T slot) pairs
+    int64_t next_vec;              // vector block on the next level
+    int64_t dense;                 // coarsest level: dense inverse (rows x rows)
+};
+
+struct LevelArgs {
+    const LvSys* ls;
+    const Tile* tiles;
+    const Tile* next_tiles;
+    const int* grp_active;
+    int n_tiles, n_next_tiles, rp, groups, n_sys, coarsest, fine, pad;
+    // matrix of this level (finest level: the caller's arrays)
+    const int32_t* ptr;
+    const int32_t* idx;
+    const double* val;
+    const int* diag;
+    const double* dinv;
+    // prolongator, restriction, Galerkin product lists
+    const int* pptr;
+    const int* pidx;
+    const int* pmap_ptr;
+    const int* pmap;
+    const int* pself;
+    double* pval;
+    const int* rptr;
+    const int* rrow;
+    const int* rslot;
+    const int* tpair_ptr;
+    const int* tpair_a;
+    const int* tpair_p;
+    double* tval;
+    const int* cpair_ptr;
+    const int* cpair_p;
+    const int* cpair_t;
+    double* next_valp;       // next level values (all systems)
+    double* dinv_out;        // this level's inverse diagonal (written by the numeric setup)
+    const double* omega_slot;  // finest level: per-slot learned damping (NULL: constant omega)
+    double* dense;           // coarsest: dense inverses
+    // vectors
+    const double* b;   // right-hand side of this level (finest: the Krylov residual r)
+    double* r1;        // residual after pre-smoothing (finest: aliases the z buffer)
+    double* e;         // correction computed on this level (finest: z)
+    double* z2;        // pre-smoothed + prolonged correction (levels >= 1, or finest without LDS)
+    double* next_b;
+    const double* next_e;
+};
+
+constexpr int kRowsPerPass(int L) { return 256 / L; }
+
+// Sum over the threads of a 256-wide block that share lane = t % L; thread t < L gets the
+// lane total.  Two independent values are reduced together.  Fixed order.
+template <int L>
+__device__ __forceinline__ void block_sum2(double& v0, double& v1) {
+    __shared__ double sh[2][4][64];
+    v0 = adt::wave_sum_strided<L>(v0);
+    v1 = adt::wave_sum_strided<L>(v1);
+    const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
+    if (wl < L) {
+        sh[0][wave][wl] = v0;
+        sh[1][wave][wl] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x < L) {
+        const int l = threadIdx.x;
+        v0 = ((sh[0][0][l] + sh[0][1][l]) + sh[0][2][l]) + sh[0][3][l];
+        v1 = ((sh[1][0][l] + sh[1][1][l]) + sh[1][2][l]) + sh[1][3][l];
+    }
+    __syncthreads();
+}
+
+template <int L>
+__device__ __forceinline__ double block_sum1(double v) {
+    double dummy = 0.0;
+    block_sum2<L>(v, dummy);
+    return v;
+}
+
+// Fixed-order sum of one lane group's tile partials; thread l < L receives lane l's total.
+template <int L>
+__device__ __forceinline__ double group_partials(const double* part, int tile0, int ntiles) {
+    const int lane = threadIdx.x % L, chunk = threadIdx.x / L;
+    double s = 0.0;
+    for (int i = chunk; i < ntiles; i += kRowsPerPass(L)) s += part[static_cast<int64_t>(tile0 + i) * L + lane];
+    return block_sum1<L>(s);
+}
+
+template <int L>
+__device__ __forceinline__ void refresh_group(const PcgArgs& a, int gid, bool lane_active) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (threadIdx.x < L && lane_active) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) a.grp_active[gid] = any;
+}
+
+inline int64_t lds_bytes_for(int staged, int nnz, int own, int halo, int lanes) {
+    return static_cast<int64_t>(staged) * lanes * 8 + static_cast<int64_t>(nnz) * 8 + (own + 1 + halo) * 4 +
+           static_cast<int64_t>(nnz) * 2;
+}
+
+}  // namespace krylov
+}  // namespace adt

@@ -1,20 +1,29 @@
-// (b) Batched multi-RHS FP64 sparse products and preconditioned conjugate gradients.
+// (b) Batched multi-RHS FP64 sparse products, Jacobi-preconditioned conjugate gradients and
+// multigrid-preconditioned flexible conjugate gradients.
 //
-// Layout: each system's vectors are an (N, Rp) row-major block (sources contiguous).  A
-// workgroup owns a tile = (system, lane group of L right-hand sides, row range).  Lanes
-// t % L walk the L right-hand sides of one row, so every nonzero reads one contiguous
-// L*8-byte run of the gathered vector; the CSR row itself is read once per lane group.
-//
-// One CG iteration is four launches (captured once in a hipGraph and replayed):
+// Jacobi PCG, one iteration = four launches (captured once in a hipGraph and replayed):
 //   spmv   : p_new = z + beta p_old (formed on the fly for every neighbour, z = D^-1 r),
 //            q = A p_new, partial sums of p_new.q per (tile, lane)
 //   alpha  : one workgroup per lane group reduces the tile partials in a fixed order
 //   update : x += alpha p_new, r -= alpha q, partials of r.z and r.r
 //   beta   : reduce, convergence test ||r|| <= tol ||b||, beta = rho_new / rho
+// Flexible CG (Y. Notay, "Flexible conjugate gradients", SIAM J. Sci. Comput. 22 (2000)
+// 1444-1460, with one retained direction) preconditioned by one smoothed-aggregation V-cycle
+// (amg_kernels.hpp) per iteration:
+//   cycle  : z = B r, partials z.r and z.q_old
+//   fbeta  : beta = -(z.q_old) / (p_old.q_old), rho = z.r
+//   spmv   : p_new = z + beta p_old, q = A p_new, p_new.q
+//   alpha  : alpha = rho / (p_new.q)
+//   update : x += alpha p_new, r -= alpha q, ||r||^2
+//   check  : convergence; a right-hand side whose residual norm has not dropped by 10% for
+//            `stall_limit` iterations is marked stalled and finished by Jacobi PCG from its
+//            current iterate (as is any breakdown), to the same tolerance.
 // p is double-buffered (ping-pong) because neighbours' p_old is still being read while
 // p_new is written.  Converged lane groups skip all work; the host polls one counter of
 // active right-hand sides between graph replays (one replay in flight ahead).
-#include "adt_common.hpp"
+#include "krylov.hpp"
+#include "amg_kernels.hpp"
+#include "amg_setup.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -24,83 +33,9 @@
 
 namespace {
 
-struct Sys {
-    int64_t rows, ptr_off, idx_off, val_off, vec_off, diag_off, b_off, lidx_off;
-};
+using namespace adt::krylov;
 
-// ptile: index of the tile's row range in the halo tables of its CSR pattern (-1: none)
-struct Tile {
-    int sys, grp, row0, row1, ptile, pad;
-};
-
-struct PcgArgs {
-    const uint16_t* lidx;   // per nonzero: LDS slot of its column within the row's tile
-    const int* halo_ptr;    // per pattern tile: range in halo_rows
-    const int* halo_rows;   // rows outside a tile that its rows reference
-    const Sys* sys;
-    const Tile* tiles;
-    const int* grp_tile0;
-    const int* grp_ntiles;
-    const int32_t* indptr;
-    const int32_t* indices;
-    const double* values;
-    const double* b;
-    double* x;
-    double* r;
-    double* p[2];
-    double* q;
-    double* z;  // flexible CG: preconditioned residual of the learned network
-    double* dinv;
-    double* part0;
-    double* part1;
-    double* part2;
-    double* alpha;
-    double* beta;
-    double* rho;
-    double* bb;
-    double* rr;
-    double* rz_old;
-    int* iters;
-    int* flags;  // 1 active, 0 converged, 2 zero right-hand side, -1 breakdown / non-finite
-    int* stall;
-    int* grp_active;
-    int* grp_iters;
-    int* active_total;
-    double tol2;
-    int rp, groups, r_real, n_tiles;
-    int lds_rows, lds_nnz, lds_own, lds_halo;  // LDS-staged product capacities
-};
-
-constexpr int kRowsPerPass(int L) { return 256 / L; }
 constexpr int kSpmvUnroll = 8;
-
-// Sum over the threads of a 256-wide block that share lane = t % L; thread t < L gets the
-// lane total.  Two independent values are reduced together.  Fixed order.
-template <int L>
-__device__ __forceinline__ void block_sum2(double& v0, double& v1) {
-    __shared__ double sh[2][4][64];
-    v0 = adt::wave_sum_strided<L>(v0);
-    v1 = adt::wave_sum_strided<L>(v1);
-    const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
-    if (wl < L) {
-        sh[0][wave][wl] = v0;
-        sh[1][wave][wl] = v1;
-    }
-    __syncthreads();
-    if (threadIdx.x < L) {
-        const int l = threadIdx.x;
-        v0 = ((sh[0][0][l] + sh[0][1][l]) + sh[0][2][l]) + sh[0][3][l];
-        v1 = ((sh[1][0][l] + sh[1][1][l]) + sh[1][2][l]) + sh[1][3][l];
-    }
-    __syncthreads();
-}
-
-template <int L>
-__device__ __forceinline__ double block_sum1(double v) {
-    double dummy = 0.0;
-    block_sum2<L>(v, dummy);
-    return v;
-}
 
 // ------------------------------------------------------------------------------------------
 // plain multi-RHS product (adt_spmm)
@@ -168,16 +103,14 @@ __device__ __forceinline__ bool load_tile(const PcgArgs& a, TileCtx& c) {
 
 __device__ __forceinline__ int tile_index(const PcgArgs& a) { return adt::xcd_tile(blockIdx.x, a.n_tiles); }
 
-template <int L>
-__device__ __forceinline__ double diag_inverse(const int32_t* ptr, const int32_t* idx, const double* val,
-                                               int row) {
+__device__ __forceinline__ double diag_inverse(const int32_t* ptr, const int32_t* idx, const double* val, int row) {
     double d = 0.0;
     for (int k = ptr[row]; k < ptr[row + 1]; ++k)
         if (idx[k] == row) d = val[k];
     return 1.0 / d;
 }
 
-// r = b - A x, dinv, p_0 = 0; partials ||b||^2, r.z, ||r||^2.
+// r = b - A x, dinv, p_0 = q = 0; partials ||b||^2, r.z, ||r||^2.
 template <int L>
 __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
@@ -195,14 +128,16 @@ __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
     double* R = a.r + c.S.vec_off + c.col;
     double* P0 = a.p[0] + c.S.vec_off + c.col;
     double* P1 = a.p[1] + c.S.vec_off + c.col;
+    double* Q = a.q + c.S.vec_off + c.col;
     double* D = a.dinv + c.S.diag_off;
     double sb = 0.0, sz = 0.0, sr = 0.0;
     for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-        const double di = diag_inverse<L>(ptr, idx, val, row);
+        const double di = diag_inverse(ptr, idx, val, row);
         if (first_group && (threadIdx.x % L) == 0) D[row] = di;
         const int64_t o = static_cast<int64_t>(row) * rp;
         P0[o] = 0.0;
         P1[o] = 0.0;
+        Q[o] = 0.0;
         if (!real) {
             R[o] = 0.0;
             continue;
@@ -224,25 +159,6 @@ __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
         a.part1[static_cast<int64_t>(tix) * L + threadIdx.x] = s2;
         a.part2[static_cast<int64_t>(tix) * L + threadIdx.x] = s3;
     }
-}
-
-// Fixed-order sum of one lane group's tile partials; thread l < L receives lane l's total.
-template <int L>
-__device__ __forceinline__ double group_partials(const double* part, int tile0, int ntiles) {
-    const int lane = threadIdx.x % L, chunk = threadIdx.x / L;
-    double s = 0.0;
-    for (int i = chunk; i < ntiles; i += kRowsPerPass(L)) s += part[static_cast<int64_t>(tile0 + i) * L + lane];
-    return block_sum1<L>(s);
-}
-
-template <int L>
-__device__ __forceinline__ void refresh_group(const PcgArgs& a, int gid, bool lane_active) {
-    __shared__ int any;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    if (threadIdx.x < L && lane_active) atomicOr(&any, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) a.grp_active[gid] = any;
 }
 
 template <int L>
@@ -270,10 +186,12 @@ __global__ __launch_bounds__(256) void cg_init_reduce(const PcgArgs* __restrict_
         a.flags[sc] = flag;
         a.bb[sc] = bb;
         a.rr[sc] = rr;
+        a.rr_best[sc] = rr;
         a.rho[sc] = rz;
         a.rz_old[sc] = rz;
         a.beta[sc] = 0.0;
         a.alpha[sc] = 0.0;
+        a.pq_old[sc] = 1.0;
         a.iters[sc] = 0;
         a.stall[sc] = 0;
         act = flag == 1;
@@ -295,8 +213,16 @@ __global__ __launch_bounds__(256) void cg_zero_solution(const PcgArgs* __restric
         X[static_cast<int64_t>(row) * a.rp] = 0.0;
 }
 
-// Jacobi PCG step 1: p_new = D^-1 r + beta p_old on the fly, q = A p_new, partial p_new.q.
-template <int L>
+// Search direction of row j: Jacobi p_new = D^-1 r + beta p_old; flexible p_new = z + beta p_old.
+template <bool FLEX>
+__device__ __forceinline__ double direction(const double* D, const double* R, const double* Z, const double* Pold,
+                                            double beta, int64_t j, int rp) {
+    if (FLEX) return fma(beta, Pold[j * rp], Z[j * rp]);
+    return fma(beta, Pold[j * rp], D[j] * R[j * rp]);
+}
+
+// Step 1: p_new on the fly, q = A p_new, partial p_new.q.
+template <int L, bool FLEX>
 __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, int parity) {
     const PcgArgs& a = *pa;
     const int tix = tile_index(a);
@@ -312,6 +238,7 @@ __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, i
     const double* val = a.values + S.val_off;
     const int rp = a.rp;
     const double* R = a.r + S.vec_off + col;
+    const double* Z = a.z + S.vec_off + col;
     const double* Pold = a.p[parity] + S.vec_off + col;
     double* Pnew = a.p[parity ^ 1] + S.vec_off + col;
     double* Q = a.q + S.vec_off + col;
@@ -334,14 +261,11 @@ __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, i
                     vv[u] = ok ? val[kb + u] : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < kSpmvUnroll; ++u) {
-                    const double zj = D[jj[u]] * R[jj[u] * rp];
-                    acc = fma(vv[u], fma(beta, Pold[jj[u] * rp], zj), acc);
-                }
+                for (int u = 0; u < kSpmvUnroll; ++u)
+                    acc = fma(vv[u], direction<FLEX>(D, R, Z, Pold, beta, jj[u], rp), acc);
             }
             const int64_t o = static_cast<int64_t>(row) * rp;
-            const double zi = D[row] * R[o];
-            const double pi = fma(beta, Pold[o], zi);
+            const double pi = direction<FLEX>(D, R, Z, Pold, beta, row, rp);
             Pnew[o] = pi;
             Q[o] = acc;
             pq = fma(pi, acc, pq);
@@ -354,12 +278,12 @@ __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, i
 // LDS-staged variant of cg_spmv, three phases per tile:
 //   1. coalesced bulk copy of the tile's CSR slice (values, tile-local column slots, row
 //      pointers) and halo row list into LDS;
-//   2. stage p_new = D^-1 r + beta p_old for the tile's rows and halo rows (independent
-//      loads, unrolled) into LDS, writing p_new of the own rows;
+//   2. stage p_new for the tile's rows and halo rows (independent loads, unrolled) into LDS,
+//      writing p_new of the own rows;
 //   3. q = A p_new entirely from LDS (slot-major with L lanes contiguous: conflict-free).
 // Every vector row is read from memory once per tile and the inner product loop has no
 // global-memory latency in it.
-template <int L>
+template <int L, bool FLEX>
 __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ pa, int parity) {
     extern __shared__ double smem[];
     const PcgArgs& a = *pa;
@@ -372,56 +296,33 @@ __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ p
     const int64_t sc = static_cast<int64_t>(t.sys) * a.rp + col;
     const bool act = a.flags[sc] == 1;
     const double beta = act ? a.beta[sc] : 0.0;
-    const int32_t* ptr = a.indptr + S.ptr_off;
     const int rp = a.rp;
     const double* R = a.r + S.vec_off + col;
+    const double* Z = a.z + S.vec_off + col;
     const double* Pold = a.p[parity] + S.vec_off + col;
     double* Pnew = a.p[parity ^ 1] + S.vec_off + col;
     double* Q = a.q + S.vec_off + col;
     const double* D = a.dinv + S.diag_off;
-    const int own = t.row1 - t.row0;
-    const int h0 = a.halo_ptr[t.ptile], nh = a.halo_ptr[t.ptile + 1] - h0;
-    const int staged = own + nh;
-    const int e0 = ptr[t.row0], ne = ptr[t.row1] - e0;
-    double* stage = smem;                                      // lds_rows x L
-    double* sval = smem + static_cast<int64_t>(a.lds_rows) * L;  // lds_nnz
-    int* sptr = reinterpret_cast<int*>(sval + a.lds_nnz);        // lds_own + 1
-    int* shalo = sptr + a.lds_own + 1;                           // lds_halo
-    uint16_t* sidx = reinterpret_cast<uint16_t*>(shalo + a.lds_halo);  // lds_nnz
-    // phase 1
-    const double* val = a.values + S.val_off + e0;
-    const uint16_t* lidx = a.lidx + S.lidx_off + e0;
-    for (int e = static_cast<int>(threadIdx.x); e < ne; e += 256) {
-        sval[e] = val[e];
-        sidx[e] = lidx[e];
-    }
-    for (int i = static_cast<int>(threadIdx.x); i <= own; i += 256) sptr[i] = ptr[t.row0 + i] - e0;
-    for (int i = static_cast<int>(threadIdx.x); i < nh; i += 256) shalo[i] = a.halo_rows[h0 + i];
-    __syncthreads();
+    const LdsTile s = stage_csr(a, t, S, L, smem);
     // phase 2
     if (act) {
         constexpr int U = 4;
-        for (int s0 = rsub; s0 < staged; s0 += U * kRowsPerPass(L)) {
+        for (int s0 = rsub; s0 < s.staged; s0 += U * kRowsPerPass(L)) {
             int64_t jj[U];
-            double dj[U], rj[U], pj[U];
+            double v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int s = s0 + u * kRowsPerPass(L);
-                jj[u] = s < own ? t.row0 + s : (s < staged ? shalo[s - own] : t.row0);
+                const int q = s0 + u * kRowsPerPass(L);
+                jj[u] = q < s.own ? t.row0 + q : (q < s.staged ? s.shalo[q - s.own] : t.row0);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                dj[u] = D[jj[u]];
-                rj[u] = R[jj[u] * rp];
-                pj[u] = Pold[jj[u] * rp];
-            }
+            for (int u = 0; u < U; ++u) v[u] = direction<FLEX>(D, R, Z, Pold, beta, jj[u], rp);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int s = s0 + u * kRowsPerPass(L);
-                if (s < staged) {
-                    const double v = fma(beta, pj[u], dj[u] * rj[u]);
-                    stage[s * L + lane] = v;
-                    if (s < own) Pnew[jj[u] * rp] = v;
+                const int q = s0 + u * kRowsPerPass(L);
+                if (q < s.staged) {
+                    s.stage[q * L + lane] = v[u];
+                    if (q < s.own) Pnew[jj[u] * rp] = v[u];
                 }
             }
         }
@@ -430,12 +331,12 @@ __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ p
     // phase 3
     double pq = 0.0;
     if (act) {
-        for (int s = rsub; s < own; s += kRowsPerPass(L)) {
-            const int k1 = sptr[s + 1];
+        for (int q = rsub; q < s.own; q += kRowsPerPass(L)) {
+            const int k1 = s.sptr[q + 1];
             double acc = 0.0;
-            for (int k = sptr[s]; k < k1; ++k) acc = fma(sval[k], stage[static_cast<int>(sidx[k]) * L + lane], acc);
-            Q[static_cast<int64_t>(t.row0 + s) * rp] = acc;
-            pq = fma(stage[s * L + lane], acc, pq);
+            for (int k = s.sptr[q]; k < k1; ++k) acc = fma(s.sval[k], s.stage[static_cast<int>(s.sidx[k]) * L + lane], acc);
+            Q[static_cast<int64_t>(t.row0 + q) * rp] = acc;
+            pq = fma(s.stage[q * L + lane], acc, pq);
         }
     }
     pq = block_sum1<L>(pq);
@@ -459,6 +360,7 @@ __global__ __launch_bounds__(256) void cg_alpha(const PcgArgs* __restrict__ pa) 
                 atomicSub(a.active_total, 1);
             } else {
                 a.alpha[sc] = alpha;
+                a.pq_old[sc] = pq;
                 act = true;
             }
         }
@@ -502,6 +404,7 @@ __global__ __launch_bounds__(256) void cg_update(const PcgArgs* __restrict__ pa,
     }
 }
 
+// Jacobi: convergence test and beta = rho_new / rho.
 template <int L>
 __global__ __launch_bounds__(256) void cg_beta(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
@@ -526,6 +429,73 @@ __global__ __launch_bounds__(256) void cg_beta(const PcgArgs* __restrict__ pa) {
             } else {
                 a.beta[sc] = rz / a.rho[sc];
                 a.rho[sc] = rz;
+                act = true;
+            }
+        }
+    }
+    refresh_group<L>(a, gid, act);
+}
+
+// Flexible CG: convergence test and stall detection (coefficients come from fcg_beta).
+template <int L>
+__global__ __launch_bounds__(256) void cg_check(const PcgArgs* __restrict__ pa) {
+    const PcgArgs& a = *pa;
+    const int gid = blockIdx.x;
+    if (!a.grp_active[gid]) return;
+    if (threadIdx.x == 0) a.grp_iters[gid] += 1;
+    const double rr = group_partials<L>(a.part2, a.grp_tile0[gid], a.grp_ntiles[gid]);
+    bool act = false;
+    if (threadIdx.x < L) {
+        const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
+        if (a.flags[sc] == 1) {
+            a.iters[sc] += 1;
+            a.rr[sc] = rr;
+            if (!std::isfinite(rr)) {
+                a.flags[sc] = -1;
+                atomicSub(a.active_total, 1);
+            } else if (rr <= a.tol2 * a.bb[sc]) {
+                a.flags[sc] = 0;
+                atomicSub(a.active_total, 1);
+            } else {
+                if (rr < 0.81 * a.rr_best[sc]) {
+                    a.rr_best[sc] = rr;
+                    a.stall[sc] = 0;
+                } else {
+                    a.stall[sc] += 1;
+                }
+                if (a.stall_limit > 0 && a.stall[sc] >= a.stall_limit) {
+                    a.flags[sc] = -2;
+                    atomicSub(a.active_total, 1);
+                } else {
+                    act = true;
+                }
+            }
+        }
+    }
+    refresh_group<L>(a, gid, act);
+}
+
+// Flexible CG coefficients from the cycle's partials: beta = -(z.q_old)/(p_old.q_old), rho = z.r.
+template <int L>
+__global__ __launch_bounds__(256) void fcg_beta(const PcgArgs* __restrict__ pa) {
+    const PcgArgs& a = *pa;
+    const int gid = blockIdx.x;
+    if (!a.grp_active[gid]) return;
+    const int t0 = a.grp_tile0[gid], nt = a.grp_ntiles[gid];
+    const double zq = group_partials<L>(a.part0, t0, nt);
+    const double zr = group_partials<L>(a.part1, t0, nt);
+    bool act = false;
+    if (threadIdx.x < L) {
+        const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
+        if (a.flags[sc] == 1) {
+            const bool first = a.iters[sc] == 0;
+            const double beta = first ? 0.0 : -zq / a.pq_old[sc];
+            if (!(zr > 0.0) || !std::isfinite(zr) || !std::isfinite(beta)) {
+                a.flags[sc] = -2;  // preconditioner not positive on this residual: finish with Jacobi
+                atomicSub(a.active_total, 1);
+            } else {
+                a.beta[sc] = beta;
+                a.rho[sc] = zr;
                 act = true;
             }
         }
@@ -570,6 +540,18 @@ __global__ __launch_bounds__(256) void cg_true_reduce(const PcgArgs* __restrict_
     }
 }
 
+// One multigrid level (all systems) on the device.
+struct AmgLevel {
+    std::vector<LvSys> ls_h;
+    std::vector<Tile> tiles_h;
+    LvSys* ls_d = nullptr;
+    Tile* tiles_d = nullptr;
+    int n_tiles = 0, max_rows = 0, max_p = 0, max_t = 0, max_next = 0;
+    LevelArgs args_h{};
+    LevelArgs* args_d = nullptr;
+    int64_t dense_len = 0;
+};
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -577,12 +559,14 @@ __global__ __launch_bounds__(256) void cg_true_reduce(const PcgArgs* __restrict_
 // ------------------------------------------------------------------------------------------
 
 constexpr int kMaxChunk = 64;
+constexpr int kEvents = 6;  // per profiled iteration: cycle, spmv and update brackets
 
 struct adt_pcg_plan {
     int n_sys = 0, r = 0, L = 0, rp = 0, rb = 0, groups = 0, n_tiles = 0, n_groups = 0;
     int64_t vec_len = 0, diag_len = 0;
     std::vector<Sys> sys_h;
     std::vector<int64_t> rows_h;
+    std::vector<int> pattern_of_sys;
     Sys* sys_d = nullptr;
     Tile* tiles_d = nullptr;
     int* grp_tile0_d = nullptr;
@@ -596,10 +580,10 @@ struct adt_pcg_plan {
     PcgArgs args_h{};
     PcgArgs* args_d = nullptr;
     hipStream_t cap = nullptr;
-    // two identical iteration graphs replayed alternately, so a replay in flight never touches
-    // the profiling events of the replay being read
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_iters = 0;
+    // per mode (0 Jacobi, 1 flexible/multigrid): two identical iteration graphs replayed
+    // alternately, so a replay in flight never touches the profiling events of the replay being read
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int graph_iters[2] = {0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
     int profile = 0;
     int lds_rows = 0;  // staged rows of the LDS spmv (0: gather kernel)
@@ -607,15 +591,36 @@ struct adt_pcg_plan {
     uint16_t* lidx_d = nullptr;
     int* halo_ptr_d = nullptr;
     int* halo_rows_d = nullptr;
-    hipEvent_t pev[2][kMaxChunk][4] = {};
-    double prof_ms[2] = {0.0, 0.0};
+    hipEvent_t pev[2][kMaxChunk][kEvents] = {};
+    double prof_ms[3] = {0.0, 0.0, 0.0};
     int64_t prof_launches = 0;
+    // multigrid hierarchy (empty: none attached)
+    std::vector<AmgLevel> amg;
+    std::vector<void*> amg_buffers;
+    int stall_limit = 0;
+    int last_fallback = 0;
+    int last_fcg_iters_max = 0;
     ~adt_pcg_plan();
+    void free_amg();
 };
 
+void adt_pcg_plan::free_amg() {
+    for (void* b : amg_buffers)
+        if (b) (void)hipFree(b);
+    amg_buffers.clear();
+    amg.clear();
+    for (auto& g : graph[1])
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    graph_iters[1] = 0;
+}
+
 adt_pcg_plan::~adt_pcg_plan() {
-    for (auto& g : graph)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& set : graph)
+        for (auto& g : set)
+            if (g) (void)hipGraphExecDestroy(g);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& set : pev)
@@ -628,6 +633,8 @@ adt_pcg_plan::~adt_pcg_plan() {
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h_active) (void)hipHostFree(h_active);
+    for (void* b : amg_buffers)
+        if (b) (void)hipFree(b);
 }
 
 namespace {
@@ -641,31 +648,77 @@ int dalloc(T** p, int64_t count) {
     return ADT_OK;
 }
 
-int64_t lds_bytes_for(int staged, int nnz, int own, int halo, int lanes);
+int64_t fine_lds_bytes(const adt_pcg_plan* p) {
+    return lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->lds_halo, p->L);
+}
+
+// One V-cycle: z = B r (flexible-CG partials z.r, z.q_old written by the finest post-smoothing).
+template <int L>
+void launch_cycle(const adt_pcg_plan* p, hipStream_t st) {
+    const int m = static_cast<int>(p->amg.size());
+    const PcgArgs* pa = p->args_d;
+    const auto& f = p->amg[0];
+    const size_t lds = p->lds_rows > 0 ? static_cast<size_t>(fine_lds_bytes(p)) : 0;
+    if (p->lds_rows > 0)
+        hipLaunchKernelGGL(amg_smooth_residual_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d);
+    else
+        hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, f.args_d);
+    hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[1].n_tiles), dim3(256), 0, st, f.args_d);
+    for (int l = 1; l < m - 1; ++l) {
+        hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+        hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[l + 1].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+    }
+    hipLaunchKernelGGL(amg_coarse_solve<L>, dim3(p->n_groups), dim3(256), 0, st, p->amg[m - 1].args_d);
+    for (int l = m - 2; l >= 1; --l) {
+        hipLaunchKernelGGL(amg_prolong<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+        hipLaunchKernelGGL(amg_post_smooth<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+    }
+    if (p->lds_rows > 0) {
+        hipLaunchKernelGGL(amg_post_smooth_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d);
+    } else {
+        hipLaunchKernelGGL(amg_prolong<L>, dim3(p->n_tiles), dim3(256), 0, st, f.args_d);
+        hipLaunchKernelGGL(amg_post_smooth_fine<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d);
+    }
+}
 
 template <int L>
-void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, hipEvent_t* events) {
+void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mode, hipEvent_t* events) {
     const PcgArgs* pa = p->args_d;
+    if (mode == 1) {
+        if (events) (void)hipEventRecord(events[4], st);
+        launch_cycle<L>(p, st);
+        if (events) (void)hipEventRecord(events[5], st);
+        hipLaunchKernelGGL(fcg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    }
     if (events) (void)hipEventRecord(events[0], st);
-    if (p->lds_rows > 0)
-        hipLaunchKernelGGL(cg_spmv_lds<L>, dim3(p->n_tiles), dim3(256),
-                           static_cast<size_t>(lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->lds_halo, L)), st,
-                           pa, parity);
-    else
-        hipLaunchKernelGGL(cg_spmv<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+    if (p->lds_rows > 0) {
+        const size_t lds = static_cast<size_t>(fine_lds_bytes(p));
+        if (mode == 1)
+            hipLaunchKernelGGL((cg_spmv_lds<L, true>), dim3(p->n_tiles), dim3(256), lds, st, pa, parity);
+        else
+            hipLaunchKernelGGL((cg_spmv_lds<L, false>), dim3(p->n_tiles), dim3(256), lds, st, pa, parity);
+    } else {
+        if (mode == 1)
+            hipLaunchKernelGGL((cg_spmv<L, true>), dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+        else
+            hipLaunchKernelGGL((cg_spmv<L, false>), dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+    }
     if (events) (void)hipEventRecord(events[1], st);
     hipLaunchKernelGGL(cg_alpha<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
     if (events) (void)hipEventRecord(events[2], st);
     hipLaunchKernelGGL(cg_update<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
     if (events) (void)hipEventRecord(events[3], st);
-    hipLaunchKernelGGL(cg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    if (mode == 1)
+        hipLaunchKernelGGL(cg_check<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    else
+        hipLaunchKernelGGL(cg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
 }
 
 // Capture `chunk` (even) iterations; replays keep the p ping-pong parity at 0.
 template <int L>
-int ensure_graphs(adt_pcg_plan* p, int chunk) {
-    if (p->graph[0] && p->graph_iters == chunk) return ADT_OK;
-    for (auto& g : p->graph)
+int ensure_graphs(adt_pcg_plan* p, int chunk, int mode) {
+    if (p->graph[mode][0] && p->graph_iters[mode] == chunk) return ADT_OK;
+    for (auto& g : p->graph[mode])
         if (g) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
@@ -673,24 +726,28 @@ int ensure_graphs(adt_pcg_plan* p, int chunk) {
     for (int set = 0; set < 2; ++set) {
         hipGraph_t g = nullptr;
         ADT_HIP_CHECK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeRelaxed));
-        for (int i = 0; i < chunk; ++i) launch_iteration<L>(p, p->cap, i & 1, nullptr);
+        for (int i = 0; i < chunk; ++i) launch_iteration<L>(p, p->cap, i & 1, mode, nullptr);
         ADT_HIP_CHECK(hipStreamEndCapture(p->cap, &g));
-        hipError_t e = hipGraphInstantiate(&p->graph[set], g, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&p->graph[mode][set], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return adt::set_error(ADT_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    p->graph_iters = chunk;
+    p->graph_iters[mode] = chunk;
     return ADT_OK;
 }
 
 // Profiling mode: the same kernels launched directly on the caller's stream with timing events
-// around every spmv and update launch (HIP event nodes inside graphs cannot be timed).
+// around every cycle, spmv and update launch (HIP event nodes inside graphs cannot be timed).
 template <int L>
-int launch_profiled_chunk(adt_pcg_plan* p, hipStream_t st, int chunk, int set) {
+int launch_profiled_chunk(adt_pcg_plan* p, hipStream_t st, int chunk, int set, int mode) {
     for (int i = 0; i < chunk; ++i) {
         for (auto& e : p->pev[set][i])
             if (!e) ADT_HIP_CHECK(hipEventCreate(&e));
-        launch_iteration<L>(p, st, i & 1, p->pev[set][i]);
+        if (mode == 0) {  // keep the cycle bracket empty
+            ADT_HIP_CHECK(hipEventRecord(p->pev[set][i][4], st));
+            ADT_HIP_CHECK(hipEventRecord(p->pev[set][i][5], st));
+        }
+        launch_iteration<L>(p, st, i & 1, mode, p->pev[set][i]);
     }
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
@@ -698,18 +755,41 @@ int launch_profiled_chunk(adt_pcg_plan* p, hipStream_t st, int chunk, int set) {
 
 int accumulate_profile(adt_pcg_plan* p, int set, int chunk) {
     for (int i = 0; i < chunk; ++i) {
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, c = 0.f;
         ADT_HIP_CHECK(hipEventElapsedTime(&a, p->pev[set][i][0], p->pev[set][i][1]));
         ADT_HIP_CHECK(hipEventElapsedTime(&b, p->pev[set][i][2], p->pev[set][i][3]));
+        ADT_HIP_CHECK(hipEventElapsedTime(&c, p->pev[set][i][4], p->pev[set][i][5]));
         p->prof_ms[0] += a;
         p->prof_ms[1] += b;
+        p->prof_ms[2] += c;
         p->prof_launches += 1;
     }
     return ADT_OK;
 }
 
+// Numeric multigrid setup for the current matrix values: P, A P, Galerkin products per level
+// and the dense inverse of the coarsest operator, for every system.
+int amg_numeric(adt_pcg_plan* p, hipStream_t st) {
+    const int m = static_cast<int>(p->amg.size());
+    for (int l = 0; l < m - 1; ++l) {
+        const AmgLevel& A = p->amg[l];
+        hipLaunchKernelGGL(amg_pvals, dim3(adt::blocks_for(A.max_rows, 256), p->n_sys), dim3(256), 0, st, A.args_d);
+        if (A.max_t > 0)
+            hipLaunchKernelGGL(amg_tvals, dim3(adt::blocks_for(A.max_t, 256), p->n_sys), dim3(256), 0, st, A.args_d);
+        if (A.max_next > 0)
+            hipLaunchKernelGGL(amg_cvals, dim3(adt::blocks_for(A.max_next, 256), p->n_sys), dim3(256), 0, st,
+                               A.args_d);
+    }
+    const AmgLevel& C = p->amg[m - 1];
+    hipLaunchKernelGGL(amg_pvals, dim3(adt::blocks_for(C.max_rows, 256), p->n_sys), dim3(256), 0, st, C.args_d);
+    hipLaunchKernelGGL(amg_dense_inverse, dim3(p->n_sys), dim3(256), static_cast<size_t>(C.max_rows) * 8, st,
+                       C.args_d);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
 template <int L>
-int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk) {
+int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mode) {
     const PcgArgs* pa = p->args_d;
     ADT_HIP_CHECK(hipMemsetAsync(p->args_h.active_total, 0, sizeof(int), st));
     hipLaunchKernelGGL(cg_init<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
@@ -717,16 +797,16 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk) {
     hipLaunchKernelGGL(cg_zero_solution<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     ADT_HIP_CHECK(hipGetLastError());
     if (max_iter > 0) {
-        int rc = p->profile ? ADT_OK : ensure_graphs<L>(p, chunk);
+        int rc = p->profile ? ADT_OK : ensure_graphs<L>(p, chunk, mode);
         if (rc) return rc;
         int k = 0;
         bool stop = false;
         for (int it = 0; it < max_iter && !stop; it += chunk, ++k) {
             const int set = k & 1;
             if (p->profile) {
-                if ((rc = launch_profiled_chunk<L>(p, st, chunk, set))) return rc;
+                if ((rc = launch_profiled_chunk<L>(p, st, chunk, set, mode))) return rc;
             } else {
-                ADT_HIP_CHECK(hipGraphLaunch(p->graph[set], st));
+                ADT_HIP_CHECK(hipGraphLaunch(p->graph[mode][set], st));
             }
             ADT_HIP_CHECK(hipMemcpyAsync(&p->h_active[set], p->args_h.active_total, sizeof(int),
                                          hipMemcpyDeviceToHost, st));
@@ -748,6 +828,15 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk) {
     return ADT_OK;
 }
 
+int solve_dispatch(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mode) {
+    switch (p->L) {
+        case 8: return solve_impl<8>(p, st, max_iter, chunk, mode);
+        case 16: return solve_impl<16>(p, st, max_iter, chunk, mode);
+        case 32: return solve_impl<32>(p, st, max_iter, chunk, mode);
+        default: return solve_impl<64>(p, st, max_iter, chunk, mode);
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -759,11 +848,6 @@ struct PatternTiles {
     std::vector<int> halo_rows;
     int max_staged = 0, max_nnz = 0, max_halo = 0;
 };
-
-int64_t lds_bytes_for(int staged, int nnz, int own, int halo, int lanes) {
-    return static_cast<int64_t>(staged) * lanes * 8 + static_cast<int64_t>(nnz) * 8 + (own + 1 + halo) * 4 +
-           static_cast<int64_t>(nnz) * 2;
-}
 
 PatternTiles build_pattern_tiles(const int32_t* ptr, const int32_t* idx, int rows, int rb) {
     PatternTiles pt;
@@ -793,6 +877,8 @@ PatternTiles build_pattern_tiles(const int32_t* ptr, const int32_t* idx, int row
     return pt;
 }
 
+typedef std::tuple<int64_t, int64_t, int> PatKey;
+
 }  // namespace
 
 extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, int rows_per_tile,
@@ -801,6 +887,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
                               const int32_t* h_indices, int lds_bytes) {
     ADT_REQUIRE(out && h_rows && h_ptr_off && h_idx_off && h_val_off, "adt_pcg_create: null pointer");
     ADT_REQUIRE(n_sys >= 1 && r >= 1, "adt_pcg_create: need at least one system and one right-hand side");
+    ADT_REQUIRE(n_sys <= 65535, "adt_pcg_create: at most 65535 systems per plan");
     ADT_REQUIRE(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, "adt_pcg_create: lanes must be 8/16/32/64");
     ADT_REQUIRE(rows_per_tile >= 1, "adt_pcg_create: rows_per_tile must be positive");
     auto* p = new adt_pcg_plan();
@@ -817,7 +904,6 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
             return adt::set_error(ADT_ERR_INVALID, "adt_pcg_create: system %d has no rows", s);
         }
     // Halo tables per distinct pattern; shrink the tile height until the staged rows fit in LDS.
-    typedef std::tuple<int64_t, int64_t, int> PatKey;
     std::map<PatKey, int> pattern_of;
     std::vector<PatKey> patterns;
     for (int s = 0; s < n_sys; ++s) {
@@ -826,6 +912,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
             pattern_of[key] = static_cast<int>(patterns.size());
             patterns.push_back(key);
         }
+        p->pattern_of_sys.push_back(pattern_of[key]);
     }
     std::vector<PatternTiles> ptiles;
     if (h_indptr && h_indices && lds_bytes > 0) {
@@ -871,7 +958,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     std::vector<int> t0, nt;
     int64_t vo = 0, dvo = 0;
     for (int s = 0; s < n_sys; ++s) {
-        const int q = pattern_of[PatKey{h_ptr_off[s], h_idx_off[s], h_rows[s]}];
+        const int q = p->pattern_of_sys[s];
         Sys S{h_rows[s], h_ptr_off[s], h_idx_off[s], h_val_off[s], vo, dvo, h_b_off ? h_b_off[s] : vo,
               p->lds_rows ? lidx_base[q] : 0};
         p->sys_h.push_back(S);
@@ -896,7 +983,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
         (rc = dalloc(&p->grp_tile0_d, p->n_groups)) || (rc = dalloc(&p->grp_ntiles_d, p->n_groups)) ||
         (rc = dalloc(&p->vec, 5 * vo)) || (rc = dalloc(&p->dinv, dvo)) ||
         (rc = dalloc(&p->part, 3 * static_cast<int64_t>(p->n_tiles) * lanes)) ||
-        (rc = dalloc(&p->scal, 7 * nsc)) || (rc = dalloc(&p->ints, 3 * nsc + 2 * p->n_groups + 1)) ||
+        (rc = dalloc(&p->scal, 9 * nsc)) || (rc = dalloc(&p->ints, 3 * nsc + 2 * p->n_groups + 1)) ||
         (rc = dalloc(&p->args_d, 1)) || (rc = dalloc(&p->lidx_d, static_cast<int64_t>(lidx_all.size()))) ||
         (rc = dalloc(&p->halo_ptr_d, static_cast<int64_t>(halo_ptr_all.size()))) ||
         (rc = dalloc(&p->halo_rows_d, static_cast<int64_t>(halo_rows_all.size())))) {
@@ -950,6 +1037,8 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     a.bb = p->scal + 3 * nsc;
     a.rr = p->scal + 4 * nsc;
     a.rz_old = p->scal + 5 * nsc;
+    a.pq_old = p->scal + 6 * nsc;
+    a.rr_best = p->scal + 7 * nsc;
     a.iters = p->ints;
     a.flags = p->ints + nsc;
     a.stall = p->ints + 2 * nsc;
@@ -964,7 +1053,350 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     a.groups = p->groups;
     a.r_real = r;
     a.n_tiles = p->n_tiles;
+    a.stall_limit = 0;
     *out = p;
+    return ADT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// multigrid hierarchy
+// ------------------------------------------------------------------------------------------
+
+namespace {
+
+// Device copy of a host vector, owned by the plan's multigrid buffer list.
+template <typename T>
+int upload(adt_pcg_plan* p, const std::vector<T>& v, T** out) {
+    T* d = nullptr;
+    int rc = dalloc(&d, static_cast<int64_t>(v.size()));
+    if (rc) return rc;
+    p->amg_buffers.push_back(d);
+    if (!v.empty() && hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return adt::set_error(ADT_ERR_HIP, "multigrid upload failed");
+    *out = d;
+    return ADT_OK;
+}
+
+template <typename T>
+int zeros(adt_pcg_plan* p, int64_t count, T** out) {
+    T* d = nullptr;
+    int rc = dalloc(&d, count);
+    if (rc) return rc;
+    p->amg_buffers.push_back(d);
+    if (hipMemset(d, 0, sizeof(T) * static_cast<size_t>(std::max<int64_t>(count, 1))) != hipSuccess)
+        return adt::set_error(ADT_ERR_HIP, "multigrid buffer clear failed");
+    *out = d;
+    return ADT_OK;
+}
+
+template <typename T>
+void append(std::vector<T>& dst, const std::vector<T>& src) {
+    dst.insert(dst.end(), src.begin(), src.end());
+}
+
+int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indices, const double* h_values,
+               const int64_t* h_val_off, const adt::amg::Options& opt) {
+    using adt::amg::HostLevel;
+    const int n_pat = 1 + *std::max_element(p->pattern_of_sys.begin(), p->pattern_of_sys.end());
+    std::vector<int> rep_sys(n_pat, -1);
+    for (int s = 0; s < p->n_sys; ++s)
+        if (rep_sys[p->pattern_of_sys[s]] < 0) rep_sys[p->pattern_of_sys[s]] = s;
+    // hierarchies per pattern, all of the same depth
+    std::vector<std::vector<HostLevel>> H(n_pat);
+    int depth = 0;
+    for (int q = 0; q < n_pat; ++q) {
+        const Sys& S = p->sys_h[rep_sys[q]];
+        H[q] = adt::amg::build_hierarchy(static_cast<int>(S.rows), h_indptr + S.ptr_off, h_indices + S.idx_off,
+                                         h_values + h_val_off[rep_sys[q]], opt);
+        if (H[q].size() < 2)
+            return adt::set_error(ADT_ERR_INVALID,
+                                  "multigrid setup: pattern %d has no coarse level (matrix too small, not SPD or "
+                                  "without strong connections)", q);
+        if (H[q].back().n > std::max(opt.coarse_max, 2048))
+            return adt::set_error(ADT_ERR_INVALID,
+                                  "multigrid setup: coarsening of pattern %d stalled at %d rows", q, H[q].back().n);
+        depth = std::max(depth, static_cast<int>(H[q].size()));
+    }
+    for (int q = 0; q < n_pat; ++q)
+        if (static_cast<int>(H[q].size()) < depth) {
+            adt::amg::Options o2 = opt;
+            o2.coarse_max = 0;
+            o2.max_levels = depth;
+            const Sys& S = p->sys_h[rep_sys[q]];
+            H[q] = adt::amg::build_hierarchy(static_cast<int>(S.rows), h_indptr + S.ptr_off, h_indices + S.idx_off,
+                                             h_values + h_val_off[rep_sys[q]], o2);
+            if (static_cast<int>(H[q].size()) != depth)
+                return adt::set_error(ADT_ERR_INVALID, "multigrid setup: patterns coarsen to different depths");
+        }
+    p->free_amg();
+    p->amg.resize(depth);
+    const int rp = p->rp, L = p->L;
+    const int rbc = std::max(256 / L, 32);  // coarse-level tile height
+    // pattern-level offsets, per level
+    std::vector<std::vector<int64_t>> off_ptr(depth, std::vector<int64_t>(n_pat)), off_idx = off_ptr,
+        off_diag = off_ptr, off_pptr = off_ptr, off_pidx = off_ptr, off_pmp = off_ptr, off_pmap = off_ptr,
+        off_rptr = off_ptr, off_rent = off_ptr, off_tpp = off_ptr, off_tp = off_ptr, off_cpp = off_ptr,
+        off_cp = off_ptr;
+    for (int l = 0; l < depth; ++l) {
+        AmgLevel& A = p->amg[l];
+        std::vector<int> ptr, idx, diag, pptr, pidx, pmap_ptr, pmap, pself, rptr, rrow, rslot, tpp, ta, tq, cpp, cp,
+            ct;
+        for (int q = 0; q < n_pat; ++q) {
+            const HostLevel& h = H[q][l];
+            off_ptr[l][q] = static_cast<int64_t>(ptr.size());
+            off_idx[l][q] = static_cast<int64_t>(idx.size());
+            off_diag[l][q] = static_cast<int64_t>(diag.size());
+            off_pptr[l][q] = static_cast<int64_t>(pptr.size());
+            off_pidx[l][q] = static_cast<int64_t>(pidx.size());
+            off_pmp[l][q] = static_cast<int64_t>(pmap_ptr.size());
+            off_pmap[l][q] = static_cast<int64_t>(pmap.size());
+            off_rptr[l][q] = static_cast<int64_t>(rptr.size());
+            off_rent[l][q] = static_cast<int64_t>(rrow.size());
+            off_tpp[l][q] = static_cast<int64_t>(tpp.size());
+            off_tp[l][q] = static_cast<int64_t>(ta.size());
+            off_cpp[l][q] = static_cast<int64_t>(cpp.size());
+            off_cp[l][q] = static_cast<int64_t>(cp.size());
+            if (l > 0) {
+                append(ptr, h.ptr);
+                append(idx, h.idx);
+            }
+            append(diag, h.diag);
+            append(pptr, h.pptr);
+            append(pidx, h.pidx);
+            append(pself, h.pself);
+            append(pmap_ptr, h.pmap_ptr);
+            append(pmap, h.pmap);
+            append(rptr, h.rptr);
+            append(rrow, h.rrow);
+            append(rslot, h.rslot);
+            append(tpp, h.tpair_ptr);
+            append(ta, h.tpair_a);
+            append(tq, h.tpair_p);
+            append(cpp, h.cpair_ptr);
+            append(cp, h.cpair_p);
+            append(ct, h.cpair_t);
+        }
+        LevelArgs& a = A.args_h;
+        int32_t* d32 = nullptr;
+        int* di = nullptr;
+        int rc = ADT_OK;
+        if (l > 0) {
+            if ((rc = upload(p, ptr, &d32))) return rc;
+            a.ptr = d32;
+            if ((rc = upload(p, idx, &d32))) return rc;
+            a.idx = d32;
+        }
+#define ADT_UP(vec, field)                         \
+    if ((rc = upload(p, vec, &di))) return rc;     \
+    a.field = di;
+        ADT_UP(diag, diag)
+        ADT_UP(pptr, pptr)
+        ADT_UP(pidx, pidx)
+        ADT_UP(pself, pself)
+        ADT_UP(pmap_ptr, pmap_ptr)
+        ADT_UP(pmap, pmap)
+        ADT_UP(rptr, rptr)
+        ADT_UP(rrow, rrow)
+        ADT_UP(rslot, rslot)
+        ADT_UP(tpp, tpair_ptr)
+        ADT_UP(ta, tpair_a)
+        ADT_UP(tq, tpair_p)
+        ADT_UP(cpp, cpair_ptr)
+        ADT_UP(cp, cpair_p)
+        ADT_UP(ct, cpair_t)
+#undef ADT_UP
+    }
+    // per-system records, tiles and numeric buffers
+    std::vector<int64_t> val_len(depth, 0), vec_len(depth, 0), dinv_len(depth, 0), pv_len(depth, 0),
+        tv_len(depth, 0);
+    for (int l = 0; l < depth; ++l) {
+        AmgLevel& A = p->amg[l];
+        for (int s = 0; s < p->n_sys; ++s) {
+            const int q = p->pattern_of_sys[s];
+            const HostLevel& h = H[q][l];
+            const Sys& F = p->sys_h[s];
+            LvSys S{};
+            S.rows = h.n;
+            S.nc = h.nc;
+            S.n_p = static_cast<int>(h.pidx.size());
+            S.n_t = static_cast<int>(h.tidx.size());
+            S.n_next = l + 1 < depth ? static_cast<int>(H[q][l + 1].idx.size()) : 0;
+            S.omega = h.omega;
+            S.diag = off_diag[l][q];
+            S.pptr = off_pptr[l][q];
+            S.pidx = off_pidx[l][q];
+            S.pmap_ptr = off_pmp[l][q];
+            S.pmap = off_pmap[l][q];
+            S.rptr = off_rptr[l][q];
+            S.rent = off_rent[l][q];
+            S.tpair_ptr = off_tpp[l][q];
+            S.tpair = off_tp[l][q];
+            S.cpair_ptr = off_cpp[l][q];
+            S.cpair = off_cp[l][q];
+            if (l == 0) {
+                S.ptr = F.ptr_off;
+                S.idx = F.idx_off;
+                S.val = F.val_off;
+                S.vec = F.vec_off;
+                S.dinv = F.diag_off;
+            } else {
+                S.ptr = off_ptr[l][q];
+                S.idx = off_idx[l][q];
+                S.val = val_len[l];
+                S.vec = vec_len[l];
+                S.dinv = dinv_len[l];
+                val_len[l] += static_cast<int64_t>(h.idx.size());
+                vec_len[l] += static_cast<int64_t>(h.n) * rp;
+                dinv_len[l] += h.n;
+            }
+            S.pval = pv_len[l];
+            S.tval = tv_len[l];
+            pv_len[l] += S.n_p;
+            tv_len[l] += S.n_t;
+            if (l + 1 == depth) {
+                S.dense = A.dense_len;
+                A.dense_len += static_cast<int64_t>(h.n) * h.n;
+            }
+            A.ls_h.push_back(S);
+            A.max_rows = std::max(A.max_rows, h.n);
+            A.max_p = std::max(A.max_p, S.n_p);
+            A.max_t = std::max(A.max_t, S.n_t);
+            A.max_next = std::max(A.max_next, S.n_next);
+            if (l > 0)
+                for (int g = 0; g < p->groups; ++g)
+                    for (int r0 = 0; r0 < h.n; r0 += rbc)
+                        A.tiles_h.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + rbc, h.n), -1, 0});
+        }
+    }
+    // next-level offsets (value and vector blocks of the same system one level down)
+    for (int l = 0; l + 1 < depth; ++l)
+        for (int s = 0; s < p->n_sys; ++s) {
+            p->amg[l].ls_h[s].next_val = p->amg[l + 1].ls_h[s].val;
+            p->amg[l].ls_h[s].next_vec = p->amg[l + 1].ls_h[s].vec;
+        }
+    std::vector<double*> val_d(depth, nullptr), dinv_d(depth, nullptr), b_d(depth, nullptr), r1_d(depth, nullptr),
+        e_d(depth, nullptr), z2_d(depth, nullptr);
+    for (int l = 0; l < depth; ++l) {
+        AmgLevel& A = p->amg[l];
+        LevelArgs& a = A.args_h;
+        int rc = ADT_OK;
+        if (l == 0) {
+            A.n_tiles = p->n_tiles;
+            A.tiles_d = p->tiles_d;
+            if (p->lds_rows == 0 && (rc = zeros(p, p->vec_len, &z2_d[0]))) return rc;
+        } else {
+            A.n_tiles = static_cast<int>(A.tiles_h.size());
+            if ((rc = upload(p, A.tiles_h, &A.tiles_d)) || (rc = zeros(p, val_len[l], &val_d[l])) ||
+                (rc = zeros(p, dinv_len[l], &dinv_d[l])) || (rc = zeros(p, vec_len[l], &b_d[l])) ||
+                (rc = zeros(p, vec_len[l], &r1_d[l])) || (rc = zeros(p, vec_len[l], &e_d[l])) ||
+                (rc = zeros(p, vec_len[l], &z2_d[l])))
+                return rc;
+        }
+        double* pv = nullptr;
+        double* tv = nullptr;
+        if ((rc = upload(p, A.ls_h, &A.ls_d)) || (rc = zeros(p, pv_len[l], &pv)) || (rc = zeros(p, tv_len[l], &tv)))
+            return rc;
+        a.ls = A.ls_d;
+        a.tiles = A.tiles_d;
+        a.grp_active = p->args_h.grp_active;
+        a.n_tiles = A.n_tiles;
+        a.rp = rp;
+        a.groups = p->groups;
+        a.n_sys = p->n_sys;
+        a.coarsest = l + 1 == depth;
+        a.fine = l == 0;
+        a.pval = pv;
+        a.tval = tv;
+        if (a.coarsest) {
+            double* dn = nullptr;
+            if ((rc = zeros(p, A.dense_len, &dn))) return rc;
+            a.dense = dn;
+        }
+    }
+    for (int l = 0; l < depth; ++l) {
+        LevelArgs& a = p->amg[l].args_h;
+        if (l == 0) {
+            a.dinv = p->args_h.dinv;
+            a.dinv_out = nullptr;
+            a.b = p->args_h.r;
+            a.r1 = p->args_h.z;
+            a.e = p->args_h.z;
+            a.z2 = z2_d[0];
+        } else {
+            a.val = val_d[l];
+            a.dinv = dinv_d[l];
+            a.dinv_out = dinv_d[l];
+            a.b = b_d[l];
+            a.r1 = r1_d[l];
+            a.e = e_d[l];
+            a.z2 = z2_d[l];
+        }
+        if (l + 1 < depth) {
+            a.next_tiles = p->amg[l + 1].tiles_d;
+            a.n_next_tiles = p->amg[l + 1].n_tiles;
+            a.next_valp = val_d[l + 1];
+            a.next_b = b_d[l + 1];
+            a.next_e = e_d[l + 1];
+        }
+        int rc = dalloc(&p->amg[l].args_d, 1);
+        if (rc) return rc;
+        p->amg_buffers.push_back(p->amg[l].args_d);
+    }
+    return ADT_OK;
+}
+
+}  // namespace
+
+extern "C" int adt_pcg_attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indices,
+                                  const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
+                                  int max_levels, int stall_limit, int* h_levels) {
+    ADT_REQUIRE(p && h_indptr && h_indices && h_values && h_val_off, "adt_pcg_attach_amg: null pointer");
+    ADT_REQUIRE(theta >= 0.0 && theta < 1.0, "adt_pcg_attach_amg: theta must be in [0, 1)");
+    ADT_REQUIRE(coarse_max >= 1 && coarse_max <= 4096, "adt_pcg_attach_amg: coarse_max must be in [1, 4096]");
+    ADT_REQUIRE(max_levels >= 2 && max_levels <= 32, "adt_pcg_attach_amg: max_levels must be in [2, 32]");
+    ADT_REQUIRE(stall_limit >= 0, "adt_pcg_attach_amg: stall_limit must be >= 0");
+    adt::amg::Options opt;
+    opt.theta = theta;
+    opt.coarse_max = coarse_max;
+    opt.max_levels = max_levels;
+    const int rc = attach_amg(p, h_indptr, h_indices, h_values, h_val_off, opt);
+    if (rc) {
+        p->free_amg();
+        return rc;
+    }
+    p->stall_limit = stall_limit;
+    if (h_levels) *h_levels = static_cast<int>(p->amg.size());
+    return ADT_OK;
+}
+
+extern "C" int adt_pcg_amg_level(const adt_pcg_plan* p, int sys, int level, int* h_rows, int64_t* h_nnz,
+                                 double* h_omega) {
+    ADT_REQUIRE(p, "adt_pcg_amg_level: null plan");
+    ADT_REQUIRE(sys >= 0 && sys < p->n_sys, "adt_pcg_amg_level: system out of range");
+    ADT_REQUIRE(level >= 0 && level < static_cast<int>(p->amg.size()), "adt_pcg_amg_level: level out of range");
+    const LvSys& S = p->amg[level].ls_h[sys];
+    if (h_rows) *h_rows = S.rows;
+    if (h_nnz) *h_nnz = level == 0 ? -1 : static_cast<int64_t>(p->amg[level - 1].ls_h[sys].n_next);  // -1: caller's
+    if (h_omega) *h_omega = S.omega;
+    return ADT_OK;
+}
+
+extern "C" int adt_amg_hierarchy_info(int n, const int32_t* h_indptr, const int32_t* h_indices,
+                                      const double* h_values, double theta, int coarse_max, int max_levels,
+                                      int* h_levels, int* h_rows, int64_t* h_nnz) {
+    ADT_REQUIRE(n >= 1 && h_indptr && h_indices && h_values && h_levels, "adt_amg_hierarchy_info: invalid input");
+    ADT_REQUIRE(max_levels >= 1 && max_levels <= 32, "adt_amg_hierarchy_info: max_levels must be in [1, 32]");
+    adt::amg::Options opt;
+    opt.theta = theta;
+    opt.coarse_max = coarse_max;
+    opt.max_levels = max_levels;
+    const auto H = adt::amg::build_hierarchy(n, h_indptr, h_indices, h_values, opt);
+    ADT_REQUIRE(!H.empty(), "adt_amg_hierarchy_info: matrix is not SPD-like (missing or non-positive diagonal)");
+    *h_levels = static_cast<int>(H.size());
+    for (size_t l = 0; l < H.size(); ++l) {
+        if (h_rows) h_rows[l] = H[l].n;
+        if (h_nnz) h_nnz[l] = static_cast<int64_t>(H[l].idx.size());
+    }
     return ADT_OK;
 }
 
@@ -990,15 +1422,59 @@ extern "C" int adt_pcg_destroy(adt_pcg_plan* p) {
     return ADT_OK;
 }
 
+namespace {
+
+struct SolveReport {
+    std::vector<int> iters, flags;
+    std::vector<double> rr, bb;
+};
+
+int read_report(adt_pcg_plan* p, hipStream_t st, SolveReport& out) {
+    const int64_t nsc = static_cast<int64_t>(p->n_sys) * p->rp;
+    out.iters.resize(nsc);
+    out.flags.resize(nsc);
+    out.rr.resize(nsc);
+    out.bb.resize(nsc);
+    const PcgArgs& a = p->args_h;
+    ADT_HIP_CHECK(hipMemcpyAsync(out.iters.data(), a.iters, sizeof(int) * nsc, hipMemcpyDeviceToHost, st));
+    ADT_HIP_CHECK(hipMemcpyAsync(out.flags.data(), a.flags, sizeof(int) * nsc, hipMemcpyDeviceToHost, st));
+    ADT_HIP_CHECK(hipMemcpyAsync(out.rr.data(), a.rr, sizeof(double) * nsc, hipMemcpyDeviceToHost, st));
+    ADT_HIP_CHECK(hipMemcpyAsync(out.bb.data(), a.bb, sizeof(double) * nsc, hipMemcpyDeviceToHost, st));
+    ADT_HIP_CHECK(hipStreamSynchronize(st));
+    return ADT_OK;
+}
+
+int upload_args(adt_pcg_plan* p, hipStream_t st, const double* omega_slot) {
+    PcgArgs& a = p->args_h;
+    ADT_HIP_CHECK(hipMemcpyAsync(p->args_d, &a, sizeof(PcgArgs), hipMemcpyHostToDevice, st));
+    for (size_t l = 0; l < p->amg.size(); ++l) {
+        LevelArgs& la = p->amg[l].args_h;
+        if (l == 0) {
+            la.ptr = a.indptr;
+            la.idx = a.indices;
+            la.val = a.values;
+            la.omega_slot = omega_slot;
+        }
+        ADT_HIP_CHECK(hipMemcpyAsync(p->amg[l].args_d, &la, sizeof(LevelArgs), hipMemcpyHostToDevice, st));
+    }
+    return ADT_OK;
+}
+
+}  // namespace
+
 extern "C" int adt_pcg_solve(adt_pcg_plan* p, const int32_t* d_indptr, const int32_t* d_indices,
                              const double* d_values, const double* d_b, double* d_x, double tol, int max_iter,
-                             int precond, const void* gnn, int check_every, int32_t* h_iters, double* h_relres,
-                             void* stream) {
+                             int precond, const void* precond_data, int check_every, int32_t* h_iters,
+                             double* h_relres, void* stream) {
     ADT_REQUIRE(p && d_indptr && d_indices && d_values && d_b && d_x, "adt_pcg_solve: null pointer");
     ADT_REQUIRE(tol > 0.0 && std::isfinite(tol), "adt_pcg_solve: tol must be positive");
     ADT_REQUIRE(max_iter >= 0, "adt_pcg_solve: max_iter must be >= 0");
-    ADT_REQUIRE(precond == ADT_PRECOND_JACOBI, "adt_pcg_solve: unsupported preconditioner %d", precond);
-    (void)gnn;
+    ADT_REQUIRE(precond == ADT_PRECOND_JACOBI || precond == ADT_PRECOND_LEARNED || precond == ADT_PRECOND_AMG,
+                "adt_pcg_solve: unsupported preconditioner %d", precond);
+    ADT_REQUIRE(precond == ADT_PRECOND_JACOBI || !p->amg.empty(),
+                "adt_pcg_solve: the multigrid preconditioner needs adt_pcg_attach_amg first");
+    ADT_REQUIRE(precond != ADT_PRECOND_LEARNED || precond_data,
+                "adt_pcg_solve: the learned preconditioner needs its per-slot damping array");
     const int chunk = std::min(kMaxChunk, std::max(2, (check_every + 1) / 2 * 2));
     hipStream_t st = adt::as_stream(stream);
     PcgArgs& a = p->args_h;
@@ -1008,39 +1484,59 @@ extern "C" int adt_pcg_solve(adt_pcg_plan* p, const int32_t* d_indptr, const int
     a.b = d_b;
     a.x = d_x;
     a.tol2 = tol * tol;
+    const int mode = precond == ADT_PRECOND_JACOBI ? 0 : 1;
+    a.stall_limit = mode == 1 ? p->stall_limit : 0;
     // Kernels read the arguments from device memory so one captured graph serves every solve.
-    ADT_HIP_CHECK(hipMemcpyAsync(p->args_d, &a, sizeof(PcgArgs), hipMemcpyHostToDevice, st));
-    int rc = ADT_OK;
-    switch (p->L) {
-        case 8: rc = solve_impl<8>(p, st, max_iter, chunk); break;
-        case 16: rc = solve_impl<16>(p, st, max_iter, chunk); break;
-        case 32: rc = solve_impl<32>(p, st, max_iter, chunk); break;
-        default: rc = solve_impl<64>(p, st, max_iter, chunk); break;
-    }
+    int rc = upload_args(p, st, precond == ADT_PRECOND_LEARNED ? static_cast<const double*>(precond_data) : nullptr);
     if (rc) return rc;
-    const int64_t nsc = static_cast<int64_t>(p->n_sys) * p->rp;
-    std::vector<int> iters(nsc), flags(nsc);
-    std::vector<double> rr(nsc), bb(nsc);
-    ADT_HIP_CHECK(hipMemcpyAsync(iters.data(), a.iters, sizeof(int) * nsc, hipMemcpyDeviceToHost, st));
-    ADT_HIP_CHECK(hipMemcpyAsync(flags.data(), a.flags, sizeof(int) * nsc, hipMemcpyDeviceToHost, st));
-    ADT_HIP_CHECK(hipMemcpyAsync(rr.data(), a.rr, sizeof(double) * nsc, hipMemcpyDeviceToHost, st));
-    ADT_HIP_CHECK(hipMemcpyAsync(bb.data(), a.bb, sizeof(double) * nsc, hipMemcpyDeviceToHost, st));
-    ADT_HIP_CHECK(hipStreamSynchronize(st));
+    if (mode == 1 && (rc = amg_numeric(p, st))) return rc;
+    if ((rc = solve_dispatch(p, st, max_iter, chunk, mode))) return rc;
+    SolveReport rep;
+    if ((rc = read_report(p, st, rep))) return rc;
+    p->last_fallback = 0;
+    p->last_fcg_iters_max = 0;
+    if (mode == 1) {
+        std::vector<int> first_iters = rep.iters;
+        int need = 0;
+        for (int s = 0; s < p->n_sys; ++s)
+            for (int c = 0; c < p->r; ++c) {
+                const int64_t i = static_cast<int64_t>(s) * p->rp + c;
+                p->last_fcg_iters_max = std::max(p->last_fcg_iters_max, rep.iters[i]);
+                need += rep.flags[i] == -1 || rep.flags[i] == -2 || rep.flags[i] == 1;
+            }
+        if (need > 0 && max_iter > 0) {
+            // Jacobi PCG finishes every right-hand side from its current iterate; converged ones
+            // stop at once (their residual already meets the tolerance).
+            p->last_fallback = need;
+            a.stall_limit = 0;
+            if ((rc = upload_args(p, st, nullptr)) || (rc = solve_dispatch(p, st, max_iter, chunk, 0)) ||
+                (rc = read_report(p, st, rep)))
+                return rc;
+            for (size_t i = 0; i < rep.iters.size(); ++i) rep.iters[i] += first_iters[i];
+        }
+    }
     int worst = ADT_OK;
     for (int s = 0; s < p->n_sys; ++s)
         for (int c = 0; c < p->r; ++c) {
             const int64_t i = static_cast<int64_t>(s) * p->rp + c;
-            if (h_iters) h_iters[static_cast<int64_t>(s) * p->r + c] = iters[i];
-            const double rel = bb[i] > 0.0 ? std::sqrt(rr[i] / bb[i]) : 0.0;
+            if (h_iters) h_iters[static_cast<int64_t>(s) * p->r + c] = rep.iters[i];
+            const double rel = rep.bb[i] > 0.0 ? std::sqrt(rep.rr[i] / rep.bb[i]) : 0.0;
             if (h_relres) h_relres[static_cast<int64_t>(s) * p->r + c] = rel;
-            if (flags[i] == -1 || !std::isfinite(rel))
+            if (rep.flags[i] == -1 || !std::isfinite(rel))
                 worst = ADT_ERR_NONFINITE;
-            else if (flags[i] == 1 && worst == ADT_OK)
+            else if ((rep.flags[i] == 1 || rep.flags[i] == -2) && worst == ADT_OK)
                 worst = ADT_ERR_NOT_CONVERGED;
         }
     if (worst == ADT_ERR_NONFINITE) return adt::set_error(worst, "PCG produced non-finite values or broke down");
     if (worst == ADT_ERR_NOT_CONVERGED)
         return adt::set_error(worst, "PCG reached max_iter=%d before tol=%g", max_iter, tol);
+    return ADT_OK;
+}
+
+extern "C" int adt_pcg_stats(const adt_pcg_plan* p, int* h_levels, int* h_fallback_rhs) {
+    ADT_REQUIRE(p, "adt_pcg_stats: null plan");
+    if (h_levels) *h_levels = static_cast<int>(p->amg.size());
+    if (h_fallback_rhs) *h_fallback_rhs = p->last_fallback;
     return ADT_OK;
 }
 
@@ -1087,7 +1583,7 @@ extern "C" int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, con
 extern "C" int adt_pcg_profile(adt_pcg_plan* p, int enable) {
     ADT_REQUIRE(p, "adt_pcg_profile: null plan");
     p->profile = enable ? 1 : 0;
-    p->prof_ms[0] = p->prof_ms[1] = 0.0;
+    p->prof_ms[0] = p->prof_ms[1] = p->prof_ms[2] = 0.0;
     p->prof_launches = 0;
     return ADT_OK;
 }
@@ -1104,5 +1600,6 @@ extern "C" int adt_pcg_profile_read(const adt_pcg_plan* p, double* h_out) {
     h_out[3] = static_cast<double>(group_iters);
     h_out[4] = static_cast<double>(p->L);
     h_out[5] = static_cast<double>(p->rp);
+    h_out[6] = p->prof_ms[2];
     return ADT_OK;
 }

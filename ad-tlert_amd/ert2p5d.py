@@ -343,7 +343,7 @@ class ERTForward2p5D:
     def __init__(self, mesh, survey, wavenumbers, weights, discretization, source_cell_ids, source_node_ids,
                  numerical_h2_refined=True, numerical_p2_refined=True, topographic_geometric_factor_mode="analytic",
                  terrain_cache_dir=None, normal_field_cache_max_entries=8, series_batch_steps=8,
-                 solver_tolerance=1e-10, sensitivity_precision="float64"):
+                 solver_tolerance=1e-10, sensitivity_precision="float64", preconditioner="amg"):
         self.mesh, self.survey = mesh, survey
         self.wavenumbers, self.weights = wavenumbers, weights
         self.discretization = discretization
@@ -356,7 +356,8 @@ class ERTForward2p5D:
         self.normal_field_cache_max_entries = int(normal_field_cache_max_entries)
         self.series_batch_steps = int(series_batch_steps)
         self.sensitivity_precision = option(sensitivity_precision, ("float64", "mixed"), "sensitivity_precision")
-        self._solver = BatchedSolver(tolerance=float(solver_tolerance))
+        self._solver = BatchedSolver(tolerance=float(solver_tolerance), preconditioner=option(
+            preconditioner, ("amg", "jacobi"), "preconditioner"))
         self._cache: dict = {}
         self._dev_state: _Device | None = None
         self._field_cache_obj = FieldCache(self.normal_field_cache_max_entries)
@@ -366,9 +367,11 @@ class ERTForward2p5D:
                          numerical_p2_refined: bool = True, topographic_geometric_factor_mode: str = "analytic",
                          terrain_cache_dir=None, normal_field_cache_max_entries: int = 8, series_batch_steps: int = 8,
                          *, solver_tolerance: float = 1e-10, sensitivity_precision: str = "float64",
-                         wavenumbers=None, weights=None) -> "ERTForward2p5D":
+                         wavenumbers=None, weights=None, preconditioner: str = "amg") -> "ERTForward2p5D":
         """Build the operator (``ert2p5d.py:132-216``).  ``wavenumbers``/``weights`` override the
-        reference quadrature rule; ``solver_tolerance`` is the PCG relative residual."""
+        reference quadrature rule; ``solver_tolerance`` is the relative residual of the iterative
+        solves; ``preconditioner`` is ``"amg"`` (multigrid-preconditioned flexible CG) or
+        ``"jacobi"`` (plain Jacobi PCG)."""
 
         if not torch.cuda.is_available():
             raise RuntimeError("ADTLERT-MI355X requires an AMD GPU with HIP (no device is visible)")
@@ -393,7 +396,7 @@ class ERTForward2p5D:
         return cls(mesh, survey, k, q, disc, mesh.locate_points(survey.electrode_positions)[0].long(),
                    torch.as_tensor(source_nodes, dtype=torch.long), numerical_h2_refined, numerical_p2_refined, gf_mode,
                    terrain_cache_dir, normal_field_cache_max_entries, series_batch_steps, solver_tolerance,
-                   sensitivity_precision)
+                   sensitivity_precision, preconditioner)
 
     # -- properties ------------------------------------------------------------------------------
 

@@ -79,7 +79,8 @@ class ERTForward3D:
 
     def __init__(self, mesh: Mesh3D, survey: Survey, boundary_mode: str = "mixed", singularity_removal: bool = True,
                  element_order: int = 1, geometric_factor_mode: str = "auto", normal_field_cache_max_entries: int = 8,
-                 series_batch_steps: int = 1, *, solver_tolerance: float = 1e-10, sensitivity_precision: str = "float64"):
+                 series_batch_steps: int = 1, *, solver_tolerance: float = 1e-10, sensitivity_precision: str = "float64",
+                 preconditioner: str = "amg"):
         self.mesh, self.survey = mesh, survey
         self.boundary_mode = option(boundary_mode, ("mixed", "dirichlet"), "boundary_mode")
         self.geometric_factor_mode = option(geometric_factor_mode, ("auto", "analytic", "numerical"),
@@ -98,7 +99,8 @@ class ERTForward3D:
         self._field_cache = FieldCache(normal_field_cache_max_entries)
         self.normal_field_cache_max_entries = self._field_cache.max_entries
         self.sensitivity_precision = option(sensitivity_precision, ("float64", "mixed"), "sensitivity_precision")
-        self._solver = BatchedSolver(tolerance=float(solver_tolerance))
+        self._solver = BatchedSolver(tolerance=float(solver_tolerance),
+                                     preconditioner=option(preconditioner, ("amg", "jacobi"), "preconditioner"))
         self._cache: dict = {}
         electrodes = np.asarray(survey.electrode_positions, dtype=float)
         quads = survey.measurement_slots().numpy()

@@ -14,11 +14,11 @@ from pathlib import Path
 import numpy as np
 import torch
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().parent / "_lib" / "libadt_hip.so"))
 
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
-PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_AMG = 0, 1, 2
 LDS_BYTES = int(os.environ.get("ADT_SPMV_LDS_BYTES", 49152))  # 0 disables the LDS-staged product
 
 c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -38,6 +38,10 @@ _SIGNATURES = {
     "adt_pcg_destroy": (c_int, [P]),
     "adt_pcg_profile": (c_int, [P, c_int]),
     "adt_pcg_profile_read": (c_int, [P, P]),
+    "adt_pcg_attach_amg": (c_int, [P, P, P, P, P, c_double, c_int, c_int, c_int, P]),
+    "adt_pcg_amg_level": (c_int, [P, c_int, c_int, P, P, P]),
+    "adt_pcg_stats": (c_int, [P, P, P]),
+    "adt_amg_hierarchy_info": (c_int, [c_int, P, P, P, c_double, c_int, c_int, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
@@ -153,8 +157,11 @@ class PcgPlan:
         self.idx_off = np.ascontiguousarray(idx_off, dtype=np.int64)
         self.val_off = np.ascontiguousarray(val_off, dtype=np.int64)
         self.b_off = None if b_off is None else np.ascontiguousarray(b_off, dtype=np.int64)
+        hp = None
         if host_pattern is not None:
             hp = tuple(np.ascontiguousarray(a, dtype=np.int32) for a in host_pattern)
+        self.host_pattern = hp
+        self.amg_levels = 0
         handle = c_void_p()
         check(self._lib.adt_pcg_create(ctypes.byref(handle), self.n_sys, self.r, self.lanes, int(rows_per_tile),
                                        host(self.rows), host(self.ptr_off), host(self.idx_off), host(self.val_off),
@@ -171,22 +178,61 @@ class PcgPlan:
         check(self._lib.adt_pcg_tiling(handle, ctypes.byref(rb), ctypes.byref(tiles), ctypes.byref(lds_rows)))
         self.rows_per_tile, self.tiles, self.lds_rows = rb.value, tiles.value, lds_rows.value
 
-    def solve(self, indptr, indices, values, b, x, tol, max_iter, *, precond=PRECOND_JACOBI, gnn=None, check_every=16):
+    def attach_amg(self, values, *, theta=0.08, coarse_max=256, max_levels=12, stall_limit=64) -> int:
+        """Build the multigrid hierarchy from the first system of each CSR pattern in the device
+        ``values`` (read back to the host once); returns the number of levels."""
+
+        if self.host_pattern is None:
+            raise ValueError("the multigrid preconditioner needs the host CSR pattern")
+        indptr_h = self.host_pattern[0]
+        flat = values.reshape(-1)
+        segments, seg_of, h_val_off = [], {}, np.zeros(self.n_sys, dtype=np.int64)
+        offset = 0
+        for s in range(self.n_sys):
+            key = (int(self.ptr_off[s]), int(self.idx_off[s]), int(self.rows[s]))
+            if key not in seg_of:
+                nnz = int(indptr_h[key[0] + key[2]])
+                v0 = int(self.val_off[s])
+                segments.append(flat[v0:v0 + nnz].double().cpu().numpy())
+                seg_of[key] = offset
+                offset += nnz
+            h_val_off[s] = seg_of[key]
+        h_values = np.ascontiguousarray(np.concatenate(segments), dtype=np.float64)
+        levels = c_int(0)
+        check(self._lib.adt_pcg_attach_amg(self._handle, host(self.host_pattern[0]), host(self.host_pattern[1]),
+                                           host(h_values), host(h_val_off), float(theta), int(coarse_max),
+                                           int(max_levels), int(stall_limit), ctypes.byref(levels)))
+        self.amg_levels = levels.value
+        return self.amg_levels
+
+    def amg_level(self, level: int, sys: int = 0) -> dict:
+        rows, nnz, omega = c_int(0), c_i64(0), c_double(0.0)
+        check(self._lib.adt_pcg_amg_level(self._handle, int(sys), int(level), ctypes.byref(rows), ctypes.byref(nnz),
+                                          ctypes.byref(omega)))
+        return dict(rows=rows.value, nnz=nnz.value, omega=omega.value)
+
+    def stats(self) -> dict:
+        levels, fallback = c_int(0), c_int(0)
+        check(self._lib.adt_pcg_stats(self._handle, ctypes.byref(levels), ctypes.byref(fallback)))
+        return dict(levels=levels.value, fallback_rhs=fallback.value)
+
+    def solve(self, indptr, indices, values, b, x, tol, max_iter, *, precond=PRECOND_JACOBI, precond_data=None,
+              check_every=16):
         iters = np.zeros((self.n_sys, self.r), dtype=np.int32)
         relres = np.zeros((self.n_sys, self.r), dtype=np.float64)
         rc = self._lib.adt_pcg_solve(self._handle, ptr(indptr), ptr(indices), ptr(values), ptr(b), ptr(x), float(tol),
-                                     int(max_iter), int(precond), c_void_p(0) if gnn is None else gnn,
-                                     int(check_every), host(iters), host(relres), stream())
+                                     int(max_iter), int(precond), ptr(precond_data), int(check_every), host(iters),
+                                     host(relres), stream())
         return rc, iters, relres
 
     def profile(self, enable: bool = True) -> None:
         check(self._lib.adt_pcg_profile(self._handle, int(bool(enable))))
 
     def profile_read(self) -> dict:
-        out = np.zeros(6, dtype=np.float64)
+        out = np.zeros(7, dtype=np.float64)
         check(self._lib.adt_pcg_profile_read(self._handle, host(out)))
         return dict(spmv_ms=out[0], update_ms=out[1], iterations=int(out[2]), group_iterations=int(out[3]),
-                    lanes=int(out[4]), rp=int(out[5]))
+                    lanes=int(out[4]), rp=int(out[5]), cycle_ms=out[6])
 
     def close(self):
         if getattr(self, "_handle", None):
@@ -247,3 +293,18 @@ def aggregate_columns(inp, col_ptr, col_idx, out):
         r1 = min(rows, r0 + 65535)
         check(library().adt_aggregate_columns(r1 - r0, int(inp.shape[1]), int(out.shape[1]), ptr(inp[r0:r1]),
                                               ptr(col_ptr), ptr(col_idx), ptr(out[r0:r1]), stream()))
+
+
+def amg_hierarchy_info(indptr, indices, values, *, theta=0.08, coarse_max=256, max_levels=12):
+    """Host-only: level sizes of the multigrid hierarchy built for one CSR matrix (no GPU needed)."""
+
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    levels = c_int(0)
+    rows = np.zeros(max_levels, dtype=np.int32)
+    nnz = np.zeros(max_levels, dtype=np.int64)
+    check(library().adt_amg_hierarchy_info(int(indptr.size - 1), host(indptr), host(indices), host(values),
+                                           float(theta), int(coarse_max), int(max_levels), ctypes.byref(levels),
+                                           host(rows), host(nnz)))
+    return rows[:levels.value].tolist(), nnz[:levels.value].tolist()

@@ -1,9 +1,13 @@
 """Batched iterative SPD solves on the GPU (replaces the cuDSS ``BatchedSolver``).
 
 The reference factorizes every matrix with cuDSS and back-substitutes all right-hand sides
-(``forward/cudss.py:182-247``).  Here each system is solved by FP64 preconditioned conjugate
-gradients in the native library; a plan per batch layout (number of systems, rows, right-hand
-sides) owns the Krylov workspace and the captured iteration graph and is reused across calls.
+(``forward/cudss.py:182-247``).  Here each system is solved in the native library by FP64
+conjugate gradients: Jacobi-preconditioned (``preconditioner="jacobi"``), or flexible CG
+preconditioned by one smoothed-aggregation multigrid V-cycle per iteration (``"amg"``), whose
+hierarchy is built once per plan from the first matrices solved with it and whose numeric
+operators are recomputed for every system of every solve; right-hand sides that stall under
+the cycle are finished by Jacobi PCG.  A plan per batch layout (number of systems, rows,
+right-hand sides) owns the Krylov workspace and the captured iteration graphs and is reused.
 
 Vectors are interleaved ``(N, Rp)`` blocks (node-major, sources contiguous).  ``Rp`` pads the
 source count to a multiple of the lane-group width ``L`` chosen by :func:`choose_lanes`;
@@ -50,11 +54,16 @@ def choose_rows_per_tile(n_sys: int, rows: int, r: int, lanes: int, target_tiles
     return -(-rb // per_pass) * per_pass
 
 
+PRECONDITIONERS = ("jacobi", "amg")
+
+
 @dataclass
 class SolveStats:
     iterations: np.ndarray
     relative_residual: np.ndarray
     seconds: float
+    preconditioner: str = "jacobi"
+    fallback_rhs: int = 0
 
 
 @dataclass
@@ -65,8 +74,16 @@ class BatchedSolver:
     max_iterations: int = 50000
     check_every: int = 16
     rows_per_tile: int | None = None
+    preconditioner: str = "jacobi"
+    amg_theta: float = 0.08
+    amg_coarse_max: int = 256
+    amg_stall_limit: int = 64
     _plans: dict = field(default_factory=dict)
     last: SolveStats | None = None
+
+    def __post_init__(self):
+        if self.preconditioner not in PRECONDITIONERS:
+            raise ValueError(f"preconditioner must be one of {PRECONDITIONERS}, got {self.preconditioner!r}")
 
     def plan(self, key, rows, r: int, *, pattern_offsets=None, b_off=None, lanes=None, rows_per_tile=None,
              host_pattern=None):
@@ -96,16 +113,35 @@ class BatchedSolver:
             self._plans[full_key] = plan
         return plan
 
+    def _precond(self, plan: native.PcgPlan, values, preconditioner):
+        name = self.preconditioner if preconditioner is None else preconditioner
+        if name == "jacobi":
+            return native.PRECOND_JACOBI, "jacobi"
+        if name != "amg":
+            raise ValueError(f"preconditioner must be one of {PRECONDITIONERS}, got {name!r}")
+        if plan.amg_levels == 0 and not getattr(plan, "amg_failed", False):
+            try:
+                plan.attach_amg(values, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
+                                stall_limit=self.amg_stall_limit)
+            except ValueError:
+                # no usable hierarchy (tiny or unstructured matrices): plain Jacobi PCG for this plan
+                plan.amg_failed = True
+        if plan.amg_levels == 0:
+            return native.PRECOND_JACOBI, "jacobi"
+        return native.PRECOND_AMG, "amg"
+
     def solve(self, plan: native.PcgPlan, indptr, indices, values, b, x, *, tol=None, max_iter=None,
-              precond=native.PRECOND_JACOBI, gnn=None, raise_on_failure=True):
+              preconditioner=None, raise_on_failure=True):
         """Solve in place: ``x`` holds the initial guess on entry and the solution on return."""
 
         tol = self.tolerance if tol is None else tol
         max_iter = self.max_iterations if max_iter is None else max_iter
+        precond, used = self._precond(plan, values, preconditioner)
         t0 = time.perf_counter()
-        rc, iters, relres = plan.solve(indptr, indices, values, b, x, tol, max_iter, precond=precond, gnn=gnn,
+        rc, iters, relres = plan.solve(indptr, indices, values, b, x, tol, max_iter, precond=precond,
                                        check_every=self.check_every)
-        self.last = SolveStats(iters, relres, time.perf_counter() - t0)
+        fallback = plan.stats()["fallback_rhs"] if precond != native.PRECOND_JACOBI else 0
+        self.last = SolveStats(iters, relres, time.perf_counter() - t0, used, fallback)
         if raise_on_failure and rc == native.ADT_ERR_NONFINITE:
             raise FloatingPointError("conjugate gradients produced non-finite values")
         if raise_on_failure and rc == native.ADT_ERR_NOT_CONVERGED:

@@ -89,9 +89,12 @@ int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t*
  * offsets, lane width) and owns the Krylov workspace; solves then take the matrix values,
  * right-hand sides and the initial guess (overwritten by the solution) by pointer.
  *
- * precond: ADT_PRECOND_JACOBI (plain PCG) or ADT_PRECOND_LEARNED (flexible CG whose
- * preconditioner is the learned graph network of adt_gnn_*; Jacobi-PCG restarts from the
- * current iterate for any right-hand side whose residual stalls).
+ * precond: ADT_PRECOND_JACOBI (plain PCG), ADT_PRECOND_AMG (flexible CG preconditioned by one
+ * smoothed-aggregation V-cycle per iteration; needs adt_pcg_attach_amg) or ADT_PRECOND_LEARNED
+ * (the same cycle whose finest prolongator damping comes per matrix slot from precond_data, a
+ * device array laid out like d_values, e.g. the output of the learned graph network).  With
+ * either cycle, a right-hand side whose residual stalls or breaks down is finished by Jacobi
+ * PCG from its current iterate, to the same tolerance (adt_pcg_stats reports how many).
  * h_indptr/h_indices (optional): host copies of the concatenated CSR patterns.  With them and
  * lds_bytes > 0 the plan cuts every pattern into row tiles whose rows and halo (referenced rows
  * outside the tile) fit in lds_bytes of LDS per workgroup, and the sparse product stages each
@@ -104,7 +107,7 @@ int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t*
  * h_iters (R per system, optional) receives iteration counts, h_relres the final
  * relative residuals.  The call synchronizes the stream once per `check_every` iterations.
  * ---------------------------------------------------------------------------------------- */
-enum { ADT_PRECOND_JACOBI = 0, ADT_PRECOND_LEARNED = 1 };
+enum { ADT_PRECOND_JACOBI = 0, ADT_PRECOND_LEARNED = 1, ADT_PRECOND_AMG = 2 };
 
 typedef struct adt_pcg_plan adt_pcg_plan;
 
@@ -116,16 +119,40 @@ int adt_pcg_tiling(const adt_pcg_plan* plan, int* h_rows_per_tile, int* h_tiles,
 int adt_pcg_layout(const adt_pcg_plan* plan, int* h_rp, int64_t* h_vec_len, int64_t* h_vec_off);
 int adt_pcg_solve(adt_pcg_plan* plan, const int32_t* d_indptr, const int32_t* d_indices,
                   const double* d_values, const double* d_b, double* d_x, double tol, int max_iter,
-                  int precond, const void* gnn, int check_every, int32_t* h_iters, double* h_relres,
+                  int precond, const void* precond_data, int check_every, int32_t* h_iters, double* h_relres,
                   void* stream);
 int adt_pcg_destroy(adt_pcg_plan* plan);
+
+/* Multigrid hierarchy of a plan (no reference counterpart: it preconditions the iterative
+ * solves that replace cuDSS).  Built on the host from host copies of the CSR patterns
+ * (h_indptr/h_indices as in adt_pcg_create) and representative values h_values + h_val_off[s]
+ * (only the first system of each pattern is read): aggregation by strong connections
+ * |a_ij| >= theta sqrt(a_ii a_jj), smoothed prolongators, Galerkin coarse operators, until a
+ * level has at most coarse_max rows (dense inverse) or max_levels is reached.  Every solve
+ * recomputes the numeric prolongators and Galerkin operators of every system on the device.
+ * stall_limit: iterations without a 10% residual decrease before Jacobi PCG takes over (0: never).
+ * h_levels receives the number of levels. */
+int adt_pcg_attach_amg(adt_pcg_plan* plan, const int32_t* h_indptr, const int32_t* h_indices,
+                       const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
+                       int max_levels, int stall_limit, int* h_levels);
+/* Rows, matrix slots (-1 on the finest level: the caller's pattern) and damping of one level. */
+int adt_pcg_amg_level(const adt_pcg_plan* plan, int sys, int level, int* h_rows, int64_t* h_nnz,
+                      double* h_omega);
+/* Levels of the attached hierarchy and right-hand sides finished by Jacobi PCG in the last solve. */
+int adt_pcg_stats(const adt_pcg_plan* plan, int* h_levels, int* h_fallback_rhs);
+/* Host-only: the hierarchy that adt_pcg_attach_amg would build for one matrix (level rows and
+ * matrix slots into h_rows / h_nnz, max_levels entries each).  Needs no GPU. */
+int adt_amg_hierarchy_info(int n, const int32_t* h_indptr, const int32_t* h_indices, const double* h_values,
+                           double theta, int coarse_max, int max_levels, int* h_levels, int* h_rows,
+                           int64_t* h_nnz);
 
 /* Profiling of the captured iterations (no reference counterpart; used by the benchmark to
  * report the live roofline of the dominant kernels).  adt_pcg_profile(plan, 1) switches the
  * iterations from graph replay to direct launches of the same kernels on the caller's stream,
  * with HIP timing events around every spmv and update launch, and zeroes the accumulators;
- * adt_pcg_profile(plan, 0) returns to graph replay.  adt_pcg_profile_read fills h_out[6] = {total spmv ms, total update ms, number
- * of profiled iterations, summed lane-group iterations of the last solve, L, Rp}. */
+ * adt_pcg_profile(plan, 0) returns to graph replay.  adt_pcg_profile_read fills h_out[7] =
+ * {total spmv ms, total update ms, number of profiled iterations, summed lane-group iterations
+ * of the last solve, L, Rp, total multigrid-cycle ms}. */
 int adt_pcg_profile(adt_pcg_plan* plan, int enable);
 int adt_pcg_profile_read(const adt_pcg_plan* plan, double* h_out);
 

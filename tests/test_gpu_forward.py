@@ -73,6 +73,28 @@ def test_apparent_resistivity_matches_the_oracle(name, quad, tri):
     assert relmax(rhoa, expected) < 1e-6
 
 
+@pytest.mark.parametrize("name", ["quad", "tri"])
+def test_multigrid_and_jacobi_solves_agree_and_multigrid_iterates_less(name, quad, tri):
+    if name == "quad":
+        case, forward, _ = quad
+        sigma, mesh, survey = case.conductivity, forward.mesh, forward.survey
+    else:
+        sigma, forward, _ = tri
+        mesh, survey = forward.mesh, forward.survey
+    jacobi = adt.ERTForward2p5D.from_mesh_survey(mesh, survey, solver_tolerance=1e-12, preconditioner="jacobi")
+    sig = torch.as_tensor(sigma) * 1.1  # not a cached field
+    rhoa_j = jacobi.apparent_resistivity_values(sig).numpy()
+    stats_j = jacobi._solver.last
+    rhoa_a = forward.apparent_resistivity_values(sig).numpy()
+    stats_a = forward._solver.last
+    print(f"[{name}] iterations jacobi max {stats_j.iterations.max()} amg max {stats_a.iterations.max()}, "
+          f"fallback {stats_a.fallback_rhs}")
+    assert stats_j.preconditioner == "jacobi" and stats_a.preconditioner == "amg"
+    assert relmax(rhoa_a, rhoa_j) < 1e-9
+    assert stats_a.relative_residual.max() <= 1.5e-12
+    assert stats_a.iterations.max() < stats_j.iterations.max()
+
+
 def test_homogeneous_half_space_and_scaling(quad):
     case, forward, _ = quad
     homogeneous = forward.apparent_resistivity_values(torch.full((forward.mesh.cell_count,), 0.01)).numpy()

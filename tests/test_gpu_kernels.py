@@ -101,12 +101,11 @@ def test_multi_rhs_product_matches_dense_products_for_every_lane_width():
         assert relmax(y.cpu().numpy(), dense @ x) < 1e-13, lanes
 
 
-@pytest.mark.parametrize("staged", [False, True], ids=["gather", "lds-halo"])
-def test_pcg_solves_random_spd_batches_to_tolerance(staged):
-    rng = np.random.default_rng(2)
-    n = 400
-    # 2D Laplacian-like SPD matrices with random positive coefficients, two systems
-    side = 20
+def random_spd_batch(rng, side=20, S=2):
+    """2D Laplacian-like SPD matrices with random positive coefficients: CSR pattern, values
+    (S, nnz) and dense copies."""
+
+    n = side * side
     idx = np.arange(n).reshape(side, side)
     rows, cols = [], []
     for di, dj in ((0, 0), (0, 1), (1, 0), (0, -1), (-1, 0)):
@@ -119,7 +118,6 @@ def test_pcg_solves_random_spd_batches_to_tolerance(staged):
     keys = np.unique(rows * n + cols)
     r, c = keys // n, keys % n
     indptr = np.concatenate(([0], np.cumsum(np.bincount(r, minlength=n)))).astype(np.int32)
-    S, R = 2, 13
     values = np.zeros((S, keys.size))
     dense = np.zeros((S, n, n))
     for s in range(S):
@@ -135,19 +133,78 @@ def test_pcg_solves_random_spd_batches_to_tolerance(staged):
         diag = -dense[s].sum(axis=1) + 0.01
         values[s, ~off] = diag[r[~off]]
         dense[s, r[~off], c[~off]] = values[s, ~off]
-    plan = native.PcgPlan(np.full(S, n), R, 16, 32, np.zeros(S), np.zeros(S), np.arange(S) * keys.size,
-                          host_pattern=(indptr, c.astype(np.int32)) if staged else None, lds_bytes=8192)
+    return indptr, c.astype(np.int32), values, dense
+
+
+def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2, omega=None):
+    rng = np.random.default_rng(seed)
+    indptr, indices, values, dense = random_spd_batch(rng, side)
+    S, n = values.shape[0], side * side
+    plan = native.PcgPlan(np.full(S, n), R, lanes, 32, np.zeros(S), np.zeros(S), np.arange(S) * values.shape[1],
+                          host_pattern=(indptr, indices) if staged or precond != native.PRECOND_JACOBI else None,
+                          lds_bytes=8192 if staged else 0)
     assert (plan.lds_rows > 0) == staged
+    values_d = torch.as_tensor(values, device=dev())
+    data = None
+    if precond != native.PRECOND_JACOBI:
+        assert plan.attach_amg(values_d, coarse_max=16, stall_limit=stall_limit) >= 3
+        assert plan.amg_level(1)["rows"] < n
+    if precond == native.PRECOND_LEARNED:
+        data = torch.full_like(values_d, omega)
     b = rng.standard_normal((S, n, plan.rp))
     b[:, :, R:] = 0.0
     b[1, :, 3] = 0.0  # a zero right-hand side -> zero solution
     x = torch.zeros((S, n, plan.rp), dtype=torch.float64, device=dev())
     x[0, :, 5] = 1.0
-    rc, iters, relres = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(c.astype(np.int32), device=dev()),
-                                   torch.as_tensor(values, device=dev()), torch.as_tensor(b, device=dev()), x, 1e-10, 5000)
+    rc, iters, relres = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()),
+                                   values_d, torch.as_tensor(b, device=dev()), x, 1e-10, 5000, precond=precond,
+                                   precond_data=data)
     assert rc == 0, native.library().adt_last_error()
     assert relres.max() <= 1e-10 * 1.5
     assert iters[1, 3] == 0 and float(x[1, :, 3].abs().max()) == 0.0
     sol = np.linalg.solve(dense, b[:, :, :R])
     assert relmax(x.cpu().numpy()[:, :, :R], sol) < 1e-7
+    stats = plan.stats()
     plan.close()
+    return iters, stats
+
+
+@pytest.mark.parametrize("staged", [False, True], ids=["gather", "lds-halo"])
+@pytest.mark.parametrize("precond", [native.PRECOND_JACOBI, native.PRECOND_AMG], ids=["jacobi", "amg"])
+def test_pcg_solves_random_spd_batches_to_tolerance(staged, precond):
+    iters, stats = solve_batch(precond, staged)
+    if precond == native.PRECOND_AMG:
+        jacobi_iters, _ = solve_batch(native.PRECOND_JACOBI, staged)
+        assert stats["levels"] >= 3 and stats["fallback_rhs"] == 0
+        assert iters.max() < 0.5 * jacobi_iters.max(), (iters.max(), jacobi_iters.max())
+
+
+def test_non_finite_preconditioner_hands_over_to_jacobi_pcg():
+    """A per-slot damping array of NaN (set on the host) makes every cycle output non-finite:
+    flexible CG stops each right-hand side before using it and Jacobi PCG solves all of them
+    from their current iterates to the same tolerance."""
+
+    iters, stats = solve_batch(native.PRECOND_LEARNED, True, omega=float("nan"))
+    assert stats["fallback_rhs"] == 2 * 13 - 1  # every right-hand side except the zero one
+
+
+def test_per_slot_damping_equal_to_the_constant_reproduces_the_cycle():
+    """The learned entry point with omega_k = the hierarchy's constant damping gives the same
+    iterates as the plain multigrid preconditioner (up to the order of one product)."""
+
+    it_amg, _ = solve_batch(native.PRECOND_AMG, True)
+    rng = np.random.default_rng(2)
+    indptr, indices, values, _ = random_spd_batch(rng)
+    plan = native.PcgPlan(np.full(2, 400), 13, 16, 32, np.zeros(2), np.zeros(2), np.arange(2) * values.shape[1],
+                          host_pattern=(indptr, indices), lds_bytes=8192)
+    plan.attach_amg(torch.as_tensor(values, device=dev()), coarse_max=16)
+    omega = plan.amg_level(0)["omega"]
+    plan.close()
+    it_learned, stats = solve_batch(native.PRECOND_LEARNED, True, omega=omega)
+    assert stats["fallback_rhs"] == 0
+    assert np.abs(it_learned - it_amg).max() <= 1
+
+
+@pytest.mark.parametrize("lanes, R", [(8, 5), (64, 70)])
+def test_multigrid_flexible_cg_for_other_lane_widths(lanes, R):
+    solve_batch(native.PRECOND_AMG, True, side=24, lanes=lanes, R=R, seed=3)

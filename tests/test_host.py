@@ -254,3 +254,95 @@ def test_dof_numbering_is_a_permutation_with_compact_tiles():
     spans = [np.ptp(pts[i:i + 64], axis=0).max() for i in range(0, pts.shape[0] - 64, 64)]
     natural = [np.ptp(case.nodes[i:i + 64], axis=0).max() for i in range(0, pts.shape[0] - 64, 64)]
     assert np.median(spans) < 0.5 * np.median(natural)
+
+
+def greedy_aggregates(indptr, indices, values, theta):
+    """Restatement of the multigrid setup's three-pass greedy aggregation over strong
+    connections |a_ij| >= theta sqrt(a_ii a_jj) (rows in index order), for pinning the library."""
+
+    n = indptr.size - 1
+    diag = np.array([values[indptr[i] + np.flatnonzero(indices[indptr[i]:indptr[i + 1]] == i)[0]] for i in range(n)])
+    strong = [[] for _ in range(n)]
+    weight = [[] for _ in range(n)]
+    for i in range(n):
+        for k in range(indptr[i], indptr[i + 1]):
+            j = indices[k]
+            s = abs(values[k]) / np.sqrt(abs(diag[i] * diag[j]))
+            if j != i and values[k] != 0 and s >= theta:
+                strong[i].append(j)
+                weight[i].append(s)
+    agg = -np.ones(n, int)
+    count = 0
+    for i in range(n):
+        if agg[i] < 0 and strong[i] and all(agg[j] < 0 for j in strong[i]):
+            agg[i] = count
+            agg[strong[i]] = count
+            count += 1
+    first = agg.copy()
+    for i in range(n):
+        if agg[i] < 0 and strong[i]:
+            cand = [(w, first[j]) for j, w in zip(strong[i], weight[i]) if first[j] >= 0]
+            if cand:
+                agg[i] = max(cand, key=lambda t: t[0])[1]
+    for i in range(n):
+        if agg[i] < 0 and strong[i]:
+            agg[i] = count
+            for j in strong[i]:
+                if agg[j] < 0:
+                    agg[j] = count
+            count += 1
+    return agg, count
+
+
+def test_multigrid_hierarchy_matches_a_restated_aggregation_and_galerkin_pattern():
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(7)
+    side = 23
+    n = side * side
+    A = sp.diags([4.0], [0], shape=(n, n)).tolil()
+    for i in range(side):
+        for j in range(side):
+            p = i * side + j
+            for di, dj in ((0, 1), (1, 0)):
+                if i + di < side and j + dj < side:
+                    q = (i + di) * side + j + dj
+                    w = -rng.uniform(0.2, 1.0) * (10.0 if (di == 1 and i < 5) else 1.0)  # anisotropic band
+                    A[p, q] = A[q, p] = w
+    A = A.tocsr()
+    A.setdiag(-np.asarray(A.sum(axis=1)).ravel() + A.diagonal() + 0.05)
+    A[0, 1] = A[1, 0] = 0.0  # explicit zeros stay weak
+    A = A.tocsr()
+    A.sort_indices()
+    rows, nnz = adt.native.amg_hierarchy_info(A.indptr, A.indices, A.data, theta=0.08, coarse_max=20)
+    agg, count = greedy_aggregates(A.indptr, A.indices, A.data, 0.08)
+    assert rows[0] == n and rows[1] == count
+    assert rows[-1] <= 20 and all(a > b for a, b in zip(rows, rows[1:]))
+    # Galerkin pattern of the first coarse level: P has the aggregates of each row's neighbours
+    coo = A.tocoo()
+    keep = agg[coo.col] >= 0
+    P = sp.csr_matrix((np.ones(keep.sum()), (coo.row[keep], agg[coo.col[keep]])), shape=(n, count))
+    Ac = (P.T @ abs(A) @ P).tocsr()
+    Ac.data[:] = 1.0
+    assert nnz[1] == Ac.nnz
+
+
+def test_multigrid_hierarchy_leaves_unconnected_rows_out_and_rejects_bad_diagonals():
+    import scipy.sparse as sp
+
+    side = 16
+    n = side * side
+    lap = sp.diags([4.0, -1.0, -1.0, -1.0, -1.0], [0, 1, -1, side, -side], shape=(n, n)).tolil()
+    for r in range(0, n, 7):  # Dirichlet-like rows: unit diagonal, no couplings
+        lap[r, :] = 0.0
+        lap[:, r] = 0.0
+        lap[r, r] = 1.0
+    lap = lap.tocsr()
+    lap.eliminate_zeros()
+    rows, _ = adt.native.amg_hierarchy_info(lap.indptr, lap.indices, lap.data, coarse_max=8)
+    agg, count = greedy_aggregates(lap.indptr, lap.indices, lap.data, 0.08)
+    assert rows[1] == count and (agg[::7] == -1).all()
+    bad = lap.copy()
+    bad.data[bad.indptr[3]:bad.indptr[4]][bad.indices[bad.indptr[3]:bad.indptr[4]] == 3] = -1.0
+    with pytest.raises(ValueError, match="SPD"):
+        adt.native.amg_hierarchy_info(bad.indptr, bad.indices, bad.data)
