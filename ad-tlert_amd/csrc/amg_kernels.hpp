@@ -11,8 +11,8 @@
 //   coarsest:       e = A_c^-1 b
 //   up, level l:    z2 = w D^-1 b + P e_{l+1}
 //                   e_l = z2 + w D^-1 (b - A z2)      (post-smoothing)
-// The finest level stages w D^-1 r (down) and z2 (up) for its rows and halo in LDS, like
-// the conjugate-gradient product, so each of its two passes reads r once per tile.
+// The finest level stages w D^-1 r (down) and z2 (up) for its rows and halo in LDS, like the
+// conjugate-gradient product, so each staged pass reads its input once per tile.
 #pragma once
 
 #include "krylov.hpp"
@@ -128,6 +128,29 @@ __global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __rest
 // cycle kernels, gather versions (any level)
 // ------------------------------------------------------------------------------------------
 
+// sum_k val(k) * term(idx(k)) over k in [k0, k1), U entries at a time: all index and value
+// loads first, then all gathers, so one thread keeps U independent requests in flight.  The
+// accumulation order is k0, k0+1, ... (padding entries add 0 * term(pad) with a valid pad row).
+template <int U, typename IdxF, typename ValF, typename TermF>
+__device__ __forceinline__ double gather_sum(int k0, int k1, IdxF idx, ValF val, TermF term, int64_t pad) {
+    double acc = 0.0;
+    for (int kb = k0; kb < k1; kb += U) {
+        int64_t jj[U];
+        double vv[U], tt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool ok = kb + u < k1;
+            jj[u] = ok ? idx(kb + u) : pad;
+            vv[u] = ok ? val(kb + u) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) tt[u] = term(jj[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = fma(vv[u], tt[u], acc);
+    }
+    return acc;
+}
+
 struct LevelTile {
     Tile t;
     LvSys S;
@@ -158,11 +181,9 @@ __global__ __launch_bounds__(256) void amg_smooth_residual(const LevelArgs* __re
     double* R1 = a.r1 + c.S.vec + c.col;
     const double w = c.S.omega;
     for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-        double acc = 0.0;
-        for (int k = ptr[row]; k < ptr[row + 1]; ++k) {
-            const int64_t j = idx[k];
-            acc = fma(val[k], w * D[j] * B[j * rp], acc);
-        }
+        const double acc = gather_sum<8>(
+            ptr[row], ptr[row + 1], [&](int k) { return static_cast<int64_t>(idx[k]); }, [&](int k) { return val[k]; },
+            [&](int64_t j) { return w * D[j] * B[j * rp]; }, row);
         const int64_t o = static_cast<int64_t>(row) * rp;
         R1[o] = B[o] - acc;
     }
@@ -182,8 +203,10 @@ __global__ __launch_bounds__(256) void amg_restrict(const LevelArgs* __restrict_
     const double* R1 = a.r1 + c.S.vec + c.col;
     double* NB = a.next_b + c.S.next_vec + c.col;
     for (int J = c.t.row0 + static_cast<int>(threadIdx.x / L); J < c.t.row1; J += kRowsPerPass(L)) {
-        double acc = 0.0;
-        for (int e = rptr[J]; e < rptr[J + 1]; ++e) acc = fma(P[rslot[e]], R1[static_cast<int64_t>(rrow[e]) * rp], acc);
+        const int e0 = rptr[J];
+        const double acc = gather_sum<8>(
+            e0, rptr[J + 1], [&](int e) { return static_cast<int64_t>(e); }, [&](int e) { return P[rslot[e]]; },
+            [&](int64_t e) { return R1[static_cast<int64_t>(rrow[e]) * rp]; }, e0);
         NB[static_cast<int64_t>(J) * rp] = acc;
     }
 }
@@ -208,9 +231,11 @@ __global__ __launch_bounds__(256) void amg_coarse_solve(const LevelArgs* __restr
     }
 }
 
-// z2 = w D^-1 b + P e_{l+1}
-template <int L>
-__global__ __launch_bounds__(256) void amg_prolong(const LevelArgs* __restrict__ la) {
+// z2 = w D^-1 b + P e_{l+1}.  On the finest level (FINE) z2 goes to the direction buffer that
+// the coming product overwrites (p[parity ^ 1] holds the direction of two iterations ago).
+template <int L, bool FINE>
+__global__ __launch_bounds__(256) void amg_prolong(const PcgArgs* __restrict__ pa, const LevelArgs* __restrict__ la,
+                                                   int parity) {
     const LevelArgs& a = *la;
     LevelTile c;
     if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
@@ -221,13 +246,14 @@ __global__ __launch_bounds__(256) void amg_prolong(const LevelArgs* __restrict__
     const double* D = a.dinv + c.S.dinv;
     const double* B = a.b + c.S.vec + c.col;
     const double* NE = a.next_e + c.S.next_vec + c.col;
-    double* Z2 = a.z2 + c.S.vec + c.col;
+    double* Z2 = (FINE ? pa->p[parity ^ 1] : a.z2) + c.S.vec + c.col;
     const double w = c.S.omega;
     for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
         const int64_t o = static_cast<int64_t>(row) * rp;
-        double acc = w * D[row] * B[o];
-        for (int s = pptr[row]; s < pptr[row + 1]; ++s) acc = fma(P[s], NE[static_cast<int64_t>(pidx[s]) * rp], acc);
-        Z2[o] = acc;
+        const double acc = gather_sum<4>(
+            pptr[row], pptr[row + 1], [&](int s) { return static_cast<int64_t>(pidx[s]); }, [&](int s) { return P[s]; },
+            [&](int64_t J) { return NE[J * rp]; }, 0);
+        Z2[o] = fma(w * D[row], B[o], acc);
     }
 }
 
@@ -247,8 +273,9 @@ __global__ __launch_bounds__(256) void amg_post_smooth(const LevelArgs* __restri
     double* E = a.e + c.S.vec + c.col;
     const double w = c.S.omega;
     for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-        double acc = 0.0;
-        for (int k = ptr[row]; k < ptr[row + 1]; ++k) acc = fma(val[k], Z2[static_cast<int64_t>(idx[k]) * rp], acc);
+        const double acc = gather_sum<8>(
+            ptr[row], ptr[row + 1], [&](int k) { return static_cast<int64_t>(idx[k]); }, [&](int k) { return val[k]; },
+            [&](int64_t j) { return Z2[j * rp]; }, row);
         const int64_t o = static_cast<int64_t>(row) * rp;
         E[o] = fma(w * D[row], B[o] - acc, Z2[o]);
     }
@@ -258,7 +285,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth(const LevelArgs* __restri
 // z.r (part1) and z.q_old (part0).
 template <int L>
 __global__ __launch_bounds__(256) void amg_post_smooth_fine(const PcgArgs* __restrict__ pa,
-                                                            const LevelArgs* __restrict__ la) {
+                                                            const LevelArgs* __restrict__ la, int parity) {
     const PcgArgs& g = *pa;
     const LevelArgs& a = *la;
     const int tix = adt::xcd_tile(blockIdx.x, a.n_tiles);
@@ -272,14 +299,15 @@ __global__ __launch_bounds__(256) void amg_post_smooth_fine(const PcgArgs* __res
     const double* val = a.val + S.val;
     const double* D = a.dinv + S.dinv;
     const double* R = a.b + S.vec + col;
-    const double* Z2 = a.z2 + S.vec + col;
+    const double* Z2 = g.p[parity ^ 1] + S.vec + col;
     const double* Q = g.q + S.vec + col;
     double* Z = a.e + S.vec + col;
     const double w = S.omega;
     double zr = 0.0, zq = 0.0;
     for (int row = t.row0 + static_cast<int>(threadIdx.x / L); row < t.row1; row += kRowsPerPass(L)) {
-        double acc = 0.0;
-        for (int k = ptr[row]; k < ptr[row + 1]; ++k) acc = fma(val[k], Z2[static_cast<int64_t>(idx[k]) * rp], acc);
+        const double acc = gather_sum<8>(
+            ptr[row], ptr[row + 1], [&](int k) { return static_cast<int64_t>(idx[k]); }, [&](int k) { return val[k]; },
+            [&](int64_t j) { return Z2[j * rp]; }, row);
         const int64_t o = static_cast<int64_t>(row) * rp;
         const double ri = R[o];
         const double zi = fma(w * D[row], ri - acc, Z2[o]);
@@ -382,11 +410,12 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     }
 }
 
-// z = z2 + w D^-1 (r - A z2) with z2 = w D^-1 r + P e_1 staged for the tile and its halo;
-// partials z.q_old (part0) and z.r (part1) for the flexible-CG coefficients.
+// z = z2 + w D^-1 (r - A z2) with z2 (written by the finest prolongation into p[parity ^ 1])
+// staged for the tile and its halo; partials z.q_old (part0) and z.r (part1) for the
+// flexible-CG coefficients.
 template <int L>
 __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __restrict__ pa,
-                                                           const LevelArgs* __restrict__ la) {
+                                                           const LevelArgs* __restrict__ la, int parity) {
     extern __shared__ double smem[];
     const PcgArgs& g = *pa;
     const LevelArgs& a = *la;
@@ -394,19 +423,15 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     const Tile t = g.tiles[tix];
     if (!g.grp_active[t.grp]) return;
     const Sys S = g.sys[t.sys];
-    const LvSys LS = a.ls[t.sys];
-    const double w = LS.omega;
+    const double w = a.ls[t.sys].omega;
     const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
     const int col = (t.grp % g.groups) * L + lane;
     const int rp = g.rp;
     const double* R = g.r + S.vec_off + col;
     const double* Q = g.q + S.vec_off + col;
+    const double* Z2 = g.p[parity ^ 1] + S.vec_off + col;
     double* Z = g.z + S.vec_off + col;
     const double* D = g.dinv + S.diag_off;
-    const int* pptr = a.pptr + LS.pptr;
-    const int* pidx = a.pidx + LS.pidx;
-    const double* P = a.pval + LS.pval;
-    const double* NE = a.next_e + LS.next_vec + col;
     const LdsTile s = stage_csr(g, t, S, L, smem);
     constexpr int U = 4;
     for (int s0 = rsub; s0 < s.staged; s0 += U * kRowsPerPass(L)) {
@@ -418,12 +443,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
             jj[u] = q < s.own ? t.row0 + q : (q < s.staged ? s.shalo[q - s.own] : t.row0);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = w * D[jj[u]] * R[jj[u] * rp];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int j = static_cast<int>(jj[u]);
-            for (int q = pptr[j]; q < pptr[j + 1]; ++q) v[u] = fma(P[q], NE[static_cast<int64_t>(pidx[q]) * rp], v[u]);
-        }
+        for (int u = 0; u < U; ++u) v[u] = Z2[jj[u] * rp];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int q = s0 + u * kRowsPerPass(L);

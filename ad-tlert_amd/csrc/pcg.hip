@@ -654,7 +654,7 @@ int64_t fine_lds_bytes(const adt_pcg_plan* p) {
 
 // One V-cycle: z = B r (flexible-CG partials z.r, z.q_old written by the finest post-smoothing).
 template <int L>
-void launch_cycle(const adt_pcg_plan* p, hipStream_t st) {
+void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
     const int m = static_cast<int>(p->amg.size());
     const PcgArgs* pa = p->args_d;
     const auto& f = p->amg[0];
@@ -670,15 +670,15 @@ void launch_cycle(const adt_pcg_plan* p, hipStream_t st) {
     }
     hipLaunchKernelGGL(amg_coarse_solve<L>, dim3(p->n_groups), dim3(256), 0, st, p->amg[m - 1].args_d);
     for (int l = m - 2; l >= 1; --l) {
-        hipLaunchKernelGGL(amg_prolong<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+        hipLaunchKernelGGL((amg_prolong<L, false>), dim3(p->amg[l].n_tiles), dim3(256), 0, st, pa, p->amg[l].args_d,
+                           parity);
         hipLaunchKernelGGL(amg_post_smooth<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
     }
-    if (p->lds_rows > 0) {
-        hipLaunchKernelGGL(amg_post_smooth_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d);
-    } else {
-        hipLaunchKernelGGL(amg_prolong<L>, dim3(p->n_tiles), dim3(256), 0, st, f.args_d);
-        hipLaunchKernelGGL(amg_post_smooth_fine<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d);
-    }
+    hipLaunchKernelGGL((amg_prolong<L, true>), dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d, parity);
+    if (p->lds_rows > 0)
+        hipLaunchKernelGGL(amg_post_smooth_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d, parity);
+    else
+        hipLaunchKernelGGL(amg_post_smooth_fine<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d, parity);
 }
 
 template <int L>
@@ -686,7 +686,7 @@ void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mod
     const PcgArgs* pa = p->args_d;
     if (mode == 1) {
         if (events) (void)hipEventRecord(events[4], st);
-        launch_cycle<L>(p, st);
+        launch_cycle<L>(p, st, parity);
         if (events) (void)hipEventRecord(events[5], st);
         hipLaunchKernelGGL(fcg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
     }
@@ -1283,7 +1283,6 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         if (l == 0) {
             A.n_tiles = p->n_tiles;
             A.tiles_d = p->tiles_d;
-            if (p->lds_rows == 0 && (rc = zeros(p, p->vec_len, &z2_d[0]))) return rc;
         } else {
             A.n_tiles = static_cast<int>(A.tiles_h.size());
             if ((rc = upload(p, A.tiles_h, &A.tiles_d)) || (rc = zeros(p, val_len[l], &val_d[l])) ||
