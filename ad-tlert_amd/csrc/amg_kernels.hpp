@@ -24,30 +24,38 @@ namespace krylov {
 // numeric setup
 // ------------------------------------------------------------------------------------------
 
-// Thread per row i of system blockIdx.y: inverse diagonal, then the prolongator row
-//   P[i, J] = [J == agg(i)] - D_ii^-1 sum_{k in row i, agg(col k) = J} omega_k A_k.
+// Thread per row i of system blockIdx.y: inverse diagonal (for the smoother), then the
+// prolongator row from the filtered matrix (strong entries, weak ones lumped into D_F):
+//   P[i, J] = [J == agg(i)] (1 - omega_ii) - D_F,ii^-1 sum_{strong k in row i, agg(col k) = J} omega_k A_k
+// with omega_k the level's constant damping or, when given, the learned per-slot damping.
 __global__ __launch_bounds__(256) void amg_pvals(const LevelArgs* __restrict__ la) {
     const LevelArgs& a = *la;
     const LvSys S = a.ls[blockIdx.y];
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= S.rows) return;
     const double* V = a.val + S.val;
-    const double dinv = 1.0 / V[a.diag[S.diag + i]];
-    if (a.dinv_out) a.dinv_out[S.dinv + i] = dinv;
+    const int dslot = a.diag[S.diag + i];
+    if (a.dinv_out) a.dinv_out[S.dinv + i] = 1.0 / V[dslot];
     if (S.nc == 0) return;
+    const int* dmap_ptr = a.dmap_ptr + S.dmap_ptr;
+    const int* dmap = a.dmap + S.dmap;
+    double df = 0.0;
+    for (int m = dmap_ptr[i]; m < dmap_ptr[i + 1]; ++m) df += V[dmap[m]];
+    const double dfinv = 1.0 / (df > 0.0 ? df : V[dslot]);
     const int* pptr = a.pptr + S.pptr;
     const int* pmap_ptr = a.pmap_ptr + S.pmap_ptr;
     const int* pmap = a.pmap + S.pmap;
     const int* pself = a.pself + S.pidx;
     const double* om = a.omega_slot ? a.omega_slot + S.val : nullptr;
+    const double wself = om ? om[dslot] : S.omega_p;
     double* P = a.pval + S.pval;
     for (int t = pptr[i]; t < pptr[i + 1]; ++t) {
         double acc = 0.0;
         for (int m = pmap_ptr[t]; m < pmap_ptr[t + 1]; ++m) {
             const int k = pmap[m];
-            acc = fma(om ? om[k] : S.omega, V[k], acc);
+            acc = fma(om ? om[k] : S.omega_p, V[k], acc);
         }
-        P[t] = (pself[t] ? 1.0 : 0.0) - dinv * acc;
+        P[t] = (pself[t] ? 1.0 - wself : 0.0) - dfinv * acc;
     }
 }
 

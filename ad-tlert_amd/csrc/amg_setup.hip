@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 
 #include "adt_common.hpp"
 
@@ -10,68 +11,76 @@ namespace adt {
 namespace amg {
 namespace {
 
-// Greedy aggregation over strong connections, rows visited in index order (the device
-// numbering follows a space-filling curve, so aggregates are spatially compact):
-//   1. a row whose strong neighbours are all free starts an aggregate with them;
-//   2. a free row joins the aggregate of its strongest already-aggregated strong neighbour;
-//   3. a row still free starts an aggregate with its free strong neighbours.
-// Rows without any strong off-diagonal connection (e.g. Dirichlet rows) stay out (-1):
-// the smoother treats them exactly.
-int aggregate(const HostLevel& L, double theta, std::vector<int>& agg) {
-    const int n = L.n;
-    std::vector<int> sptr(n + 1, 0), sidx;
-    std::vector<double> sval;
-    sidx.reserve(L.idx.size());
-    sval.reserve(L.idx.size());
-    for (int i = 0; i < n; ++i) {
+// strong[k] for every slot k of A: off-diagonal with |a_ij| >= theta sqrt(|a_ii a_jj|), a_ij != 0.
+std::vector<char> strength(const HostLevel& L, double theta) {
+    std::vector<char> strong(L.idx.size(), 0);
+    for (int i = 0; i < L.n; ++i) {
         const double aii = std::fabs(L.rep[L.diag[i]]);
         for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) {
             const int j = L.idx[k];
             if (j == i) continue;
-            const double ajj = std::fabs(L.rep[L.diag[j]]);
             const double a = std::fabs(L.rep[k]);
-            if (a > 0.0 && a >= theta * std::sqrt(aii * ajj)) {
-                sidx.push_back(j);
-                sval.push_back(a / std::sqrt(aii * ajj));
-            }
+            strong[k] = a > 0.0 && a >= theta * std::sqrt(aii * std::fabs(L.rep[L.diag[j]]));
         }
-        sptr[i + 1] = static_cast<int>(sidx.size());
     }
+    return strong;
+}
+
+// Greedy aggregation over strong connections, rows visited in index order (the device
+// numbering follows a space-filling curve, so aggregates are spatially compact):
+//   1. a row whose strong neighbours are all free starts an aggregate with them;
+//   2. a free row joins the aggregate of its strongest pass-1 strong neighbour;
+//   3. a row still free starts an aggregate with its free strong neighbours.
+// Rows without any strong off-diagonal connection (e.g. Dirichlet rows) stay out (-1):
+// the smoother treats them exactly.
+int aggregate(const HostLevel& L, const std::vector<char>& strong, std::vector<int>& agg) {
+    const int n = L.n;
+    auto has_strong = [&](int i) {
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+            if (strong[k]) return true;
+        return false;
+    };
+    auto weight = [&](int i, int k) {
+        return std::fabs(L.rep[k]) / std::sqrt(std::fabs(L.rep[L.diag[i]] * L.rep[L.diag[L.idx[k]]]));
+    };
     agg.assign(n, -1);
     int count = 0;
     for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0 || sptr[i] == sptr[i + 1]) continue;
+        if (agg[i] >= 0 || !has_strong(i)) continue;
         bool free = true;
-        for (int k = sptr[i]; k < sptr[i + 1] && free; ++k) free = agg[sidx[k]] < 0;
+        for (int k = L.ptr[i]; k < L.ptr[i + 1] && free; ++k)
+            if (strong[k]) free = agg[L.idx[k]] < 0;
         if (!free) continue;
         agg[i] = count;
-        for (int k = sptr[i]; k < sptr[i + 1]; ++k) agg[sidx[k]] = count;
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+            if (strong[k]) agg[L.idx[k]] = count;
         ++count;
     }
-    std::vector<int> first = agg;  // pass-1 aggregates only
+    const std::vector<int> first = agg;
     for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0 || sptr[i] == sptr[i + 1]) continue;
+        if (agg[i] >= 0 || !has_strong(i)) continue;
         double best = -1.0;
         int pick = -1;
-        for (int k = sptr[i]; k < sptr[i + 1]; ++k)
-            if (first[sidx[k]] >= 0 && sval[k] > best) {
-                best = sval[k];
-                pick = first[sidx[k]];
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+            if (strong[k] && first[L.idx[k]] >= 0 && weight(i, k) > best) {
+                best = weight(i, k);
+                pick = first[L.idx[k]];
             }
         agg[i] = pick;
     }
     for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0 || sptr[i] == sptr[i + 1]) continue;
+        if (agg[i] >= 0 || !has_strong(i)) continue;
         agg[i] = count;
-        for (int k = sptr[i]; k < sptr[i + 1]; ++k)
-            if (agg[sidx[k]] < 0) agg[sidx[k]] = count;
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+            if (strong[k] && agg[L.idx[k]] < 0) agg[L.idx[k]] = count;
         ++count;
     }
     return count;
 }
 
-// lambda_max(D^-1 A) by power iteration (D^-1 A is similar to the SPD D^-1/2 A D^-1/2).
-double lambda_max(const HostLevel& L) {
+// lambda_max(D^-1 B) by power iteration for a matrix B on A's pattern with values bv (D^-1 B is
+// similar to the SPD D^-1/2 B D^-1/2; Rayleigh quotient in the D inner product).
+double lambda_max(const HostLevel& L, const std::vector<double>& bv, const std::vector<double>& d) {
     const int n = L.n;
     std::vector<double> v(n), w(n);
     uint64_t s = 0x9E3779B97F4A7C15ull;
@@ -84,14 +93,12 @@ double lambda_max(const HostLevel& L) {
         double nw = 0.0, vw = 0.0, vv = 0.0;
         for (int i = 0; i < n; ++i) {
             double acc = 0.0;
-            for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) acc += L.rep[k] * v[L.idx[k]];
-            w[i] = acc / L.rep[L.diag[i]];
+            for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) acc += bv[k] * v[L.idx[k]];
+            w[i] = acc / d[i];
         }
-        // Rayleigh quotient in the D inner product
         for (int i = 0; i < n; ++i) {
-            const double d = L.rep[L.diag[i]];
-            vw += v[i] * d * w[i];
-            vv += v[i] * d * v[i];
+            vw += v[i] * d[i] * w[i];
+            vv += v[i] * d[i] * v[i];
             nw += w[i] * w[i];
         }
         lam = vv > 0.0 ? vw / vv : 1.0;
@@ -101,28 +108,47 @@ double lambda_max(const HostLevel& L) {
     return lam > 0.0 && std::isfinite(lam) ? lam : 2.0;
 }
 
-// Prolongator pattern, its gather map and the transpose.
-void prolongator(HostLevel& L) {
+// Dampings, the filtered diagonal map, the prolongator pattern with its gather map, and R = P^T.
+void prolongator(HostLevel& L, const std::vector<char>& strong) {
     const int n = L.n, nc = L.nc;
+    std::vector<double> d(n);
+    for (int i = 0; i < n; ++i) d[i] = L.rep[L.diag[i]];
+    L.omega = 4.0 / (3.0 * lambda_max(L, L.rep, d));
+    // filtered matrix: strong off-diagonals, weak ones lumped into the diagonal
+    L.dmap_ptr.assign(n + 1, 0);
+    L.dmap.clear();
+    std::vector<double> fv(L.rep.size(), 0.0), fd(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        L.dmap.push_back(L.diag[i]);
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+            if (k != L.diag[i] && !strong[k]) L.dmap.push_back(k);
+        L.dmap_ptr[i + 1] = static_cast<int>(L.dmap.size());
+        double di = 0.0;
+        for (int m = L.dmap_ptr[i]; m < L.dmap_ptr[i + 1]; ++m) di += L.rep[L.dmap[m]];
+        fd[i] = di > 0.0 ? di : d[i];
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) fv[k] = strong[k] ? L.rep[k] : 0.0;
+        fv[L.diag[i]] = fd[i];
+    }
+    L.omega_p = 4.0 / (3.0 * lambda_max(L, fv, fd));
     L.pptr.assign(n + 1, 0);
     L.pidx.clear();
     L.pmap_ptr.assign(1, 0);
     L.pmap.clear();
     L.pself.clear();
-    std::vector<std::pair<int, int>> cols;  // (aggregate, A slot)
+    std::vector<std::pair<int, int>> cols;  // (aggregate, strong A slot or -1 for the row itself)
     for (int i = 0; i < n; ++i) {
         cols.clear();
-        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) {
-            const int J = L.agg[L.idx[k]];
-            if (J >= 0) cols.emplace_back(J, k);
-        }
+        if (L.agg[i] >= 0) cols.emplace_back(L.agg[i], -1);
+        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+            if (strong[k] && L.agg[L.idx[k]] >= 0) cols.emplace_back(L.agg[L.idx[k]], k);
         std::stable_sort(cols.begin(), cols.end(),
                          [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first < b.first; });
         for (size_t c = 0; c < cols.size();) {
             const int J = cols[c].first;
             L.pidx.push_back(J);
             L.pself.push_back(J == L.agg[i] ? 1 : 0);
-            for (; c < cols.size() && cols[c].first == J; ++c) L.pmap.push_back(cols[c].second);
+            for (; c < cols.size() && cols[c].first == J; ++c)
+                if (cols[c].second >= 0) L.pmap.push_back(cols[c].second);
             L.pmap_ptr.push_back(static_cast<int>(L.pmap.size()));
         }
         L.pptr[i + 1] = static_cast<int>(L.pidx.size());
@@ -142,83 +168,86 @@ void prolongator(HostLevel& L) {
         }
 }
 
-// T = A P and A_next = R T: patterns and product pairs (fixed order: A slots, then P slots).
+// Sparse product pattern with pair lists: for every output row, the (x, y) contributions
+// produced by `visit(row, emit)` (emit(col, x, y)) are grouped by output column (ascending),
+// keeping the visiting order inside each column.
+struct PairBuilder {
+    std::vector<int> mark, loc, cols, count, start, order, tx, ty, tl;
+    explicit PairBuilder(int ncols) : mark(ncols, -1), loc(ncols, 0) {}
+    template <typename Visit>
+    void build(int nrows, Visit visit, std::vector<int>& optr, std::vector<int>& oidx, std::vector<int>& pptr,
+               std::vector<int>& px, std::vector<int>& py) {
+        optr.assign(nrows + 1, 0);
+        oidx.clear();
+        pptr.assign(1, 0);
+        px.clear();
+        py.clear();
+        for (int r = 0; r < nrows; ++r) {
+            cols.clear();
+            count.clear();
+            tx.clear();
+            ty.clear();
+            tl.clear();
+            visit(r, [&](int c, int x, int y) {
+                if (mark[c] != r) {
+                    mark[c] = r;
+                    loc[c] = static_cast<int>(cols.size());
+                    cols.push_back(c);
+                    count.push_back(0);
+                }
+                count[loc[c]]++;
+                tx.push_back(x);
+                ty.push_back(y);
+                tl.push_back(loc[c]);
+            });
+            order.resize(cols.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::sort(order.begin(), order.end(), [&](int a, int b) { return cols[a] < cols[b]; });
+            // start offset of each local column in sorted order
+            start.assign(cols.size(), 0);
+            int base = static_cast<int>(px.size());
+            for (int o : order) {
+                oidx.push_back(cols[o]);
+                start[o] = base;
+                base += count[o];
+                pptr.push_back(base);
+            }
+            px.resize(base);
+            py.resize(base);
+            for (size_t e = 0; e < tx.size(); ++e) {
+                const int at = start[tl[e]]++;
+                px[at] = tx[e];
+                py[at] = ty[e];
+            }
+            optr[r + 1] = static_cast<int>(oidx.size());
+        }
+        std::fill(mark.begin(), mark.end(), -1);
+    }
+};
+
+// T = A P and A_next = R T (patterns, pair lists) and the representative next-level values.
 void galerkin(HostLevel& L, HostLevel& next) {
     const int n = L.n, nc = L.nc;
-    std::vector<int> mark(nc, -1), pos(nc, 0);
-    L.tptr.assign(n + 1, 0);
-    L.tidx.clear();
-    L.tpair_ptr.assign(1, 0);
-    L.tpair_a.clear();
-    L.tpair_p.clear();
-    std::vector<int> row_cols;
-    std::vector<std::vector<std::pair<int, int>>> row_pairs;
-    for (int i = 0; i < n; ++i) {
-        row_cols.clear();
-        row_pairs.clear();
-        for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) {
-            const int j = L.idx[k];
-            for (int s = L.pptr[j]; s < L.pptr[j + 1]; ++s) {
-                const int K = L.pidx[s];
-                if (mark[K] != i) {
-                    mark[K] = i;
-                    pos[K] = static_cast<int>(row_cols.size());
-                    row_cols.push_back(K);
-                    row_pairs.emplace_back();
-                }
-                row_pairs[pos[K]].emplace_back(k, s);
+    PairBuilder pb(nc);
+    pb.build(
+        n,
+        [&](int i, auto emit) {
+            for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) {
+                const int j = L.idx[k];
+                for (int s = L.pptr[j]; s < L.pptr[j + 1]; ++s) emit(L.pidx[s], k, s);
             }
-        }
-        std::vector<int> order(row_cols.size());
-        for (size_t c = 0; c < order.size(); ++c) order[c] = static_cast<int>(c);
-        std::sort(order.begin(), order.end(), [&](int a, int b) { return row_cols[a] < row_cols[b]; });
-        for (int c : order) {
-            L.tidx.push_back(row_cols[c]);
-            for (const auto& pr : row_pairs[c]) {
-                L.tpair_a.push_back(pr.first);
-                L.tpair_p.push_back(pr.second);
-            }
-            L.tpair_ptr.push_back(static_cast<int>(L.tpair_a.size()));
-        }
-        L.tptr[i + 1] = static_cast<int>(L.tidx.size());
-    }
-    // next level: row J of R T
+        },
+        L.tptr, L.tidx, L.tpair_ptr, L.tpair_a, L.tpair_p);
     next.n = nc;
-    next.ptr.assign(nc + 1, 0);
-    next.idx.clear();
-    L.cpair_ptr.assign(1, 0);
-    L.cpair_p.clear();
-    L.cpair_t.clear();
-    std::fill(mark.begin(), mark.end(), -1);
-    for (int J = 0; J < nc; ++J) {
-        row_cols.clear();
-        row_pairs.clear();
-        for (int e = L.rptr[J]; e < L.rptr[J + 1]; ++e) {
-            const int i = L.rrow[e], s = L.rslot[e];
-            for (int t = L.tptr[i]; t < L.tptr[i + 1]; ++t) {
-                const int K = L.tidx[t];
-                if (mark[K] != J) {
-                    mark[K] = J;
-                    pos[K] = static_cast<int>(row_cols.size());
-                    row_cols.push_back(K);
-                    row_pairs.emplace_back();
-                }
-                row_pairs[pos[K]].emplace_back(s, t);
+    pb.build(
+        nc,
+        [&](int J, auto emit) {
+            for (int e = L.rptr[J]; e < L.rptr[J + 1]; ++e) {
+                const int i = L.rrow[e], s = L.rslot[e];
+                for (int t = L.tptr[i]; t < L.tptr[i + 1]; ++t) emit(L.tidx[t], s, t);
             }
-        }
-        std::vector<int> order(row_cols.size());
-        for (size_t c = 0; c < order.size(); ++c) order[c] = static_cast<int>(c);
-        std::sort(order.begin(), order.end(), [&](int a, int b) { return row_cols[a] < row_cols[b]; });
-        for (int c : order) {
-            next.idx.push_back(row_cols[c]);
-            for (const auto& pr : row_pairs[c]) {
-                L.cpair_p.push_back(pr.first);
-                L.cpair_t.push_back(pr.second);
-            }
-            L.cpair_ptr.push_back(static_cast<int>(L.cpair_p.size()));
-        }
-        next.ptr[J + 1] = static_cast<int>(next.idx.size());
-    }
+        },
+        next.ptr, next.idx, L.cpair_ptr, L.cpair_p, L.cpair_t);
     next.diag.assign(nc, -1);
     for (int J = 0; J < nc; ++J)
         for (int k = next.ptr[J]; k < next.ptr[J + 1]; ++k)
@@ -226,11 +255,13 @@ void galerkin(HostLevel& L, HostLevel& next) {
     // representative numeric values (same formulas and orders as the device kernels)
     std::vector<double> pv(L.pidx.size()), tv(L.tidx.size());
     for (int i = 0; i < n; ++i) {
-        const double dinv = 1.0 / L.rep[L.diag[i]];
+        double df = 0.0;
+        for (int m = L.dmap_ptr[i]; m < L.dmap_ptr[i + 1]; ++m) df += L.rep[L.dmap[m]];
+        const double dinv = 1.0 / (df > 0.0 ? df : L.rep[L.diag[i]]);
         for (int s = L.pptr[i]; s < L.pptr[i + 1]; ++s) {
             double acc = 0.0;
-            for (int m = L.pmap_ptr[s]; m < L.pmap_ptr[s + 1]; ++m) acc += L.rep[L.pmap[m]];
-            pv[s] = (L.pself[s] ? 1.0 : 0.0) - L.omega * dinv * acc;
+            for (int m = L.pmap_ptr[s]; m < L.pmap_ptr[s + 1]; ++m) acc += L.omega_p * L.rep[L.pmap[m]];
+            pv[s] = (L.pself[s] ? 1.0 - L.omega_p : 0.0) - dinv * acc;
         }
     }
     for (size_t t = 0; t < tv.size(); ++t) {
@@ -266,13 +297,13 @@ std::vector<HostLevel> build_hierarchy(int n, const int32_t* ptr, const int32_t*
         if (L0.diag[i] < 0 || !(L0.rep[L0.diag[i]] > 0.0)) return {};
     while (static_cast<int>(levels.size()) < opt.max_levels && levels.back().n > opt.coarse_max) {
         HostLevel& L = levels.back();
+        const std::vector<char> strong = strength(L, opt.theta);
         std::vector<int> agg;
-        const int nc = aggregate(L, opt.theta, agg);
+        const int nc = aggregate(L, strong, agg);
         if (nc == 0 || nc > L.n * 9 / 10) break;  // coarsening stalled: this level is the coarsest
         L.agg = std::move(agg);
         L.nc = nc;
-        L.omega = 4.0 / (3.0 * lambda_max(L));
-        prolongator(L);
+        prolongator(L, strong);
         HostLevel next;
         galerkin(L, next);
         bool ok = true;

@@ -3,13 +3,16 @@
 // One hierarchy is built per CSR pattern from a representative matrix of that pattern
 // (strength of connection and the damping weights need values).  Each level l holds the
 // pattern of A_l, its aggregation, the pattern of the smoothed prolongator
-//     P_l = (I - omega_l D_l^-1 A_l) P0_l      (P0_l: piecewise-constant on aggregates)
-// its transpose (restriction), and the gather lists with which the device recomputes, for
-// every system and every solve, the numeric P_l, T_l = A_l P_l and A_{l+1} = P_l^T T_l in
-// a fixed order (so the Galerkin operators are bitwise repeatable).
+//     P_l = (I - omega_l D_F^-1 A_F) P0_l
+// (P0_l piecewise constant on aggregates; A_F the filtered matrix: strong off-diagonal
+// entries only, the weak ones lumped into the diagonal D_F), its transpose (restriction), and
+// the gather lists with which the device recomputes, for every system and every solve, the
+// numeric P_l, T_l = A_l P_l and A_{l+1} = P_l^T T_l in a fixed order (so the Galerkin
+// operators are bitwise repeatable).  Filtering keeps the coarse operators sparse in 3D.
 //
 // Method: P. Vanek, J. Mandel, M. Brezina, "Algebraic multigrid by smoothed aggregation
-// for second and fourth order elliptic problems", Computing 56 (1996) 179-196.
+// for second and fourth order elliptic problems", Computing 56 (1996) 179-196 (aggregation,
+// prolongator smoothing with the filtered matrix).
 #pragma once
 
 #include <cstdint>
@@ -23,11 +26,13 @@ struct HostLevel {
     int nc = 0;  // rows of A_{l+1} (aggregates); 0 on the coarsest level
     std::vector<int> ptr, idx, diag;  // pattern of A_l and the slot of each diagonal entry
     std::vector<double> rep;          // representative values of A_l
-    double omega = 2.0 / 3.0;         // smoother and prolongation damping 4 / (3 lambda_max(D^-1 A))
+    double omega = 2.0 / 3.0;         // smoother damping 4 / (3 lambda_max(D^-1 A))
+    double omega_p = 2.0 / 3.0;       // prolongation damping 4 / (3 lambda_max(D_F^-1 A_F))
     std::vector<int> agg;             // aggregate of each row, -1: left out (no strong connection)
+    std::vector<int> dmap_ptr, dmap;  // per row: slots summed into D_F (diagonal + weak entries)
     // prolongator P (n x nc)
     std::vector<int> pptr, pidx;
-    std::vector<int> pmap_ptr, pmap;  // per P slot: the A_l slots of row i whose column lies in aggregate J
+    std::vector<int> pmap_ptr, pmap;  // per P slot: strong off-diagonal A slots of row i in aggregate J
     std::vector<int> pself;           // per P slot: 1 if J is the row's own aggregate
     // restriction R = P^T (nc x n): per coarse row, (fine row, P slot) entries in fine-row order
     std::vector<int> rptr, rrow, rslot;

@@ -69,8 +69,10 @@ struct LvSys {
     int rows, nc;           // rows on this level, rows on the next (0 on the coarsest)
     int n_p, n_t, n_next;   // slots of P, of T = A P and of the next level's matrix
     int pad;
-    double omega;           // smoother and prolongation damping
+    double omega;           // smoother damping
+    double omega_p;         // prolongation damping (constant; per slot when learned)
     int64_t ptr, idx, diag, val;   // matrix pattern, diagonal slots, values
+    int64_t dmap_ptr, dmap;        // filtered diagonal: summed slots per row
     int64_t vec, dinv;             // (rows, Rp) vector block, inverse diagonal
     int64_t pptr, pidx, pmap_ptr, pmap, pval;  // prolongator pattern / gather map / values
     int64_t rptr, rent;            // restriction rows (nc + 1) and (fine row, P slot) entries
@@ -92,6 +94,8 @@ struct LevelArgs {
     const double* val;
     const int* diag;
     const double* dinv;
+    const int* dmap_ptr;
+    const int* dmap;
     // prolongator, restriction, Galerkin product lists
     const int* pptr;
     const int* pidx;

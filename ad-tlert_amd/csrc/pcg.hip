@@ -1136,11 +1136,11 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
     std::vector<std::vector<int64_t>> off_ptr(depth, std::vector<int64_t>(n_pat)), off_idx = off_ptr,
         off_diag = off_ptr, off_pptr = off_ptr, off_pidx = off_ptr, off_pmp = off_ptr, off_pmap = off_ptr,
         off_rptr = off_ptr, off_rent = off_ptr, off_tpp = off_ptr, off_tp = off_ptr, off_cpp = off_ptr,
-        off_cp = off_ptr;
+        off_cp = off_ptr, off_dmp = off_ptr, off_dmap = off_ptr;
     for (int l = 0; l < depth; ++l) {
         AmgLevel& A = p->amg[l];
         std::vector<int> ptr, idx, diag, pptr, pidx, pmap_ptr, pmap, pself, rptr, rrow, rslot, tpp, ta, tq, cpp, cp,
-            ct;
+            ct, dmp, dmap;
         for (int q = 0; q < n_pat; ++q) {
             const HostLevel& h = H[q][l];
             off_ptr[l][q] = static_cast<int64_t>(ptr.size());
@@ -1156,6 +1156,10 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
             off_tp[l][q] = static_cast<int64_t>(ta.size());
             off_cpp[l][q] = static_cast<int64_t>(cpp.size());
             off_cp[l][q] = static_cast<int64_t>(cp.size());
+            off_dmp[l][q] = static_cast<int64_t>(dmp.size());
+            off_dmap[l][q] = static_cast<int64_t>(dmap.size());
+            append(dmp, h.dmap_ptr);
+            append(dmap, h.dmap);
             if (l > 0) {
                 append(ptr, h.ptr);
                 append(idx, h.idx);
@@ -1204,6 +1208,8 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         ADT_UP(cpp, cpair_ptr)
         ADT_UP(cp, cpair_p)
         ADT_UP(ct, cpair_t)
+        ADT_UP(dmp, dmap_ptr)
+        ADT_UP(dmap, dmap)
 #undef ADT_UP
     }
     // per-system records, tiles and numeric buffers
@@ -1222,7 +1228,10 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
             S.n_t = static_cast<int>(h.tidx.size());
             S.n_next = l + 1 < depth ? static_cast<int>(H[q][l + 1].idx.size()) : 0;
             S.omega = h.omega;
+            S.omega_p = h.omega_p;
             S.diag = off_diag[l][q];
+            S.dmap_ptr = off_dmp[l][q];
+            S.dmap = off_dmap[l][q];
             S.pptr = off_pptr[l][q];
             S.pidx = off_pidx[l][q];
             S.pmap_ptr = off_pmp[l][q];
@@ -1376,7 +1385,7 @@ extern "C" int adt_pcg_amg_level(const adt_pcg_plan* p, int sys, int level, int*
     const LvSys& S = p->amg[level].ls_h[sys];
     if (h_rows) *h_rows = S.rows;
     if (h_nnz) *h_nnz = level == 0 ? -1 : static_cast<int64_t>(p->amg[level - 1].ls_h[sys].n_next);  // -1: caller's
-    if (h_omega) *h_omega = S.omega;
+    if (h_omega) *h_omega = S.omega_p;
     return ADT_OK;
 }
 

@@ -80,6 +80,7 @@ class BatchedSolver:
     amg_stall_limit: int = 64
     _plans: dict = field(default_factory=dict)
     last: SolveStats | None = None
+    last_plan: native.PcgPlan | None = None
 
     def __post_init__(self):
         if self.preconditioner not in PRECONDITIONERS:
@@ -142,6 +143,7 @@ class BatchedSolver:
                                        check_every=self.check_every)
         fallback = plan.stats()["fallback_rhs"] if precond != native.PRECOND_JACOBI else 0
         self.last = SolveStats(iters, relres, time.perf_counter() - t0, used, fallback)
+        self.last_plan = plan
         if raise_on_failure and rc == native.ADT_ERR_NONFINITE:
             raise FloatingPointError("conjugate gradients produced non-finite values")
         if raise_on_failure and rc == native.ADT_ERR_NOT_CONVERGED:

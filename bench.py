@@ -1,15 +1,22 @@
 #!/usr/bin/env python
 """Throughput of the MI355X ERT forward-and-sensitivity path on one GPU.
 
-Workload (default): the batched 2.5D time-lapse configuration -- a ~20k-node graded P1
-triangle mesh, 96 surface electrodes at 1 m, ~3000 Wenner-Schlumberger quadrupoles, the
-reference wavenumber rule (19 wavenumbers for this survey), and a seeded 365-step
-log-conductivity series.  One timed step solves every right-hand side (wavenumber x source
-electrode) of ``--batch-steps`` consecutive time steps to a relative residual of 1e-10 and
-reduces them to apparent resistivities.  The main solver (``--preconditioner``, default
-multigrid-preconditioned flexible CG) and plain Jacobi PCG are timed on the same batches.
-After the timed steps, a second timed phase materializes the full normal-sensitivity
-Jacobian (all quadrupole rows x all cells) of one time step with the fused kernel.
+Workloads (``--workload``):
+
+* ``timelapse`` (default): a ~20k-node graded P1 triangle mesh, 96 surface electrodes at 1 m,
+  ~3000 Wenner-Schlumberger quadrupoles, the reference wavenumber rule (19 wavenumbers for
+  this survey) and a seeded 365-step log-conductivity series.  One timed step solves every
+  right-hand side (wavenumber x source electrode) of ``--batch-steps`` consecutive time steps.
+* ``box3d``: a 200 m x 200 m x 100 m box of ~179k nodes / ~1.0M P1 tetrahedra refined under
+  an 8 x 8 electrode grid at 5 m, 2000 random quadrupoles, mixed boundary, analytic half-space
+  singularity removal, log-normal conductivity fields.  One timed step solves the 64 source
+  right-hand sides of ``--batch-steps`` conductivity models.
+
+Every step solves to a relative residual of 1e-10 and reduces the fields to apparent
+resistivities.  The main solver (``--preconditioner``, default multigrid-preconditioned
+flexible CG) and plain Jacobi PCG are timed on the same batches.  After the timed steps a
+second timed phase materializes the full normal-sensitivity Jacobian (all quadrupole rows x
+all cells) of one model with the fused kernel.
 
 Prints ONE JSON line.  ``value`` = RHS solves/s of the main solver over the timed steps (wall
 clock between two synchronizations, graph-replayed iterations); ``solver.rhs_solves_per_s_pcg``
@@ -35,6 +42,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "RHS solves/s + Jacobian rows/s, learned-FCG vs plain PCG at rel-res 1e-10"
 
 
 def log(msg: str) -> None:
@@ -44,8 +52,6 @@ def log(msg: str) -> None:
 def cpu_baseline(case, series, wavenumbers, weights, sample_steps: int, rows: int):
     """FP64 CPU oracle (scipy splu direct solves, the restatement of the reference path) on a
     bounded sample: ``sample_steps`` time steps (all wavenumbers x sources) and ``rows`` Jacobian rows."""
-
-    import numpy as np
 
     from oracle import ert2p5d as oracle2p5d
 
@@ -63,12 +69,132 @@ def cpu_baseline(case, series, wavenumbers, weights, sample_steps: int, rows: in
     return rhs / solve_s, rows / jac_s, solve_s, jac_s
 
 
+class Timelapse:
+    """Batched 2.5D time-lapse workload."""
+
+    name = "2.5D time-lapse batch: ~20k-node P1 mesh, 96 electrodes, ~3000 Wenner-Schlumberger rows"
+    data = "synthetic (seeded graded P1 mesh, Wenner-Schlumberger survey, GRF time-lapse conductivity)"
+
+    def __init__(self, adt, torch, args, models: int):
+        self.args = args
+        self.torch = torch
+        self.case, self.series = adt.cases.timelapse_ws_20k(steps=max(models, 2), node_target=args.nodes)
+        mesh = adt.Mesh.from_arrays(self.case.nodes, self.case.cells)
+        survey = adt.Survey.from_arrays(self.case.electrodes, self.case.measurements)
+        self.op = adt.ERTForward2p5D.from_mesh_survey(mesh, survey, solver_tolerance=args.tol,
+                                                      preconditioner=args.preconditioner)
+        d = self.op._dev
+        self.W, self.E, self.N, self.nnz = d.W, d.E, d.N, d.nnz
+        self.D, self.C = survey.measurement_count, mesh.cell_count
+        self.rhs_per_step = args.batch_steps * self.W * self.E
+        self.sig = torch.as_tensor(self.series, dtype=torch.float64, device=adt.native.device())
+        self.config = {"nodes": self.N, "cells": self.C, "nnz": self.nnz, "electrodes": self.E,
+                       "quadrupoles": self.D, "wavenumbers": self.W, "wavenumber_rule": "reference (integration.py)",
+                       "time_steps_per_step": args.batch_steps, "rhs_per_step": self.rhs_per_step}
+
+    def step(self, i: int):
+        op, n = self.op, self.args.batch_steps
+        fields = op._solve_fields(self.sig[i * n:(i + 1) * n])
+        stats = op._solver.last
+        pots = op._potentials(self.torch.stack([f.blocks for f in fields]))
+        normal, reciprocal = op._normal_reciprocal(pots)
+        op._geometric_factors().to(self.sig.device, self.torch.float64) * op._combine(normal, reciprocal)
+        return stats
+
+    def jacobian(self):
+        op = self.op
+        fields = op._solve_fields(self.sig[:1])
+        columns = op._columns()
+
+        def run():
+            return op._normal_jacobian(fields[0], self.D, columns, return_device=True)
+
+        K = 3
+        bytes_ = 8 * self.D * self.C + 8 * self.W * self.E * self.N + 8 * 2 * self.C * K * K + 4 * self.C * K + 16 * self.D
+        flops = self.D * self.C * self.W * (4 * K * K + 6 * K)
+        return run, bytes_, flops, "sensitivity_kernel<3,double>"
+
+    def cpu(self):
+        args = self.args
+        if args.cpu_steps <= 0:
+            return None
+        op = self.op
+        wn, wt = op.wavenumbers.double().numpy(), op.weights.double().numpy()
+        rhs, rows, solve_s, jac_s = cpu_baseline(self.case, self.series, wn, wt, args.cpu_steps, args.cpu_rows)
+        return {"value": rhs, "unit": "RHS solves/s", "cores": 1, "kind": "port",
+                "sample": f"{args.cpu_steps} time steps x {self.W} wavenumbers x {self.E} sources, scipy splu FP64 "
+                          f"({solve_s:.1f}s); Jacobian {args.cpu_rows} rows x {self.C} cells einsum ({jac_s:.1f}s)",
+                "jacobian_rows_per_s": rows, "cpu": platform.processor() or platform.machine(),
+                "host_cores_available": len(os.sched_getaffinity(0))}
+
+
+class Box3D:
+    """3D P1 box workload: batches of conductivity models, 64 source right-hand sides each."""
+
+    name = "3D P1 box: ~179k nodes / ~1.0M tetrahedra, 8x8 electrodes at 5 m, 2000 random quadrupoles"
+    data = "synthetic (seeded graded tetrahedral box, random ABMN survey, log-normal GRF conductivity models)"
+
+    def __init__(self, adt, torch, args, models: int):
+        import numpy as np
+
+        self.args = args
+        self.torch = torch
+        self.case = adt.cases.box_p1_200k()
+        mesh = adt.Mesh3D.from_arrays(self.case.nodes, self.case.cells)
+        survey = adt.Survey.from_arrays(self.case.electrodes, self.case.measurements)
+        self.op = adt.ERTForward3D.from_mesh_survey(mesh, survey, element_order=1, boundary_mode="mixed",
+                                                    solver_tolerance=args.tol, series_batch_steps=args.batch_steps,
+                                                    preconditioner=args.preconditioner)
+        s = self.op._dev  # builds the device state (pattern, templates, primaries)
+        self.N, self.nnz, self.C = s.N, s.nnz, mesh.cell_count
+        self.D, self.E = survey.measurement_count, survey.electrode_count
+        self.rhs_per_step = args.batch_steps * self.op._dev.nc
+        # models: the seeded base field times two smooth log-perturbations with varying weights
+        rng = np.random.default_rng(44)
+        centers = self.case.nodes[self.case.cells].mean(axis=1)
+        g1 = np.log10(adt.cases.grf_3d(centers, rng, mean=0.0, std=1.0))
+        g2 = np.log10(adt.cases.grf_3d(centers, rng, mean=0.0, std=1.0))
+        t = np.arange(max(models, 2))[:, None]
+        logs = np.log10(self.case.conductivity)[None] + 0.2 * (np.sin(0.7 * t) * g1[None] + np.cos(0.3 * t) * g2[None])
+        self.sig = torch.as_tensor(10.0 ** logs, dtype=torch.float64, device=adt.native.device())
+        self.config = {"nodes": self.N, "cells": self.C, "nnz": self.nnz, "electrodes": self.E,
+                       "quadrupoles": self.D, "source_rhs_per_model": self.op._dev.nc,
+                       "models_per_step": args.batch_steps, "rhs_per_step": self.rhs_per_step,
+                       "boundary": "mixed", "singularity_removal": True}
+
+    def step(self, i: int):
+        op, n = self.op, self.args.batch_steps
+        fields = op._solve_fields(self.sig[i * n:(i + 1) * n])
+        stats = op._solver.last
+        current = self.torch.stack([f.current for f in fields])
+        op._geometric_factors().to(self.sig.device, self.torch.float64) * op._measurement_resistances(current)
+        return stats
+
+    def jacobian(self):
+        op = self.op
+        sigma = self.sig[0].cpu()
+        op.jacobian(sigma, return_device=True, batch_size=self.D)  # fields into the cache
+
+        def run():
+            return op.jacobian(sigma, return_device=True, batch_size=self.D)
+
+        K, S = 4, 2 * op._dev.nc
+        bytes_ = 8 * self.D * self.C + 8 * S * self.N + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
+        flops = self.D * self.C * (4 * K * K + 6 * K)
+        return run, bytes_, flops, "sensitivity_kernel<4,double>"
+
+    def cpu(self):
+        return None  # a 3D direct factorization of this size does not fit a bounded CPU sample
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch-steps", type=int, default=4, help="time steps per timed step")
+    ap.add_argument("--workload", choices=("timelapse", "box3d"), default="timelapse")
+    ap.add_argument("--batch-steps", type=int, default=None,
+                    help="time steps (timelapse, default 4) or models (box3d, default 2) per timed step")
     ap.add_argument("--nodes", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check-every", type=int, default=16)
@@ -84,6 +210,8 @@ def main() -> int:
               "(the path shards by wavenumber/time step with one tiny all-reduce; see the design notes)",
               file=sys.stderr)
         return 2
+    if args.batch_steps is None:
+        args.batch_steps = 4 if args.workload == "timelapse" else 2
 
     import numpy as np
     import torch
@@ -91,58 +219,47 @@ def main() -> int:
     adt = importlib.import_module("ad-tlert_amd")
     dev = adt.native.device()
     cus, arch = adt.native.device_info(0)
-    total_models = (max(args.warmup + args.steps, 1 + args.pcg_steps) + 1) * args.batch_steps
+    models = (max(args.warmup + args.steps, 1 + args.pcg_steps) + 1) * args.batch_steps
     t0 = time.perf_counter()
-    case, series = adt.cases.timelapse_ws_20k(steps=max(total_models, 2), node_target=args.nodes)
-    mesh = adt.Mesh.from_arrays(case.nodes, case.cells)
-    survey = adt.Survey.from_arrays(case.electrodes, case.measurements)
-    op = adt.ERTForward2p5D.from_mesh_survey(mesh, survey, solver_tolerance=args.tol)
+    wl = (Timelapse if args.workload == "timelapse" else Box3D)(adt, torch, args, models)
+    op = wl.op
     op._solver.check_every = args.check_every
     op._solver.rows_per_tile = args.rows_per_tile
-    d = op._dev
-    W, E, N, D = d.W, d.E, d.N, survey.measurement_count
-    C = mesh.cell_count
-    log(f"setup {time.perf_counter() - t0:.1f}s: N={N} C={C} nnz={d.nnz} W={W} E={E} D={D} device={arch} CUs={cus}")
-    sig = torch.as_tensor(series, dtype=torch.float64, device=dev)
-
-    def step(i: int):
-        fields = op._solve_fields(sig[i * args.batch_steps:(i + 1) * args.batch_steps])
-        stats = op._solver.last
-        pots = op._potentials(torch.stack([f.blocks for f in fields]))
-        normal, reciprocal = op._normal_reciprocal(pots)
-        rhoa = op._geometric_factors().to(dev, torch.float64) * op._combine(normal, reciprocal)
-        return fields, rhoa, stats
+    log(f"setup {time.perf_counter() - t0:.1f}s: {args.workload} N={wl.N} C={wl.C} nnz={wl.nnz} D={wl.D} "
+        f"rhs/step={wl.rhs_per_step} device={arch} CUs={cus}")
 
     def timed(preconditioner: str, first: int, steps: int, warmup: int):
         """Wall-clock of `steps` batches solved with `preconditioner` (after `warmup` untimed ones)."""
 
         op._solver.preconditioner = preconditioner
         for i in range(first, first + warmup):
-            _, _, stats = step(i)
+            stats = wl.step(i)
             log(f"[{preconditioner}] warmup {i}: iters mean {stats.iterations.mean():.0f} max "
                 f"{stats.iterations.max()} relres max {stats.relative_residual.max():.2e}")
         iters, worst, fallback = [], 0.0, 0
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for i in range(first + warmup, first + warmup + steps):
-            fields, rhoa, stats = step(i)
+            stats = wl.step(i)
             iters.append(stats.iterations.mean())
             worst = max(worst, float(stats.relative_residual.max()))
             fallback += stats.fallback_rhs
             log(f"[{preconditioner}] step {i}: iters mean {stats.iterations.mean():.1f} max {stats.iterations.max()} "
                 f"relres max {stats.relative_residual.max():.2e} fallback {stats.fallback_rhs}")
         torch.cuda.synchronize()
-        return time.perf_counter() - t_start, float(np.mean(iters)), worst, fallback
+        return time.perf_counter() - t_start, float(np.mean(iters)) if iters else None, worst, fallback
 
     # the same batches (steps 0 .. warmup+steps-1) for both solvers
     elapsed, mean_iters, worst_relres, fallback = timed(args.preconditioner, 0, args.steps, args.warmup)
-    rhs_total = args.steps * args.batch_steps * W * E
-    rhs_per_s = rhs_total / elapsed
-    pcg_s, pcg_iters, pcg_relres, _ = timed("jacobi", 0, args.pcg_steps, 1 if args.pcg_steps else 0)
-    pcg_per_s = args.pcg_steps * args.batch_steps * W * E / pcg_s if args.pcg_steps else None
+    rhs_per_s = args.steps * wl.rhs_per_step / elapsed
+    pcg_per_s = pcg_iters = None
+    pcg_relres = 0.0
+    if args.pcg_steps > 0 and args.preconditioner != "jacobi":
+        pcg_s, pcg_iters, pcg_relres, _ = timed("jacobi", 0, args.pcg_steps, 1)
+        pcg_per_s = args.pcg_steps * wl.rhs_per_step / pcg_s
+    elif args.preconditioner == "jacobi":
+        pcg_per_s, pcg_iters = rhs_per_s, mean_iters
     op._solver.preconditioner = args.preconditioner
-    plan = next(iter(op._solver._plans.values()))
-    levels = [plan.amg_level(l)["rows"] for l in range(plan.amg_levels)]
     # a copy of known size (256 MiB read + 256 MiB written): hardware-counter runs of this command
     # scale the solver kernels' FETCH_SIZE / WRITE_SIZE by what the counters read for it
     calib_src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device=dev)
@@ -150,21 +267,24 @@ def main() -> int:
     calib_dst.copy_(calib_src)
     del calib_src, calib_dst
     # profiled step: direct launches with HIP events around every cycle / spmv / update launch
+    plan = op._solver.last_plan
     plan.profile(True)
     torch.cuda.synchronize()
     tp = time.perf_counter()
-    _, _, stats = step(args.warmup + args.steps)
+    stats = wl.step(args.warmup + args.steps)
     torch.cuda.synchronize()
     profiled_s = time.perf_counter() - tp
     prof = plan.profile_read()
     plan.profile(False)
+    levels = [plan.amg_level(lv)["rows"] for lv in range(plan.amg_levels)]
     it = stats.iterations
     rhs_iters = int(it.sum())
+    r = it.shape[1]
     padded = np.zeros((it.shape[0], plan.rp), dtype=np.int64)
-    padded[:, :E] = it
+    padded[:, :r] = it
     group_iters = int(padded.reshape(it.shape[0], -1, plan.lanes).max(axis=2).sum())
     # algorithmic bytes of the two streaming Krylov kernels over the profiled step
-    nnz = d.nnz
+    N, nnz = wl.N, wl.nnz
     matrix_bytes = 12 * nnz + 4 * (N + 1) + 8 * N  # values+indices, indptr, D^-1 (gathered)
     spmv_bytes = rhs_iters * N * 32 + group_iters * matrix_bytes  # read r (z), p_old; write p_new, q
     update_bytes = rhs_iters * N * 48 + group_iters * 8 * N  # read x, p, r, q; write x, r
@@ -173,10 +293,14 @@ def main() -> int:
     dominant = ("cg_spmv", spmv_gbs, spmv_bytes, prof["spmv_ms"]) if prof["spmv_ms"] >= prof["update_ms"] else \
         ("cg_update", update_gbs, update_bytes, prof["update_ms"])
     launches = max(prof["iterations"], 1)
-    solver_name = {"amg": "FP64 flexible CG, smoothed-aggregation V(1,1) cycle, graph-replayed",
+    solver_name = {"amg": "FP64 flexible CG, filtered smoothed-aggregation V(1,1) cycle, graph-replayed",
                    "jacobi": "FP64 Jacobi-PCG, graph-replayed"}[args.preconditioner]
+    config = {"workload": wl.name, **wl.config, "solver": solver_name, "tolerance": args.tol,
+              "lanes": plan.lanes, "rp": plan.rp, "n_tiles": plan.tiles, "rows_per_tile": plan.rows_per_tile,
+              "lds_staged_rows": plan.lds_rows, "dof_order": os.environ.get("ADT_DOF_ORDER", "hilbert"),
+              "multigrid_levels": levels}
     result = {
-        "metric": "RHS solves/s + Jacobian rows/s, learned-FCG vs plain PCG at rel-res 1e-10",
+        "metric": METRIC,
         "value": rhs_per_s,
         "unit": "RHS solves/s",
         "n_gpus": 1,
@@ -187,19 +311,12 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded graded P1 mesh, Wenner-Schlumberger survey, GRF time-lapse conductivity)",
-        "config": {"workload": "2.5D time-lapse batch: ~20k-node P1 mesh, 96 electrodes, ~3000 Wenner-Schlumberger rows",
-                   "nodes": N, "cells": C, "nnz": nnz,
-                   "electrodes": E, "quadrupoles": D, "wavenumbers": W, "wavenumber_rule": "reference (integration.py)",
-                   "time_steps_per_step": args.batch_steps, "rhs_per_step": args.batch_steps * W * E,
-                   "solver": solver_name, "tolerance": args.tol,
-                   "lanes": plan.lanes, "rp": plan.rp, "n_tiles": plan.tiles, "rows_per_tile": plan.rows_per_tile,
-                   "lds_staged_rows": plan.lds_rows, "dof_order": os.environ.get("ADT_DOF_ORDER", "hilbert"),
-                   "multigrid_levels": levels},
+        "data": wl.data,
+        "config": config,
         "solver": {"preconditioner": args.preconditioner, "rhs_solves_per_s": rhs_per_s,
                    "rhs_solves_per_s_pcg": pcg_per_s,
                    "speedup_vs_pcg": rhs_per_s / pcg_per_s if pcg_per_s else None,
-                   "mean_iterations": mean_iters, "mean_iterations_pcg": pcg_iters if args.pcg_steps else None,
+                   "mean_iterations": mean_iters, "mean_iterations_pcg": pcg_iters,
                    "worst_true_relres": max(worst_relres, pcg_relres), "jacobi_fallback_rhs": fallback,
                    "rhs_solves_per_s_learned_fcg": None},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": dominant[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -209,48 +326,38 @@ def main() -> int:
         "kernels": {"cg_spmv": {"gbs": spmv_gbs, "avg_ms": prof["spmv_ms"] / launches},
                     "cg_update": {"gbs": update_gbs, "avg_ms": prof["update_ms"] / launches},
                     "multigrid_cycle": {"avg_ms": prof["cycle_ms"] / launches}},
+        "profiled_step": {"seconds": profiled_s, "rhs_solves_per_s": wl.rhs_per_step / profiled_s,
+                          "iterations_profiled": prof["iterations"]},
     }
-    result["profiled_step"] = {"seconds": profiled_s, "rhs_solves_per_s": args.batch_steps * W * E / profiled_s,
-                               "iterations_profiled": prof["iterations"]}
     log(f"timed: {elapsed:.2f}s, {rhs_per_s:.0f} RHS/s ({args.preconditioner}); plain PCG "
         f"{pcg_per_s or 0:.0f} RHS/s; profiled step {profiled_s:.2f}s; cycle {prof['cycle_ms'] / launches:.3f} ms, "
         f"spmv {spmv_gbs:.0f} GB/s ({prof['spmv_ms'] / launches:.3f} ms), "
         f"update {update_gbs:.0f} GB/s ({prof['update_ms'] / launches:.3f} ms)")
 
     if not args.no_jacobian:
-        fields = op._solve_fields(sig[:1])
-        columns = op._columns()
-        op._normal_jacobian(fields[0], D, columns, return_device=True)  # warm
+        run, sens_bytes, sens_flops, kernel = wl.jacobian()
+        run()  # warm
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps = 3
         start.record()
         for _ in range(reps):
-            jac = op._normal_jacobian(fields[0], D, columns, return_device=True)
+            jac = run()
         stop.record()
         torch.cuda.synchronize()
         ms = start.elapsed_time(stop) / reps
-        result["jacobian_rows_per_s"] = D / (ms * 1e-3)
-        K = 3
-        sens_bytes = 8 * D * C + 8 * W * E * N + 8 * 2 * C * K * K + 4 * C * K + 16 * D
-        sens_flops = D * C * W * (4 * K * K + 6 * K)
+        result["jacobian_rows_per_s"] = wl.D / (ms * 1e-3)
         result["roofline_sensitivity"] = {
-            "bound": "vector-fp64", "kernel": "sensitivity_kernel<3,double>", "avg_launch_ms": ms,
+            "bound": "vector-fp64", "kernel": kernel, "avg_launch_ms": ms,
             "achieved_gbs": sens_bytes / (ms * 1e-3) / 1e9, "hbm_frac": sens_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "achieved_tflops": sens_flops / (ms * 1e-3) / 1e12, "fp64_vector_peak_tflops": 78.6,
             "flops_per_launch": sens_flops, "bytes_per_launch": sens_bytes}
-        log(f"jacobian {D}x{C}: {ms:.1f} ms/launch -> {result['jacobian_rows_per_s']:.0f} rows/s")
+        log(f"jacobian {wl.D}x{wl.C}: {ms:.1f} ms/launch -> {result['jacobian_rows_per_s']:.0f} rows/s")
         del jac
 
-    if args.cpu_steps > 0:
-        wn, wt = op.wavenumbers.double().numpy(), op.weights.double().numpy()
-        cpu_rhs, cpu_rows, cpu_solve_s, cpu_jac_s = cpu_baseline(case, series, wn, wt, args.cpu_steps, args.cpu_rows)
-        result["cpu_baseline"] = {
-            "value": cpu_rhs, "unit": "RHS solves/s", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_steps} time steps x {W} wavenumbers x {E} sources, scipy splu FP64 "
-                      f"({cpu_solve_s:.1f}s); Jacobian {args.cpu_rows} rows x {C} cells einsum ({cpu_jac_s:.1f}s)",
-            "jacobian_rows_per_s": cpu_rows, "cpu": platform.processor() or platform.machine(),
-            "host_cores_available": len(os.sched_getaffinity(0))}
-        log(f"cpu baseline: {cpu_rhs:.0f} RHS/s, {cpu_rows:.1f} rows/s")
+    baseline = wl.cpu()
+    if baseline is not None:
+        result["cpu_baseline"] = baseline
+        log(f"cpu baseline: {baseline['value']:.0f} RHS/s, {baseline['jacobian_rows_per_s']:.1f} rows/s")
     print(json.dumps(result), flush=True)
     return 0
 

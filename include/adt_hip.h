@@ -127,7 +127,8 @@ int adt_pcg_destroy(adt_pcg_plan* plan);
  * solves that replace cuDSS).  Built on the host from host copies of the CSR patterns
  * (h_indptr/h_indices as in adt_pcg_create) and representative values h_values + h_val_off[s]
  * (only the first system of each pattern is read): aggregation by strong connections
- * |a_ij| >= theta sqrt(a_ii a_jj), smoothed prolongators, Galerkin coarse operators, until a
+ * |a_ij| >= theta sqrt(a_ii a_jj), prolongators smoothed with the filtered matrix (strong
+ * entries, weak ones lumped into the diagonal), Galerkin coarse operators, until a
  * level has at most coarse_max rows (dense inverse) or max_levels is reached.  Every solve
  * recomputes the numeric prolongators and Galerkin operators of every system on the device.
  * stall_limit: iterations without a 10% residual decrease before Jacobi PCG takes over (0: never).
@@ -135,7 +136,8 @@ int adt_pcg_destroy(adt_pcg_plan* plan);
 int adt_pcg_attach_amg(adt_pcg_plan* plan, const int32_t* h_indptr, const int32_t* h_indices,
                        const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
                        int max_levels, int stall_limit, int* h_levels);
-/* Rows, matrix slots (-1 on the finest level: the caller's pattern) and damping of one level. */
+/* Rows, matrix slots (-1 on the finest level: the caller's pattern) and the constant prolongation
+ * damping of one level (the value a learned per-slot damping array replaces). */
 int adt_pcg_amg_level(const adt_pcg_plan* plan, int sys, int level, int* h_rows, int64_t* h_nnz,
                       double* h_omega);
 /* Levels of the attached hierarchy and right-hand sides finished by Jacobi PCG in the last solve. */

@@ -318,9 +318,12 @@ def test_multigrid_hierarchy_matches_a_restated_aggregation_and_galerkin_pattern
     agg, count = greedy_aggregates(A.indptr, A.indices, A.data, 0.08)
     assert rows[0] == n and rows[1] == count
     assert rows[-1] <= 20 and all(a > b for a, b in zip(rows, rows[1:]))
-    # Galerkin pattern of the first coarse level: P has the aggregates of each row's neighbours
+    # Galerkin pattern of the first coarse level: P row i has the aggregates of i and of its
+    # strong neighbours (the prolongator is smoothed with the filtered matrix)
     coo = A.tocoo()
-    keep = agg[coo.col] >= 0
+    d = A.diagonal()
+    strong = (coo.row == coo.col) | (np.abs(coo.data) >= 0.08 * np.sqrt(d[coo.row] * d[coo.col]))
+    keep = (agg[coo.col] >= 0) & strong & (agg[coo.row] >= 0)
     P = sp.csr_matrix((np.ones(keep.sum()), (coo.row[keep], agg[coo.col[keep]])), shape=(n, count))
     Ac = (P.T @ abs(A) @ P).tocsr()
     Ac.data[:] = 1.0
