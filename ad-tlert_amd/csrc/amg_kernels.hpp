@@ -219,7 +219,8 @@ __global__ __launch_bounds__(256) void amg_restrict(const LevelArgs* __restrict_
     }
 }
 
-// e = A_c^-1 b on the coarsest level: one workgroup per (system, lane group)
+// e = A_c^-1 b on the coarsest level: workgroup (lane group, block of 256 / L rows), one
+// thread per (row, lane)
 template <int L>
 __global__ __launch_bounds__(256) void amg_coarse_solve(const LevelArgs* __restrict__ la) {
     const LevelArgs& a = *la;
@@ -228,15 +229,18 @@ __global__ __launch_bounds__(256) void amg_coarse_solve(const LevelArgs* __restr
     const LvSys S = a.ls[gid / a.groups];
     const int col = (gid % a.groups) * L + static_cast<int>(threadIdx.x % L);
     const int n = S.rows, rp = a.rp;
-    const double* M = a.dense + S.dense;
+    const int i = static_cast<int>(blockIdx.y) * kRowsPerPass(L) + static_cast<int>(threadIdx.x / L);
+    if (i >= n) return;
+    const double* Mi = a.dense + S.dense + static_cast<int64_t>(i) * n;
     const double* B = a.b + S.vec + col;
-    double* E = a.e + S.vec + col;
-    for (int i = static_cast<int>(threadIdx.x / L); i < n; i += kRowsPerPass(L)) {
-        const double* Mi = M + static_cast<int64_t>(i) * n;
-        double acc = 0.0;
-        for (int j = 0; j < n; ++j) acc = fma(Mi[j], B[static_cast<int64_t>(j) * rp], acc);
-        E[static_cast<int64_t>(i) * rp] = acc;
+    double a0 = 0.0, a1 = 0.0;
+    int j = 0;
+    for (; j + 1 < n; j += 2) {
+        a0 = fma(Mi[j], B[static_cast<int64_t>(j) * rp], a0);
+        a1 = fma(Mi[j + 1], B[static_cast<int64_t>(j + 1) * rp], a1);
     }
+    if (j < n) a0 = fma(Mi[j], B[static_cast<int64_t>(j) * rp], a0);
+    a.e[S.vec + col + static_cast<int64_t>(i) * rp] = a0 + a1;
 }
 
 // z2 = w D^-1 b + P e_{l+1}.  On the finest level (FINE) z2 goes to the direction buffer that

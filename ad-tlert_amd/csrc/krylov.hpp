@@ -29,6 +29,13 @@ struct PcgArgs {
     const Tile* tiles;
     const int* grp_tile0;
     const int* grp_ntiles;
+    // segmented group sums (groups of many tiles): segment = run of consecutive tiles of one group
+    const int* seg_tile0;
+    const int* seg_ntiles;
+    const int* seg_grp;
+    const int* grp_seg0;
+    const int* grp_nseg;
+    double* spart;  // 3 x (n_segs, L) segment partials
     const int32_t* indptr;
     const int32_t* indices;
     const double* values;
@@ -60,6 +67,7 @@ struct PcgArgs {
     int rp, groups, r_real, n_tiles;
     int lds_rows, lds_nnz, lds_own, lds_halo;  // LDS-staged product capacities
     int stall_limit;                           // iterations without a 10% residual decrease (0: off)
+    int segmented, n_segs;
     int pad_;
 };
 
@@ -156,13 +164,52 @@ __device__ __forceinline__ double block_sum1(double v) {
     return v;
 }
 
+// Per-lane-group reduction kernels run kReduceThreads threads: a group's tile partials (one
+// contiguous block of ntiles x L doubles) are summed by kReduceThreads / L parallel chunks with
+// four independent accumulators each, so groups of many tiles (large 3D systems) do not
+// serialize on load latency.
+constexpr int kReduceThreads = 1024;
+
 // Fixed-order sum of one lane group's tile partials; thread l < L receives lane l's total.
+// Must be called by all kReduceThreads threads of the block.
 template <int L>
 __device__ __forceinline__ double group_partials(const double* part, int tile0, int ntiles) {
-    const int lane = threadIdx.x % L, chunk = threadIdx.x / L;
-    double s = 0.0;
-    for (int i = chunk; i < ntiles; i += kRowsPerPass(L)) s += part[static_cast<int64_t>(tile0 + i) * L + lane];
-    return block_sum1<L>(s);
+    constexpr int chunks = kReduceThreads / L;
+    constexpr int waves = kReduceThreads / 64;
+    __shared__ double sh[waves][64];
+    const int lane = static_cast<int>(threadIdx.x) % L, chunk = static_cast<int>(threadIdx.x) / L;
+    const double* base = part + static_cast<int64_t>(tile0) * L + lane;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = chunk;
+    for (; i + 3 * chunks < ntiles; i += 4 * chunks) {
+        s0 += base[static_cast<int64_t>(i) * L];
+        s1 += base[static_cast<int64_t>(i + chunks) * L];
+        s2 += base[static_cast<int64_t>(i + 2 * chunks) * L];
+        s3 += base[static_cast<int64_t>(i + 3 * chunks) * L];
+    }
+    for (; i < ntiles; i += chunks) s0 += base[static_cast<int64_t>(i) * L];
+    double s = adt::wave_sum_strided<L>((s0 + s1) + (s2 + s3));
+    const int wave = static_cast<int>(threadIdx.x) >> 6, wl = static_cast<int>(threadIdx.x) & 63;
+    if (wl < L) sh[wave][wl] = s;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < L) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < waves; ++w) t += sh[w][threadIdx.x];
+        s = t;
+    }
+    __syncthreads();
+    return s;
+}
+
+// Lane totals of partial array k (0, 1, 2) over lane group gid: from the tile partials, or from
+// the segment partials when the plan pre-reduces runs of tiles (reduce_segments).
+template <int L>
+__device__ __forceinline__ double group_sum(const PcgArgs& a, int k, int gid) {
+    if (a.segmented)
+        return group_partials<L>(a.spart + static_cast<int64_t>(k) * a.n_segs * L, a.grp_seg0[gid], a.grp_nseg[gid]);
+    const double* part = k == 0 ? a.part0 : (k == 1 ? a.part1 : a.part2);
+    return group_partials<L>(part, a.grp_tile0[gid], a.grp_ntiles[gid]);
 }
 
 template <int L>

@@ -162,13 +162,12 @@ __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void cg_init_reduce(const PcgArgs* __restrict__ pa) {
+__global__ __launch_bounds__(kReduceThreads) void cg_init_reduce(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
-    const int t0 = a.grp_tile0[gid], nt = a.grp_ntiles[gid];
-    const double bb = group_partials<L>(a.part0, t0, nt);
-    const double rz = group_partials<L>(a.part1, t0, nt);
-    const double rr = group_partials<L>(a.part2, t0, nt);
+    const double bb = group_sum<L>(a, 0, gid);
+    const double rz = group_sum<L>(a, 1, gid);
+    const double rr = group_sum<L>(a, 2, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int sys = gid / a.groups, g = gid % a.groups;
@@ -344,11 +343,11 @@ __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ p
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void cg_alpha(const PcgArgs* __restrict__ pa) {
+__global__ __launch_bounds__(kReduceThreads) void cg_alpha(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
     if (!a.grp_active[gid]) return;
-    const double pq = group_partials<L>(a.part0, a.grp_tile0[gid], a.grp_ntiles[gid]);
+    const double pq = group_sum<L>(a, 0, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
@@ -406,14 +405,13 @@ __global__ __launch_bounds__(256) void cg_update(const PcgArgs* __restrict__ pa,
 
 // Jacobi: convergence test and beta = rho_new / rho.
 template <int L>
-__global__ __launch_bounds__(256) void cg_beta(const PcgArgs* __restrict__ pa) {
+__global__ __launch_bounds__(kReduceThreads) void cg_beta(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
     if (!a.grp_active[gid]) return;
     if (threadIdx.x == 0) a.grp_iters[gid] += 1;
-    const int t0 = a.grp_tile0[gid], nt = a.grp_ntiles[gid];
-    const double rz = group_partials<L>(a.part1, t0, nt);
-    const double rr = group_partials<L>(a.part2, t0, nt);
+    const double rz = group_sum<L>(a, 1, gid);
+    const double rr = group_sum<L>(a, 2, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
@@ -438,12 +436,12 @@ __global__ __launch_bounds__(256) void cg_beta(const PcgArgs* __restrict__ pa) {
 
 // Flexible CG: convergence test and stall detection (coefficients come from fcg_beta).
 template <int L>
-__global__ __launch_bounds__(256) void cg_check(const PcgArgs* __restrict__ pa) {
+__global__ __launch_bounds__(kReduceThreads) void cg_check(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
     if (!a.grp_active[gid]) return;
     if (threadIdx.x == 0) a.grp_iters[gid] += 1;
-    const double rr = group_partials<L>(a.part2, a.grp_tile0[gid], a.grp_ntiles[gid]);
+    const double rr = group_sum<L>(a, 2, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
@@ -477,13 +475,12 @@ __global__ __launch_bounds__(256) void cg_check(const PcgArgs* __restrict__ pa) 
 
 // Flexible CG coefficients from the cycle's partials: beta = -(z.q_old)/(p_old.q_old), rho = z.r.
 template <int L>
-__global__ __launch_bounds__(256) void fcg_beta(const PcgArgs* __restrict__ pa) {
+__global__ __launch_bounds__(kReduceThreads) void fcg_beta(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
     if (!a.grp_active[gid]) return;
-    const int t0 = a.grp_tile0[gid], nt = a.grp_ntiles[gid];
-    const double zq = group_partials<L>(a.part0, t0, nt);
-    const double zr = group_partials<L>(a.part1, t0, nt);
+    const double zq = group_sum<L>(a, 0, gid);
+    const double zr = group_sum<L>(a, 1, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
@@ -530,13 +527,35 @@ __global__ __launch_bounds__(256) void cg_true_residual(const PcgArgs* __restric
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void cg_true_reduce(const PcgArgs* __restrict__ pa) {
+__global__ __launch_bounds__(kReduceThreads) void cg_true_reduce(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
-    const double rr = group_partials<L>(a.part2, a.grp_tile0[gid], a.grp_ntiles[gid]);
+    const double rr = group_sum<L>(a, 2, gid);
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
         a.rr[sc] = rr;
+    }
+}
+
+// Segment pre-reduction: workgroup per segment (run of <= kSegTiles tiles of one lane group)
+// sums the selected partial arrays (mask bits 0..2) into the segment partials, fixed order.
+// Bit 3 also reduces segments of inactive groups (before the flags exist, and after the solve).
+constexpr int kSegTiles = 64;
+
+template <int L>
+__global__ __launch_bounds__(256) void reduce_segments(const PcgArgs* __restrict__ pa, int mask) {
+    const PcgArgs& a = *pa;
+    const int seg = blockIdx.x;
+    if (!(mask & 8) && !a.grp_active[a.seg_grp[seg]]) return;
+    const int t0 = a.seg_tile0[seg], nt = a.seg_ntiles[seg];
+    const int lane = static_cast<int>(threadIdx.x % L), chunk = static_cast<int>(threadIdx.x / L);
+    for (int k = 0; k < 3; ++k) {
+        if (!((mask >> k) & 1)) continue;
+        const double* part = k == 0 ? a.part0 : (k == 1 ? a.part1 : a.part2);
+        double v = 0.0;
+        for (int i = chunk; i < nt; i += kRowsPerPass(L)) v += part[static_cast<int64_t>(t0 + i) * L + lane];
+        v = block_sum1<L>(v);
+        if (threadIdx.x < L) a.spart[(static_cast<int64_t>(k) * a.n_segs + seg) * L + threadIdx.x] = v;
     }
 }
 
@@ -571,6 +590,9 @@ struct adt_pcg_plan {
     Tile* tiles_d = nullptr;
     int* grp_tile0_d = nullptr;
     int* grp_ntiles_d = nullptr;
+    int* seg_d = nullptr;  // seg_tile0, seg_ntiles, seg_grp, grp_seg0, grp_nseg
+    double* spart_d = nullptr;
+    int n_segs = 0;
     double* vec = nullptr;  // r, p0, p1, q, z
     double* dinv = nullptr;
     double* part = nullptr;
@@ -629,7 +651,7 @@ adt_pcg_plan::~adt_pcg_plan() {
                 if (e) (void)hipEventDestroy(e);
     if (cap) (void)hipStreamDestroy(cap);
     void* bufs[] = {sys_d, tiles_d, grp_tile0_d, grp_ntiles_d, vec, dinv, part, scal, ints, args_d, lidx_d, halo_ptr_d,
-                    halo_rows_d};
+                    halo_rows_d, seg_d, spart_d};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h_active) (void)hipHostFree(h_active);
@@ -668,7 +690,8 @@ void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
         hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
         hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[l + 1].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
     }
-    hipLaunchKernelGGL(amg_coarse_solve<L>, dim3(p->n_groups), dim3(256), 0, st, p->amg[m - 1].args_d);
+    hipLaunchKernelGGL(amg_coarse_solve<L>, dim3(p->n_groups, adt::blocks_for(p->amg[m - 1].max_rows, 256 / L)),
+                       dim3(256), 0, st, p->amg[m - 1].args_d);
     for (int l = m - 2; l >= 1; --l) {
         hipLaunchKernelGGL((amg_prolong<L, false>), dim3(p->amg[l].n_tiles), dim3(256), 0, st, pa, p->amg[l].args_d,
                            parity);
@@ -681,6 +704,14 @@ void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
         hipLaunchKernelGGL(amg_post_smooth_fine<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d, parity);
 }
 
+// Reduction kernel for lane groups, preceded by the segment pre-reduction in segmented plans.
+template <int L>
+void launch_reduce(const adt_pcg_plan* p, hipStream_t st, void (*kernel)(const PcgArgs*), int mask) {
+    if (p->n_segs > 0)
+        hipLaunchKernelGGL(reduce_segments<L>, dim3(p->n_segs), dim3(256), 0, st, p->args_d, mask);
+    hipLaunchKernelGGL(kernel, dim3(p->n_groups), dim3(kReduceThreads), 0, st, p->args_d);
+}
+
 template <int L>
 void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mode, hipEvent_t* events) {
     const PcgArgs* pa = p->args_d;
@@ -688,7 +719,7 @@ void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mod
         if (events) (void)hipEventRecord(events[4], st);
         launch_cycle<L>(p, st, parity);
         if (events) (void)hipEventRecord(events[5], st);
-        hipLaunchKernelGGL(fcg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+        launch_reduce<L>(p, st, fcg_beta<L>, 3);
     }
     if (events) (void)hipEventRecord(events[0], st);
     if (p->lds_rows > 0) {
@@ -704,14 +735,14 @@ void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mod
             hipLaunchKernelGGL((cg_spmv<L, false>), dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
     }
     if (events) (void)hipEventRecord(events[1], st);
-    hipLaunchKernelGGL(cg_alpha<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    launch_reduce<L>(p, st, cg_alpha<L>, 1);
     if (events) (void)hipEventRecord(events[2], st);
     hipLaunchKernelGGL(cg_update<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
     if (events) (void)hipEventRecord(events[3], st);
     if (mode == 1)
-        hipLaunchKernelGGL(cg_check<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+        launch_reduce<L>(p, st, cg_check<L>, 4);
     else
-        hipLaunchKernelGGL(cg_beta<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+        launch_reduce<L>(p, st, cg_beta<L>, 6);
 }
 
 // Capture `chunk` (even) iterations; replays keep the p ping-pong parity at 0.
@@ -793,7 +824,7 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mod
     const PcgArgs* pa = p->args_d;
     ADT_HIP_CHECK(hipMemsetAsync(p->args_h.active_total, 0, sizeof(int), st));
     hipLaunchKernelGGL(cg_init<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
-    hipLaunchKernelGGL(cg_init_reduce<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    launch_reduce<L>(p, st, cg_init_reduce<L>, 15);
     hipLaunchKernelGGL(cg_zero_solution<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     ADT_HIP_CHECK(hipGetLastError());
     if (max_iter > 0) {
@@ -823,7 +854,7 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mod
         if (p->profile && (rc = accumulate_profile(p, last, chunk))) return rc;
     }
     hipLaunchKernelGGL(cg_true_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
-    hipLaunchKernelGGL(cg_true_reduce<L>, dim3(p->n_groups), dim3(256), 0, st, pa);
+    launch_reduce<L>(p, st, cg_true_reduce<L>, 12);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -1013,7 +1044,47 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
         delete p;
         return adt::set_error(ADT_ERR_HIP, "adt_pcg_create: metadata upload failed");
     }
+    // groups of many tiles: segment partials pre-reduced by many workgroups
+    const int max_nt = *std::max_element(nt.begin(), nt.end());
+    if (max_nt > 512) {
+        std::vector<int> st0, snt, sgrp, gs0(p->n_groups), gns(p->n_groups);
+        for (int g = 0; g < p->n_groups; ++g) {
+            gs0[g] = static_cast<int>(st0.size());
+            for (int off = 0; off < nt[g]; off += kSegTiles) {
+                st0.push_back(t0[g] + off);
+                snt.push_back(std::min(kSegTiles, nt[g] - off));
+                sgrp.push_back(g);
+            }
+            gns[g] = static_cast<int>(st0.size()) - gs0[g];
+        }
+        p->n_segs = static_cast<int>(st0.size());
+        std::vector<int> meta;
+        meta.insert(meta.end(), st0.begin(), st0.end());
+        meta.insert(meta.end(), snt.begin(), snt.end());
+        meta.insert(meta.end(), sgrp.begin(), sgrp.end());
+        meta.insert(meta.end(), gs0.begin(), gs0.end());
+        meta.insert(meta.end(), gns.begin(), gns.end());
+        if ((rc = dalloc(&p->seg_d, static_cast<int64_t>(meta.size()))) ||
+            (rc = dalloc(&p->spart_d, 3 * static_cast<int64_t>(p->n_segs) * lanes))) {
+            delete p;
+            return rc;
+        }
+        if (hipMemcpy(p->seg_d, meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            delete p;
+            return adt::set_error(ADT_ERR_HIP, "adt_pcg_create: segment table upload failed");
+        }
+    }
     PcgArgs& a = p->args_h;
+    if (p->n_segs > 0) {
+        a.seg_tile0 = p->seg_d;
+        a.seg_ntiles = p->seg_d + p->n_segs;
+        a.seg_grp = p->seg_d + 2 * p->n_segs;
+        a.grp_seg0 = p->seg_d + 3 * p->n_segs;
+        a.grp_nseg = p->seg_d + 3 * p->n_segs + p->n_groups;
+        a.spart = p->spart_d;
+        a.segmented = 1;
+        a.n_segs = p->n_segs;
+    }
     a.lidx = p->lidx_d;
     a.halo_ptr = p->halo_ptr_d;
     a.halo_rows = p->halo_rows_d;

@@ -136,11 +136,11 @@ def random_spd_batch(rng, side=20, S=2):
     return indptr, c.astype(np.int32), values, dense
 
 
-def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2, omega=None):
+def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2, omega=None, rows_per_tile=32):
     rng = np.random.default_rng(seed)
     indptr, indices, values, dense = random_spd_batch(rng, side)
     S, n = values.shape[0], side * side
-    plan = native.PcgPlan(np.full(S, n), R, lanes, 32, np.zeros(S), np.zeros(S), np.arange(S) * values.shape[1],
+    plan = native.PcgPlan(np.full(S, n), R, lanes, rows_per_tile, np.zeros(S), np.zeros(S), np.arange(S) * values.shape[1],
                           host_pattern=(indptr, indices) if staged or precond != native.PRECOND_JACOBI else None,
                           lds_bytes=8192 if staged else 0)
     assert (plan.lds_rows > 0) == staged
@@ -208,3 +208,11 @@ def test_per_slot_damping_equal_to_the_constant_reproduces_the_cycle():
 @pytest.mark.parametrize("lanes, R", [(8, 5), (64, 70)])
 def test_multigrid_flexible_cg_for_other_lane_widths(lanes, R):
     solve_batch(native.PRECOND_AMG, True, side=24, lanes=lanes, R=R, seed=3)
+
+
+@pytest.mark.parametrize("precond", [native.PRECOND_JACOBI, native.PRECOND_AMG], ids=["jacobi", "amg"])
+def test_lane_groups_of_many_tiles_use_segmented_sums(precond):
+    """Four-row tiles on 2304 rows give 576 tiles per lane group: the group sums then go through
+    the segment pre-reduction; results must still meet the tolerance and match dense solves."""
+
+    solve_batch(precond, True, side=48, lanes=64, R=64, seed=5, rows_per_tile=4)
