@@ -1,0 +1,558 @@
+// Learned multigrid damping: a 3-layer graph message-passing network on the matrix graph that
+// predicts, for every matrix slot, the damping used by the finest prolongator of the
+// smoothed-aggregation preconditioner (ADT_PRECOND_LEARNED).  Weights are stored as bfloat16
+// (staged in LDS, widened to FP32), activations and every sum are FP32 with fixed orders.
+//
+//   node features  x_i = [log10(a_ii / mean a), sum_{j!=i} |a_ij| / a_ii, sum_j a_ij / a_ii, 1]
+//   messages       w_ij = |a_ij| / sum_{j'!=i} |a_ij'|
+//   layer 0        h0 = relu(x W0 + b0)                                      (64 channels)
+//   layers 1..3    m = Wmsg h_{l-1};  h_l = relu([h_{l-1}, m] W_l + b_l)
+//   slot head      z_k = [h3_i, h3_j, a_ij / sqrt(a_ii a_jj), [i == j], same aggregate, strong]
+//                  s_k = relu(z_k U + c) . v + d;  omega_k = omega_p (1 + tanh(s_k) / 2)
+//
+// The backward pass (for training) turns dL/domega into FP32 weight gradients: per-block
+// partial sums of every outer product, reduced over blocks in a fixed order.  Transposed
+// message and head contributions are gathered through the mirror slot (j, i) of each slot
+// (i, j), so no atomics are used anywhere.
+#include "adt_common.hpp"
+
+#include <cmath>
+
+namespace {
+
+constexpr int H = 64;         // hidden channels
+constexpr int HD = 32;        // head hidden units
+constexpr int NF = 4;         // node features
+constexpr int EF = 4;         // slot features
+constexpr int ZIN = 2 * H + EF;  // head input
+constexpr int OFF_W0 = 0, OFF_B0 = NF * H;
+constexpr int OFF_L1 = OFF_B0 + H;              // per message layer: W (2H x H) then b (H)
+constexpr int LAYER_SIZE = 2 * H * H + H;
+constexpr int OFF_U = OFF_L1 + 3 * LAYER_SIZE;
+constexpr int OFF_C = OFF_U + ZIN * HD, OFF_V = OFF_C + HD, OFF_D = OFF_V + HD;
+constexpr int PARAMS = OFF_D + 1;
+constexpr int NODES_PER_BLOCK = 32;  // rows per block of the node kernels (4 per round)
+constexpr int SLOTS_PER_BLOCK = 64;  // slots per block of the head kernels (8 per round)
+
+__device__ __forceinline__ float bf16_to_float(uint16_t v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
+
+// per-system FP32 work layout
+struct Work {
+    float *x, *w, *e, *h[4], *m[4], *u, *s, *mean;
+};
+
+__host__ __device__ inline int64_t work_floats(int n, int nnz) {
+    return static_cast<int64_t>(n) * (NF + 4 * H + 3 * H) + static_cast<int64_t>(nnz) * (1 + EF + HD + 1) + 4;
+}
+
+__host__ __device__ inline Work work_at(float* base, int n, int nnz) {
+    Work W;
+    const int64_t N = n, Z = nnz;
+    W.x = base;
+    W.w = W.x + N * NF;
+    W.e = W.w + Z;
+    W.h[0] = W.e + Z * EF;
+    for (int l = 1; l < 4; ++l) W.h[l] = W.h[l - 1] + N * H;
+    W.m[0] = nullptr;
+    W.m[1] = W.h[3] + N * H;
+    W.m[2] = W.m[1] + N * H;
+    W.m[3] = W.m[2] + N * H;
+    W.u = W.m[3] + N * H;
+    W.s = W.u + Z * HD;
+    W.mean = W.s + Z;
+    return W;
+}
+
+struct Graph {
+    const int32_t* ptr;
+    const int32_t* idx;
+    const int32_t* rows;
+    const int32_t* mirror;
+    const int32_t* flags;  // bit 1: same aggregate
+    int n, nnz;
+};
+
+// ---------------------------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ int diag_slot(const Graph& g, int i) {
+    for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k)
+        if (g.idx[k] == i) return k;
+    return -1;
+}
+
+// mean of the diagonal per system (one block, fixed order)
+__global__ __launch_bounds__(256) void gnn_mean_diag(Graph g, const double* values, int64_t stride, float* work,
+                                                     int64_t wstride) {
+    __shared__ double sh[256];
+    const double* V = values + blockIdx.x * stride;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < g.n; i += 256) {
+        const int k = diag_slot(g, i);
+        acc += k >= 0 ? V[k] : 0.0;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) work_at(work + blockIdx.x * wstride, g.n, g.nnz).mean[0] = static_cast<float>(sh[0] / g.n);
+}
+
+// node features, message weights and slot features: thread per row
+__global__ __launch_bounds__(256) void gnn_features(Graph g, const double* values, int64_t stride, float* work,
+                                                    int64_t wstride, float theta) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.n) return;
+    const double* V = values + blockIdx.y * stride;
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const int kd = diag_slot(g, i);
+    const double aii = kd >= 0 ? V[kd] : 1.0;
+    double s_off = 0.0, s_row = 0.0;
+    for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+        s_row += V[k];
+        if (k != kd) s_off += fabs(V[k]);
+    }
+    W.x[static_cast<int64_t>(i) * NF + 0] = static_cast<float>(log10(aii / static_cast<double>(W.mean[0])));
+    W.x[static_cast<int64_t>(i) * NF + 1] = static_cast<float>(s_off / aii);
+    W.x[static_cast<int64_t>(i) * NF + 2] = static_cast<float>(s_row / aii);
+    W.x[static_cast<int64_t>(i) * NF + 3] = 1.0f;
+    for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+        const int j = g.idx[k];
+        const int kj = diag_slot(g, j);
+        const double ajj = kj >= 0 ? V[kj] : 1.0;
+        const double a = V[k];
+        const bool diag = k == kd;
+        W.w[k] = diag || s_off <= 0.0 ? 0.0f : static_cast<float>(fabs(a) / s_off);
+        const double scaled = a / sqrt(aii * ajj);
+        const bool strong = !diag && a != 0.0 && fabs(scaled) >= theta;
+        float* e = W.e + static_cast<int64_t>(k) * EF;
+        e[0] = static_cast<float>(scaled);
+        e[1] = diag ? 1.0f : 0.0f;
+        e[2] = static_cast<float>((g.flags[k] >> 1) & 1);
+        e[3] = strong ? 1.0f : 0.0f;
+    }
+}
+
+// layer 0: h0 = relu(x W0 + b0); block = 4 rows x 64 channels per round
+__global__ __launch_bounds__(256) void gnn_layer0(Graph g, const uint16_t* params, float* work, int64_t wstride) {
+    __shared__ float Wl[NF * H + H];
+    for (int t = threadIdx.x; t < NF * H + H; t += 256) Wl[t] = bf16_to_float(params[OFF_W0 + t]);
+    __syncthreads();
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const int q = threadIdx.x / H, c = threadIdx.x % H;
+    for (int r = 0; r < NODES_PER_BLOCK; r += 4) {
+        const int i = blockIdx.x * NODES_PER_BLOCK + r + q;
+        if (i >= g.n) break;
+        float acc = Wl[NF * H + c];
+        for (int t = 0; t < NF; ++t) acc = fmaf(W.x[static_cast<int64_t>(i) * NF + t], Wl[t * H + c], acc);
+        W.h[0][static_cast<int64_t>(i) * H + c] = fmaxf(acc, 0.0f);
+    }
+}
+
+// message layer l: m = Wmsg h_{l-1}, h_l = relu([h_{l-1}, m] W_l + b_l)
+__global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params, float* work, int64_t wstride,
+                                                 int l) {
+    __shared__ uint16_t Wl[2 * H * H];
+    __shared__ float bl[H];
+    __shared__ float in[4][2 * H];
+    const uint16_t* P = params + OFF_L1 + (l - 1) * LAYER_SIZE;
+    for (int t = threadIdx.x; t < 2 * H * H; t += 256) Wl[t] = P[t];
+    if (threadIdx.x < H) bl[threadIdx.x] = bf16_to_float(P[2 * H * H + threadIdx.x]);
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const float* hp = W.h[l - 1];
+    const int q = threadIdx.x / H, c = threadIdx.x % H;
+    for (int r = 0; r < NODES_PER_BLOCK; r += 4) {
+        const int i = blockIdx.x * NODES_PER_BLOCK + r + q;
+        __syncthreads();
+        if (i < g.n) {
+            float m = 0.0f;
+            for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k) m = fmaf(W.w[k], hp[static_cast<int64_t>(g.idx[k]) * H + c], m);
+            in[q][c] = hp[static_cast<int64_t>(i) * H + c];
+            in[q][H + c] = m;
+            W.m[l][static_cast<int64_t>(i) * H + c] = m;
+        }
+        __syncthreads();
+        if (i < g.n) {
+            float acc = bl[c];
+            for (int t = 0; t < 2 * H; ++t) acc = fmaf(in[q][t], bf16_to_float(Wl[t * H + c]), acc);
+            W.h[l][static_cast<int64_t>(i) * H + c] = fmaxf(acc, 0.0f);
+        }
+    }
+}
+
+// slot head: block = 8 slots x 32 hidden units per round
+__global__ __launch_bounds__(256) void gnn_head(Graph g, const uint16_t* params, float* work, int64_t wstride,
+                                                const double* omega_p, double* omega, int64_t ostride) {
+    __shared__ uint16_t Us[ZIN * HD];
+    __shared__ float cs[HD], vs[HD];
+    __shared__ float z[8][ZIN];
+    for (int t = threadIdx.x; t < ZIN * HD; t += 256) Us[t] = params[OFF_U + t];
+    if (threadIdx.x < HD) {
+        cs[threadIdx.x] = bf16_to_float(params[OFF_C + threadIdx.x]);
+        vs[threadIdx.x] = bf16_to_float(params[OFF_V + threadIdx.x]);
+    }
+    const float d0 = bf16_to_float(params[OFF_D]);
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const double wp = omega_p[blockIdx.y];
+    double* O = omega + blockIdx.y * ostride;
+    const int q = threadIdx.x / HD, j = threadIdx.x % HD;
+    for (int r = 0; r < SLOTS_PER_BLOCK; r += 8) {
+        const int k = blockIdx.x * SLOTS_PER_BLOCK + r + q;
+        __syncthreads();
+        if (k < g.nnz) {
+            const int64_t i = g.rows[k], col = g.idx[k];
+            for (int t = j; t < H; t += HD) {
+                z[q][t] = W.h[3][i * H + t];
+                z[q][H + t] = W.h[3][col * H + t];
+            }
+            if (j < EF) z[q][2 * H + j] = W.e[static_cast<int64_t>(k) * EF + j];
+        }
+        __syncthreads();
+        float part = 0.0f;
+        if (k < g.nnz) {
+            float acc = cs[j];
+            for (int t = 0; t < ZIN; ++t) acc = fmaf(z[q][t], bf16_to_float(Us[t * HD + j]), acc);
+            const float u = fmaxf(acc, 0.0f);
+            W.u[static_cast<int64_t>(k) * HD + j] = u;
+            part = u * vs[j];
+        }
+        // sum over the 32 hidden units (lanes j of one half wavefront), fixed butterfly order
+#pragma unroll
+        for (int off = 1; off < HD; off <<= 1) part += __shfl_xor(part, off, 64);
+        if (k < g.nnz && j == 0) {
+            const float s = part + d0;
+            W.s[k] = s;
+            O[k] = wp * (1.0 + 0.5 * tanh(static_cast<double>(s)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// backward (one system)
+// ---------------------------------------------------------------------------------------------
+
+struct Scratch {
+    float *dz, *dh, *dn, *dm, *da, *part;
+};
+
+__host__ __device__ inline int head_blocks(int nnz) { return (nnz + SLOTS_PER_BLOCK - 1) / SLOTS_PER_BLOCK; }
+__host__ __device__ inline int node_blocks(int n) { return (n + NODES_PER_BLOCK - 1) / NODES_PER_BLOCK; }
+constexpr int HEAD_PARAMS = ZIN * HD + HD + HD + 1;  // U, c, v, d
+constexpr int LAYER0_PARAMS = NF * H + H;
+
+__host__ __device__ inline int64_t partial_floats(int n, int nnz) {
+    const int64_t hb = head_blocks(nnz), nb = node_blocks(n);
+    const int64_t a = hb * HEAD_PARAMS, b = nb * LAYER_SIZE;
+    return a > b ? a : b;
+}
+
+__host__ __device__ inline int64_t scratch_floats(int n, int nnz) {
+    return static_cast<int64_t>(nnz) * 2 * H + 4 * static_cast<int64_t>(n) * H + partial_floats(n, nnz);
+}
+
+__host__ __device__ inline Scratch scratch_at(float* base, int n, int nnz) {
+    Scratch S;
+    S.dz = base;
+    S.dh = S.dz + static_cast<int64_t>(nnz) * 2 * H;
+    S.dn = S.dh + static_cast<int64_t>(n) * H;
+    S.dm = S.dn + static_cast<int64_t>(n) * H;
+    S.da = S.dm + static_cast<int64_t>(n) * H;
+    S.part = S.da + static_cast<int64_t>(n) * H;
+    return S;
+}
+
+// head backward: du, dz per slot, block partials of (U, c, v, d)
+__global__ __launch_bounds__(256) void gnn_head_bwd(Graph g, const uint16_t* params, const float* work,
+                                                    double omega_p, const double* domega, float* scratch) {
+    __shared__ float z[8][ZIN];
+    __shared__ float du[8][HD];
+    __shared__ float dsv[8];
+    __shared__ uint16_t Us[ZIN * HD];
+    __shared__ float vs[HD];
+    for (int t = threadIdx.x; t < ZIN * HD; t += 256) Us[t] = params[OFF_U + t];
+    if (threadIdx.x < HD) vs[threadIdx.x] = bf16_to_float(params[OFF_V + threadIdx.x]);
+    const Work W = work_at(const_cast<float*>(work), g.n, g.nnz);
+    Scratch S = scratch_at(scratch, g.n, g.nnz);
+    const int q = threadIdx.x / HD, j = threadIdx.x % HD;
+    // entries owned by this thread in the block partial: e = threadIdx.x + 256 r
+    constexpr int OWN = (HEAD_PARAMS + 255) / 256;
+    float acc[OWN];
+#pragma unroll
+    for (int r = 0; r < OWN; ++r) acc[r] = 0.0f;
+    for (int r0 = 0; r0 < SLOTS_PER_BLOCK; r0 += 8) {
+        const int k = blockIdx.x * SLOTS_PER_BLOCK + r0 + q;
+        __syncthreads();
+        if (k < g.nnz) {
+            const int64_t i = g.rows[k], col = g.idx[k];
+            for (int t = j; t < H; t += HD) {
+                z[q][t] = W.h[3][i * H + t];
+                z[q][H + t] = W.h[3][col * H + t];
+            }
+            if (j < EF) z[q][2 * H + j] = W.e[static_cast<int64_t>(k) * EF + j];
+            const float s = W.s[k];
+            const float th = tanhf(s);
+            const float ds = static_cast<float>(domega[k] * omega_p * 0.5) * (1.0f - th * th);
+            const float u = W.u[static_cast<int64_t>(k) * HD + j];
+            du[q][j] = u > 0.0f ? ds * vs[j] : 0.0f;
+            if (j == 0) dsv[q] = ds;
+        } else {
+            for (int t = j; t < ZIN; t += HD) z[q][t] = 0.0f;
+            du[q][j] = 0.0f;
+            if (j == 0) dsv[q] = 0.0f;
+        }
+        __syncthreads();
+        if (k < g.nnz) {  // dz[t] = sum_j U[t][j] du[j] for t < 2H
+            for (int t = j; t < 2 * H; t += HD) {
+                float a = 0.0f;
+                for (int jj = 0; jj < HD; ++jj) a = fmaf(bf16_to_float(Us[t * HD + jj]), du[q][jj], a);
+                S.dz[static_cast<int64_t>(k) * 2 * H + t] = a;
+            }
+        }
+        // block partials over the 8 slots of this round, fixed slot order
+#pragma unroll
+        for (int r = 0; r < OWN; ++r) {
+            const int e = threadIdx.x + 256 * r;
+            if (e >= HEAD_PARAMS) continue;
+            float a = acc[r];
+            for (int qq = 0; qq < 8; ++qq) {
+                float term;
+                if (e < ZIN * HD) {
+                    term = z[qq][e / HD] * du[qq][e % HD];
+                } else if (e < ZIN * HD + HD) {
+                    term = du[qq][e - ZIN * HD];
+                } else if (e < ZIN * HD + 2 * HD) {
+                    const int jj = e - ZIN * HD - HD;
+                    const int kk = blockIdx.x * SLOTS_PER_BLOCK + r0 + qq;
+                    term = kk < g.nnz ? dsv[qq] * W.u[static_cast<int64_t>(kk) * HD + jj] : 0.0f;
+                } else {
+                    term = dsv[qq];
+                }
+                a += term;
+            }
+            acc[r] = a;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < OWN; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        if (e < HEAD_PARAMS) S.part[static_cast<int64_t>(blockIdx.x) * HEAD_PARAMS + e] = acc[r];
+    }
+}
+
+// fixed-order sum of block partials: out[e] = sum_b part[b][e]
+__global__ __launch_bounds__(256) void gnn_reduce_partials(const float* part, int blocks, int entries, float* out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= entries) return;
+    float a = 0.0f;
+    for (int b = 0; b < blocks; ++b) a += part[static_cast<int64_t>(b) * entries + e];
+    out[e] = a;
+}
+
+// dh3_i = sum_{k in row i} dz_k[0:H] + dz_{mirror(k)}[H:2H]  (thread per (row, channel))
+__global__ __launch_bounds__(256) void gnn_head_to_nodes(Graph g, float* scratch) {
+    Scratch S = scratch_at(scratch, g.n, g.nnz);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int i = static_cast<int>(t / H), c = static_cast<int>(t % H);
+    if (i >= g.n) return;
+    float a = 0.0f;
+    for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+        a += S.dz[static_cast<int64_t>(k) * 2 * H + c];
+        a += S.dz[static_cast<int64_t>(g.mirror[k]) * 2 * H + H + c];
+    }
+    S.dh[static_cast<int64_t>(i) * H + c] = a;
+}
+
+// message layer backward, part 1 (thread per (row, channel) in rounds of 4 rows):
+// da = dh * [h_l > 0];  [dn, dm] = da W_l^T;  block partials of dW_l, db_l
+__global__ __launch_bounds__(256) void gnn_layer_bwd(Graph g, const uint16_t* params, const float* work,
+                                                     float* scratch, int l) {
+    __shared__ uint16_t Wl[2 * H * H];
+    __shared__ float in[4][2 * H];
+    __shared__ float das[4][H];
+    const uint16_t* P = params + OFF_L1 + (l - 1) * LAYER_SIZE;
+    for (int t = threadIdx.x; t < 2 * H * H; t += 256) Wl[t] = P[t];
+    const Work W = work_at(const_cast<float*>(work), g.n, g.nnz);
+    Scratch S = scratch_at(scratch, g.n, g.nnz);
+    const int q = threadIdx.x / H, c = threadIdx.x % H;
+    constexpr int OWN = (LAYER_SIZE + 255) / 256;
+    float acc[OWN];
+#pragma unroll
+    for (int r = 0; r < OWN; ++r) acc[r] = 0.0f;
+    for (int r0 = 0; r0 < NODES_PER_BLOCK; r0 += 4) {
+        const int i = blockIdx.x * NODES_PER_BLOCK + r0 + q;
+        __syncthreads();
+        if (i < g.n) {
+            const int64_t o = static_cast<int64_t>(i) * H + c;
+            const float da = W.h[l][o] > 0.0f ? S.dh[o] : 0.0f;
+            das[q][c] = da;
+            S.da[o] = da;
+            in[q][c] = W.h[l - 1][o];
+            in[q][H + c] = W.m[l][o];
+        } else {
+            das[q][c] = 0.0f;
+            in[q][c] = 0.0f;
+            in[q][H + c] = 0.0f;
+        }
+        __syncthreads();
+        if (i < g.n) {  // dinp[t] = sum_c W[t][c] da[c], t = c and c + H
+            for (int t = c; t < 2 * H; t += H) {
+                float a = 0.0f;
+                for (int cc = 0; cc < H; ++cc) a = fmaf(bf16_to_float(Wl[t * H + cc]), das[q][cc], a);
+                if (t < H)
+                    S.dn[static_cast<int64_t>(i) * H + t] = a;
+                else
+                    S.dm[static_cast<int64_t>(i) * H + t - H] = a;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < OWN; ++r) {
+            const int e = threadIdx.x + 256 * r;
+            if (e >= LAYER_SIZE) continue;
+            float a = acc[r];
+            if (e < 2 * H * H) {
+                const int t = e / H, cc = e % H;
+                for (int qq = 0; qq < 4; ++qq) a += in[qq][t] * das[qq][cc];
+            } else {
+                for (int qq = 0; qq < 4; ++qq) a += das[qq][e - 2 * H * H];
+            }
+            acc[r] = a;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < OWN; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        if (e < LAYER_SIZE) S.part[static_cast<int64_t>(blockIdx.x) * LAYER_SIZE + e] = acc[r];
+    }
+}
+
+// message layer backward, part 2: dh_{l-1}[j] = dn[j] + sum_{k in row j} w_{mirror(k)} dm[col k]
+__global__ __launch_bounds__(256) void gnn_msg_bwd(Graph g, const float* work, float* scratch) {
+    const Work W = work_at(const_cast<float*>(work), g.n, g.nnz);
+    Scratch S = scratch_at(scratch, g.n, g.nnz);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int j = static_cast<int>(t / H), c = static_cast<int>(t % H);
+    if (j >= g.n) return;
+    float a = S.dn[static_cast<int64_t>(j) * H + c];
+    for (int k = g.ptr[j]; k < g.ptr[j + 1]; ++k)
+        a = fmaf(W.w[g.mirror[k]], S.dm[static_cast<int64_t>(g.idx[k]) * H + c], a);
+    S.dh[static_cast<int64_t>(j) * H + c] = a;
+}
+
+// layer 0 backward: da = dh * [h0 > 0]; block partials of dW0 (x da^T) and db0
+__global__ __launch_bounds__(256) void gnn_layer0_bwd(Graph g, const float* work, float* scratch) {
+    __shared__ float xs[4][NF];
+    __shared__ float das[4][H];
+    const Work W = work_at(const_cast<float*>(work), g.n, g.nnz);
+    Scratch S = scratch_at(scratch, g.n, g.nnz);
+    const int q = threadIdx.x / H, c = threadIdx.x % H;
+    constexpr int OWN = (LAYER0_PARAMS + 255) / 256;
+    float acc[OWN];
+#pragma unroll
+    for (int r = 0; r < OWN; ++r) acc[r] = 0.0f;
+    for (int r0 = 0; r0 < NODES_PER_BLOCK; r0 += 4) {
+        const int i = blockIdx.x * NODES_PER_BLOCK + r0 + q;
+        __syncthreads();
+        if (i < g.n) {
+            const int64_t o = static_cast<int64_t>(i) * H + c;
+            das[q][c] = W.h[0][o] > 0.0f ? S.dh[o] : 0.0f;
+            if (c < NF) xs[q][c] = W.x[static_cast<int64_t>(i) * NF + c];
+        } else {
+            das[q][c] = 0.0f;
+            if (c < NF) xs[q][c] = 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < OWN; ++r) {
+            const int e = threadIdx.x + 256 * r;
+            if (e >= LAYER0_PARAMS) continue;
+            float a = acc[r];
+            if (e < NF * H) {
+                for (int qq = 0; qq < 4; ++qq) a += xs[qq][e / H] * das[qq][e % H];
+            } else {
+                for (int qq = 0; qq < 4; ++qq) a += das[qq][e - NF * H];
+            }
+            acc[r] = a;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < OWN; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        if (e < LAYER0_PARAMS) S.part[static_cast<int64_t>(blockIdx.x) * LAYER0_PARAMS + e] = acc[r];
+    }
+}
+
+Graph make_graph(int n, int nnz, const int32_t* ptr, const int32_t* idx, const int32_t* rows, const int32_t* mirror,
+                 const int32_t* flags) {
+    Graph g;
+    g.ptr = ptr;
+    g.idx = idx;
+    g.rows = rows;
+    g.mirror = mirror;
+    g.flags = flags;
+    g.n = n;
+    g.nnz = nnz;
+    return g;
+}
+
+}  // namespace
+
+extern "C" int adt_gnn_param_count(void) { return PARAMS; }
+
+extern "C" int64_t adt_gnn_work_floats(int n, int nnz) { return work_floats(n, nnz); }
+
+extern "C" int64_t adt_gnn_scratch_floats(int n, int nnz) { return scratch_floats(n, nnz); }
+
+extern "C" int adt_gnn_forward(int n_sys, int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
+                               const int32_t* d_rows, const int32_t* d_flags, const double* d_values,
+                               int64_t val_stride, double theta, const double* d_omega_p, const uint16_t* d_params,
+                               float* d_work, double* d_omega, void* stream) {
+    ADT_REQUIRE(n_sys >= 1 && n_sys <= 65535 && n >= 1 && nnz >= n, "adt_gnn_forward: invalid sizes");
+    ADT_REQUIRE(d_indptr && d_indices && d_rows && d_flags && d_values && d_omega_p && d_params && d_work && d_omega,
+                "adt_gnn_forward: null pointer");
+    ADT_REQUIRE(val_stride >= nnz, "adt_gnn_forward: val_stride must be >= nnz");
+    hipStream_t st = adt::as_stream(stream);
+    const Graph g = make_graph(n, nnz, d_indptr, d_indices, d_rows, nullptr, d_flags);
+    const int64_t ws = work_floats(n, nnz);
+    hipLaunchKernelGGL(gnn_mean_diag, dim3(n_sys), dim3(256), 0, st, g, d_values, val_stride, d_work, ws);
+    hipLaunchKernelGGL(gnn_features, dim3(adt::blocks_for(n, 256), n_sys), dim3(256), 0, st, g, d_values, val_stride,
+                       d_work, ws, static_cast<float>(theta));
+    const dim3 nodes(node_blocks(n), n_sys);
+    hipLaunchKernelGGL(gnn_layer0, nodes, dim3(256), 0, st, g, d_params, d_work, ws);
+    for (int l = 1; l <= 3; ++l) hipLaunchKernelGGL(gnn_layer, nodes, dim3(256), 0, st, g, d_params, d_work, ws, l);
+    hipLaunchKernelGGL(gnn_head, dim3(head_blocks(nnz), n_sys), dim3(256), 0, st, g, d_params, d_work, ws, d_omega_p,
+                       d_omega, val_stride);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
+extern "C" int adt_gnn_backward(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
+                                const int32_t* d_rows, const int32_t* d_mirror, double omega_p,
+                                const uint16_t* d_params, const float* d_work, const double* d_domega,
+                                float* d_scratch, float* d_grad, void* stream) {
+    ADT_REQUIRE(n >= 1 && nnz >= n, "adt_gnn_backward: invalid sizes");
+    ADT_REQUIRE(d_indptr && d_indices && d_rows && d_mirror && d_params && d_work && d_domega && d_scratch && d_grad,
+                "adt_gnn_backward: null pointer");
+    hipStream_t st = adt::as_stream(stream);
+    const Graph g = make_graph(n, nnz, d_indptr, d_indices, d_rows, d_mirror, nullptr);
+    Scratch S = scratch_at(d_scratch, n, nnz);
+    const int hb = head_blocks(nnz), nb = node_blocks(n);
+    const int node_channels = adt::blocks_for(static_cast<int64_t>(n) * H, 256);
+    hipLaunchKernelGGL(gnn_head_bwd, dim3(hb), dim3(256), 0, st, g, d_params, d_work, omega_p, d_domega, d_scratch);
+    hipLaunchKernelGGL(gnn_reduce_partials, dim3(adt::blocks_for(HEAD_PARAMS, 256)), dim3(256), 0, st, S.part, hb,
+                       HEAD_PARAMS, d_grad + OFF_U);
+    hipLaunchKernelGGL(gnn_head_to_nodes, dim3(node_channels), dim3(256), 0, st, g, d_scratch);
+    for (int l = 3; l >= 1; --l) {
+        hipLaunchKernelGGL(gnn_layer_bwd, dim3(nb), dim3(256), 0, st, g, d_params, d_work, d_scratch, l);
+        hipLaunchKernelGGL(gnn_reduce_partials, dim3(adt::blocks_for(LAYER_SIZE, 256)), dim3(256), 0, st, S.part, nb,
+                           LAYER_SIZE, d_grad + OFF_L1 + (l - 1) * LAYER_SIZE);
+        hipLaunchKernelGGL(gnn_msg_bwd, dim3(node_channels), dim3(256), 0, st, g, d_work, d_scratch);
+    }
+    hipLaunchKernelGGL(gnn_layer0_bwd, dim3(nb), dim3(256), 0, st, g, d_work, d_scratch);
+    hipLaunchKernelGGL(gnn_reduce_partials, dim3(adt::blocks_for(LAYER0_PARAMS, 256)), dim3(256), 0, st, S.part, nb,
+                       LAYER0_PARAMS, d_grad + OFF_W0);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}

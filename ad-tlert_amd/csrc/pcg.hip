@@ -1479,6 +1479,19 @@ extern "C" int adt_amg_hierarchy_info(int n, const int32_t* h_indptr, const int3
     return ADT_OK;
 }
 
+extern "C" int adt_amg_aggregates(int n, const int32_t* h_indptr, const int32_t* h_indices, const double* h_values,
+                                  double theta, int32_t* h_agg) {
+    ADT_REQUIRE(n >= 1 && h_indptr && h_indices && h_values && h_agg, "adt_amg_aggregates: invalid input");
+    adt::amg::Options opt;
+    opt.theta = theta;
+    opt.coarse_max = 0;
+    opt.max_levels = 2;
+    const auto H = adt::amg::build_hierarchy(n, h_indptr, h_indices, h_values, opt);
+    ADT_REQUIRE(!H.empty(), "adt_amg_aggregates: matrix is not SPD-like (missing or non-positive diagonal)");
+    for (int i = 0; i < n; ++i) h_agg[i] = H[0].agg.empty() ? -1 : H[0].agg[i];
+    return ADT_OK;
+}
+
 extern "C" int adt_pcg_tiling(const adt_pcg_plan* p, int* h_rows_per_tile, int* h_tiles, int* h_lds_rows) {
     ADT_REQUIRE(p, "adt_pcg_tiling: null plan");
     if (h_rows_per_tile) *h_rows_per_tile = p->rb;

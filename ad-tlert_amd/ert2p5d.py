@@ -357,7 +357,7 @@ class ERTForward2p5D:
         self.series_batch_steps = int(series_batch_steps)
         self.sensitivity_precision = option(sensitivity_precision, ("float64", "mixed"), "sensitivity_precision")
         self._solver = BatchedSolver(tolerance=float(solver_tolerance), preconditioner=option(
-            preconditioner, ("amg", "jacobi"), "preconditioner"))
+            preconditioner, ("amg", "jacobi", "learned"), "preconditioner"))
         self._cache: dict = {}
         self._dev_state: _Device | None = None
         self._field_cache_obj = FieldCache(self.normal_field_cache_max_entries)
@@ -370,8 +370,8 @@ class ERTForward2p5D:
                          wavenumbers=None, weights=None, preconditioner: str = "amg") -> "ERTForward2p5D":
         """Build the operator (``ert2p5d.py:132-216``).  ``wavenumbers``/``weights`` override the
         reference quadrature rule; ``solver_tolerance`` is the relative residual of the iterative
-        solves; ``preconditioner`` is ``"amg"`` (multigrid-preconditioned flexible CG) or
-        ``"jacobi"`` (plain Jacobi PCG)."""
+        solves; ``preconditioner`` is ``"amg"`` (multigrid-preconditioned flexible CG),
+        ``"learned"`` (the same with the trained damping network) or ``"jacobi"`` (plain Jacobi PCG)."""
 
         if not torch.cuda.is_available():
             raise RuntimeError("ADTLERT-MI355X requires an AMD GPU with HIP (no device is visible)")

@@ -42,6 +42,12 @@ _SIGNATURES = {
     "adt_pcg_amg_level": (c_int, [P, c_int, c_int, P, P, P]),
     "adt_pcg_stats": (c_int, [P, P, P]),
     "adt_amg_hierarchy_info": (c_int, [c_int, P, P, P, c_double, c_int, c_int, P, P, P]),
+    "adt_amg_aggregates": (c_int, [c_int, P, P, P, c_double, P]),
+    "adt_gnn_param_count": (c_int, []),
+    "adt_gnn_work_floats": (c_i64, [c_int, c_int]),
+    "adt_gnn_scratch_floats": (c_i64, [c_int, c_int]),
+    "adt_gnn_forward": (c_int, [c_int, c_int, c_int, P, P, P, P, P, c_i64, c_double, P, P, P, P, P]),
+    "adt_gnn_backward": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
@@ -198,6 +204,7 @@ class PcgPlan:
                 offset += nnz
             h_val_off[s] = seg_of[key]
         h_values = np.ascontiguousarray(np.concatenate(segments), dtype=np.float64)
+        self.amg_representative = (h_values, h_val_off)
         levels = c_int(0)
         check(self._lib.adt_pcg_attach_amg(self._handle, host(self.host_pattern[0]), host(self.host_pattern[1]),
                                            host(h_values), host(h_val_off), float(theta), int(coarse_max),
@@ -308,3 +315,29 @@ def amg_hierarchy_info(indptr, indices, values, *, theta=0.08, coarse_max=256, m
                                            float(theta), int(coarse_max), int(max_levels), ctypes.byref(levels),
                                            host(rows), host(nnz)))
     return rows[:levels.value].tolist(), nnz[:levels.value].tolist()
+
+
+def amg_aggregates(indptr, indices, values, *, theta=0.08):
+    """Host-only: finest-level aggregates (-1: left out) of one CSR matrix."""
+
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    out = np.zeros(indptr.size - 1, dtype=np.int32)
+    check(library().adt_amg_aggregates(int(indptr.size - 1), host(indptr),
+                                       host(np.ascontiguousarray(indices, dtype=np.int32)),
+                                       host(np.ascontiguousarray(values, dtype=np.float64)), float(theta), host(out)))
+    return out
+
+
+def gnn_forward(graph, values, val_stride, theta, omega_p, params, work, omega):
+    """``omega`` (n_sys rows, ``val_stride`` apart) from matrix values; see ``adt_gnn_forward``."""
+
+    n_sys = int(omega_p.numel())
+    check(library().adt_gnn_forward(n_sys, graph.n, graph.nnz, ptr(graph.indptr), ptr(graph.indices), ptr(graph.rows),
+                                    ptr(graph.flags), ptr(values), int(val_stride), float(theta), ptr(omega_p),
+                                    ptr(params), ptr(work), ptr(omega), stream()))
+
+
+def gnn_backward(graph, omega_p, params, work, domega, scratch, grad):
+    check(library().adt_gnn_backward(graph.n, graph.nnz, ptr(graph.indptr), ptr(graph.indices), ptr(graph.rows),
+                                     ptr(graph.mirror), float(omega_p), ptr(params), ptr(work), ptr(domega),
+                                     ptr(scratch), ptr(grad), stream()))

@@ -54,7 +54,7 @@ def choose_rows_per_tile(n_sys: int, rows: int, r: int, lanes: int, target_tiles
     return -(-rb // per_pass) * per_pass
 
 
-PRECONDITIONERS = ("jacobi", "amg")
+PRECONDITIONERS = ("jacobi", "amg", "learned")
 
 
 @dataclass
@@ -78,6 +78,7 @@ class BatchedSolver:
     amg_theta: float = 0.08
     amg_coarse_max: int = 256
     amg_stall_limit: int = 64
+    damping_network: object = None  # learned.DampingNetwork for preconditioner="learned"
     _plans: dict = field(default_factory=dict)
     last: SolveStats | None = None
     last_plan: native.PcgPlan | None = None
@@ -115,10 +116,12 @@ class BatchedSolver:
         return plan
 
     def _precond(self, plan: native.PcgPlan, values, preconditioner):
+        """(native preconditioner id, name used, per-slot damping array or None)."""
+
         name = self.preconditioner if preconditioner is None else preconditioner
         if name == "jacobi":
-            return native.PRECOND_JACOBI, "jacobi"
-        if name != "amg":
+            return native.PRECOND_JACOBI, "jacobi", None
+        if name not in ("amg", "learned"):
             raise ValueError(f"preconditioner must be one of {PRECONDITIONERS}, got {name!r}")
         if plan.amg_levels == 0 and not getattr(plan, "amg_failed", False):
             try:
@@ -128,8 +131,36 @@ class BatchedSolver:
                 # no usable hierarchy (tiny or unstructured matrices): plain Jacobi PCG for this plan
                 plan.amg_failed = True
         if plan.amg_levels == 0:
-            return native.PRECOND_JACOBI, "jacobi"
-        return native.PRECOND_AMG, "amg"
+            return native.PRECOND_JACOBI, "jacobi", None
+        if name == "amg":
+            return native.PRECOND_AMG, "amg", None
+        return native.PRECOND_LEARNED, "learned", self._learned_damping(plan, values)
+
+    def _learned_damping(self, plan: native.PcgPlan, values):
+        from . import learned
+
+        if self.damping_network is None:
+            if not learned.DEFAULT_WEIGHTS.exists():
+                raise RuntimeError(f"no trained damping network at {learned.DEFAULT_WEIGHTS}")
+            self.damping_network = learned.DampingNetwork.load()
+        graph = getattr(plan, "gnn_graph", None)
+        if graph is None:
+            same = np.all(plan.ptr_off == plan.ptr_off[0]) and np.all(plan.idx_off == plan.idx_off[0]) and \
+                np.all(plan.rows == plan.rows[0])
+            nnz = int(plan.host_pattern[0][int(plan.ptr_off[0]) + int(plan.rows[0])])
+            if not same or not np.array_equal(plan.val_off, plan.val_off[0] + np.arange(plan.n_sys) * nnz):
+                raise ValueError("the learned preconditioner needs systems that share one CSR pattern")
+            p0, i0, n = int(plan.ptr_off[0]), int(plan.idx_off[0]), int(plan.rows[0])
+            indptr = plan.host_pattern[0][p0:p0 + n + 1] - plan.host_pattern[0][p0]
+            indices = plan.host_pattern[1][i0:i0 + nnz]
+            rep_values, rep_off = plan.amg_representative
+            agg = native.amg_aggregates(indptr, indices, rep_values[rep_off[0]:rep_off[0] + nnz], theta=self.amg_theta)
+            graph = learned.GnnGraph.from_pattern(indptr, indices, agg, values.device)
+            plan.gnn_graph = graph
+            plan.gnn_omega_p = torch.as_tensor([plan.amg_level(0, s)["omega"] for s in range(plan.n_sys)],
+                                               dtype=torch.float64, device=values.device)
+        flat = values.reshape(-1)[int(plan.val_off[0]):]
+        return self.damping_network.omega(graph, flat, plan.gnn_omega_p, val_stride=graph.nnz)
 
     def solve(self, plan: native.PcgPlan, indptr, indices, values, b, x, *, tol=None, max_iter=None,
               preconditioner=None, raise_on_failure=True):
@@ -137,10 +168,10 @@ class BatchedSolver:
 
         tol = self.tolerance if tol is None else tol
         max_iter = self.max_iterations if max_iter is None else max_iter
-        precond, used = self._precond(plan, values, preconditioner)
+        precond, used, data = self._precond(plan, values, preconditioner)
         t0 = time.perf_counter()
         rc, iters, relres = plan.solve(indptr, indices, values, b, x, tol, max_iter, precond=precond,
-                                       check_every=self.check_every)
+                                       precond_data=data, check_every=self.check_every)
         fallback = plan.stats()["fallback_rhs"] if precond != native.PRECOND_JACOBI else 0
         self.last = SolveStats(iters, relres, time.perf_counter() - t0, used, fallback)
         self.last_plan = plan

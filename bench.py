@@ -204,6 +204,8 @@ def main() -> int:
     ap.add_argument("--rows-per-tile", type=int, default=None, help="PCG workgroup tile height (default: auto)")
     ap.add_argument("--preconditioner", choices=("amg", "jacobi"), default="amg")
     ap.add_argument("--pcg-steps", type=int, default=2, help="timed steps of plain Jacobi PCG on the same batches (0: skip)")
+    ap.add_argument("--learned-steps", type=int, default=2,
+                    help="timed steps of flexible CG with the trained damping network on the same batches (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -219,7 +221,7 @@ def main() -> int:
     adt = importlib.import_module("ad-tlert_amd")
     dev = adt.native.device()
     cus, arch = adt.native.device_info(0)
-    models = (max(args.warmup + args.steps, 1 + args.pcg_steps) + 1) * args.batch_steps
+    models = (max(args.warmup + args.steps, 1 + args.pcg_steps, 1 + args.learned_steps) + 1) * args.batch_steps
     t0 = time.perf_counter()
     wl = (Timelapse if args.workload == "timelapse" else Box3D)(adt, torch, args, models)
     op = wl.op
@@ -259,6 +261,11 @@ def main() -> int:
         pcg_per_s = args.pcg_steps * wl.rhs_per_step / pcg_s
     elif args.preconditioner == "jacobi":
         pcg_per_s, pcg_iters = rhs_per_s, mean_iters
+    learned_per_s = learned_iters = None
+    if args.learned_steps > 0 and args.preconditioner == "amg":
+        learned_s, learned_iters, learned_relres, _ = timed("learned", 0, args.learned_steps, 1)
+        learned_per_s = args.learned_steps * wl.rhs_per_step / learned_s
+        pcg_relres = max(pcg_relres, learned_relres)
     op._solver.preconditioner = args.preconditioner
     # a copy of known size (256 MiB read + 256 MiB written): hardware-counter runs of this command
     # scale the solver kernels' FETCH_SIZE / WRITE_SIZE by what the counters read for it
@@ -318,7 +325,8 @@ def main() -> int:
                    "speedup_vs_pcg": rhs_per_s / pcg_per_s if pcg_per_s else None,
                    "mean_iterations": mean_iters, "mean_iterations_pcg": pcg_iters,
                    "worst_true_relres": max(worst_relres, pcg_relres), "jacobi_fallback_rhs": fallback,
-                   "rhs_solves_per_s_learned_fcg": None},
+                   "rhs_solves_per_s_learned_fcg": learned_per_s, "mean_iterations_learned_fcg": learned_iters,
+                   "speedup_learned_fcg_vs_pcg": learned_per_s / pcg_per_s if learned_per_s and pcg_per_s else None},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": dominant[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dominant[1] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": dominant[2] / launches, "avg_launch_ms": dominant[3] / launches,
@@ -330,7 +338,7 @@ def main() -> int:
                           "iterations_profiled": prof["iterations"]},
     }
     log(f"timed: {elapsed:.2f}s, {rhs_per_s:.0f} RHS/s ({args.preconditioner}); plain PCG "
-        f"{pcg_per_s or 0:.0f} RHS/s; profiled step {profiled_s:.2f}s; cycle {prof['cycle_ms'] / launches:.3f} ms, "
+        f"{pcg_per_s or 0:.0f} RHS/s; learned FCG {learned_per_s or 0:.0f} RHS/s; profiled step {profiled_s:.2f}s; cycle {prof['cycle_ms'] / launches:.3f} ms, "
         f"spmv {spmv_gbs:.0f} GB/s ({prof['spmv_ms'] / launches:.3f} ms), "
         f"update {update_gbs:.0f} GB/s ({prof['update_ms'] / launches:.3f} ms)")
 

@@ -223,12 +223,40 @@ int adt_aggregate_columns(int n_rows, int64_t in_cols, int n_out, const double* 
                           const int32_t* d_col_ptr, const int32_t* d_col_idx, double* d_out,
                           void* stream);
 
+/* Host-only: finest-level aggregate of every row (-1: left out) for strength threshold theta,
+ * the aggregation adt_pcg_attach_amg uses for these values.  Needs no GPU. */
+int adt_amg_aggregates(int n, const int32_t* h_indptr, const int32_t* h_indices, const double* h_values,
+                       double theta, int32_t* h_agg);
+
 /* ------------------------------------------------------------------------------------------
- * Learned preconditioner: a 3-layer graph message-passing network on the matrix graph
- * (hidden width 64, BF16 weights, FP32 accumulation).  See adt_gnn.h-level description in
- * the package documentation; the opaque handle is created from host weights.
+ * Learned multigrid damping (no reference counterpart; it feeds ADT_PRECOND_LEARNED).
+ *
+ * A 3-layer graph message-passing network on the matrix graph (64 channels, bfloat16 weights,
+ * FP32 activations and sums) maps each matrix to a damping factor per slot for the finest
+ * prolongator:  x_i = [log10(a_ii / mean a), sum_{j!=i}|a_ij| / a_ii, sum_j a_ij / a_ii, 1],
+ * w_ij = |a_ij| / sum_{j'!=i}|a_ij'|, h0 = relu(x W0 + b0), h_l = relu([h_{l-1}, W_msg h_{l-1}] W_l + b_l)
+ * (l = 1..3), z_k = [h3_i, h3_j, a_ij / sqrt(a_ii a_jj), [i==j], same aggregate, strong],
+ * omega_k = omega_p (1 + tanh(relu(z_k U + c) . v + d) / 2).
+ * Parameters are packed (W0, b0, W1, b1, W2, b2, W3, b3, U, c, v, d), row-major, as bfloat16
+ * bit patterns (uint16); adt_gnn_param_count gives their number.  The pattern must be
+ * symmetric: d_rows gives each slot's row, d_mirror the slot of (j, i) for slot (i, j),
+ * d_flags bit 1 marks slots whose two rows share an aggregate.  Systems share the pattern and
+ * are val_stride apart in d_values / d_omega; d_omega_p holds each system's constant damping.
+ * d_work holds n_sys * adt_gnn_work_floats(n, nnz) floats of activations, kept for the backward
+ * pass (one system): given d_domega = dL/domega it writes the FP32 parameter gradients
+ * (packed like the parameters) using adt_gnn_scratch_floats(n, nnz) floats of scratch.
+ * Every sum runs in a fixed order (no atomics): results are bitwise repeatable.
  * ---------------------------------------------------------------------------------------- */
-typedef struct adt_gnn adt_gnn;
+int adt_gnn_param_count(void);
+int64_t adt_gnn_work_floats(int n, int nnz);
+int64_t adt_gnn_scratch_floats(int n, int nnz);
+int adt_gnn_forward(int n_sys, int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
+                    const int32_t* d_rows, const int32_t* d_flags, const double* d_values, int64_t val_stride,
+                    double theta, const double* d_omega_p, const uint16_t* d_params, float* d_work,
+                    double* d_omega, void* stream);
+int adt_gnn_backward(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices, const int32_t* d_rows,
+                     const int32_t* d_mirror, double omega_p, const uint16_t* d_params, const float* d_work,
+                     const double* d_domega, float* d_scratch, float* d_grad, void* stream);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ R = -1
 
 def test_c_abi_library_exports_every_declared_function():
     header = (ROOT / "include" / "adt_hip.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(adt_\w+)\s*\(", header, flags=re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(adt_\w+)\s*\(", header, flags=re.M))
     assert declared, "no declarations parsed"
     lib = adt.native.library()
     missing = [name for name in declared if not hasattr(lib, name)]
@@ -349,3 +349,21 @@ def test_multigrid_hierarchy_leaves_unconnected_rows_out_and_rejects_bad_diagona
     bad.data[bad.indptr[3]:bad.indptr[4]][bad.indices[bad.indptr[3]:bad.indptr[4]] == 3] = -1.0
     with pytest.raises(ValueError, match="SPD"):
         adt.native.amg_hierarchy_info(bad.indptr, bad.indices, bad.data)
+
+
+def test_mirror_slots_of_symmetric_patterns_and_rejection_of_others():
+    import importlib
+
+    import scipy.sparse as sp
+
+    learned = importlib.import_module("ad-tlert_amd.learned")
+    A = sp.random(40, 40, density=0.1, random_state=3, format="csr")
+    A = (A + A.T + sp.identity(40)).tocsr()
+    A.sort_indices()
+    mirror = learned.mirror_slots(A.indptr, A.indices)
+    rows = np.repeat(np.arange(40), np.diff(A.indptr))
+    assert np.array_equal(rows[mirror], A.indices) and np.array_equal(A.indices[mirror], rows)
+    assert np.array_equal(mirror[mirror], np.arange(A.nnz))
+    B = sp.csr_matrix(np.triu(np.ones((5, 5))))
+    with pytest.raises(ValueError, match="symmetric"):
+        learned.mirror_slots(B.indptr, B.indices)

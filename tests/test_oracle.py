@@ -315,3 +315,40 @@ def test_3d_jacobian_fd_dot_tests_poles_and_orders(box):
     p2 = ert3d.Forward3D(box.nodes, box.cells, box.electrodes, box.measurements, element_order=2)
     r1, r2 = f.apparent_resistivity(s), p2.apparent_resistivity(s)
     assert np.allclose(r2, r1, rtol=0.15) and not np.allclose(r2, r1, rtol=1e-9)
+
+
+def test_damping_network_oracle_backward_matches_central_differences(monkeypatch):
+    from oracle import gnn
+    from tests.test_gpu_kernels import random_spd_batch
+
+    rng = np.random.default_rng(0)
+    indptr, indices, values, _ = random_spd_batch(rng, side=6, S=1)
+    flags = (rng.integers(0, 2, indices.size) << 1).astype(np.int64)
+    flat = rng.standard_normal(gnn.PARAM_COUNT) * 0.2
+    monkeypatch.setattr(gnn, "bf16_round", lambda x: np.asarray(x, dtype=np.float64))
+    omega, cache = gnn.forward(flat, indptr, indices, values[0], flags, 0.6)
+    domega = rng.standard_normal(omega.size)
+    grad = gnn.backward(cache, domega)
+    for i in rng.choice(gnn.PARAM_COUNT, 60, replace=False):
+        e = np.zeros_like(flat)
+        e[i] = 1e-6
+        fp = gnn.forward(flat + e, indptr, indices, values[0], flags, 0.6)[0] @ domega
+        fm = gnn.forward(flat - e, indptr, indices, values[0], flags, 0.6)[0] @ domega
+        assert abs((fp - fm) / 2e-6 - grad[i]) < 1e-7 + 1e-6 * abs(grad[i])
+
+
+def test_damping_network_oracle_rounds_like_torch_bfloat16_and_is_constant_without_output_layer():
+    import torch
+
+    from oracle import gnn
+
+    x = np.random.default_rng(1).standard_normal(10000) * 10.0 ** np.random.default_rng(2).uniform(-8, 8, 10000)
+    assert np.array_equal(gnn.bf16_round(x), torch.as_tensor(x, dtype=torch.float32).to(torch.bfloat16).double().numpy())
+    from tests.test_gpu_kernels import random_spd_batch
+
+    indptr, indices, values, _ = random_spd_batch(np.random.default_rng(3), side=5, S=1)
+    p = gnn.unpack(np.random.default_rng(4).standard_normal(gnn.PARAM_COUNT))
+    p["v"][:] = 0.0
+    p["d"][:] = 0.0
+    omega, _ = gnn.forward(gnn.pack(p), indptr, indices, values[0], np.zeros(indices.size, np.int64), 0.7)
+    assert np.all(omega == 0.7)
