@@ -337,6 +337,16 @@ def main() -> int:
         "profiled_step": {"seconds": profiled_s, "rhs_solves_per_s": wl.rhs_per_step / profiled_s,
                           "iterations_profiled": prof["iterations"]},
     }
+    # HBM traffic of the dominant kernel from the committed hardware-counter runs of this same
+    # configuration (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known size)
+    traffic_file = ROOT / "profiles" / "timelapse_cg_spmv_traffic.json"
+    if args.workload == "timelapse" and dominant[0] == "cg_spmv" and traffic_file.exists():
+        t = json.loads(traffic_file.read_text())
+        c = t["config"]
+        if (c["nodes"], c["nnz"], c["rp"], c["time_steps_per_step"], c["preconditioner"]) == \
+                (N, nnz, plan.rp, args.batch_steps, args.preconditioner):
+            result["roofline"]["traffic"] = t["traffic_bytes_per_dispatch"]
+            result["roofline"]["traffic_source"] = f"profiles/{traffic_file.name}"
     log(f"timed: {elapsed:.2f}s, {rhs_per_s:.0f} RHS/s ({args.preconditioner}); plain PCG "
         f"{pcg_per_s or 0:.0f} RHS/s; learned FCG {learned_per_s or 0:.0f} RHS/s; profiled step {profiled_s:.2f}s; cycle {prof['cycle_ms'] / launches:.3f} ms, "
         f"spmv {spmv_gbs:.0f} GB/s ({prof['spmv_ms'] / launches:.3f} ms), "
