@@ -2,6 +2,7 @@
 #include "adt_common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 namespace {
@@ -98,6 +99,120 @@ int launch_sensitivity(const SensArgs& a, int precision, hipStream_t st) {
         hipLaunchKernelGGL((sensitivity_kernel<K, float>), grid, dim3(256), 0, st, a);
     else
         hipLaunchKernelGGL((sensitivity_kernel<K, double>), grid, dim3(256), 0, st, a);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// (c') Gram form of the normal sensitivity on interleaved (N, Rp) fields, for many wavenumbers:
+//   G_c[e][f] = sum_w sum_i phi_w[e, dof_ci] (T_wc phi_w[f, dof_c.])_i,   T_wc = ws_w K_c + wm_w M_c
+//   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])          (row S: remote, zero)
+// which equals the per-output formula of sensitivity_kernel; the element templates are symmetric
+// (stiffness and mass), so G_c is symmetric and only its upper triangle is kept.  One workgroup
+// per cell: 16 x 16 threads each own a B x B block of G_c (S <= 16 B); the cell's field rows
+// Phi[(w, i)][e] (one contiguous interleaved row per node) and Psi = T Phi are staged in LDS in
+// chunks of wavenumbers; then the upper triangle of G_c replaces the staging area in LDS and
+// the workgroup writes column c of every measurement row.  Cells are mapped XCD-contiguously,
+// so neighbouring columns of a row are written by workgroups that share one L2.  FP64 and
+// fixed orders: k = (w, i) then j; bitwise repeatable.
+// ------------------------------------------------------------------------------------------
+
+struct GramArgs {
+    const double* x;
+    const int32_t* dofs;
+    const double* stiff;
+    const double* mass;
+    const int32_t* a;
+    const int32_t* b;
+    const int32_t* m;
+    const int32_t* n;
+    double* out;
+    int S, Sp, K, W, rp, wchunk, C, D;
+    int64_t vec_off[kMaxWaves];
+    double ws[kMaxWaves];
+    double wm[kMaxWaves];
+};
+
+__device__ __forceinline__ int tri_index(int e, int f, int S) {  // e <= f
+    return e * S - (e * (e - 1)) / 2 + (f - e);
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
+    extern __shared__ double lds[];
+    const int c = adt::xcd_tile(static_cast<int>(blockIdx.x), A.C);
+    const int K = A.K, Sp = A.Sp, S = A.S;
+    const int rows_max = A.wchunk * K;
+    double* phi = lds;                                        // rows_max x Sp
+    double* psi = lds + static_cast<int64_t>(rows_max) * Sp;  // rows_max x Sp
+    const int te = static_cast<int>(threadIdx.x) / 16, tf = static_cast<int>(threadIdx.x) % 16;
+    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
+    double acc[B][B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) acc[i][j] = 0.0;
+    for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
+        const int nw = min(A.wchunk, A.W - w0);
+        const int rows = nw * K;
+        __syncthreads();
+        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
+            const int r = t / Sp, e = t % Sp;
+            const int w = w0 + r / K, i = r % K;
+            phi[t] = e < S ? A.x[A.vec_off[w] + static_cast<int64_t>(dof[i]) * A.rp + e] : 0.0;
+        }
+        __syncthreads();
+        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
+            const int r = t / Sp, f = t % Sp;
+            const int wl = r / K, i = r % K, w = w0 + wl;
+            double v = 0.0;
+            for (int j = 0; j < K; ++j) {
+                const double tij = Mc ? fma(A.wm[w], Mc[i * K + j], A.ws[w] * Kc[i * K + j]) : A.ws[w] * Kc[i * K + j];
+                v = fma(tij, phi[(wl * K + j) * Sp + f], v);
+            }
+            psi[t] = v;
+        }
+        __syncthreads();
+        for (int r = 0; r < rows; ++r) {
+            double av[B], bv[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) av[i] = phi[r * Sp + te * B + i];
+#pragma unroll
+            for (int j = 0; j < B; ++j) bv[j] = psi[r * Sp + tf * B + j];
+#pragma unroll
+            for (int i = 0; i < B; ++i)
+#pragma unroll
+                for (int j = 0; j < B; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+        }
+    }
+    __syncthreads();
+    double* G = lds;  // upper triangle of G_c, S (S + 1) / 2 entries
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int e = te * B + i;
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int f = tf * B + j;
+            if (e < S && f < S && e <= f) G[tri_index(e, f, S)] = acc[i][j];
+        }
+    }
+    __syncthreads();
+    auto g = [&](int e, int f) {
+        if (e >= S || f >= S) return 0.0;
+        return e <= f ? G[tri_index(e, f, S)] : G[tri_index(f, e, S)];
+    };
+    for (int d = static_cast<int>(threadIdx.x); d < A.D; d += 256) {
+        const int ia = A.a[d], ib = A.b[d], im = A.m[d], in = A.n[d];
+        const double v = ((g(im, ia) - g(im, ib)) - g(in, ia)) + g(in, ib);
+        A.out[static_cast<int64_t>(d) * A.C + c] = -v;
+    }
+}
+
+template <int B>
+int launch_gram(GramArgs& A, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL(gram_kernel<B>, dim3(A.C), dim3(256), lds, st, A);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -419,4 +534,62 @@ extern "C" int adt_aggregate_columns(int n_rows, int64_t in_cols, int n_out, con
                        d_col_idx, d_out, n_out);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
+}
+
+extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x,
+                                    int n_meas, int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b,
+                                    const int32_t* d_m, const int32_t* d_n, const int32_t* d_cell_dofs,
+                                    const double* d_stiff, const double* d_mass, const double* h_ws,
+                                    const double* h_wm, double* d_out, void* stream) {
+    ADT_REQUIRE(n_waves >= 1 && n_waves <= kMaxWaves, "adt_sensitivity_gram: 1 <= n_waves <= %d", kMaxWaves);
+    ADT_REQUIRE(s_rows >= 1 && s_rows <= 128 && s_rows <= rp, "adt_sensitivity_gram: 1 <= sources <= min(128, rp)");
+    ADT_REQUIRE(k_local >= 1 && k_local <= 10, "adt_sensitivity_gram: unsupported local size %d", k_local);
+    ADT_REQUIRE(h_vec_off && h_ws && (d_mass == nullptr || h_wm), "adt_sensitivity_gram: missing host arrays");
+    if (n_meas == 0 || n_cells == 0) return ADT_OK;
+    ADT_REQUIRE(d_x && d_a && d_b && d_m && d_n && d_cell_dofs && d_stiff && d_out, "adt_sensitivity_gram: null pointer");
+    GramArgs A{};
+    A.x = d_x;
+    A.dofs = d_cell_dofs;
+    A.stiff = d_stiff;
+    A.mass = d_mass;
+    A.a = d_a;
+    A.b = d_b;
+    A.m = d_m;
+    A.n = d_n;
+    A.out = d_out;
+    A.S = s_rows;
+    const int B = (s_rows + 15) / 16;
+    A.Sp = 16 * B;
+    A.K = k_local;
+    A.W = n_waves;
+    A.rp = rp;
+    A.C = n_cells;
+    A.D = n_meas;
+    // wavenumbers per LDS chunk: the two staged (rows x Sp) blocks fit the budget (default 16 KB,
+    // several workgroups per CU); the Gram triangle reuses the same LDS afterwards
+    static const int lds_budget = [] {
+        const char* e = std::getenv("ADT_GRAM_LDS_BYTES");
+        const int v = e ? std::atoi(e) : 16384;
+        return std::min(65536, std::max(4096, v));
+    }();
+    A.wchunk = std::max(1, std::min(n_waves, (lds_budget / (2 * 8 * A.Sp)) / k_local));
+    const size_t staging = static_cast<size_t>(2) * A.wchunk * k_local * A.Sp * 8;
+    const size_t tri = static_cast<size_t>(s_rows) * (s_rows + 1) / 2 * 8;
+    const size_t lds = std::max(staging, tri);
+    for (int w = 0; w < n_waves; ++w) {
+        A.vec_off[w] = h_vec_off[w];
+        A.ws[w] = h_ws[w];
+        A.wm[w] = d_mass ? h_wm[w] : 0.0;
+    }
+    hipStream_t st = adt::as_stream(stream);
+    switch (B) {
+        case 1: return launch_gram<1>(A, lds, st);
+        case 2: return launch_gram<2>(A, lds, st);
+        case 3: return launch_gram<3>(A, lds, st);
+        case 4: return launch_gram<4>(A, lds, st);
+        case 5: return launch_gram<5>(A, lds, st);
+        case 6: return launch_gram<6>(A, lds, st);
+        case 7: return launch_gram<7>(A, lds, st);
+        default: return launch_gram<8>(A, lds, st);
+    }
 }

@@ -764,14 +764,26 @@ class ERTForward2p5D:
         """``-sum_w w_w (u_M - u_N)^T (K + k_w^2 M) (u_A - u_B)`` by the fused kernel (``ert2p5d.py:984-1010``)."""
 
         dev = self._dev
-        phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
         stiff = columns.select(dev.stiff, 0).contiguous()
         mass = columns.select(dev.mass, 0).contiguous()
-        dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
         ws, wm = dev.q_host, dev.q_host * dev.k2_host
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         D = self.survey.measurement_count
         cols = columns.count
+        if dev.E <= 128 and precision == 0:
+            # Gram form on the interleaved device-order blocks: per cell G_c = Phi^T T Phi once,
+            # then four gathers per row (FP64; the mixed mode keeps the per-output FP32 kernel)
+            dofs = columns.select(dev.cell_dofs, 0).contiguous()
+            blocks = fields.blocks.contiguous()
+            n_cells = dofs.shape[0]
+            part = torch.empty((D, n_cells), dtype=torch.float64, device=dev.device)
+            vec_off = np.arange(dev.W, dtype=np.int64) * dev.N * blocks.shape[-1]
+            native.sensitivity_gram(blocks, vec_off, blocks.shape[-1], dev.E, dev.a, dev.b, dev.m, dev.n, dofs, stiff,
+                                    mass, ws, wm, part)
+            result = columns.columns(part)
+            return result if return_device else result.cpu().to(FLOAT_DTYPE)
+        phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
+        dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
         result = torch.empty((D, cols), dtype=torch.float64, device=dev.device)
         for start in range(0, D, batch_size):
             stop = min(D, start + batch_size)

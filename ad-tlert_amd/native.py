@@ -49,6 +49,8 @@ _SIGNATURES = {
     "adt_gnn_forward": (c_int, [c_int, c_int, c_int, P, P, P, P, P, c_i64, c_double, P, P, P, P, P]),
     "adt_gnn_backward": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
+    "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
+                                     P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
     "adt_electrode_potentials": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P]),
@@ -260,6 +262,19 @@ def sensitivity(phi, a, b, m, n, cell_dofs, stiff, mass, ws, wm, out, precision=
     wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
     check(library().adt_sensitivity(W, S, N, D, C, K, ptr(phi), ptr(a), ptr(b), ptr(m), ptr(n), ptr(cell_dofs),
                                     ptr(stiff), ptr(mass), host(ws), host(wm), int(precision), ptr(out), stream()))
+
+
+def sensitivity_gram(x, vec_off, rp, s_rows, a, b, m, n, cell_dofs, stiff, mass, ws, wm, out):
+    """Jacobian rows from interleaved fields in Gram form (see ``adt_sensitivity_gram``)."""
+
+    W = len(vec_off)
+    C, K = cell_dofs.shape
+    vec_off = np.ascontiguousarray(vec_off, dtype=np.int64)
+    ws = np.ascontiguousarray(ws, dtype=np.float64)
+    wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
+    check(library().adt_sensitivity_gram(W, int(rp), int(s_rows), host(vec_off), ptr(x), int(a.numel()), C, K,
+                                         ptr(a), ptr(b), ptr(m), ptr(n), ptr(cell_dofs), ptr(stiff), ptr(mass),
+                                         host(ws), host(wm), ptr(out), stream()))
 
 
 def cell_gradient(lam, rp_lam, lam_off, phi, rp_phi, phi_off, r, cell_dofs, stiff, mass, ws, wm, out, *,

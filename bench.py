@@ -109,10 +109,11 @@ class Timelapse:
         def run():
             return op._normal_jacobian(fields[0], self.D, columns, return_device=True)
 
-        K = 3
-        bytes_ = 8 * self.D * self.C + 8 * self.W * self.E * self.N + 8 * 2 * self.C * K * K + 4 * self.C * K + 16 * self.D
-        flops = self.D * self.C * self.W * (4 * K * K + 6 * K)
-        return run, bytes_, flops, "sensitivity_kernel<3,double>"
+        # Gram form: per cell G_c = Phi^T (T Phi) over W*K rows of E sources, then 4 reads per output
+        K, S, rp = 3, self.E, self.op._dev.rp
+        bytes_ = 8 * self.D * self.C + 8 * self.W * self.N * rp + 8 * 2 * self.C * K * K + 4 * self.C * K + 16 * self.D
+        flops = self.C * (2 * self.W * K * S * S + 2 * self.W * K * K * S) + 4 * self.D * self.C
+        return run, bytes_, flops, "gram_kernel<6> (Gram-form sensitivity)"
 
     def cpu(self):
         args = self.args

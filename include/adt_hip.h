@@ -179,6 +179,20 @@ int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_c
                     const double* d_mass, const double* h_ws, const double* h_wm, int precision,
                     double* d_out, void* stream);
 
+/* Same Jacobian rows from interleaved fields, in Gram form (for many wavenumbers):
+ *   G_c[e][f] = sum_w phi_w[e, dofs_c]^T (ws[w] stiff_c + wm[w] mass_c) phi_w[f, dofs_c]
+ *   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])      (index s_rows = remote: 0)
+ * phi_w is the (N, rp) block at d_x + h_vec_off[w]; its first s_rows (<= 128) columns are the
+ * sources; d_cell_dofs index its rows.  The element templates must be symmetric (stiffness and
+ * mass templates are), so G_c is symmetric and one workgroup per cell keeps its upper triangle
+ * in LDS (no workspace).
+ * FP64, every sum in a fixed order: bitwise repeatable.  out is (D, C) row-major.  Replaces the
+ * same reference kernel as adt_sensitivity. */
+int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x, int n_meas,
+                         int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
+                         const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff, const double* d_mass,
+                         const double* h_ws, const double* h_wm, double* d_out, void* stream);
+
 /* ------------------------------------------------------------------------------------------
  * (d) Adjoint cell gradient:  g[c] = - sum_w sum_s lam_w[:, s]^T (ws[w] K_c + wm[w] M_c) phi_w[:, s]
  *     (+ Robin edges:  - sum_w sum_s lam_w[e]^T (balpha[w,b] * bws[w] * Mb_b) phi_w[e]).

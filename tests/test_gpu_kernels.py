@@ -216,3 +216,30 @@ def test_lane_groups_of_many_tiles_use_segmented_sums(precond):
     the segment pre-reduction; results must still meet the tolerance and match dense solves."""
 
     solve_batch(precond, True, side=48, lanes=64, R=64, seed=5, rows_per_tile=4)
+
+
+@pytest.mark.parametrize("k, S, rp", [(3, 9, 16), (4, 40, 64), (10, 128, 128), (3, 96, 96)])
+def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp):
+    """Gram-form Jacobian rows from interleaved (N, Rp) blocks equal the per-output formula
+    (sources 9..128, local sizes 3..10, padding columns ignored); bitwise repeatable.  Element
+    templates are symmetric (as every stiffness and mass template is), which the Gram form uses."""
+
+    phi, a, b, m, n, cells, stiff, mass, ws, wm = random_problem(k, W=7, S=S, N=90, C=53, D=41, seed=k + S)
+    stiff, mass = 0.5 * (stiff + stiff.transpose(1, 2)), 0.5 * (mass + mass.transpose(1, 2))
+    problem = (phi, a, b, m, n, cells, stiff, mass, ws, wm)
+    expected = einsum_reference(*problem)
+    W, _, N = phi.shape
+    blocks = torch.zeros((W, N, rp), dtype=torch.float64)
+    blocks[:, :, :S] = phi.transpose(1, 2)
+    blocks[:, :, S:] = 7.0  # padding columns must not be read
+    g = lambda t, dt=None: t.to(dev(), dtype=dt or t.dtype).contiguous()  # noqa: E731
+
+    def run():
+        out = torch.empty((a.numel(), cells.shape[0]), dtype=torch.float64, device=dev())
+        native.sensitivity_gram(g(blocks), np.arange(W) * N * rp, rp, S, *(g(x, torch.int32) for x in (a, b, m, n)),
+                                g(cells, torch.int32), g(stiff), g(mass), ws.numpy(), wm.numpy(), out)
+        return out
+
+    out = run()
+    assert relmax(out.cpu(), expected) < 1e-12
+    assert torch.equal(run(), out)
