@@ -34,6 +34,24 @@ int set_error(int code, const char* fmt, ...);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Global-memory view of a device pointer.  Pointers that kernels read out of argument blocks
+// in device memory are generic, and accesses through generic pointers compile to flat_*
+// instructions, which count against both the vector-memory and the LDS/scalar counters: every
+// wait for an LDS or scalar result then also waits for all loads in flight.  Accesses through
+// the view compile to global_* instructions.  Only for pointers into global memory.  (The host
+// compilation pass, which only parses kernel bodies, sees the identity.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p) {
+    return (__attribute__((address_space(1))) T*)(p);
+}
+#else
+template <typename T>
+__device__ __forceinline__ T* gview(T* p) {
+    return p;
+}
+#endif
+
 // Workgroup b runs on XCD b % 8.  Remap so that each XCD receives a contiguous range of
 // tiles (neighbouring row ranges of one system share its L2).  Bijective for any count.
 __device__ __forceinline__ int xcd_tile(int b, int n) {

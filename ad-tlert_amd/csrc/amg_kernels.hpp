@@ -30,25 +30,25 @@ namespace krylov {
 // with omega_k the level's constant damping or, when given, the learned per-slot damping.
 __global__ __launch_bounds__(256) void amg_pvals(const LevelArgs* __restrict__ la) {
     const LevelArgs& a = *la;
-    const LvSys S = a.ls[blockIdx.y];
+    const LvSys S = adt::gview(a.ls)[blockIdx.y];
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= S.rows) return;
-    const double* V = a.val + S.val;
-    const int dslot = a.diag[S.diag + i];
-    if (a.dinv_out) a.dinv_out[S.dinv + i] = 1.0 / V[dslot];
+    const auto V = adt::gview(a.val + S.val);
+    const int dslot = adt::gview(a.diag)[S.diag + i];
+    if (a.dinv_out) adt::gview(a.dinv_out)[S.dinv + i] = 1.0 / V[dslot];
     if (S.nc == 0) return;
-    const int* dmap_ptr = a.dmap_ptr + S.dmap_ptr;
-    const int* dmap = a.dmap + S.dmap;
+    const auto dmap_ptr = adt::gview(a.dmap_ptr + S.dmap_ptr);
+    const auto dmap = adt::gview(a.dmap + S.dmap);
     double df = 0.0;
     for (int m = dmap_ptr[i]; m < dmap_ptr[i + 1]; ++m) df += V[dmap[m]];
     const double dfinv = 1.0 / (df > 0.0 ? df : V[dslot]);
-    const int* pptr = a.pptr + S.pptr;
-    const int* pmap_ptr = a.pmap_ptr + S.pmap_ptr;
-    const int* pmap = a.pmap + S.pmap;
-    const int* pself = a.pself + S.pidx;
-    const double* om = a.omega_slot ? a.omega_slot + S.val : nullptr;
+    const auto pptr = adt::gview(a.pptr + S.pptr);
+    const auto pmap_ptr = adt::gview(a.pmap_ptr + S.pmap_ptr);
+    const auto pmap = adt::gview(a.pmap + S.pmap);
+    const auto pself = adt::gview(a.pself + S.pidx);
+    const auto om = adt::gview(a.omega_slot ? a.omega_slot + S.val : nullptr);
     const double wself = om ? om[dslot] : S.omega_p;
-    double* P = a.pval + S.pval;
+    const auto P = adt::gview(a.pval + S.pval);
     for (int t = pptr[i]; t < pptr[i + 1]; ++t) {
         double acc = 0.0;
         for (int m = pmap_ptr[t]; m < pmap_ptr[t + 1]; ++m) {
@@ -62,33 +62,33 @@ __global__ __launch_bounds__(256) void amg_pvals(const LevelArgs* __restrict__ l
 // Thread per slot of T = A P.
 __global__ __launch_bounds__(256) void amg_tvals(const LevelArgs* __restrict__ la) {
     const LevelArgs& a = *la;
-    const LvSys S = a.ls[blockIdx.y];
+    const LvSys S = adt::gview(a.ls)[blockIdx.y];
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= S.n_t) return;
-    const int* tp = a.tpair_ptr + S.tpair_ptr;
-    const int* ta = a.tpair_a + S.tpair;
-    const int* tq = a.tpair_p + S.tpair;
-    const double* V = a.val + S.val;
-    const double* P = a.pval + S.pval;
+    const auto tp = adt::gview(a.tpair_ptr + S.tpair_ptr);
+    const auto ta = adt::gview(a.tpair_a + S.tpair);
+    const auto tq = adt::gview(a.tpair_p + S.tpair);
+    const auto V = adt::gview(a.val + S.val);
+    const auto P = adt::gview(a.pval + S.pval);
     double acc = 0.0;
     for (int m = tp[t]; m < tp[t + 1]; ++m) acc = fma(V[ta[m]], P[tq[m]], acc);
-    a.tval[S.tval + t] = acc;
+    adt::gview(a.tval)[S.tval + t] = acc;
 }
 
 // Thread per slot of the next level's matrix: A_{l+1}[J, K] = sum_i P[i, J] T[i, K].
 __global__ __launch_bounds__(256) void amg_cvals(const LevelArgs* __restrict__ la) {
     const LevelArgs& a = *la;
-    const LvSys S = a.ls[blockIdx.y];
+    const LvSys S = adt::gview(a.ls)[blockIdx.y];
     const int c = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (c >= S.n_next) return;
-    const int* cp = a.cpair_ptr + S.cpair_ptr;
-    const int* cs = a.cpair_p + S.cpair;
-    const int* ct = a.cpair_t + S.cpair;
-    const double* P = a.pval + S.pval;
-    const double* T = a.tval + S.tval;
+    const auto cp = adt::gview(a.cpair_ptr + S.cpair_ptr);
+    const auto cs = adt::gview(a.cpair_p + S.cpair);
+    const auto ct = adt::gview(a.cpair_t + S.cpair);
+    const auto P = adt::gview(a.pval + S.pval);
+    const auto T = adt::gview(a.tval + S.tval);
     double acc = 0.0;
     for (int m = cp[c]; m < cp[c + 1]; ++m) acc = fma(P[cs[m]], T[ct[m]], acc);
-    a.next_valp[S.next_val + c] = acc;
+    adt::gview(a.next_valp)[S.next_val + c] = acc;
 }
 
 // One workgroup per system: dense copy of the coarsest operator inverted in place by
@@ -98,12 +98,12 @@ __global__ __launch_bounds__(256) void amg_cvals(const LevelArgs* __restrict__ l
 __global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __restrict__ la) {
     extern __shared__ double colk[];
     const LevelArgs& a = *la;
-    const LvSys S = a.ls[blockIdx.x];
+    const LvSys S = adt::gview(a.ls)[blockIdx.x];
     const int n = S.rows;
-    double* M = a.dense + S.dense;
-    const int32_t* ptr = a.ptr + S.ptr;
-    const int32_t* idx = a.idx + S.idx;
-    const double* V = a.val + S.val;
+    const auto M = adt::gview(a.dense + S.dense);
+    const auto ptr = adt::gview(a.ptr + S.ptr);
+    const auto idx = adt::gview(a.idx + S.idx);
+    const auto V = adt::gview(a.val + S.val);
     const int64_t nn = static_cast<int64_t>(n) * n;
     for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) M[e] = 0.0;
     __syncthreads();
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __rest
         for (int k = ptr[i]; k < ptr[i + 1]; ++k) M[static_cast<int64_t>(i) * n + idx[k]] = V[k];
     __syncthreads();
     for (int k = 0; k < n; ++k) {
-        double* rowk = M + static_cast<int64_t>(k) * n;
+        const auto rowk = adt::gview(M + static_cast<int64_t>(k) * n);
         const double piv = rowk[k];
         const double inv = piv > 0.0 && std::isfinite(piv) ? 1.0 / piv : 0.0;
         for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = M[static_cast<int64_t>(i) * n + k];
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __rest
         for (int i = static_cast<int>(threadIdx.x) / 16; i < n; i += 16) {
             if (i == k) continue;
             const double f = colk[i];
-            double* Mi = M + static_cast<int64_t>(i) * n;
+            const auto Mi = adt::gview(M + static_cast<int64_t>(i) * n);
             for (int j = static_cast<int>(threadIdx.x) % 16; j < n; j += 16) {
                 const double m = (j == k) ? 0.0 : Mi[j];
                 Mi[j] = fma(-f, rowk[j], m);
@@ -135,29 +135,12 @@ __global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __rest
 // ------------------------------------------------------------------------------------------
 // cycle kernels, gather versions (any level)
 // ------------------------------------------------------------------------------------------
+// A thread owns the rows row0 + t / L + i * (256 / L) of its tile and works on kGatherRows of
+// them at once (gather_rows), so the index, value and gathered-vector loads of several rows
+// are in flight together.
 
-// sum_k val(k) * term(idx(k)) over k in [k0, k1), U entries at a time: all index and value
-// loads first, then all gathers, so one thread keeps U independent requests in flight.  The
-// accumulation order is k0, k0+1, ... (padding entries add 0 * term(pad) with a valid pad row).
-template <int U, typename IdxF, typename ValF, typename TermF>
-__device__ __forceinline__ double gather_sum(int k0, int k1, IdxF idx, ValF val, TermF term, int64_t pad) {
-    double acc = 0.0;
-    for (int kb = k0; kb < k1; kb += U) {
-        int64_t jj[U];
-        double vv[U], tt[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const bool ok = kb + u < k1;
-            jj[u] = ok ? idx(kb + u) : pad;
-            vv[u] = ok ? val(kb + u) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) tt[u] = term(jj[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc = fma(vv[u], tt[u], acc);
-    }
-    return acc;
-}
+constexpr int kGatherRows = 4;
+constexpr int kGatherSlots = 4;
 
 struct LevelTile {
     Tile t;
@@ -167,11 +150,26 @@ struct LevelTile {
 
 template <int L>
 __device__ __forceinline__ bool level_tile(const LevelArgs& a, const Tile* tiles, int n_tiles, LevelTile& c) {
-    c.t = tiles[adt::xcd_tile(blockIdx.x, n_tiles)];
-    if (!a.grp_active[c.t.grp]) return false;
-    c.S = a.ls[c.t.sys];
+    c.t = adt::gview(tiles)[adt::xcd_tile(blockIdx.x, n_tiles)];
+    if (!adt::gview(a.grp_active)[c.t.grp]) return false;
+    c.S = adt::gview(a.ls)[c.t.sys];
     c.col = (c.t.grp % a.groups) * L + static_cast<int>(threadIdx.x % L);
     return true;
+}
+
+// Row ranges [k0, k1) of a batch of kGatherRows rows of one thread (empty past row1).  The
+// row-pointer loads are unconditional (rows past row1 read row r0's), selected afterwards.
+template <typename PtrT>
+__device__ __forceinline__ void batch_ranges(PtrT ptr, int r0, int step, int row1, int (&k0)[kGatherRows],
+                                             int (&k1)[kGatherRows]) {
+#pragma unroll
+    for (int i = 0; i < kGatherRows; ++i) {
+        const int row = r0 + i * step;
+        const int rs = row < row1 ? row : r0;
+        const int a0 = ptr[rs], a1 = ptr[rs + 1];
+        k0[i] = row < row1 ? a0 : 0;
+        k1[i] = row < row1 ? a1 : 0;
+    }
 }
 
 // r1 = b - A (w D^-1 b)
@@ -180,20 +178,33 @@ __global__ __launch_bounds__(256) void amg_smooth_residual(const LevelArgs* __re
     const LevelArgs& a = *la;
     LevelTile c;
     if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
     const int rp = a.rp;
-    const int32_t* ptr = a.ptr + c.S.ptr;
-    const int32_t* idx = a.idx + c.S.idx;
-    const double* val = a.val + c.S.val;
-    const double* D = a.dinv + c.S.dinv;
-    const double* B = a.b + c.S.vec + c.col;
-    double* R1 = a.r1 + c.S.vec + c.col;
+    const auto ptr = adt::gview(a.ptr + c.S.ptr);
+    const auto idx = adt::gview(a.idx + c.S.idx);
+    const auto val = adt::gview(a.val + c.S.val);
+    const auto D = adt::gview(a.dinv + c.S.dinv);
+    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto R1 = adt::gview(a.r1 + c.S.vec + c.col);
     const double w = c.S.omega;
-    for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-        const double acc = gather_sum<8>(
-            ptr[row], ptr[row + 1], [&](int k) { return static_cast<int64_t>(idx[k]); }, [&](int k) { return val[k]; },
-            [&](int64_t j) { return w * D[j] * B[j * rp]; }, row);
-        const int64_t o = static_cast<int64_t>(row) * rp;
-        R1[o] = B[o] - acc;
+    for (int r0 = c.t.row0 + static_cast<int>(threadIdx.x / L); r0 < c.t.row1; r0 += kGatherRows * RPP) {
+        int k0[kGatherRows], k1[kGatherRows];
+        double acc[kGatherRows], bo[kGatherRows];
+        batch_ranges(ptr, r0, RPP, c.t.row1, k0, k1);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int row = r0 + i * RPP;
+            bo[i] = B[static_cast<int64_t>(row < c.t.row1 ? row : r0) * rp];  // unused past row1
+            acc[i] = 0.0;
+        }
+        gather_rows<kGatherRows, kGatherSlots>(
+            k0, k1, [&](int k) { return idx[k]; }, [&](int k) { return val[k]; },
+            [&](int j) { return w * D[j] * B[static_cast<int64_t>(j) * rp]; }, acc);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int row = r0 + i * RPP;
+            if (row < c.t.row1) R1[static_cast<int64_t>(row) * rp] = bo[i] - acc[i];
+        }
     }
 }
 
@@ -203,19 +214,26 @@ __global__ __launch_bounds__(256) void amg_restrict(const LevelArgs* __restrict_
     const LevelArgs& a = *la;
     LevelTile c;
     if (!level_tile<L>(a, a.next_tiles, a.n_next_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
     const int rp = a.rp;
-    const int* rptr = a.rptr + c.S.rptr;
-    const int* rrow = a.rrow + c.S.rent;
-    const int* rslot = a.rslot + c.S.rent;
-    const double* P = a.pval + c.S.pval;
-    const double* R1 = a.r1 + c.S.vec + c.col;
-    double* NB = a.next_b + c.S.next_vec + c.col;
-    for (int J = c.t.row0 + static_cast<int>(threadIdx.x / L); J < c.t.row1; J += kRowsPerPass(L)) {
-        const int e0 = rptr[J];
-        const double acc = gather_sum<8>(
-            e0, rptr[J + 1], [&](int e) { return static_cast<int64_t>(e); }, [&](int e) { return P[rslot[e]]; },
-            [&](int64_t e) { return R1[static_cast<int64_t>(rrow[e]) * rp]; }, e0);
-        NB[static_cast<int64_t>(J) * rp] = acc;
+    const auto rptr = adt::gview(a.rptr + c.S.rptr);
+    const auto rrow = adt::gview(a.rrow + c.S.rent);
+    const auto rslot = adt::gview(a.rslot + c.S.rent);
+    const auto P = adt::gview(a.pval + c.S.pval);
+    const auto R1 = adt::gview(a.r1 + c.S.vec + c.col);
+    const auto NB = adt::gview(a.next_b + c.S.next_vec + c.col);
+    for (int J0 = c.t.row0 + static_cast<int>(threadIdx.x / L); J0 < c.t.row1; J0 += kGatherRows * RPP) {
+        int k0[kGatherRows], k1[kGatherRows];
+        double acc[kGatherRows] = {};
+        batch_ranges(rptr, J0, RPP, c.t.row1, k0, k1);
+        gather_rows<kGatherRows, kGatherSlots>(
+            k0, k1, [&](int e) { return rrow[e]; }, [&](int e) { return P[rslot[e]]; },
+            [&](int i) { return R1[static_cast<int64_t>(i) * rp]; }, acc);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int J = J0 + i * RPP;
+            if (J < c.t.row1) NB[static_cast<int64_t>(J) * rp] = acc[i];
+        }
     }
 }
 
@@ -225,14 +243,14 @@ template <int L>
 __global__ __launch_bounds__(256) void amg_coarse_solve(const LevelArgs* __restrict__ la) {
     const LevelArgs& a = *la;
     const int gid = blockIdx.x;
-    if (!a.grp_active[gid]) return;
-    const LvSys S = a.ls[gid / a.groups];
+    if (!adt::gview(a.grp_active)[gid]) return;
+    const LvSys S = adt::gview(a.ls)[gid / a.groups];
     const int col = (gid % a.groups) * L + static_cast<int>(threadIdx.x % L);
     const int n = S.rows, rp = a.rp;
     const int i = static_cast<int>(blockIdx.y) * kRowsPerPass(L) + static_cast<int>(threadIdx.x / L);
     if (i >= n) return;
-    const double* Mi = a.dense + S.dense + static_cast<int64_t>(i) * n;
-    const double* B = a.b + S.vec + col;
+    const auto Mi = adt::gview(a.dense + S.dense + static_cast<int64_t>(i) * n);
+    const auto B = adt::gview(a.b + S.vec + col);
     double a0 = 0.0, a1 = 0.0;
     int j = 0;
     for (; j + 1 < n; j += 2) {
@@ -240,7 +258,7 @@ __global__ __launch_bounds__(256) void amg_coarse_solve(const LevelArgs* __restr
         a1 = fma(Mi[j + 1], B[static_cast<int64_t>(j + 1) * rp], a1);
     }
     if (j < n) a0 = fma(Mi[j], B[static_cast<int64_t>(j) * rp], a0);
-    a.e[S.vec + col + static_cast<int64_t>(i) * rp] = a0 + a1;
+    adt::gview(a.e)[S.vec + col + static_cast<int64_t>(i) * rp] = a0 + a1;
 }
 
 // z2 = w D^-1 b + P e_{l+1}.  On the finest level (FINE) z2 goes to the direction buffer that
@@ -251,21 +269,35 @@ __global__ __launch_bounds__(256) void amg_prolong(const PcgArgs* __restrict__ p
     const LevelArgs& a = *la;
     LevelTile c;
     if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
     const int rp = a.rp;
-    const int* pptr = a.pptr + c.S.pptr;
-    const int* pidx = a.pidx + c.S.pidx;
-    const double* P = a.pval + c.S.pval;
-    const double* D = a.dinv + c.S.dinv;
-    const double* B = a.b + c.S.vec + c.col;
-    const double* NE = a.next_e + c.S.next_vec + c.col;
-    double* Z2 = (FINE ? pa->p[parity ^ 1] : a.z2) + c.S.vec + c.col;
+    const auto pptr = adt::gview(a.pptr + c.S.pptr);
+    const auto pidx = adt::gview(a.pidx + c.S.pidx);
+    const auto P = adt::gview(a.pval + c.S.pval);
+    const auto D = adt::gview(a.dinv + c.S.dinv);
+    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto NE = adt::gview(a.next_e + c.S.next_vec + c.col);
+    const auto Z2 = adt::gview((FINE ? pa->p[parity ^ 1] : a.z2) + c.S.vec + c.col);
     const double w = c.S.omega;
-    for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-        const int64_t o = static_cast<int64_t>(row) * rp;
-        const double acc = gather_sum<4>(
-            pptr[row], pptr[row + 1], [&](int s) { return static_cast<int64_t>(pidx[s]); }, [&](int s) { return P[s]; },
-            [&](int64_t J) { return NE[J * rp]; }, 0);
-        Z2[o] = fma(w * D[row], B[o], acc);
+    for (int r0 = c.t.row0 + static_cast<int>(threadIdx.x / L); r0 < c.t.row1; r0 += kGatherRows * RPP) {
+        int k0[kGatherRows], k1[kGatherRows];
+        double acc[kGatherRows], bo[kGatherRows], dv[kGatherRows];
+        batch_ranges(pptr, r0, RPP, c.t.row1, k0, k1);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int rs = r0 + i * RPP < c.t.row1 ? r0 + i * RPP : r0;  // rows past row1: unused
+            bo[i] = B[static_cast<int64_t>(rs) * rp];
+            dv[i] = D[rs];
+            acc[i] = 0.0;
+        }
+        gather_rows<kGatherRows, kGatherSlots>(
+            k0, k1, [&](int s) { return pidx[s]; }, [&](int s) { return P[s]; },
+            [&](int J) { return NE[static_cast<int64_t>(J) * rp]; }, acc);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int row = r0 + i * RPP;
+            if (row < c.t.row1) Z2[static_cast<int64_t>(row) * rp] = fma(w * dv[i], bo[i], acc[i]);
+        }
     }
 }
 
@@ -275,21 +307,37 @@ __global__ __launch_bounds__(256) void amg_post_smooth(const LevelArgs* __restri
     const LevelArgs& a = *la;
     LevelTile c;
     if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
     const int rp = a.rp;
-    const int32_t* ptr = a.ptr + c.S.ptr;
-    const int32_t* idx = a.idx + c.S.idx;
-    const double* val = a.val + c.S.val;
-    const double* D = a.dinv + c.S.dinv;
-    const double* B = a.b + c.S.vec + c.col;
-    const double* Z2 = a.z2 + c.S.vec + c.col;
-    double* E = a.e + c.S.vec + c.col;
+    const auto ptr = adt::gview(a.ptr + c.S.ptr);
+    const auto idx = adt::gview(a.idx + c.S.idx);
+    const auto val = adt::gview(a.val + c.S.val);
+    const auto D = adt::gview(a.dinv + c.S.dinv);
+    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto Z2 = adt::gview(a.z2 + c.S.vec + c.col);
+    const auto E = adt::gview(a.e + c.S.vec + c.col);
     const double w = c.S.omega;
-    for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-        const double acc = gather_sum<8>(
-            ptr[row], ptr[row + 1], [&](int k) { return static_cast<int64_t>(idx[k]); }, [&](int k) { return val[k]; },
-            [&](int64_t j) { return Z2[j * rp]; }, row);
-        const int64_t o = static_cast<int64_t>(row) * rp;
-        E[o] = fma(w * D[row], B[o] - acc, Z2[o]);
+    for (int r0 = c.t.row0 + static_cast<int>(threadIdx.x / L); r0 < c.t.row1; r0 += kGatherRows * RPP) {
+        int k0[kGatherRows], k1[kGatherRows];
+        double acc[kGatherRows], bo[kGatherRows], zo[kGatherRows], dv[kGatherRows];
+        batch_ranges(ptr, r0, RPP, c.t.row1, k0, k1);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int rs = r0 + i * RPP < c.t.row1 ? r0 + i * RPP : r0;  // rows past row1: unused
+            const int64_t o = static_cast<int64_t>(rs) * rp;
+            bo[i] = B[o];
+            zo[i] = Z2[o];
+            dv[i] = D[rs];
+            acc[i] = 0.0;
+        }
+        gather_rows<kGatherRows, kGatherSlots>(
+            k0, k1, [&](int k) { return idx[k]; }, [&](int k) { return val[k]; },
+            [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; }, acc);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int row = r0 + i * RPP;
+            if (row < c.t.row1) E[static_cast<int64_t>(row) * rp] = fma(w * dv[i], bo[i] - acc[i], zo[i]);
+        }
     }
 }
 
@@ -301,76 +349,210 @@ __global__ __launch_bounds__(256) void amg_post_smooth_fine(const PcgArgs* __res
     const PcgArgs& g = *pa;
     const LevelArgs& a = *la;
     const int tix = adt::xcd_tile(blockIdx.x, a.n_tiles);
-    const Tile t = a.tiles[tix];
-    if (!a.grp_active[t.grp]) return;
-    const LvSys S = a.ls[t.sys];
+    const Tile t = adt::gview(a.tiles)[tix];
+    if (!adt::gview(a.grp_active)[t.grp]) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const LvSys S = adt::gview(a.ls)[t.sys];
     const int col = (t.grp % a.groups) * L + static_cast<int>(threadIdx.x % L);
     const int rp = a.rp;
-    const int32_t* ptr = a.ptr + S.ptr;
-    const int32_t* idx = a.idx + S.idx;
-    const double* val = a.val + S.val;
-    const double* D = a.dinv + S.dinv;
-    const double* R = a.b + S.vec + col;
-    const double* Z2 = g.p[parity ^ 1] + S.vec + col;
-    const double* Q = g.q + S.vec + col;
-    double* Z = a.e + S.vec + col;
+    const auto ptr = adt::gview(a.ptr + S.ptr);
+    const auto idx = adt::gview(a.idx + S.idx);
+    const auto val = adt::gview(a.val + S.val);
+    const auto D = adt::gview(a.dinv + S.dinv);
+    const auto R = adt::gview(a.b + S.vec + col);
+    const auto Z2 = adt::gview(g.p[parity ^ 1] + S.vec + col);
+    const auto Q = adt::gview(g.q + S.vec + col);
+    const auto Z = adt::gview(a.e + S.vec + col);
     const double w = S.omega;
     double zr = 0.0, zq = 0.0;
-    for (int row = t.row0 + static_cast<int>(threadIdx.x / L); row < t.row1; row += kRowsPerPass(L)) {
-        const double acc = gather_sum<8>(
-            ptr[row], ptr[row + 1], [&](int k) { return static_cast<int64_t>(idx[k]); }, [&](int k) { return val[k]; },
-            [&](int64_t j) { return Z2[j * rp]; }, row);
-        const int64_t o = static_cast<int64_t>(row) * rp;
-        const double ri = R[o];
-        const double zi = fma(w * D[row], ri - acc, Z2[o]);
-        Z[o] = zi;
-        zr = fma(zi, ri, zr);
-        zq = fma(zi, Q[o], zq);
+    for (int r0 = t.row0 + static_cast<int>(threadIdx.x / L); r0 < t.row1; r0 += kGatherRows * RPP) {
+        int k0[kGatherRows], k1[kGatherRows];
+        double acc[kGatherRows], rv[kGatherRows], qv[kGatherRows], zo[kGatherRows], dv[kGatherRows];
+        batch_ranges(ptr, r0, RPP, t.row1, k0, k1);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int rs = r0 + i * RPP < t.row1 ? r0 + i * RPP : r0;  // rows past row1: unused
+            const int64_t o = static_cast<int64_t>(rs) * rp;
+            rv[i] = R[o];
+            qv[i] = Q[o];
+            zo[i] = Z2[o];
+            dv[i] = D[rs];
+            acc[i] = 0.0;
+        }
+        gather_rows<kGatherRows, kGatherSlots>(
+            k0, k1, [&](int k) { return idx[k]; }, [&](int k) { return val[k]; },
+            [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; }, acc);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int row = r0 + i * RPP;
+            if (row < t.row1) {
+                const double zi = fma(w * dv[i], rv[i] - acc[i], zo[i]);
+                Z[static_cast<int64_t>(row) * rp] = zi;
+                zr = fma(zi, rv[i], zr);
+                zq = fma(zi, qv[i], zq);
+            }
+        }
     }
     block_sum2<L>(zq, zr);
     if (threadIdx.x < L) {
-        g.part0[static_cast<int64_t>(tix) * L + threadIdx.x] = zq;
-        g.part1[static_cast<int64_t>(tix) * L + threadIdx.x] = zr;
+        adt::gview(g.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = zq;
+        adt::gview(g.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = zr;
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // finest level, LDS-staged versions (tiles and halo tables of the conjugate-gradient product)
 // ------------------------------------------------------------------------------------------
+// One pass per tile, one barrier:
+//   * the tile's CSR slice (values, 16-bit tile-local column slots, row pointers) is copied
+//     into LDS while
+//   * each thread loads its own rows' operands into registers (kept for the product pass) and
+//     the staged values of its share of the halo rows (row ids straight from the global halo
+//     list, halo_batch(L) rows with loads in flight together);
+//   * barrier; the product reads matrix and staged vector from LDS only.
 
 struct LdsTile {
-    double* stage;
+    double* stage;  // (own + halo rows) x L staged values
     double* sval;
     int* sptr;
-    int* shalo;
     uint16_t* sidx;
-    int own, nh, staged;
+    int own, nh, h0, e0, ne;
 };
 
-// Phase 1 of every staged product: the tile's CSR slice and halo list into LDS.
-__device__ __forceinline__ LdsTile stage_csr(const PcgArgs& g, const Tile& t, const Sys& S, int L, double* smem) {
+__device__ __forceinline__ LdsTile lds_tile(const PcgArgs& g, const Tile& t, const Sys& S, int L, double* smem) {
     LdsTile s;
-    const int32_t* ptr = g.indptr + S.ptr_off;
+    const auto ptr = adt::gview(g.indptr + S.ptr_off);
     s.own = t.row1 - t.row0;
-    const int h0 = g.halo_ptr[t.ptile];
-    s.nh = g.halo_ptr[t.ptile + 1] - h0;
-    s.staged = s.own + s.nh;
-    const int e0 = ptr[t.row0], ne = ptr[t.row1] - e0;
+    s.h0 = adt::gview(g.halo_ptr)[t.ptile];
+    s.nh = adt::gview(g.halo_ptr)[t.ptile + 1] - s.h0;
+    s.e0 = ptr[t.row0];
+    s.ne = ptr[t.row1] - s.e0;
     s.stage = smem;
     s.sval = smem + static_cast<int64_t>(g.lds_rows) * L;
     s.sptr = reinterpret_cast<int*>(s.sval + g.lds_nnz);
-    s.shalo = s.sptr + g.lds_own + 1;
-    s.sidx = reinterpret_cast<uint16_t*>(s.shalo + g.lds_halo);
-    const double* val = g.values + S.val_off + e0;
-    const uint16_t* lidx = g.lidx + S.lidx_off + e0;
-    for (int e = static_cast<int>(threadIdx.x); e < ne; e += 256) {
-        s.sval[e] = val[e];
-        s.sidx[e] = lidx[e];
-    }
-    for (int i = static_cast<int>(threadIdx.x); i <= s.own; i += 256) s.sptr[i] = ptr[t.row0 + i] - e0;
-    for (int i = static_cast<int>(threadIdx.x); i < s.nh; i += 256) s.shalo[i] = g.halo_rows[h0 + i];
-    __syncthreads();
+    s.sidx = reinterpret_cast<uint16_t*>(s.sptr + g.lds_own + 1);
     return s;
+}
+
+// The tile's CSR slice (values, local column slots, row pointers) in registers: the first 512
+// entries and row pointers of thread t (loaded unconditionally, stored predicated).
+struct CsrRegs {
+    double va, vb;
+    int pi;
+    uint16_t ia, ib;
+};
+
+__device__ __forceinline__ CsrRegs load_csr(const PcgArgs& g, const Tile& t, const Sys& S, const LdsTile& s) {
+    const auto ptr = adt::gview(g.indptr + S.ptr_off);
+    const auto val = adt::gview(g.values + S.val_off + s.e0);
+    const auto lidx = adt::gview(g.lidx + S.lidx_off + s.e0);
+    const int tid = static_cast<int>(threadIdx.x);
+    const int sa = tid < s.ne ? tid : 0, sb = 256 + tid < s.ne ? 256 + tid : 0;
+    CsrRegs c;
+    c.va = val[sa];
+    c.vb = val[sb];
+    c.ia = lidx[sa];
+    c.ib = lidx[sb];
+    c.pi = ptr[t.row0 + (tid <= s.own ? tid : 0)];
+    return c;
+}
+
+// Stores the slice into LDS (published by the caller's barrier); slices longer than 512 entries
+// or tiles of more than 255 rows continue with plain loops.
+__device__ __forceinline__ void store_csr(const PcgArgs& g, const Tile& t, const Sys& S, const LdsTile& s,
+                                          const CsrRegs& c) {
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < s.ne) {
+        s.sval[tid] = c.va;
+        s.sidx[tid] = c.ia;
+    }
+    if (256 + tid < s.ne) {
+        s.sval[256 + tid] = c.vb;
+        s.sidx[256 + tid] = c.ib;
+    }
+    if (tid <= s.own) s.sptr[tid] = c.pi - s.e0;
+    if (s.ne > 512 || s.own >= 256) {
+        const auto ptr = adt::gview(g.indptr + S.ptr_off);
+        const auto val = adt::gview(g.values + S.val_off + s.e0);
+        const auto lidx = adt::gview(g.lidx + S.lidx_off + s.e0);
+        for (int e = 512 + tid; e < s.ne; e += 256) {
+            s.sval[e] = val[e];
+            s.sidx[e] = lidx[e];
+        }
+        for (int i = 256 + tid; i <= s.own; i += 256) s.sptr[i] = ptr[t.row0 + i] - s.e0;
+    }
+}
+
+// Halo row ids of one staging batch: rows base + t / L + u * (256 / L), u < halo_batch(L).
+// Positions past the halo get the tile's first row (a valid row whose value is not stored).
+template <int L>
+__device__ __forceinline__ void halo_ids(const PcgArgs& g, const Tile& t, const LdsTile& s, int base,
+                                         int (&jj)[halo_batch(L)]) {
+    constexpr int RPP = kRowsPerPass(L);
+    const auto halo = adt::gview(g.halo_rows);
+    const int h0 = base + static_cast<int>(threadIdx.x / L);
+#pragma unroll
+    for (int u = 0; u < halo_batch(L); ++u) {
+        const bool in = h0 + u * RPP < s.nh;
+        const int raw = halo[in ? s.h0 + h0 + u * RPP : 0];
+        jj[u] = in ? raw : t.row0;
+    }
+}
+
+// Staged values of one halo batch (loads only).
+template <int L, typename ValueF>
+__device__ __forceinline__ void halo_values(const int (&jj)[halo_batch(L)], double (&v)[halo_batch(L)], ValueF value) {
+#pragma unroll
+    for (int u = 0; u < halo_batch(L); ++u) v[u] = value(jj[u]);
+}
+
+template <int L>
+__device__ __forceinline__ void store_halo(const LdsTile& s, int base, const double (&v)[halo_batch(L)]) {
+    constexpr int RPP = kRowsPerPass(L);
+    const int lane = static_cast<int>(threadIdx.x % L), h0 = base + static_cast<int>(threadIdx.x / L);
+#pragma unroll
+    for (int u = 0; u < halo_batch(L); ++u)
+        if (h0 + u * RPP < s.nh) s.stage[(s.own + h0 + u * RPP) * L + lane] = v[u];
+}
+
+// Halo batches after the first (tiles with a large halo only).
+template <int L, typename ValueF>
+__device__ __forceinline__ void stage_halo_rest(const PcgArgs& g, const Tile& t, const LdsTile& s, ValueF value) {
+    constexpr int step = halo_batch(L) * kRowsPerPass(L);
+    for (int base = step; base < s.nh; base += step) {
+        int jj[halo_batch(L)];
+        double v[halo_batch(L)];
+        halo_ids<L>(g, t, s, base, jj);
+        halo_values<L>(jj, v, value);
+        store_halo<L>(s, base, v);
+    }
+}
+
+// sum_k sval[k] * stage[sidx[k]] over staged row q (never empty: it holds the diagonal), in
+// slot order, 8 slots in flight; slots past the row re-read its first slot and are discarded.
+template <int L>
+__device__ __forceinline__ double lds_row_product(const LdsTile& s, int q, int lane) {
+    constexpr int U = 8;
+    const int k0 = s.sptr[q], k1 = s.sptr[q + 1];
+    double acc = 0.0;
+    for (int kb = k0; kb < k1; kb += U) {
+        int jj[U];
+        double vv[U], tt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = kb + u < k1 ? kb + u : k0;
+            jj[u] = static_cast<int>(s.sidx[k]);
+            vv[u] = s.sval[k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) tt[u] = s.stage[jj[u] * L + lane];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double next = fma(vv[u], tt[u], acc);
+            acc = kb + u < k1 ? next : acc;
+        }
+    }
+    return acc;
 }
 
 // r1 = r - A (w D^-1 r), written into the z buffer.
@@ -381,44 +563,44 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     const PcgArgs& g = *pa;
     const LevelArgs& a = *la;
     const int tix = adt::xcd_tile(blockIdx.x, g.n_tiles);
-    const Tile t = g.tiles[tix];
-    if (!g.grp_active[t.grp]) return;
-    const Sys S = g.sys[t.sys];
-    const double w = a.ls[t.sys].omega;
+    const Tile t = adt::gview(g.tiles)[tix];
+    if (!adt::gview(g.grp_active)[t.grp]) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const Sys S = adt::gview(g.sys)[t.sys];
+    const double w = adt::gview(a.ls)[t.sys].omega;
     const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
     const int col = (t.grp % g.groups) * L + lane;
     const int rp = g.rp;
-    const double* R = g.r + S.vec_off + col;
-    double* R1 = g.z + S.vec_off + col;
-    const double* D = g.dinv + S.diag_off;
-    const LdsTile s = stage_csr(g, t, S, L, smem);
-    constexpr int U = 4;
-    for (int s0 = rsub; s0 < s.staged; s0 += U * kRowsPerPass(L)) {
-        int64_t jj[U];
-        double dj[U], rj[U];
+    const auto R = adt::gview(g.r + S.vec_off + col);
+    const auto R1 = adt::gview(g.z + S.vec_off + col);
+    const auto D = adt::gview(g.dinv + S.diag_off);
+    const LdsTile s = lds_tile(g, t, S, L, smem);
+    const auto value = [&](int j) { return w * D[j] * R[static_cast<int64_t>(j) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(g, t, s, 0, hid);
+    double rv[kOwnMax], dv[kOwnMax];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = s0 + u * kRowsPerPass(L);
-            jj[u] = q < s.own ? t.row0 + q : (q < s.staged ? s.shalo[q - s.own] : t.row0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            dj[u] = D[jj[u]];
-            rj[u] = R[jj[u] * rp];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = s0 + u * kRowsPerPass(L);
-            if (q < s.staged) s.stage[q * L + lane] = w * dj[u] * rj[u];
-        }
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int row = t.row0 + (rsub + i * RPP < s.own ? rsub + i * RPP : 0);  // past the tile: unused
+        rv[i] = R[static_cast<int64_t>(row) * rp];
+        dv[i] = D[row];
     }
+    const CsrRegs cr = load_csr(g, t, S, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(g, t, S, s, cr);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.own) s.stage[q * L + lane] = w * dv[i] * rv[i];
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(g, t, s, value);
     __syncthreads();
-    for (int q = rsub; q < s.own; q += kRowsPerPass(L)) {
-        const int k1 = s.sptr[q + 1];
-        double acc = 0.0;
-        for (int k = s.sptr[q]; k < k1; ++k) acc = fma(s.sval[k], s.stage[static_cast<int>(s.sidx[k]) * L + lane], acc);
-        const int64_t o = static_cast<int64_t>(t.row0 + q) * rp;
-        R1[o] = R[o] - acc;
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.own) R1[static_cast<int64_t>(t.row0 + q) * rp] = rv[i] - lds_row_product<L>(s, q, lane);
     }
 }
 
@@ -432,54 +614,61 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     const PcgArgs& g = *pa;
     const LevelArgs& a = *la;
     const int tix = adt::xcd_tile(blockIdx.x, g.n_tiles);
-    const Tile t = g.tiles[tix];
-    if (!g.grp_active[t.grp]) return;
-    const Sys S = g.sys[t.sys];
-    const double w = a.ls[t.sys].omega;
+    const Tile t = adt::gview(g.tiles)[tix];
+    if (!adt::gview(g.grp_active)[t.grp]) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const Sys S = adt::gview(g.sys)[t.sys];
+    const double w = adt::gview(a.ls)[t.sys].omega;
     const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
     const int col = (t.grp % g.groups) * L + lane;
     const int rp = g.rp;
-    const double* R = g.r + S.vec_off + col;
-    const double* Q = g.q + S.vec_off + col;
-    const double* Z2 = g.p[parity ^ 1] + S.vec_off + col;
-    double* Z = g.z + S.vec_off + col;
-    const double* D = g.dinv + S.diag_off;
-    const LdsTile s = stage_csr(g, t, S, L, smem);
-    constexpr int U = 4;
-    for (int s0 = rsub; s0 < s.staged; s0 += U * kRowsPerPass(L)) {
-        int64_t jj[U];
-        double v[U];
+    const auto R = adt::gview(g.r + S.vec_off + col);
+    const auto Q = adt::gview(g.q + S.vec_off + col);
+    const auto Z2 = adt::gview(g.p[parity ^ 1] + S.vec_off + col);
+    const auto Z = adt::gview(g.z + S.vec_off + col);
+    const auto D = adt::gview(g.dinv + S.diag_off);
+    const LdsTile s = lds_tile(g, t, S, L, smem);
+    const auto value = [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(g, t, s, 0, hid);
+    double zv[kOwnMax], rv[kOwnMax], qv[kOwnMax], dv[kOwnMax];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = s0 + u * kRowsPerPass(L);
-            jj[u] = q < s.own ? t.row0 + q : (q < s.staged ? s.shalo[q - s.own] : t.row0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = Z2[jj[u] * rp];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = s0 + u * kRowsPerPass(L);
-            if (q < s.staged) s.stage[q * L + lane] = v[u];
-        }
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int row = t.row0 + (rsub + i * RPP < s.own ? rsub + i * RPP : 0);  // past the tile: unused
+        const int64_t o = static_cast<int64_t>(row) * rp;
+        zv[i] = Z2[o];
+        rv[i] = R[o];
+        qv[i] = Q[o];
+        dv[i] = D[row];
     }
+    const CsrRegs cr = load_csr(g, t, S, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(g, t, S, s, cr);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.own) s.stage[q * L + lane] = zv[i];
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(g, t, s, value);
     __syncthreads();
     double zr = 0.0, zq = 0.0;
-    for (int q = rsub; q < s.own; q += kRowsPerPass(L)) {
-        const int k1 = s.sptr[q + 1];
-        double acc = 0.0;
-        for (int k = s.sptr[q]; k < k1; ++k) acc = fma(s.sval[k], s.stage[static_cast<int>(s.sidx[k]) * L + lane], acc);
-        const int row = t.row0 + q;
-        const int64_t o = static_cast<int64_t>(row) * rp;
-        const double ri = R[o];
-        const double zi = fma(w * D[row], ri - acc, s.stage[q * L + lane]);
-        Z[o] = zi;
-        zr = fma(zi, ri, zr);
-        zq = fma(zi, Q[o], zq);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.own) {
+            const double acc = lds_row_product<L>(s, q, lane);
+            const double zi = fma(w * dv[i], rv[i] - acc, zv[i]);
+            Z[static_cast<int64_t>(t.row0 + q) * rp] = zi;
+            zr = fma(zi, rv[i], zr);
+            zq = fma(zi, qv[i], zq);
+        }
     }
     block_sum2<L>(zq, zr);
     if (threadIdx.x < L) {
-        g.part0[static_cast<int64_t>(tix) * L + threadIdx.x] = zq;
-        g.part1[static_cast<int64_t>(tix) * L + threadIdx.x] = zr;
+        adt::gview(g.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = zq;
+        adt::gview(g.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = zr;
     }
 }
 

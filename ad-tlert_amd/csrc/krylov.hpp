@@ -65,7 +65,7 @@ struct PcgArgs {
     int* active_total;
     double tol2;
     int rp, groups, r_real, n_tiles;
-    int lds_rows, lds_nnz, lds_own, lds_halo;  // LDS-staged product capacities
+    int lds_rows, lds_nnz, lds_own, lds_pad_;  // LDS-staged product capacities
     int stall_limit;                           // iterations without a 10% residual decrease (0: off)
     int segmented, n_segs;
     int pad_;
@@ -136,6 +136,52 @@ struct LevelArgs {
 
 constexpr int kRowsPerPass(int L) { return 256 / L; }
 
+// LDS-staged tiles keep each thread's own-row operands in registers from the staging pass to
+// the product pass: a staged tile has at most kOwnMax rows per thread (rows_per_tile <=
+// kOwnMax * 256 / L).  Halo rows are staged halo_batch(L) rows per thread at a time, all of
+// their loads in flight together (one batch covers the halo of the usual 2D and 3D tiles).
+constexpr int kOwnMax = 8;
+constexpr int halo_batch(int L) { return L >= 64 ? 24 : 12; }
+constexpr int max_staged_tile_rows(int L) { return kOwnMax * kRowsPerPass(L); }
+
+// Row-batched CSR gather: for the RB rows of one thread, acc[b] += sum_k val(k) * term(idx(k))
+// over k in [k0[b], k1[b]), accumulated per row in slot order.  U slots of every row are
+// requested before any is used, so up to RB * U independent loads are in flight per thread
+// (a row-at-a-time loop would pay index -> value -> gather latency once per row).  Loads are
+// unconditional (slots past a row's end read slot 0 of the arrays and are discarded by a
+// select), so no branch makes the compiler wait for the loads in flight.
+template <int RB, int U, typename IdxF, typename ValF, typename TermF>
+__device__ __forceinline__ void gather_rows(const int (&k0)[RB], const int (&k1)[RB], IdxF idx, ValF val, TermF term,
+                                            double (&acc)[RB]) {
+    int span = 0;
+#pragma unroll
+    for (int b = 0; b < RB; ++b) span = max(span, k1[b] - k0[b]);
+    for (int kb = 0; kb < span; kb += U) {
+        int jj[RB][U];
+        double vv[RB][U], tt[RB][U];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0[b] + kb + u;
+                const int ks = k < k1[b] ? k : 0;
+                jj[b][u] = idx(ks);
+                vv[b][u] = val(ks);
+            }
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int u = 0; u < U; ++u) tt[b][u] = term(jj[b][u]);
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double next = fma(vv[b][u], tt[b][u], acc[b]);
+                acc[b] = k0[b] + kb + u < k1[b] ? next : acc[b];
+            }
+    }
+}
+
 // Sum over the threads of a 256-wide block that share lane = t % L; thread t < L gets the
 // lane total.  Two independent values are reduced together.  Fixed order.
 template <int L>
@@ -178,7 +224,7 @@ __device__ __forceinline__ double group_partials(const double* part, int tile0, 
     constexpr int waves = kReduceThreads / 64;
     __shared__ double sh[waves][64];
     const int lane = static_cast<int>(threadIdx.x) % L, chunk = static_cast<int>(threadIdx.x) / L;
-    const double* base = part + static_cast<int64_t>(tile0) * L + lane;
+    const auto base = adt::gview(part + static_cast<int64_t>(tile0) * L + lane);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int i = chunk;
     for (; i + 3 * chunks < ntiles; i += 4 * chunks) {
@@ -207,9 +253,10 @@ __device__ __forceinline__ double group_partials(const double* part, int tile0, 
 template <int L>
 __device__ __forceinline__ double group_sum(const PcgArgs& a, int k, int gid) {
     if (a.segmented)
-        return group_partials<L>(a.spart + static_cast<int64_t>(k) * a.n_segs * L, a.grp_seg0[gid], a.grp_nseg[gid]);
+        return group_partials<L>(a.spart + static_cast<int64_t>(k) * a.n_segs * L, adt::gview(a.grp_seg0)[gid],
+                                 adt::gview(a.grp_nseg)[gid]);
     const double* part = k == 0 ? a.part0 : (k == 1 ? a.part1 : a.part2);
-    return group_partials<L>(part, a.grp_tile0[gid], a.grp_ntiles[gid]);
+    return group_partials<L>(part, adt::gview(a.grp_tile0)[gid], adt::gview(a.grp_ntiles)[gid]);
 }
 
 template <int L>
@@ -219,11 +266,13 @@ __device__ __forceinline__ void refresh_group(const PcgArgs& a, int gid, bool la
     __syncthreads();
     if (threadIdx.x < L && lane_active) atomicOr(&any, 1);
     __syncthreads();
-    if (threadIdx.x == 0) a.grp_active[gid] = any;
+    if (threadIdx.x == 0) adt::gview(a.grp_active)[gid] = any;
 }
 
-inline int64_t lds_bytes_for(int staged, int nnz, int own, int halo, int lanes) {
-    return static_cast<int64_t>(staged) * lanes * 8 + static_cast<int64_t>(nnz) * 8 + (own + 1 + halo) * 4 +
+// Dynamic LDS of a staged tile: staged rows x L doubles, the CSR slice (values, 16-bit local
+// columns) and its row pointers.
+inline int64_t lds_bytes_for(int staged, int nnz, int own, int lanes) {
+    return static_cast<int64_t>(staged) * lanes * 8 + static_cast<int64_t>(nnz) * 8 + (own + 1) * 4 +
            static_cast<int64_t>(nnz) * 2;
 }
 

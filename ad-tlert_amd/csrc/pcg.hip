@@ -35,7 +35,6 @@ namespace {
 
 using namespace adt::krylov;
 
-constexpr int kSpmvUnroll = 8;
 
 // ------------------------------------------------------------------------------------------
 // plain multi-RHS product (adt_spmm)
@@ -53,11 +52,11 @@ __global__ __launch_bounds__(256) void spmm_kernel(const Sys* __restrict__ sys, 
     const int col = blockIdx.z * L + lane;
     const int row0 = blockIdx.x * rb;
     const int row1 = static_cast<int>(std::min<int64_t>(row0 + rb, S.rows));
-    const int32_t* ptr = indptr + S.ptr_off;
-    const int32_t* idx = indices + S.idx_off;
-    const double* val = values + S.val_off;
-    const double* X = x + S.vec_off + col;
-    double* Y = y + S.vec_off + col;
+    const auto ptr = adt::gview(indptr + S.ptr_off);
+    const auto idx = adt::gview(indices + S.idx_off);
+    const auto val = adt::gview(values + S.val_off);
+    const auto X = adt::gview(x + S.vec_off + col);
+    const auto Y = adt::gview(y + S.vec_off + col);
     for (int row = row0 + rsub; row < row1; row += kRowsPerPass(L)) {
         double acc = 0.0;
         const int k1 = ptr[row + 1];
@@ -93,8 +92,8 @@ struct TileCtx {
 template <int L>
 __device__ __forceinline__ bool load_tile(const PcgArgs& a, TileCtx& c) {
     const int tix = adt::xcd_tile(blockIdx.x, a.n_tiles);
-    c.t = a.tiles[tix];
-    c.S = a.sys[c.t.sys];
+    c.t = adt::gview(a.tiles)[tix];
+    c.S = adt::gview(a.sys)[c.t.sys];
     const int g = c.t.grp % a.groups;
     c.col = g * L + static_cast<int>(threadIdx.x % L);
     c.sc = static_cast<int64_t>(c.t.sys) * a.rp + c.col;
@@ -103,7 +102,8 @@ __device__ __forceinline__ bool load_tile(const PcgArgs& a, TileCtx& c) {
 
 __device__ __forceinline__ int tile_index(const PcgArgs& a) { return adt::xcd_tile(blockIdx.x, a.n_tiles); }
 
-__device__ __forceinline__ double diag_inverse(const int32_t* ptr, const int32_t* idx, const double* val, int row) {
+template <typename IP, typename VP>
+__device__ __forceinline__ double diag_inverse(IP ptr, IP idx, VP val, int row) {
     double d = 0.0;
     for (int k = ptr[row]; k < ptr[row + 1]; ++k)
         if (idx[k] == row) d = val[k];
@@ -117,19 +117,19 @@ __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
     TileCtx c;
     load_tile<L>(a, c);
     const int tix = tile_index(a);
-    const int32_t* ptr = a.indptr + c.S.ptr_off;
-    const int32_t* idx = a.indices + c.S.idx_off;
-    const double* val = a.values + c.S.val_off;
+    const auto ptr = adt::gview(a.indptr + c.S.ptr_off);
+    const auto idx = adt::gview(a.indices + c.S.idx_off);
+    const auto val = adt::gview(a.values + c.S.val_off);
     const int rp = a.rp;
     const bool real = c.col < a.r_real;
     const bool first_group = (c.t.grp % a.groups) == 0;
-    const double* B = a.b + c.S.b_off + c.col;
-    const double* X = a.x + c.S.vec_off + c.col;
-    double* R = a.r + c.S.vec_off + c.col;
-    double* P0 = a.p[0] + c.S.vec_off + c.col;
-    double* P1 = a.p[1] + c.S.vec_off + c.col;
-    double* Q = a.q + c.S.vec_off + c.col;
-    double* D = a.dinv + c.S.diag_off;
+    const auto B = adt::gview(a.b + c.S.b_off + c.col);
+    const auto X = adt::gview(a.x + c.S.vec_off + c.col);
+    const auto R = adt::gview(a.r + c.S.vec_off + c.col);
+    const auto P0 = adt::gview(a.p[0] + c.S.vec_off + c.col);
+    const auto P1 = adt::gview(a.p[1] + c.S.vec_off + c.col);
+    const auto Q = adt::gview(a.q + c.S.vec_off + c.col);
+    const auto D = adt::gview(a.dinv + c.S.diag_off);
     double sb = 0.0, sz = 0.0, sr = 0.0;
     for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
         const double di = diag_inverse(ptr, idx, val, row);
@@ -155,9 +155,9 @@ __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
     block_sum2<L>(s1, s2);
     double s3 = block_sum1<L>(sr);
     if (threadIdx.x < L) {
-        a.part0[static_cast<int64_t>(tix) * L + threadIdx.x] = s1;
-        a.part1[static_cast<int64_t>(tix) * L + threadIdx.x] = s2;
-        a.part2[static_cast<int64_t>(tix) * L + threadIdx.x] = s3;
+        adt::gview(a.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = s1;
+        adt::gview(a.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = s2;
+        adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = s3;
     }
 }
 
@@ -182,21 +182,21 @@ __global__ __launch_bounds__(kReduceThreads) void cg_init_reduce(const PcgArgs* 
             else if (rr > a.tol2 * bb)
                 flag = 1;
         }
-        a.flags[sc] = flag;
-        a.bb[sc] = bb;
-        a.rr[sc] = rr;
-        a.rr_best[sc] = rr;
-        a.rho[sc] = rz;
-        a.rz_old[sc] = rz;
-        a.beta[sc] = 0.0;
-        a.alpha[sc] = 0.0;
-        a.pq_old[sc] = 1.0;
-        a.iters[sc] = 0;
-        a.stall[sc] = 0;
+        adt::gview(a.flags)[sc] = flag;
+        adt::gview(a.bb)[sc] = bb;
+        adt::gview(a.rr)[sc] = rr;
+        adt::gview(a.rr_best)[sc] = rr;
+        adt::gview(a.rho)[sc] = rz;
+        adt::gview(a.rz_old)[sc] = rz;
+        adt::gview(a.beta)[sc] = 0.0;
+        adt::gview(a.alpha)[sc] = 0.0;
+        adt::gview(a.pq_old)[sc] = 1.0;
+        adt::gview(a.iters)[sc] = 0;
+        adt::gview(a.stall)[sc] = 0;
         act = flag == 1;
         if (act) atomicAdd(a.active_total, 1);
     }
-    if (threadIdx.x == 0) a.grp_iters[gid] = 0;
+    if (threadIdx.x == 0) adt::gview(a.grp_iters)[gid] = 0;
     refresh_group<L>(a, gid, act);
 }
 
@@ -206,16 +206,15 @@ __global__ __launch_bounds__(256) void cg_zero_solution(const PcgArgs* __restric
     const PcgArgs& a = *pa;
     TileCtx c;
     load_tile<L>(a, c);
-    if (a.flags[c.sc] != 2) return;
-    double* X = a.x + c.S.vec_off + c.col;
+    if (adt::gview(a.flags)[c.sc] != 2) return;
+    const auto X = adt::gview(a.x + c.S.vec_off + c.col);
     for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L))
         X[static_cast<int64_t>(row) * a.rp] = 0.0;
 }
 
 // Search direction of row j: Jacobi p_new = D^-1 r + beta p_old; flexible p_new = z + beta p_old.
-template <bool FLEX>
-__device__ __forceinline__ double direction(const double* D, const double* R, const double* Z, const double* Pold,
-                                            double beta, int64_t j, int rp) {
+template <bool FLEX, typename DP, typename VP>
+__device__ __forceinline__ double direction(DP D, VP R, VP Z, VP Pold, double beta, int64_t j, int rp) {
     if (FLEX) return fma(beta, Pold[j * rp], Z[j * rp]);
     return fma(beta, Pold[j * rp], D[j] * R[j * rp]);
 }
@@ -225,141 +224,144 @@ template <int L, bool FLEX>
 __global__ __launch_bounds__(256) void cg_spmv(const PcgArgs* __restrict__ pa, int parity) {
     const PcgArgs& a = *pa;
     const int tix = tile_index(a);
-    const Tile t = a.tiles[tix];
-    if (!a.grp_active[t.grp]) return;
-    const Sys S = a.sys[t.sys];
+    const Tile t = adt::gview(a.tiles)[tix];
+    if (!adt::gview(a.grp_active)[t.grp]) return;
+    const Sys S = adt::gview(a.sys)[t.sys];
     const int col = (t.grp % a.groups) * L + static_cast<int>(threadIdx.x % L);
     const int64_t sc = static_cast<int64_t>(t.sys) * a.rp + col;
-    const bool act = a.flags[sc] == 1;
-    const double beta = act ? a.beta[sc] : 0.0;
-    const int32_t* ptr = a.indptr + S.ptr_off;
-    const int32_t* idx = a.indices + S.idx_off;
-    const double* val = a.values + S.val_off;
+    const bool act = adt::gview(a.flags)[sc] == 1;
+    const double beta = act ? adt::gview(a.beta)[sc] : 0.0;
+    const auto ptr = adt::gview(a.indptr + S.ptr_off);
+    const auto idx = adt::gview(a.indices + S.idx_off);
+    const auto val = adt::gview(a.values + S.val_off);
     const int rp = a.rp;
-    const double* R = a.r + S.vec_off + col;
-    const double* Z = a.z + S.vec_off + col;
-    const double* Pold = a.p[parity] + S.vec_off + col;
-    double* Pnew = a.p[parity ^ 1] + S.vec_off + col;
-    double* Q = a.q + S.vec_off + col;
-    const double* D = a.dinv + S.diag_off;
+    const auto R = adt::gview(a.r + S.vec_off + col);
+    const auto Z = adt::gview(a.z + S.vec_off + col);
+    const auto Pold = adt::gview(a.p[parity] + S.vec_off + col);
+    const auto Pnew = adt::gview(a.p[parity ^ 1] + S.vec_off + col);
+    const auto Q = adt::gview(a.q + S.vec_off + col);
+    const auto D = adt::gview(a.dinv + S.diag_off);
     double pq = 0.0;
     if (act) {
-        for (int row = t.row0 + static_cast<int>(threadIdx.x / L); row < t.row1; row += kRowsPerPass(L)) {
-            double acc = 0.0;
-            const int k1 = ptr[row + 1];
-            // kSpmvUnroll entries at a time: all index loads, then all gathers, so one thread keeps
-            // several independent memory requests in flight (padding entries re-read the own row
-            // with a zero coefficient).
-            for (int kb = ptr[row]; kb < k1; kb += kSpmvUnroll) {
-                int64_t jj[kSpmvUnroll];
-                double vv[kSpmvUnroll];
+        constexpr int RPP = kRowsPerPass(L);
+        for (int r0 = t.row0 + static_cast<int>(threadIdx.x / L); r0 < t.row1; r0 += kGatherRows * RPP) {
+            int k0[kGatherRows], k1[kGatherRows];
+            double acc[kGatherRows], pv[kGatherRows];
+            batch_ranges(ptr, r0, RPP, t.row1, k0, k1);
 #pragma unroll
-                for (int u = 0; u < kSpmvUnroll; ++u) {
-                    const bool ok = kb + u < k1;
-                    jj[u] = ok ? idx[kb + u] : row;
-                    vv[u] = ok ? val[kb + u] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < kSpmvUnroll; ++u)
-                    acc = fma(vv[u], direction<FLEX>(D, R, Z, Pold, beta, jj[u], rp), acc);
+            for (int i = 0; i < kGatherRows; ++i) {
+                const int rs = r0 + i * RPP < t.row1 ? r0 + i * RPP : r0;  // rows past row1: unused
+                pv[i] = direction<FLEX>(D, R, Z, Pold, beta, rs, rp);
+                acc[i] = 0.0;
             }
-            const int64_t o = static_cast<int64_t>(row) * rp;
-            const double pi = direction<FLEX>(D, R, Z, Pold, beta, row, rp);
-            Pnew[o] = pi;
-            Q[o] = acc;
-            pq = fma(pi, acc, pq);
+            gather_rows<kGatherRows, kGatherSlots>(
+                k0, k1, [&](int k) { return idx[k]; }, [&](int k) { return val[k]; },
+                [&](int j) { return direction<FLEX>(D, R, Z, Pold, beta, j, rp); }, acc);
+#pragma unroll
+            for (int i = 0; i < kGatherRows; ++i) {
+                const int row = r0 + i * RPP;
+                if (row < t.row1) {
+                    const int64_t o = static_cast<int64_t>(row) * rp;
+                    Pnew[o] = pv[i];
+                    Q[o] = acc[i];
+                    pq = fma(pv[i], acc[i], pq);
+                }
+            }
         }
     }
     pq = block_sum1<L>(pq);
-    if (threadIdx.x < L) a.part0[static_cast<int64_t>(tix) * L + threadIdx.x] = pq;
+    if (threadIdx.x < L) adt::gview(a.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = pq;
 }
 
-// LDS-staged variant of cg_spmv, three phases per tile:
-//   1. coalesced bulk copy of the tile's CSR slice (values, tile-local column slots, row
-//      pointers) and halo row list into LDS;
-//   2. stage p_new for the tile's rows and halo rows (independent loads, unrolled) into LDS,
-//      writing p_new of the own rows;
-//   3. q = A p_new entirely from LDS (slot-major with L lanes contiguous: conflict-free).
-// Every vector row is read from memory once per tile and the inner product loop has no
-// global-memory latency in it.
+// LDS-staged variant of cg_spmv, one barrier per tile: the tile's CSR slice is copied into
+// LDS while each thread computes p_new for its own rows (kept in registers, written out) and
+// its share of the halo rows (halo_batch(L) rows with loads in flight together), all staged in
+// LDS; after the barrier q = A p_new is formed from LDS only (slot-major with L lanes
+// contiguous: conflict-free).  Every vector row is read from memory once per tile.
 template <int L, bool FLEX>
 __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ pa, int parity) {
     extern __shared__ double smem[];
     const PcgArgs& a = *pa;
     const int tix = tile_index(a);
-    const Tile t = a.tiles[tix];
-    if (!a.grp_active[t.grp]) return;
-    const Sys S = a.sys[t.sys];
+    const Tile t = adt::gview(a.tiles)[tix];
+    if (!adt::gview(a.grp_active)[t.grp]) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const Sys S = adt::gview(a.sys)[t.sys];
     const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
     const int col = (t.grp % a.groups) * L + lane;
     const int64_t sc = static_cast<int64_t>(t.sys) * a.rp + col;
-    const bool act = a.flags[sc] == 1;
-    const double beta = act ? a.beta[sc] : 0.0;
+    const bool act = adt::gview(a.flags)[sc] == 1;
+    const double beta_sc = adt::gview(a.beta)[sc];
+    const double beta = act ? beta_sc : 0.0;
     const int rp = a.rp;
-    const double* R = a.r + S.vec_off + col;
-    const double* Z = a.z + S.vec_off + col;
-    const double* Pold = a.p[parity] + S.vec_off + col;
-    double* Pnew = a.p[parity ^ 1] + S.vec_off + col;
-    double* Q = a.q + S.vec_off + col;
-    const double* D = a.dinv + S.diag_off;
-    const LdsTile s = stage_csr(a, t, S, L, smem);
-    // phase 2
-    if (act) {
-        constexpr int U = 4;
-        for (int s0 = rsub; s0 < s.staged; s0 += U * kRowsPerPass(L)) {
-            int64_t jj[U];
-            double v[U];
+    const auto R = adt::gview(a.r + S.vec_off + col);
+    const auto Z = adt::gview(a.z + S.vec_off + col);
+    const auto Pold = adt::gview(a.p[parity] + S.vec_off + col);
+    const auto Pnew = adt::gview(a.p[parity ^ 1] + S.vec_off + col);
+    const auto Q = adt::gview(a.q + S.vec_off + col);
+    const auto D = adt::gview(a.dinv + S.diag_off);
+    const LdsTile s = lds_tile(a, t, S, L, smem);
+    // every lane stages (inactive lanes' values are never read); only active lanes write p_new
+    const auto value = [&](int j) { return direction<FLEX>(D, R, Z, Pold, beta, j, rp); };
+    int hid[halo_batch(L)];
+    halo_ids<L>(a, t, s, 0, hid);
+    double v[kOwnMax];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int q = s0 + u * kRowsPerPass(L);
-                jj[u] = q < s.own ? t.row0 + q : (q < s.staged ? s.shalo[q - s.own] : t.row0);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = direction<FLEX>(D, R, Z, Pold, beta, jj[u], rp);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int q = s0 + u * kRowsPerPass(L);
-                if (q < s.staged) {
-                    s.stage[q * L + lane] = v[u];
-                    if (q < s.own) Pnew[jj[u] * rp] = v[u];
-                }
-            }
-        }
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP < s.own ? rsub + i * RPP : 0;  // past the tile: unused
+        v[i] = value(t.row0 + q);
     }
+    const CsrRegs cr = load_csr(a, t, S, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(a, t, S, s, cr);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.own) s.stage[q * L + lane] = v[i];
+    }
+    store_halo<L>(s, 0, hv);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (act && q < s.own) Pnew[static_cast<int64_t>(t.row0 + q) * rp] = v[i];
+    }
+    stage_halo_rest<L>(a, t, s, value);
     __syncthreads();
-    // phase 3
     double pq = 0.0;
     if (act) {
-        for (int q = rsub; q < s.own; q += kRowsPerPass(L)) {
-            const int k1 = s.sptr[q + 1];
-            double acc = 0.0;
-            for (int k = s.sptr[q]; k < k1; ++k) acc = fma(s.sval[k], s.stage[static_cast<int>(s.sidx[k]) * L + lane], acc);
-            Q[static_cast<int64_t>(t.row0 + q) * rp] = acc;
-            pq = fma(s.stage[q * L + lane], acc, pq);
+#pragma unroll
+        for (int i = 0; i < kOwnMax; ++i) {
+            const int q = rsub + i * RPP;
+            if (q < s.own) {
+                const double acc = lds_row_product<L>(s, q, lane);
+                Q[static_cast<int64_t>(t.row0 + q) * rp] = acc;
+                pq = fma(v[i], acc, pq);
+            }
         }
     }
     pq = block_sum1<L>(pq);
-    if (threadIdx.x < L) a.part0[static_cast<int64_t>(tix) * L + threadIdx.x] = pq;
+    if (threadIdx.x < L) adt::gview(a.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = pq;
 }
 
 template <int L>
 __global__ __launch_bounds__(kReduceThreads) void cg_alpha(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
-    if (!a.grp_active[gid]) return;
+    if (!adt::gview(a.grp_active)[gid]) return;
     const double pq = group_sum<L>(a, 0, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
-        if (a.flags[sc] == 1) {
-            const double alpha = a.rho[sc] / pq;
+        if (adt::gview(a.flags)[sc] == 1) {
+            const double alpha = adt::gview(a.rho)[sc] / pq;
             if (!(pq > 0.0) || !std::isfinite(alpha)) {
-                a.flags[sc] = -1;
-                a.alpha[sc] = 0.0;
+                adt::gview(a.flags)[sc] = -1;
+                adt::gview(a.alpha)[sc] = 0.0;
                 atomicSub(a.active_total, 1);
             } else {
-                a.alpha[sc] = alpha;
-                a.pq_old[sc] = pq;
+                adt::gview(a.alpha)[sc] = alpha;
+                adt::gview(a.pq_old)[sc] = pq;
                 act = true;
             }
         }
@@ -372,21 +374,21 @@ template <int L>
 __global__ __launch_bounds__(256) void cg_update(const PcgArgs* __restrict__ pa, int parity) {
     const PcgArgs& a = *pa;
     const int tix = tile_index(a);
-    const Tile t = a.tiles[tix];
-    if (!a.grp_active[t.grp]) return;
-    const Sys S = a.sys[t.sys];
+    const Tile t = adt::gview(a.tiles)[tix];
+    if (!adt::gview(a.grp_active)[t.grp]) return;
+    const Sys S = adt::gview(a.sys)[t.sys];
     const int col = (t.grp % a.groups) * L + static_cast<int>(threadIdx.x % L);
     const int64_t sc = static_cast<int64_t>(t.sys) * a.rp + col;
-    const bool act = a.flags[sc] == 1;
+    const bool act = adt::gview(a.flags)[sc] == 1;
     const int rp = a.rp;
-    const double* P = a.p[parity ^ 1] + S.vec_off + col;
-    const double* Q = a.q + S.vec_off + col;
-    double* X = a.x + S.vec_off + col;
-    double* R = a.r + S.vec_off + col;
-    const double* D = a.dinv + S.diag_off;
+    const auto P = adt::gview(a.p[parity ^ 1] + S.vec_off + col);
+    const auto Q = adt::gview(a.q + S.vec_off + col);
+    const auto X = adt::gview(a.x + S.vec_off + col);
+    const auto R = adt::gview(a.r + S.vec_off + col);
+    const auto D = adt::gview(a.dinv + S.diag_off);
     double sz = 0.0, sr = 0.0;
     if (act) {
-        const double alpha = a.alpha[sc];
+        const double alpha = adt::gview(a.alpha)[sc];
         for (int row = t.row0 + static_cast<int>(threadIdx.x / L); row < t.row1; row += kRowsPerPass(L)) {
             const int64_t o = static_cast<int64_t>(row) * rp;
             X[o] = fma(alpha, P[o], X[o]);
@@ -398,8 +400,8 @@ __global__ __launch_bounds__(256) void cg_update(const PcgArgs* __restrict__ pa,
     }
     block_sum2<L>(sz, sr);
     if (threadIdx.x < L) {
-        a.part1[static_cast<int64_t>(tix) * L + threadIdx.x] = sz;
-        a.part2[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
+        adt::gview(a.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = sz;
+        adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
     }
 }
 
@@ -408,25 +410,25 @@ template <int L>
 __global__ __launch_bounds__(kReduceThreads) void cg_beta(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
-    if (!a.grp_active[gid]) return;
-    if (threadIdx.x == 0) a.grp_iters[gid] += 1;
+    if (!adt::gview(a.grp_active)[gid]) return;
+    if (threadIdx.x == 0) adt::gview(a.grp_iters)[gid] += 1;
     const double rz = group_sum<L>(a, 1, gid);
     const double rr = group_sum<L>(a, 2, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
-        if (a.flags[sc] == 1) {
-            a.iters[sc] += 1;
-            a.rr[sc] = rr;
+        if (adt::gview(a.flags)[sc] == 1) {
+            adt::gview(a.iters)[sc] += 1;
+            adt::gview(a.rr)[sc] = rr;
             if (!std::isfinite(rr) || !std::isfinite(rz)) {
-                a.flags[sc] = -1;
+                adt::gview(a.flags)[sc] = -1;
                 atomicSub(a.active_total, 1);
-            } else if (rr <= a.tol2 * a.bb[sc]) {
-                a.flags[sc] = 0;
+            } else if (rr <= a.tol2 * adt::gview(a.bb)[sc]) {
+                adt::gview(a.flags)[sc] = 0;
                 atomicSub(a.active_total, 1);
             } else {
-                a.beta[sc] = rz / a.rho[sc];
-                a.rho[sc] = rz;
+                adt::gview(a.beta)[sc] = rz / adt::gview(a.rho)[sc];
+                adt::gview(a.rho)[sc] = rz;
                 act = true;
             }
         }
@@ -439,30 +441,30 @@ template <int L>
 __global__ __launch_bounds__(kReduceThreads) void cg_check(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
-    if (!a.grp_active[gid]) return;
-    if (threadIdx.x == 0) a.grp_iters[gid] += 1;
+    if (!adt::gview(a.grp_active)[gid]) return;
+    if (threadIdx.x == 0) adt::gview(a.grp_iters)[gid] += 1;
     const double rr = group_sum<L>(a, 2, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
-        if (a.flags[sc] == 1) {
-            a.iters[sc] += 1;
-            a.rr[sc] = rr;
+        if (adt::gview(a.flags)[sc] == 1) {
+            adt::gview(a.iters)[sc] += 1;
+            adt::gview(a.rr)[sc] = rr;
             if (!std::isfinite(rr)) {
-                a.flags[sc] = -1;
+                adt::gview(a.flags)[sc] = -1;
                 atomicSub(a.active_total, 1);
-            } else if (rr <= a.tol2 * a.bb[sc]) {
-                a.flags[sc] = 0;
+            } else if (rr <= a.tol2 * adt::gview(a.bb)[sc]) {
+                adt::gview(a.flags)[sc] = 0;
                 atomicSub(a.active_total, 1);
             } else {
-                if (rr < 0.81 * a.rr_best[sc]) {
-                    a.rr_best[sc] = rr;
-                    a.stall[sc] = 0;
+                if (rr < 0.81 * adt::gview(a.rr_best)[sc]) {
+                    adt::gview(a.rr_best)[sc] = rr;
+                    adt::gview(a.stall)[sc] = 0;
                 } else {
-                    a.stall[sc] += 1;
+                    adt::gview(a.stall)[sc] += 1;
                 }
-                if (a.stall_limit > 0 && a.stall[sc] >= a.stall_limit) {
-                    a.flags[sc] = -2;
+                if (a.stall_limit > 0 && adt::gview(a.stall)[sc] >= a.stall_limit) {
+                    adt::gview(a.flags)[sc] = -2;
                     atomicSub(a.active_total, 1);
                 } else {
                     act = true;
@@ -478,21 +480,21 @@ template <int L>
 __global__ __launch_bounds__(kReduceThreads) void fcg_beta(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
     const int gid = blockIdx.x;
-    if (!a.grp_active[gid]) return;
+    if (!adt::gview(a.grp_active)[gid]) return;
     const double zq = group_sum<L>(a, 0, gid);
     const double zr = group_sum<L>(a, 1, gid);
     bool act = false;
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
-        if (a.flags[sc] == 1) {
-            const bool first = a.iters[sc] == 0;
-            const double beta = first ? 0.0 : -zq / a.pq_old[sc];
+        if (adt::gview(a.flags)[sc] == 1) {
+            const bool first = adt::gview(a.iters)[sc] == 0;
+            const double beta = first ? 0.0 : -zq / adt::gview(a.pq_old)[sc];
             if (!(zr > 0.0) || !std::isfinite(zr) || !std::isfinite(beta)) {
-                a.flags[sc] = -2;  // preconditioner not positive on this residual: finish with Jacobi
+                adt::gview(a.flags)[sc] = -2;  // preconditioner not positive on this residual: finish with Jacobi
                 atomicSub(a.active_total, 1);
             } else {
-                a.beta[sc] = beta;
-                a.rho[sc] = zr;
+                adt::gview(a.beta)[sc] = beta;
+                adt::gview(a.rho)[sc] = zr;
                 act = true;
             }
         }
@@ -507,11 +509,11 @@ __global__ __launch_bounds__(256) void cg_true_residual(const PcgArgs* __restric
     TileCtx c;
     load_tile<L>(a, c);
     const int tix = tile_index(a);
-    const int32_t* ptr = a.indptr + c.S.ptr_off;
-    const int32_t* idx = a.indices + c.S.idx_off;
-    const double* val = a.values + c.S.val_off;
-    const double* B = a.b + c.S.b_off + c.col;
-    const double* X = a.x + c.S.vec_off + c.col;
+    const auto ptr = adt::gview(a.indptr + c.S.ptr_off);
+    const auto idx = adt::gview(a.indices + c.S.idx_off);
+    const auto val = adt::gview(a.values + c.S.val_off);
+    const auto B = adt::gview(a.b + c.S.b_off + c.col);
+    const auto X = adt::gview(a.x + c.S.vec_off + c.col);
     double sr = 0.0;
     if (c.col < a.r_real) {
         for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(256) void cg_true_residual(const PcgArgs* __restric
         }
     }
     sr = block_sum1<L>(sr);
-    if (threadIdx.x < L) a.part2[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
+    if (threadIdx.x < L) adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
 }
 
 template <int L>
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(kReduceThreads) void cg_true_reduce(const PcgArgs* 
     const double rr = group_sum<L>(a, 2, gid);
     if (threadIdx.x < L) {
         const int64_t sc = static_cast<int64_t>(gid / a.groups) * a.rp + (gid % a.groups) * L + threadIdx.x;
-        a.rr[sc] = rr;
+        adt::gview(a.rr)[sc] = rr;
     }
 }
 
@@ -546,16 +548,16 @@ template <int L>
 __global__ __launch_bounds__(256) void reduce_segments(const PcgArgs* __restrict__ pa, int mask) {
     const PcgArgs& a = *pa;
     const int seg = blockIdx.x;
-    if (!(mask & 8) && !a.grp_active[a.seg_grp[seg]]) return;
-    const int t0 = a.seg_tile0[seg], nt = a.seg_ntiles[seg];
+    if (!(mask & 8) && !adt::gview(a.grp_active)[adt::gview(a.seg_grp)[seg]]) return;
+    const int t0 = adt::gview(a.seg_tile0)[seg], nt = adt::gview(a.seg_ntiles)[seg];
     const int lane = static_cast<int>(threadIdx.x % L), chunk = static_cast<int>(threadIdx.x / L);
     for (int k = 0; k < 3; ++k) {
         if (!((mask >> k) & 1)) continue;
-        const double* part = k == 0 ? a.part0 : (k == 1 ? a.part1 : a.part2);
+        const auto part = adt::gview(k == 0 ? a.part0 : (k == 1 ? a.part1 : a.part2));
         double v = 0.0;
         for (int i = chunk; i < nt; i += kRowsPerPass(L)) v += part[static_cast<int64_t>(t0 + i) * L + lane];
         v = block_sum1<L>(v);
-        if (threadIdx.x < L) a.spart[(static_cast<int64_t>(k) * a.n_segs + seg) * L + threadIdx.x] = v;
+        if (threadIdx.x < L) adt::gview(a.spart)[(static_cast<int64_t>(k) * a.n_segs + seg) * L + threadIdx.x] = v;
     }
 }
 
@@ -609,7 +611,7 @@ struct adt_pcg_plan {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int profile = 0;
     int lds_rows = 0;  // staged rows of the LDS spmv (0: gather kernel)
-    int lds_nnz = 0, lds_halo = 0;
+    int lds_nnz = 0;
     uint16_t* lidx_d = nullptr;
     int* halo_ptr_d = nullptr;
     int* halo_rows_d = nullptr;
@@ -671,7 +673,7 @@ int dalloc(T** p, int64_t count) {
 }
 
 int64_t fine_lds_bytes(const adt_pcg_plan* p) {
-    return lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->lds_halo, p->L);
+    return lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->L);
 }
 
 // One V-cycle: z = B r (flexible-CG partials z.r, z.q_old written by the finest post-smoothing).
@@ -877,7 +879,7 @@ struct PatternTiles {
     std::vector<uint16_t> lidx;
     std::vector<int> halo_ptr;  // one entry per tile (start), closed by the caller
     std::vector<int> halo_rows;
-    int max_staged = 0, max_nnz = 0, max_halo = 0;
+    int max_staged = 0, max_nnz = 0;
 };
 
 PatternTiles build_pattern_tiles(const int32_t* ptr, const int32_t* idx, int rows, int rb) {
@@ -894,7 +896,6 @@ PatternTiles build_pattern_tiles(const int32_t* ptr, const int32_t* idx, int row
         const int own = r1 - r0;
         pt.max_staged = std::max(pt.max_staged, own + static_cast<int>(halo.size()));
         pt.max_nnz = std::max(pt.max_nnz, ptr[r1] - ptr[r0]);
-        pt.max_halo = std::max(pt.max_halo, static_cast<int>(halo.size()));
         for (int k = ptr[r0]; k < ptr[r1]; ++k) {
             const int c = idx[k];
             const int slot = (c >= r0 && c < r1)
@@ -952,21 +953,20 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
             hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
             limit > 0)
             lds_bytes = std::min(lds_bytes, limit - 8192);  // the block reductions keep static LDS
-        for (int rb = p->rb; rb >= rpp; rb = (rb / 2 + rpp - 1) / rpp * rpp) {
+        // staged tiles keep their own rows' operands in registers: at most kOwnMax rows per thread
+        for (int rb = std::min(p->rb, max_staged_tile_rows(lanes)); rb >= rpp; rb = (rb / 2 + rpp - 1) / rpp * rpp) {
             ptiles.clear();
-            int max_staged = 0, max_nnz = 0, max_halo = 0;
+            int max_staged = 0, max_nnz = 0;
             for (const auto& key : patterns) {
                 ptiles.push_back(build_pattern_tiles(h_indptr + std::get<0>(key), h_indices + std::get<1>(key),
                                                      std::get<2>(key), rb));
                 max_staged = std::max(max_staged, ptiles.back().max_staged);
                 max_nnz = std::max(max_nnz, ptiles.back().max_nnz);
-                max_halo = std::max(max_halo, ptiles.back().max_halo);
             }
-            if (lds_bytes_for(max_staged, max_nnz, rb, max_halo, lanes) <= lds_bytes && max_staged < 65536) {
+            if (lds_bytes_for(max_staged, max_nnz, rb, lanes) <= lds_bytes && max_staged < 65536) {
                 p->rb = rb;
                 p->lds_rows = max_staged;
                 p->lds_nnz = max_nnz;
-                p->lds_halo = max_halo;
                 break;
             }
             if (rb == rpp) break;
@@ -1120,7 +1120,6 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
     a.lds_rows = p->lds_rows;
     a.lds_nnz = p->lds_nnz;
     a.lds_own = p->rb;
-    a.lds_halo = p->lds_halo;
     a.groups = p->groups;
     a.r_real = r;
     a.n_tiles = p->n_tiles;
