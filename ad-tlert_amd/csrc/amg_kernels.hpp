@@ -401,66 +401,100 @@ __global__ __launch_bounds__(256) void amg_post_smooth_fine(const PcgArgs* __res
 }
 
 // ------------------------------------------------------------------------------------------
-// finest level, LDS-staged versions (tiles and halo tables of the conjugate-gradient product)
+// LDS-staged versions
 // ------------------------------------------------------------------------------------------
-// One pass per tile, one barrier:
-//   * the tile's CSR slice (values, 16-bit tile-local column slots, row pointers) is copied
-//     into LDS while
+// A staged product y_i = f(i, sum_k val_k * x[col_k]) over the rows i of a tile reads every
+// needed row of x once per tile into LDS instead of gathering it per nonzero.  The staged rows
+// are the tile's own rows (smoothing; not for the prolongation, whose x lives on the coarser
+// level) followed by the tile's halo list (host-built per pattern tile, sorted), and every
+// slot of the tile's CSR slice carries the LDS row of its column (16-bit).  One pass, one
+// barrier:
+//   * the CSR slice (values, local column slots, row pointers) is copied into LDS while
 //   * each thread loads its own rows' operands into registers (kept for the product pass) and
 //     the staged values of its share of the halo rows (row ids straight from the global halo
 //     list, halo_batch(L) rows with loads in flight together);
 //   * barrier; the product reads matrix and staged vector from LDS only.
+// Used for the finest level's smoothing (the conjugate-gradient product's tables), the coarse
+// levels' smoothing and every level's prolongation.
+
+// Where a staged product's matrix and tables live (pointers already offset to the system).
+struct StageSrc {
+    const int32_t* ptr;     // row pointers (system rows)
+    const double* val;      // slot values
+    const uint16_t* lidx;   // slot -> staged row of its column
+    const int* halo_ptr;    // per pattern tile: first entry in halo_rows
+    const int* halo_rows;   // staged rows outside the tile (or, for the prolongation, all of them)
+    int lds_rows, lds_nnz, lds_own;  // LDS capacities: staged rows, slots, tile rows
+};
+
+__device__ __forceinline__ StageSrc fine_src(const PcgArgs& g, const Sys& S) {
+    return StageSrc{g.indptr + S.ptr_off, g.values + S.val_off, g.lidx + S.lidx_off, g.halo_ptr, g.halo_rows,
+                    g.lds_rows, g.lds_nnz, g.lds_own};
+}
+
+__device__ __forceinline__ StageSrc level_src(const LevelArgs& a, const LvSys& S) {
+    return StageSrc{a.ptr + S.ptr, a.val + S.val, a.lidx + S.idx, a.halo_ptr, a.halo_rows,
+                    a.lds_rows, a.lds_nnz, a.lds_own};
+}
+
+__device__ __forceinline__ StageSrc prolong_src(const LevelArgs& a, const LvSys& S) {
+    return StageSrc{a.pptr + S.pptr, a.pval + S.pval, a.plidx + S.pidx, a.pc_ptr, a.pc_rows,
+                    a.p_lds_rows, a.p_lds_nnz, a.p_lds_own};
+}
 
 struct LdsTile {
-    double* stage;  // (own + halo rows) x L staged values
+    double* stage;  // staged rows x L values
     double* sval;
     int* sptr;
     uint16_t* sidx;
-    int own, nh, h0, e0, ne;
+    int rows;       // rows of the tile
+    int own;        // own rows staged first (rows, or 0)
+    int nh, h0, e0, ne;
 };
 
-__device__ __forceinline__ LdsTile lds_tile(const PcgArgs& g, const Tile& t, const Sys& S, int L, double* smem) {
+__device__ __forceinline__ LdsTile lds_tile(const StageSrc& src, const Tile& t, int L, double* smem, bool stage_own) {
     LdsTile s;
-    const auto ptr = adt::gview(g.indptr + S.ptr_off);
-    s.own = t.row1 - t.row0;
-    s.h0 = adt::gview(g.halo_ptr)[t.ptile];
-    s.nh = adt::gview(g.halo_ptr)[t.ptile + 1] - s.h0;
+    const auto ptr = adt::gview(src.ptr);
+    s.rows = t.row1 - t.row0;
+    s.own = stage_own ? s.rows : 0;
+    s.h0 = adt::gview(src.halo_ptr)[t.ptile];
+    s.nh = adt::gview(src.halo_ptr)[t.ptile + 1] - s.h0;
     s.e0 = ptr[t.row0];
     s.ne = ptr[t.row1] - s.e0;
     s.stage = smem;
-    s.sval = smem + static_cast<int64_t>(g.lds_rows) * L;
-    s.sptr = reinterpret_cast<int*>(s.sval + g.lds_nnz);
-    s.sidx = reinterpret_cast<uint16_t*>(s.sptr + g.lds_own + 1);
+    s.sval = smem + static_cast<int64_t>(src.lds_rows) * L;
+    s.sptr = reinterpret_cast<int*>(s.sval + src.lds_nnz);
+    s.sidx = reinterpret_cast<uint16_t*>(s.sptr + src.lds_own + 1);
     return s;
 }
 
-// The tile's CSR slice (values, local column slots, row pointers) in registers: the first 512
-// entries and row pointers of thread t (loaded unconditionally, stored predicated).
+// The tile's CSR slice in registers: the first 512 slots and row pointer t of thread t (loaded
+// unconditionally, stored predicated).
 struct CsrRegs {
     double va, vb;
     int pi;
     uint16_t ia, ib;
 };
 
-__device__ __forceinline__ CsrRegs load_csr(const PcgArgs& g, const Tile& t, const Sys& S, const LdsTile& s) {
-    const auto ptr = adt::gview(g.indptr + S.ptr_off);
-    const auto val = adt::gview(g.values + S.val_off + s.e0);
-    const auto lidx = adt::gview(g.lidx + S.lidx_off + s.e0);
+__device__ __forceinline__ CsrRegs load_csr(const StageSrc& src, const Tile& t, const LdsTile& s) {
+    const auto ptr = adt::gview(src.ptr);
+    const auto val = adt::gview(src.val);
+    const auto lidx = adt::gview(src.lidx);
     const int tid = static_cast<int>(threadIdx.x);
-    const int sa = tid < s.ne ? tid : 0, sb = 256 + tid < s.ne ? 256 + tid : 0;
+    // positions past the slice read the system's slot 0 (the slice itself may be empty)
+    const int sa = tid < s.ne ? s.e0 + tid : 0, sb = 256 + tid < s.ne ? s.e0 + 256 + tid : 0;
     CsrRegs c;
     c.va = val[sa];
     c.vb = val[sb];
     c.ia = lidx[sa];
     c.ib = lidx[sb];
-    c.pi = ptr[t.row0 + (tid <= s.own ? tid : 0)];
+    c.pi = ptr[t.row0 + (tid <= s.rows ? tid : 0)];
     return c;
 }
 
-// Stores the slice into LDS (published by the caller's barrier); slices longer than 512 entries
+// Stores the slice into LDS (published by the caller's barrier); slices longer than 512 slots
 // or tiles of more than 255 rows continue with plain loops.
-__device__ __forceinline__ void store_csr(const PcgArgs& g, const Tile& t, const Sys& S, const LdsTile& s,
-                                          const CsrRegs& c) {
+__device__ __forceinline__ void store_csr(const StageSrc& src, const Tile& t, const LdsTile& s, const CsrRegs& c) {
     const int tid = static_cast<int>(threadIdx.x);
     if (tid < s.ne) {
         s.sval[tid] = c.va;
@@ -470,32 +504,32 @@ __device__ __forceinline__ void store_csr(const PcgArgs& g, const Tile& t, const
         s.sval[256 + tid] = c.vb;
         s.sidx[256 + tid] = c.ib;
     }
-    if (tid <= s.own) s.sptr[tid] = c.pi - s.e0;
-    if (s.ne > 512 || s.own >= 256) {
-        const auto ptr = adt::gview(g.indptr + S.ptr_off);
-        const auto val = adt::gview(g.values + S.val_off + s.e0);
-        const auto lidx = adt::gview(g.lidx + S.lidx_off + s.e0);
+    if (tid <= s.rows) s.sptr[tid] = c.pi - s.e0;
+    if (s.ne > 512 || s.rows >= 256) {
+        const auto ptr = adt::gview(src.ptr);
+        const auto val = adt::gview(src.val + s.e0);
+        const auto lidx = adt::gview(src.lidx + s.e0);
         for (int e = 512 + tid; e < s.ne; e += 256) {
             s.sval[e] = val[e];
             s.sidx[e] = lidx[e];
         }
-        for (int i = 256 + tid; i <= s.own; i += 256) s.sptr[i] = ptr[t.row0 + i] - s.e0;
+        for (int i = 256 + tid; i <= s.rows; i += 256) s.sptr[i] = ptr[t.row0 + i] - s.e0;
     }
 }
 
-// Halo row ids of one staging batch: rows base + t / L + u * (256 / L), u < halo_batch(L).
-// Positions past the halo get the tile's first row (a valid row whose value is not stored).
+// Halo row ids of one staging batch: list positions base + t / L + u * (256 / L), u <
+// halo_batch(L).  Positions past the list get row 0 (a valid row whose value is not stored).
 template <int L>
-__device__ __forceinline__ void halo_ids(const PcgArgs& g, const Tile& t, const LdsTile& s, int base,
+__device__ __forceinline__ void halo_ids(const StageSrc& src, const LdsTile& s, int base,
                                          int (&jj)[halo_batch(L)]) {
     constexpr int RPP = kRowsPerPass(L);
-    const auto halo = adt::gview(g.halo_rows);
+    const auto halo = adt::gview(src.halo_rows);
     const int h0 = base + static_cast<int>(threadIdx.x / L);
 #pragma unroll
     for (int u = 0; u < halo_batch(L); ++u) {
         const bool in = h0 + u * RPP < s.nh;
         const int raw = halo[in ? s.h0 + h0 + u * RPP : 0];
-        jj[u] = in ? raw : t.row0;
+        jj[u] = in ? raw : 0;
     }
 }
 
@@ -517,19 +551,19 @@ __device__ __forceinline__ void store_halo(const LdsTile& s, int base, const dou
 
 // Halo batches after the first (tiles with a large halo only).
 template <int L, typename ValueF>
-__device__ __forceinline__ void stage_halo_rest(const PcgArgs& g, const Tile& t, const LdsTile& s, ValueF value) {
+__device__ __forceinline__ void stage_halo_rest(const StageSrc& src, const LdsTile& s, ValueF value) {
     constexpr int step = halo_batch(L) * kRowsPerPass(L);
     for (int base = step; base < s.nh; base += step) {
         int jj[halo_batch(L)];
         double v[halo_batch(L)];
-        halo_ids<L>(g, t, s, base, jj);
+        halo_ids<L>(src, s, base, jj);
         halo_values<L>(jj, v, value);
         store_halo<L>(s, base, v);
     }
 }
 
-// sum_k sval[k] * stage[sidx[k]] over staged row q (never empty: it holds the diagonal), in
-// slot order, 8 slots in flight; slots past the row re-read its first slot and are discarded.
+// sum_k sval[k] * stage[sidx[k]] over tile row q, in slot order, 8 slots in flight; slots past
+// the row re-read its first slot and are discarded (an empty row sums to zero).
 template <int L>
 __device__ __forceinline__ double lds_row_product(const LdsTile& s, int q, int lane) {
     constexpr int U = 8;
@@ -555,7 +589,15 @@ __device__ __forceinline__ double lds_row_product(const LdsTile& s, int q, int l
     return acc;
 }
 
-// r1 = r - A (w D^-1 r), written into the z buffer.
+// Row of own-row slot i of the calling thread (rows past the tile read the tile's first row;
+// their values are never stored).
+template <int L>
+__device__ __forceinline__ int own_row(const Tile& t, int rows, int i) {
+    const int q = static_cast<int>(threadIdx.x / L) + i * kRowsPerPass(L);
+    return t.row0 + (q < rows ? q : 0);
+}
+
+// Finest level: r1 = r - A (w D^-1 r), written into the z buffer.
 template <int L>
 __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __restrict__ pa,
                                                                const LevelArgs* __restrict__ la) {
@@ -574,39 +616,40 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     const auto R = adt::gview(g.r + S.vec_off + col);
     const auto R1 = adt::gview(g.z + S.vec_off + col);
     const auto D = adt::gview(g.dinv + S.diag_off);
-    const LdsTile s = lds_tile(g, t, S, L, smem);
+    const StageSrc src = fine_src(g, S);
+    const LdsTile s = lds_tile(src, t, L, smem, true);
     const auto value = [&](int j) { return w * D[j] * R[static_cast<int64_t>(j) * rp]; };
     int hid[halo_batch(L)];
-    halo_ids<L>(g, t, s, 0, hid);
+    halo_ids<L>(src, s, 0, hid);
     double rv[kOwnMax], dv[kOwnMax];
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
-        const int row = t.row0 + (rsub + i * RPP < s.own ? rsub + i * RPP : 0);  // past the tile: unused
+        const int row = own_row<L>(t, s.rows, i);
         rv[i] = R[static_cast<int64_t>(row) * rp];
         dv[i] = D[row];
     }
-    const CsrRegs cr = load_csr(g, t, S, s);
+    const CsrRegs cr = load_csr(src, t, s);
     double hv[halo_batch(L)];
     halo_values<L>(hid, hv, value);
-    store_csr(g, t, S, s, cr);
+    store_csr(src, t, s, cr);
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
         const int q = rsub + i * RPP;
-        if (q < s.own) s.stage[q * L + lane] = w * dv[i] * rv[i];
+        if (q < s.rows) s.stage[q * L + lane] = w * dv[i] * rv[i];
     }
     store_halo<L>(s, 0, hv);
-    stage_halo_rest<L>(g, t, s, value);
+    stage_halo_rest<L>(src, s, value);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
         const int q = rsub + i * RPP;
-        if (q < s.own) R1[static_cast<int64_t>(t.row0 + q) * rp] = rv[i] - lds_row_product<L>(s, q, lane);
+        if (q < s.rows) R1[static_cast<int64_t>(t.row0 + q) * rp] = rv[i] - lds_row_product<L>(s, q, lane);
     }
 }
 
-// z = z2 + w D^-1 (r - A z2) with z2 (written by the finest prolongation into p[parity ^ 1])
-// staged for the tile and its halo; partials z.q_old (part0) and z.r (part1) for the
-// flexible-CG coefficients.
+// Finest level: z = z2 + w D^-1 (r - A z2) with z2 (written by the finest prolongation into
+// p[parity ^ 1]) staged for the tile and its halo; partials z.q_old (part0) and z.r (part1) for
+// the flexible-CG coefficients.
 template <int L>
 __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __restrict__ pa,
                                                            const LevelArgs* __restrict__ la, int parity) {
@@ -627,37 +670,38 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     const auto Z2 = adt::gview(g.p[parity ^ 1] + S.vec_off + col);
     const auto Z = adt::gview(g.z + S.vec_off + col);
     const auto D = adt::gview(g.dinv + S.diag_off);
-    const LdsTile s = lds_tile(g, t, S, L, smem);
+    const StageSrc src = fine_src(g, S);
+    const LdsTile s = lds_tile(src, t, L, smem, true);
     const auto value = [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; };
     int hid[halo_batch(L)];
-    halo_ids<L>(g, t, s, 0, hid);
+    halo_ids<L>(src, s, 0, hid);
     double zv[kOwnMax], rv[kOwnMax], qv[kOwnMax], dv[kOwnMax];
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
-        const int row = t.row0 + (rsub + i * RPP < s.own ? rsub + i * RPP : 0);  // past the tile: unused
+        const int row = own_row<L>(t, s.rows, i);
         const int64_t o = static_cast<int64_t>(row) * rp;
         zv[i] = Z2[o];
         rv[i] = R[o];
         qv[i] = Q[o];
         dv[i] = D[row];
     }
-    const CsrRegs cr = load_csr(g, t, S, s);
+    const CsrRegs cr = load_csr(src, t, s);
     double hv[halo_batch(L)];
     halo_values<L>(hid, hv, value);
-    store_csr(g, t, S, s, cr);
+    store_csr(src, t, s, cr);
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
         const int q = rsub + i * RPP;
-        if (q < s.own) s.stage[q * L + lane] = zv[i];
+        if (q < s.rows) s.stage[q * L + lane] = zv[i];
     }
     store_halo<L>(s, 0, hv);
-    stage_halo_rest<L>(g, t, s, value);
+    stage_halo_rest<L>(src, s, value);
     __syncthreads();
     double zr = 0.0, zq = 0.0;
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
         const int q = rsub + i * RPP;
-        if (q < s.own) {
+        if (q < s.rows) {
             const double acc = lds_row_product<L>(s, q, lane);
             const double zi = fma(w * dv[i], rv[i] - acc, zv[i]);
             Z[static_cast<int64_t>(t.row0 + q) * rp] = zi;
@@ -669,6 +713,146 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     if (threadIdx.x < L) {
         adt::gview(g.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = zq;
         adt::gview(g.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = zr;
+    }
+}
+
+// Coarse levels: r1 = b - A (w D^-1 b) with w D^-1 b staged.
+template <int L>
+__global__ __launch_bounds__(256) void amg_smooth_residual_st(const LevelArgs* __restrict__ la) {
+    extern __shared__ double smem[];
+    const LevelArgs& a = *la;
+    LevelTile c;
+    if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    const int rp = a.rp;
+    const auto D = adt::gview(a.dinv + c.S.dinv);
+    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto R1 = adt::gview(a.r1 + c.S.vec + c.col);
+    const double w = c.S.omega;
+    const StageSrc src = level_src(a, c.S);
+    const LdsTile s = lds_tile(src, c.t, L, smem, true);
+    const auto value = [&](int j) { return w * D[j] * B[static_cast<int64_t>(j) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(src, s, 0, hid);
+    double bv[kOwnMax], dv[kOwnMax];
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int row = own_row<L>(c.t, s.rows, i);
+        bv[i] = B[static_cast<int64_t>(row) * rp];
+        dv[i] = D[row];
+    }
+    const CsrRegs cr = load_csr(src, c.t, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(src, c.t, s, cr);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) s.stage[q * L + lane] = w * dv[i] * bv[i];
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(src, s, value);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) R1[static_cast<int64_t>(c.t.row0 + q) * rp] = bv[i] - lds_row_product<L>(s, q, lane);
+    }
+}
+
+// Coarse levels: e = z2 + w D^-1 (b - A z2) with z2 staged.
+template <int L>
+__global__ __launch_bounds__(256) void amg_post_smooth_st(const LevelArgs* __restrict__ la) {
+    extern __shared__ double smem[];
+    const LevelArgs& a = *la;
+    LevelTile c;
+    if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    const int rp = a.rp;
+    const auto D = adt::gview(a.dinv + c.S.dinv);
+    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto Z2 = adt::gview(a.z2 + c.S.vec + c.col);
+    const auto E = adt::gview(a.e + c.S.vec + c.col);
+    const double w = c.S.omega;
+    const StageSrc src = level_src(a, c.S);
+    const LdsTile s = lds_tile(src, c.t, L, smem, true);
+    const auto value = [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(src, s, 0, hid);
+    double zv[kOwnMax], bv[kOwnMax], dv[kOwnMax];
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int row = own_row<L>(c.t, s.rows, i);
+        const int64_t o = static_cast<int64_t>(row) * rp;
+        zv[i] = Z2[o];
+        bv[i] = B[o];
+        dv[i] = D[row];
+    }
+    const CsrRegs cr = load_csr(src, c.t, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(src, c.t, s, cr);
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) s.stage[q * L + lane] = zv[i];
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(src, s, value);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) {
+            const double acc = lds_row_product<L>(s, q, lane);
+            E[static_cast<int64_t>(c.t.row0 + q) * rp] = fma(w * dv[i], bv[i] - acc, zv[i]);
+        }
+    }
+}
+
+// Any level: z2 = w D^-1 b + P e_{l+1} with the coarse rows that the tile's P rows reference
+// staged (the finest level writes z2 into p[parity ^ 1], see amg_prolong).
+template <int L, bool FINE>
+__global__ __launch_bounds__(256) void amg_prolong_st(const PcgArgs* __restrict__ pa, const LevelArgs* __restrict__ la,
+                                                      int parity) {
+    extern __shared__ double smem[];
+    const LevelArgs& a = *la;
+    LevelTile c;
+    if (!level_tile<L>(a, a.tiles, a.n_tiles, c)) return;
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    constexpr int RPP = kRowsPerPass(L);
+    const int rp = a.rp;
+    const auto D = adt::gview(a.dinv + c.S.dinv);
+    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto NE = adt::gview(a.next_e + c.S.next_vec + c.col);
+    const auto Z2 = adt::gview((FINE ? pa->p[parity ^ 1] : a.z2) + c.S.vec + c.col);
+    const double w = c.S.omega;
+    const StageSrc src = prolong_src(a, c.S);
+    const LdsTile s = lds_tile(src, c.t, L, smem, false);
+    const auto value = [&](int J) { return NE[static_cast<int64_t>(J) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(src, s, 0, hid);
+    double bv[kOwnMax], dv[kOwnMax];
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int row = own_row<L>(c.t, s.rows, i);
+        bv[i] = B[static_cast<int64_t>(row) * rp];
+        dv[i] = D[row];
+    }
+    const CsrRegs cr = load_csr(src, c.t, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(src, c.t, s, cr);
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(src, s, value);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kOwnMax; ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows)
+            Z2[static_cast<int64_t>(c.t.row0 + q) * rp] = fma(w * dv[i], bv[i], lds_row_product<L>(s, q, lane));
     }
 }
 

@@ -132,6 +132,17 @@ struct LevelArgs {
     double* z2;        // pre-smoothed + prolonged correction (levels >= 1, or finest without LDS)
     double* next_b;
     const double* next_e;
+    // LDS staging of this level's smoothing (levels >= 1; lidx == NULL: gather kernels)
+    const uint16_t* lidx;   // per matrix slot: staged row of its column
+    const int* halo_ptr;    // per pattern tile: first entry in halo_rows
+    const int* halo_rows;
+    int lds_rows, lds_nnz, lds_own, lds_pad;
+    // LDS staging of this level's prolongation (plidx == NULL: gather kernel): per pattern tile
+    // the coarse rows that its P rows reference, and per P slot the staged row of its column
+    const uint16_t* plidx;
+    const int* pc_ptr;
+    const int* pc_rows;
+    int p_lds_rows, p_lds_nnz, p_lds_own, p_lds_pad;
 };
 
 constexpr int kRowsPerPass(int L) { return 256 / L; }

@@ -300,40 +300,38 @@ __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ p
     const auto Pnew = adt::gview(a.p[parity ^ 1] + S.vec_off + col);
     const auto Q = adt::gview(a.q + S.vec_off + col);
     const auto D = adt::gview(a.dinv + S.diag_off);
-    const LdsTile s = lds_tile(a, t, S, L, smem);
+    const StageSrc src = fine_src(a, S);
+    const LdsTile s = lds_tile(src, t, L, smem, true);
     // every lane stages (inactive lanes' values are never read); only active lanes write p_new
     const auto value = [&](int j) { return direction<FLEX>(D, R, Z, Pold, beta, j, rp); };
     int hid[halo_batch(L)];
-    halo_ids<L>(a, t, s, 0, hid);
+    halo_ids<L>(src, s, 0, hid);
     double v[kOwnMax];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
-        const int q = rsub + i * RPP < s.own ? rsub + i * RPP : 0;  // past the tile: unused
-        v[i] = value(t.row0 + q);
-    }
-    const CsrRegs cr = load_csr(a, t, S, s);
+    for (int i = 0; i < kOwnMax; ++i) v[i] = value(own_row<L>(t, s.rows, i));
+    const CsrRegs cr = load_csr(src, t, s);
     double hv[halo_batch(L)];
     halo_values<L>(hid, hv, value);
-    store_csr(a, t, S, s, cr);
+    store_csr(src, t, s, cr);
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
         const int q = rsub + i * RPP;
-        if (q < s.own) s.stage[q * L + lane] = v[i];
+        if (q < s.rows) s.stage[q * L + lane] = v[i];
     }
     store_halo<L>(s, 0, hv);
 #pragma unroll
     for (int i = 0; i < kOwnMax; ++i) {
         const int q = rsub + i * RPP;
-        if (act && q < s.own) Pnew[static_cast<int64_t>(t.row0 + q) * rp] = v[i];
+        if (act && q < s.rows) Pnew[static_cast<int64_t>(t.row0 + q) * rp] = v[i];
     }
-    stage_halo_rest<L>(a, t, s, value);
+    stage_halo_rest<L>(src, s, value);
     __syncthreads();
     double pq = 0.0;
     if (act) {
 #pragma unroll
         for (int i = 0; i < kOwnMax; ++i) {
             const int q = rsub + i * RPP;
-            if (q < s.own) {
+            if (q < s.rows) {
                 const double acc = lds_row_product<L>(s, q, lane);
                 Q[static_cast<int64_t>(t.row0 + q) * rp] = acc;
                 pq = fma(v[i], acc, pq);
@@ -571,6 +569,8 @@ struct AmgLevel {
     LevelArgs args_h{};
     LevelArgs* args_d = nullptr;
     int64_t dense_len = 0;
+    int smooth_lds = 0;   // dynamic LDS of the staged smoothing kernels (0: gather kernels)
+    int prolong_lds = 0;  // dynamic LDS of the staged prolongation (0: gather kernel)
 };
 
 }  // namespace
@@ -677,6 +677,7 @@ int64_t fine_lds_bytes(const adt_pcg_plan* p) {
 }
 
 // One V-cycle: z = B r (flexible-CG partials z.r, z.q_old written by the finest post-smoothing).
+// Staged (LDS) kernels wherever the plan built their tables, gather kernels elsewhere.
 template <int L>
 void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
     const int m = static_cast<int>(p->amg.size());
@@ -689,17 +690,32 @@ void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
         hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, f.args_d);
     hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[1].n_tiles), dim3(256), 0, st, f.args_d);
     for (int l = 1; l < m - 1; ++l) {
-        hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
-        hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[l + 1].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+        const auto& A = p->amg[l];
+        if (A.smooth_lds > 0)
+            hipLaunchKernelGGL(amg_smooth_residual_st<L>, dim3(A.n_tiles), dim3(256), A.smooth_lds, st, A.args_d);
+        else
+            hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(A.n_tiles), dim3(256), 0, st, A.args_d);
+        hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[l + 1].n_tiles), dim3(256), 0, st, A.args_d);
     }
     hipLaunchKernelGGL(amg_coarse_solve<L>, dim3(p->n_groups, adt::blocks_for(p->amg[m - 1].max_rows, 256 / L)),
                        dim3(256), 0, st, p->amg[m - 1].args_d);
     for (int l = m - 2; l >= 1; --l) {
-        hipLaunchKernelGGL((amg_prolong<L, false>), dim3(p->amg[l].n_tiles), dim3(256), 0, st, pa, p->amg[l].args_d,
-                           parity);
-        hipLaunchKernelGGL(amg_post_smooth<L>, dim3(p->amg[l].n_tiles), dim3(256), 0, st, p->amg[l].args_d);
+        const auto& A = p->amg[l];
+        if (A.prolong_lds > 0)
+            hipLaunchKernelGGL((amg_prolong_st<L, false>), dim3(A.n_tiles), dim3(256), A.prolong_lds, st, pa, A.args_d,
+                               parity);
+        else
+            hipLaunchKernelGGL((amg_prolong<L, false>), dim3(A.n_tiles), dim3(256), 0, st, pa, A.args_d, parity);
+        if (A.smooth_lds > 0)
+            hipLaunchKernelGGL(amg_post_smooth_st<L>, dim3(A.n_tiles), dim3(256), A.smooth_lds, st, A.args_d);
+        else
+            hipLaunchKernelGGL(amg_post_smooth<L>, dim3(A.n_tiles), dim3(256), 0, st, A.args_d);
     }
-    hipLaunchKernelGGL((amg_prolong<L, true>), dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d, parity);
+    if (f.prolong_lds > 0)
+        hipLaunchKernelGGL((amg_prolong_st<L, true>), dim3(p->n_tiles), dim3(256), f.prolong_lds, st, pa, f.args_d,
+                           parity);
+    else
+        hipLaunchKernelGGL((amg_prolong<L, true>), dim3(p->n_tiles), dim3(256), 0, st, pa, f.args_d, parity);
     if (p->lds_rows > 0)
         hipLaunchKernelGGL(amg_post_smooth_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d, parity);
     else
@@ -909,6 +925,34 @@ PatternTiles build_pattern_tiles(const int32_t* ptr, const int32_t* idx, int row
     return pt;
 }
 
+// Prolongation staging tables of one pattern's level: per tile of rb rows the sorted coarse rows
+// that its P rows reference, and per P slot the position of its column in that list.
+struct ProlongTiles {
+    std::vector<uint16_t> plidx;
+    std::vector<int> pc_ptr;  // one entry per tile (start), closed by the caller
+    std::vector<int> pc_rows;
+    int max_staged = 0, max_nnz = 0;
+};
+
+ProlongTiles build_prolong_tiles(const std::vector<int>& pptr, const std::vector<int>& pidx, int rows, int rb) {
+    ProlongTiles pt;
+    pt.plidx.resize(pidx.size());
+    std::vector<int> cols;
+    for (int r0 = 0; r0 < rows; r0 += rb) {
+        const int r1 = std::min(r0 + rb, rows);
+        cols.assign(pidx.begin() + pptr[r0], pidx.begin() + pptr[r1]);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        pt.max_staged = std::max(pt.max_staged, static_cast<int>(cols.size()));
+        pt.max_nnz = std::max(pt.max_nnz, pptr[r1] - pptr[r0]);
+        for (int k = pptr[r0]; k < pptr[r1]; ++k)
+            pt.plidx[k] = static_cast<uint16_t>(std::lower_bound(cols.begin(), cols.end(), pidx[k]) - cols.begin());
+        pt.pc_ptr.push_back(static_cast<int>(pt.pc_rows.size()));
+        pt.pc_rows.insert(pt.pc_rows.end(), cols.begin(), cols.end());
+    }
+    return pt;
+}
+
 typedef std::tuple<int64_t, int64_t, int> PatKey;
 
 }  // namespace
@@ -973,12 +1017,15 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
         }
         if (!p->lds_rows) ptiles.clear();
     }
+    // pattern tiles (row ranges of one pattern) are numbered pattern by pattern; per-pattern-tile
+    // tables (halo lists here, prolongation lists of the multigrid levels) use that numbering
     std::vector<int64_t> lidx_base(patterns.size(), 0), ptile_base(patterns.size(), 0);
+    for (size_t q = 1; q < patterns.size(); ++q)
+        ptile_base[q] = ptile_base[q - 1] + (std::get<2>(patterns[q - 1]) + p->rb - 1) / p->rb;
     std::vector<uint16_t> lidx_all;
     std::vector<int> halo_ptr_all, halo_rows_all;
     for (size_t q = 0; q < ptiles.size(); ++q) {
         lidx_base[q] = static_cast<int64_t>(lidx_all.size());
-        ptile_base[q] = static_cast<int64_t>(halo_ptr_all.size());
         lidx_all.insert(lidx_all.end(), ptiles[q].lidx.begin(), ptiles[q].lidx.end());
         const int shift = static_cast<int>(halo_rows_all.size());
         for (int h : ptiles[q].halo_ptr) halo_ptr_all.push_back(h + shift);
@@ -1000,7 +1047,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
             int count = 0;
             for (int r0 = 0; r0 < h_rows[s]; r0 += p->rb, ++count)
                 tiles.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + p->rb, h_rows[s]),
-                                     p->lds_rows ? static_cast<int>(ptile_base[q]) + count : -1, 0});
+                                     static_cast<int>(ptile_base[q]) + count, 0});
             nt.push_back(count);
         }
     }
@@ -1202,6 +1249,11 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
     p->amg.resize(depth);
     const int rp = p->rp, L = p->L;
     const int rbc = std::max(256 / L, 32);  // coarse-level tile height
+    int lds_limit = 0, device = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        lds_limit = 0;
+    lds_limit = std::max(0, std::min(lds_limit - 8192, 65536));  // the block reductions keep static LDS
     // pattern-level offsets, per level
     std::vector<std::vector<int64_t>> off_ptr(depth, std::vector<int64_t>(n_pat)), off_idx = off_ptr,
         off_diag = off_ptr, off_pptr = off_ptr, off_pidx = off_ptr, off_pmp = off_ptr, off_pmap = off_ptr,
@@ -1281,6 +1333,62 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         ADT_UP(dmp, dmap_ptr)
         ADT_UP(dmap, dmap)
 #undef ADT_UP
+        // LDS staging tables: smoothing on levels >= 1, prolongation on every level but the
+        // coarsest; per pattern tile (tiles of th rows, numbered pattern by pattern), kept only
+        // when the staged tile fits the LDS budget
+        const int th = l == 0 ? p->rb : rbc;
+        if (th > max_staged_tile_rows(L)) continue;
+        std::vector<uint16_t> lidx_l, plidx_l;
+        std::vector<int> hp_l, hr_l, pcp_l, pcr_l;
+        int s_rows = 0, s_nnz = 0, p_rows = 0, p_nnz = 0;
+        for (int q = 0; q < n_pat; ++q) {
+            const HostLevel& h = H[q][l];
+            if (l > 0) {
+                PatternTiles pt = build_pattern_tiles(h.ptr.data(), h.idx.data(), h.n, th);
+                const int shift = static_cast<int>(hr_l.size());
+                for (int v : pt.halo_ptr) hp_l.push_back(v + shift);
+                append(hr_l, pt.halo_rows);
+                lidx_l.insert(lidx_l.end(), pt.lidx.begin(), pt.lidx.end());
+                s_rows = std::max(s_rows, pt.max_staged);
+                s_nnz = std::max(s_nnz, pt.max_nnz);
+            }
+            if (l + 1 < depth) {
+                ProlongTiles pr = build_prolong_tiles(h.pptr, h.pidx, h.n, th);
+                const int shift = static_cast<int>(pcr_l.size());
+                for (int v : pr.pc_ptr) pcp_l.push_back(v + shift);
+                append(pcr_l, pr.pc_rows);
+                plidx_l.insert(plidx_l.end(), pr.plidx.begin(), pr.plidx.end());
+                p_rows = std::max(p_rows, pr.max_staged);
+                p_nnz = std::max(p_nnz, pr.max_nnz);
+            }
+        }
+        hp_l.push_back(static_cast<int>(hr_l.size()));
+        pcp_l.push_back(static_cast<int>(pcr_l.size()));
+        const int64_t s_lds = lds_bytes_for(s_rows, s_nnz, th, L), p_lds = lds_bytes_for(p_rows, p_nnz, th, L);
+        if (l > 0 && s_rows < 65536 && s_lds <= lds_limit) {
+            uint16_t* d16 = nullptr;
+            if ((rc = upload(p, lidx_l, &d16)) || (rc = upload(p, hp_l, &di))) return rc;
+            a.lidx = d16;
+            a.halo_ptr = di;
+            if ((rc = upload(p, hr_l, &di))) return rc;
+            a.halo_rows = di;
+            a.lds_rows = s_rows;
+            a.lds_nnz = s_nnz;
+            a.lds_own = th;
+            A.smooth_lds = static_cast<int>(s_lds);
+        }
+        if (l + 1 < depth && p_rows < 65536 && p_lds <= lds_limit) {
+            uint16_t* d16 = nullptr;
+            if ((rc = upload(p, plidx_l, &d16)) || (rc = upload(p, pcp_l, &di))) return rc;
+            a.plidx = d16;
+            a.pc_ptr = di;
+            if ((rc = upload(p, pcr_l, &di))) return rc;
+            a.pc_rows = di;
+            a.p_lds_rows = p_rows;
+            a.p_lds_nnz = p_nnz;
+            a.p_lds_own = th;
+            A.prolong_lds = static_cast<int>(p_lds);
+        }
     }
     // per-system records, tiles and numeric buffers
     std::vector<int64_t> val_len(depth, 0), vec_len(depth, 0), dinv_len(depth, 0), pv_len(depth, 0),
@@ -1341,10 +1449,13 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
             A.max_p = std::max(A.max_p, S.n_p);
             A.max_t = std::max(A.max_t, S.n_t);
             A.max_next = std::max(A.max_next, S.n_next);
-            if (l > 0)
+            if (l > 0) {
+                int base = 0;  // pattern tiles of this level before pattern q
+                for (int q2 = 0; q2 < q; ++q2) base += (H[q2][l].n + rbc - 1) / rbc;
                 for (int g = 0; g < p->groups; ++g)
                     for (int r0 = 0; r0 < h.n; r0 += rbc)
-                        A.tiles_h.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + rbc, h.n), -1, 0});
+                        A.tiles_h.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + rbc, h.n), base + r0 / rbc, 0});
+            }
         }
     }
     // next-level offsets (value and vector blocks of the same system one level down)

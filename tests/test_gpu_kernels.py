@@ -218,6 +218,60 @@ def test_lane_groups_of_many_tiles_use_segmented_sums(precond):
     solve_batch(precond, True, side=48, lanes=64, R=64, seed=5, rows_per_tile=4)
 
 
+def test_multigrid_with_tiles_too_tall_for_staging_uses_the_gather_kernels():
+    """256-row tiles exceed what the staged kernels hold (8 rows per thread): the finest level
+    then runs the gather versions of every cycle kernel; same tolerance and dense solutions."""
+
+    solve_batch(native.PRECOND_AMG, False, side=24, lanes=16, R=13, seed=4, rows_per_tile=256)
+
+
+def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
+    """Uneven meshes in one plan: two CSR patterns (400 and 169 rows) interleaved over four
+    systems, each pattern with its own hierarchy and staging tables; every solution matches a
+    dense solve and the multigrid iterates far less than Jacobi PCG."""
+
+    rng = np.random.default_rng(11)
+    (pa, ia, va, da), (pb, ib, vb, db) = random_spd_batch(rng, 20), random_spd_batch(rng, 13)
+    indptr, indices = np.concatenate([pa, pb]), np.concatenate([ia, ib])
+    order = [(0, 0), (1, 0), (0, 1), (1, 1)]  # (pattern, system of that pattern)
+    vals = [(va, vb)[q][k] for q, k in order]
+    dense = [(da, db)[q][k] for q, k in order]
+    rows = np.array([(400, 169)[q] for q, _ in order])
+    ptr_off = np.array([(0, pa.size)[q] for q, _ in order])
+    idx_off = np.array([(0, ia.size)[q] for q, _ in order])
+    val_off = np.concatenate(([0], np.cumsum([v.size for v in vals])[:-1]))
+    values_d = torch.as_tensor(np.concatenate(vals), device=dev())
+    R = 13
+    results = {}
+    for precond in (native.PRECOND_JACOBI, native.PRECOND_AMG):
+        plan = native.PcgPlan(rows, R, 16, 32, ptr_off, idx_off, val_off, host_pattern=(indptr, indices),
+                              lds_bytes=8192)
+        assert plan.lds_rows > 0
+        if precond == native.PRECOND_AMG:
+            assert plan.attach_amg(values_d, coarse_max=16) >= 3
+            coarse = [plan.amg_level(1, sys=s)["rows"] for s in range(4)]
+            assert coarse[0] == coarse[2] < 400 and coarse[1] == coarse[3] < 169 and coarse[0] != coarse[1]
+        b = np.zeros(plan.vec_len)
+        sols = []
+        for s in range(4):
+            bs = rng.standard_normal((rows[s], plan.rp))
+            bs[:, R:] = 0.0
+            b[plan.vec_off[s]:plan.vec_off[s] + bs.size] = bs.reshape(-1)
+            sols.append(np.linalg.solve(dense[s], bs[:, :R]))
+        x = torch.zeros(plan.vec_len, dtype=torch.float64, device=dev())
+        rc, iters, relres = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()),
+                                       values_d, torch.as_tensor(b, device=dev()), x, 1e-10, 5000, precond=precond)
+        assert rc == 0, native.library().adt_last_error()
+        assert relres.max() <= 1.5e-10
+        xh = x.cpu().numpy()
+        for s in range(4):
+            xs = xh[plan.vec_off[s]:plan.vec_off[s] + rows[s] * plan.rp].reshape(rows[s], plan.rp)[:, :R]
+            assert relmax(xs, sols[s]) < 1e-7, s
+        results[precond] = iters
+        plan.close()
+    assert results[native.PRECOND_AMG].max() < 0.5 * results[native.PRECOND_JACOBI].max()
+
+
 @pytest.mark.parametrize("k, S, rp", [(3, 9, 16), (4, 40, 64), (10, 128, 128), (3, 96, 96)])
 def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp):
     """Gram-form Jacobian rows from interleaved (N, Rp) blocks equal the per-output formula
