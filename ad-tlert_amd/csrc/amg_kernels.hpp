@@ -237,6 +237,53 @@ __global__ __launch_bounds__(256) void amg_restrict(const LevelArgs* __restrict_
     }
 }
 
+// b_{l+1} = P^T r1 with the tile's restriction slice (fine row and P value of every entry of
+// its coarse rows) staged in LDS by coalesced loads: only the r1 rows are gathered per entry.
+template <int L>
+__global__ __launch_bounds__(256) void amg_restrict_st(const LevelArgs* __restrict__ la) {
+    extern __shared__ double smem[];
+    const LevelArgs& a = *la;
+    LevelTile c;
+    if (!level_tile<L>(a, a.next_tiles, a.n_next_tiles, c)) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const int rp = a.rp, tid = static_cast<int>(threadIdx.x);
+    const auto rptr = adt::gview(a.rptr + c.S.rptr);
+    const auto rrow = adt::gview(a.rrow + c.S.rent);
+    const auto rslot = adt::gview(a.rslot + c.S.rent);
+    const auto P = adt::gview(a.pval + c.S.pval);
+    const auto R1 = adt::gview(a.r1 + c.S.vec + c.col);
+    const auto NB = adt::gview(a.next_b + c.S.next_vec + c.col);
+    const int rows = c.t.row1 - c.t.row0;
+    const int e0 = rptr[c.t.row0], ne = rptr[c.t.row1] - e0;
+    double* sval = smem;
+    int* srow = reinterpret_cast<int*>(sval + a.r_lds_nnz);
+    int* sptr = srow + a.r_lds_nnz;
+    for (int e = tid; e < ne; e += 256) {
+        srow[e] = rrow[e0 + e];
+        sval[e] = P[rslot[e0 + e]];
+    }
+    for (int i = tid; i <= rows; i += 256) sptr[i] = rptr[c.t.row0 + i] - e0;
+    __syncthreads();
+    for (int q0 = tid / L; q0 < rows; q0 += kGatherRows * RPP) {
+        int k0[kGatherRows], k1[kGatherRows];
+        double acc[kGatherRows] = {};
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int q = q0 + i * RPP;
+            k0[i] = q < rows ? sptr[q] : 0;
+            k1[i] = q < rows ? sptr[q + 1] : 0;
+        }
+        gather_rows<kGatherRows, kGatherSlots>(
+            k0, k1, [&](int e) { return srow[e]; }, [&](int e) { return sval[e]; },
+            [&](int i) { return R1[static_cast<int64_t>(i) * rp]; }, acc);
+#pragma unroll
+        for (int i = 0; i < kGatherRows; ++i) {
+            const int q = q0 + i * RPP;
+            if (q < rows) NB[static_cast<int64_t>(c.t.row0 + q) * rp] = acc[i];
+        }
+    }
+}
+
 // e = A_c^-1 b on the coarsest level: workgroup (lane group, block of 256 / L rows), one
 // thread per (row, lane)
 template <int L>

@@ -143,6 +143,8 @@ struct LevelArgs {
     const int* pc_ptr;
     const int* pc_rows;
     int p_lds_rows, p_lds_nnz, p_lds_own, p_lds_pad;
+    // LDS staging of the restriction to the next level: largest slice of a next-level tile
+    int r_lds_nnz, r_lds_pad;
 };
 
 constexpr int kRowsPerPass(int L) { return 256 / L; }

@@ -571,6 +571,7 @@ struct AmgLevel {
     int64_t dense_len = 0;
     int smooth_lds = 0;   // dynamic LDS of the staged smoothing kernels (0: gather kernels)
     int prolong_lds = 0;  // dynamic LDS of the staged prolongation (0: gather kernel)
+    int restrict_lds = 0; // dynamic LDS of the staged restriction to the next level (0: gather kernel)
 };
 
 }  // namespace
@@ -676,6 +677,17 @@ int64_t fine_lds_bytes(const adt_pcg_plan* p) {
     return lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->L);
 }
 
+// Restriction from level l to level l + 1 (tiles of the next level).
+template <int L>
+void launch_restrict(const adt_pcg_plan* p, int l, hipStream_t st) {
+    const auto& A = p->amg[l];
+    const int tiles = p->amg[l + 1].n_tiles;
+    if (A.restrict_lds > 0)
+        hipLaunchKernelGGL(amg_restrict_st<L>, dim3(tiles), dim3(256), A.restrict_lds, st, A.args_d);
+    else
+        hipLaunchKernelGGL(amg_restrict<L>, dim3(tiles), dim3(256), 0, st, A.args_d);
+}
+
 // One V-cycle: z = B r (flexible-CG partials z.r, z.q_old written by the finest post-smoothing).
 // Staged (LDS) kernels wherever the plan built their tables, gather kernels elsewhere.
 template <int L>
@@ -688,14 +700,14 @@ void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
         hipLaunchKernelGGL(amg_smooth_residual_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d);
     else
         hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, f.args_d);
-    hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[1].n_tiles), dim3(256), 0, st, f.args_d);
+    launch_restrict<L>(p, 0, st);
     for (int l = 1; l < m - 1; ++l) {
         const auto& A = p->amg[l];
         if (A.smooth_lds > 0)
             hipLaunchKernelGGL(amg_smooth_residual_st<L>, dim3(A.n_tiles), dim3(256), A.smooth_lds, st, A.args_d);
         else
             hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(A.n_tiles), dim3(256), 0, st, A.args_d);
-        hipLaunchKernelGGL(amg_restrict<L>, dim3(p->amg[l + 1].n_tiles), dim3(256), 0, st, A.args_d);
+        launch_restrict<L>(p, l, st);
     }
     hipLaunchKernelGGL(amg_coarse_solve<L>, dim3(p->n_groups, adt::blocks_for(p->amg[m - 1].max_rows, 256 / L)),
                        dim3(256), 0, st, p->amg[m - 1].args_d);
@@ -1336,6 +1348,19 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         // LDS staging tables: smoothing on levels >= 1, prolongation on every level but the
         // coarsest; per pattern tile (tiles of th rows, numbered pattern by pattern), kept only
         // when the staged tile fits the LDS budget
+        if (l + 1 < depth) {  // restriction: slices of the next level's tiles (rbc coarse rows)
+            int r_nnz = 0;
+            for (int q = 0; q < n_pat; ++q) {
+                const HostLevel& h = H[q][l];
+                for (int r0 = 0; r0 < h.nc; r0 += rbc)
+                    r_nnz = std::max(r_nnz, h.rptr[std::min(r0 + rbc, h.nc)] - h.rptr[r0]);
+            }
+            const int64_t r_lds = static_cast<int64_t>(r_nnz) * 12 + (rbc + 1) * 4;
+            if (r_lds <= lds_limit) {
+                a.r_lds_nnz = r_nnz;
+                A.restrict_lds = static_cast<int>(r_lds);
+            }
+        }
         const int th = l == 0 ? p->rb : rbc;
         if (th > max_staged_tile_rows(L)) continue;
         std::vector<uint16_t> lidx_l, plidx_l;
