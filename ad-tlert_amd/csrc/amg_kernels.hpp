@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void amg_prolong(const PcgArgs* __restrict__ p
     const auto pidx = adt::gview(a.pidx + c.S.pidx);
     const auto P = adt::gview(a.pval + c.S.pval);
     const auto D = adt::gview(a.dinv + c.S.dinv);
-    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto B = adt::gview((FINE ? residual(*pa, parity) : a.b) + c.S.vec + c.col);
     const auto NE = adt::gview(a.next_e + c.S.next_vec + c.col);
     const auto Z2 = adt::gview((FINE ? pa->p[parity ^ 1] : a.z2) + c.S.vec + c.col);
     const double w = c.S.omega;
@@ -668,9 +668,9 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     const auto value = [&](int j) { return w * D[j] * R[static_cast<int64_t>(j) * rp]; };
     int hid[halo_batch(L)];
     halo_ids<L>(src, s, 0, hid);
-    double rv[kOwnMax], dv[kOwnMax];
+    double rv[own_max(L)], dv[own_max(L)];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int row = own_row<L>(t, s.rows, i);
         rv[i] = R[static_cast<int64_t>(row) * rp];
         dv[i] = D[row];
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     halo_values<L>(hid, hv, value);
     store_csr(src, t, s, cr);
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) s.stage[q * L + lane] = w * dv[i] * rv[i];
     }
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     stage_halo_rest<L>(src, s, value);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) R1[static_cast<int64_t>(t.row0 + q) * rp] = rv[i] - lds_row_product<L>(s, q, lane);
     }
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
     const int col = (t.grp % g.groups) * L + lane;
     const int rp = g.rp;
-    const auto R = adt::gview(g.r + S.vec_off + col);
+    const auto R = adt::gview(residual(g, parity) + S.vec_off + col);
     const auto Q = adt::gview(g.q + S.vec_off + col);
     const auto Z2 = adt::gview(g.p[parity ^ 1] + S.vec_off + col);
     const auto Z = adt::gview(g.z + S.vec_off + col);
@@ -722,9 +722,9 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     const auto value = [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; };
     int hid[halo_batch(L)];
     halo_ids<L>(src, s, 0, hid);
-    double zv[kOwnMax], rv[kOwnMax], qv[kOwnMax], dv[kOwnMax];
+    double zv[own_max(L)], rv[own_max(L)], qv[own_max(L)], dv[own_max(L)];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int row = own_row<L>(t, s.rows, i);
         const int64_t o = static_cast<int64_t>(row) * rp;
         zv[i] = Z2[o];
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     halo_values<L>(hid, hv, value);
     store_csr(src, t, s, cr);
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) s.stage[q * L + lane] = zv[i];
     }
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth_lds(const PcgArgs* __rest
     __syncthreads();
     double zr = 0.0, zq = 0.0;
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) {
             const double acc = lds_row_product<L>(s, q, lane);
@@ -782,9 +782,9 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_st(const LevelArgs* _
     const auto value = [&](int j) { return w * D[j] * B[static_cast<int64_t>(j) * rp]; };
     int hid[halo_batch(L)];
     halo_ids<L>(src, s, 0, hid);
-    double bv[kOwnMax], dv[kOwnMax];
+    double bv[own_max(L)], dv[own_max(L)];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int row = own_row<L>(c.t, s.rows, i);
         bv[i] = B[static_cast<int64_t>(row) * rp];
         dv[i] = D[row];
@@ -794,7 +794,7 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_st(const LevelArgs* _
     halo_values<L>(hid, hv, value);
     store_csr(src, c.t, s, cr);
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) s.stage[q * L + lane] = w * dv[i] * bv[i];
     }
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_st(const LevelArgs* _
     stage_halo_rest<L>(src, s, value);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) R1[static_cast<int64_t>(c.t.row0 + q) * rp] = bv[i] - lds_row_product<L>(s, q, lane);
     }
@@ -828,9 +828,9 @@ __global__ __launch_bounds__(256) void amg_post_smooth_st(const LevelArgs* __res
     const auto value = [&](int j) { return Z2[static_cast<int64_t>(j) * rp]; };
     int hid[halo_batch(L)];
     halo_ids<L>(src, s, 0, hid);
-    double zv[kOwnMax], bv[kOwnMax], dv[kOwnMax];
+    double zv[own_max(L)], bv[own_max(L)], dv[own_max(L)];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int row = own_row<L>(c.t, s.rows, i);
         const int64_t o = static_cast<int64_t>(row) * rp;
         zv[i] = Z2[o];
@@ -842,7 +842,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth_st(const LevelArgs* __res
     halo_values<L>(hid, hv, value);
     store_csr(src, c.t, s, cr);
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) s.stage[q * L + lane] = zv[i];
     }
@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void amg_post_smooth_st(const LevelArgs* __res
     stage_halo_rest<L>(src, s, value);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) {
             const double acc = lds_row_product<L>(s, q, lane);
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(256) void amg_prolong_st(const PcgArgs* __restrict_
     constexpr int RPP = kRowsPerPass(L);
     const int rp = a.rp;
     const auto D = adt::gview(a.dinv + c.S.dinv);
-    const auto B = adt::gview(a.b + c.S.vec + c.col);
+    const auto B = adt::gview((FINE ? residual(*pa, parity) : a.b) + c.S.vec + c.col);
     const auto NE = adt::gview(a.next_e + c.S.next_vec + c.col);
     const auto Z2 = adt::gview((FINE ? pa->p[parity ^ 1] : a.z2) + c.S.vec + c.col);
     const double w = c.S.omega;
@@ -881,9 +881,9 @@ __global__ __launch_bounds__(256) void amg_prolong_st(const PcgArgs* __restrict_
     const auto value = [&](int J) { return NE[static_cast<int64_t>(J) * rp]; };
     int hid[halo_batch(L)];
     halo_ids<L>(src, s, 0, hid);
-    double bv[kOwnMax], dv[kOwnMax];
+    double bv[own_max(L)], dv[own_max(L)];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int row = own_row<L>(c.t, s.rows, i);
         bv[i] = B[static_cast<int64_t>(row) * rp];
         dv[i] = D[row];
@@ -896,7 +896,7 @@ __global__ __launch_bounds__(256) void amg_prolong_st(const PcgArgs* __restrict_
     stage_halo_rest<L>(src, s, value);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows)
             Z2[static_cast<int64_t>(c.t.row0 + q) * rp] = fma(w * dv[i], bv[i], lds_row_product<L>(s, q, lane));

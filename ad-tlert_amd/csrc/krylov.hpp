@@ -68,8 +68,16 @@ struct PcgArgs {
     int lds_rows, lds_nnz, lds_own, lds_pad_;  // LDS-staged product capacities
     int stall_limit;                           // iterations without a 10% residual decrease (0: off)
     int segmented, n_segs;
-    int pad_;
+    // Flexible CG with the update fused to the next pre-smoothing: neighbouring tiles read the
+    // residual while it is updated, so it alternates between r (even iterations) and r_alt (odd)
+    int r_pingpong;
+    double* r_alt;
 };
+
+// The finest-level residual of the iteration with this parity.
+__device__ __forceinline__ double* residual(const PcgArgs& g, int parity) {
+    return g.r_pingpong && (parity & 1) ? g.r_alt : g.r;
+}
 
 // One multigrid level of one system.  Offsets index the level's concatenated arrays.  On the
 // finest level the matrix is the caller's (ptr/idx/val offsets are the system's CSR offsets).
@@ -150,12 +158,13 @@ struct LevelArgs {
 constexpr int kRowsPerPass(int L) { return 256 / L; }
 
 // LDS-staged tiles keep each thread's own-row operands in registers from the staging pass to
-// the product pass: a staged tile has at most kOwnMax rows per thread (rows_per_tile <=
-// kOwnMax * 256 / L).  Halo rows are staged halo_batch(L) rows per thread at a time, all of
+// the product pass: a staged tile has at most own_max(L) rows per thread (rows_per_tile <=
+// own_max(L) * 256 / L).  Halo rows are staged halo_batch(L) rows per thread at a time, all of
 // their loads in flight together (one batch covers the halo of the usual 2D and 3D tiles).
-constexpr int kOwnMax = 8;
+// 64-lane tiles (3D) are short and have long halos: fewer own rows, more halo rows per thread.
+constexpr int own_max(int L) { return L >= 64 ? 4 : 8; }
 constexpr int halo_batch(int L) { return L >= 64 ? 24 : 12; }
-constexpr int max_staged_tile_rows(int L) { return kOwnMax * kRowsPerPass(L); }
+constexpr int max_staged_tile_rows(int L) { return own_max(L) * kRowsPerPass(L); }
 
 // Row-batched CSR gather: for the RB rows of one thread, acc[b] += sum_k val(k) * term(idx(k))
 // over k in [k0[b], k1[b]), accumulated per row in slot order.  U slots of every row are

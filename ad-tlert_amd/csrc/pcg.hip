@@ -306,21 +306,21 @@ __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ p
     const auto value = [&](int j) { return direction<FLEX>(D, R, Z, Pold, beta, j, rp); };
     int hid[halo_batch(L)];
     halo_ids<L>(src, s, 0, hid);
-    double v[kOwnMax];
+    double v[own_max(L)];
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) v[i] = value(own_row<L>(t, s.rows, i));
+    for (int i = 0; i < own_max(L); ++i) v[i] = value(own_row<L>(t, s.rows, i));
     const CsrRegs cr = load_csr(src, t, s);
     double hv[halo_batch(L)];
     halo_values<L>(hid, hv, value);
     store_csr(src, t, s, cr);
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (q < s.rows) s.stage[q * L + lane] = v[i];
     }
     store_halo<L>(s, 0, hv);
 #pragma unroll
-    for (int i = 0; i < kOwnMax; ++i) {
+    for (int i = 0; i < own_max(L); ++i) {
         const int q = rsub + i * RPP;
         if (act && q < s.rows) Pnew[static_cast<int64_t>(t.row0 + q) * rp] = v[i];
     }
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void cg_spmv_lds(const PcgArgs* __restrict__ p
     double pq = 0.0;
     if (act) {
 #pragma unroll
-        for (int i = 0; i < kOwnMax; ++i) {
+        for (int i = 0; i < own_max(L); ++i) {
             const int q = rsub + i * RPP;
             if (q < s.rows) {
                 const double acc = lds_row_product<L>(s, q, lane);
@@ -400,6 +400,93 @@ __global__ __launch_bounds__(256) void cg_update(const PcgArgs* __restrict__ pa,
     if (threadIdx.x < L) {
         adt::gview(a.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = sz;
         adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
+    }
+}
+
+// Flexible CG with the multigrid cycle: the update x += alpha p, r -= alpha q (partials
+// r.D^-1 r and r.r for the convergence check, as cg_update) fused with the next cycle's finest
+// pre-smoothing r1 = r - A (w D^-1 r) of the updated residual, staged like
+// amg_smooth_residual_lds.  Halo rows are updated on the fly with the arithmetic of their own
+// tiles, so every staged value equals the residual that its own tile writes, and the residual
+// is read once per iteration instead of twice.  The updated residual goes to the other residual
+// buffer (neighbours still read this one).  Inactive lanes keep x and write no residual.
+template <int L>
+__global__ __launch_bounds__(256) void fcg_update_smooth(const PcgArgs* __restrict__ pa,
+                                                         const LevelArgs* __restrict__ la, int parity) {
+    extern __shared__ double smem[];
+    const PcgArgs& g = *pa;
+    const LevelArgs& a = *la;
+    const int tix = tile_index(g);
+    const Tile t = adt::gview(g.tiles)[tix];
+    if (!adt::gview(g.grp_active)[t.grp]) return;
+    constexpr int RPP = kRowsPerPass(L);
+    const Sys S = adt::gview(g.sys)[t.sys];
+    const double w = adt::gview(a.ls)[t.sys].omega;
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    const int col = (t.grp % g.groups) * L + lane;
+    const int64_t sc = static_cast<int64_t>(t.sys) * g.rp + col;
+    const bool act = adt::gview(g.flags)[sc] == 1;
+    const double alpha = adt::gview(g.alpha)[sc];
+    const int rp = g.rp;
+    const auto P = adt::gview(g.p[parity ^ 1] + S.vec_off + col);
+    const auto Q = adt::gview(g.q + S.vec_off + col);
+    const auto X = adt::gview(g.x + S.vec_off + col);
+    const auto R = adt::gview(residual(g, parity) + S.vec_off + col);        // this iteration's residual
+    const auto Rn = adt::gview(residual(g, parity ^ 1) + S.vec_off + col);   // the next one
+    const auto R1 = adt::gview(g.z + S.vec_off + col);
+    const auto D = adt::gview(g.dinv + S.diag_off);
+    const StageSrc src = fine_src(g, S);
+    const LdsTile s = lds_tile(src, t, L, smem, true);
+    const auto updated = [&](double r, double q) { return act ? fma(-alpha, q, r) : r; };
+    const auto value = [&](int j) {
+        const int64_t o = static_cast<int64_t>(j) * rp;
+        return w * D[j] * updated(R[o], Q[o]);
+    };
+    int hid[halo_batch(L)];
+    halo_ids<L>(src, s, 0, hid);
+    double xv[own_max(L)], pv[own_max(L)], rv[own_max(L)], qv[own_max(L)], dv[own_max(L)];
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int row = own_row<L>(t, s.rows, i);
+        const int64_t o = static_cast<int64_t>(row) * rp;
+        xv[i] = X[o];
+        pv[i] = P[o];
+        rv[i] = R[o];
+        qv[i] = Q[o];
+        dv[i] = D[row];
+    }
+    const CsrRegs cr = load_csr(src, t, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(src, t, s, cr);
+    double sz = 0.0, sr = 0.0;
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int q = rsub + i * RPP;
+        rv[i] = updated(rv[i], qv[i]);
+        if (q < s.rows) {
+            s.stage[q * L + lane] = w * dv[i] * rv[i];
+            if (act) {
+                const int64_t o = static_cast<int64_t>(t.row0 + q) * rp;
+                X[o] = fma(alpha, pv[i], xv[i]);
+                Rn[o] = rv[i];
+                sz = fma(rv[i] * rv[i], dv[i], sz);
+                sr = fma(rv[i], rv[i], sr);
+            }
+        }
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(src, s, value);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) R1[static_cast<int64_t>(t.row0 + q) * rp] = rv[i] - lds_row_product<L>(s, q, lane);
+    }
+    block_sum2<L>(sz, sr);
+    if (threadIdx.x < L) {
+        adt::gview(g.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = sz;
+        adt::gview(g.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
     }
 }
 
@@ -634,6 +721,7 @@ void adt_pcg_plan::free_amg() {
         if (b) (void)hipFree(b);
     amg_buffers.clear();
     amg.clear();
+    args_h.r_alt = nullptr;
     for (auto& g : graph[1])
         if (g) {
             (void)hipGraphExecDestroy(g);
@@ -677,6 +765,12 @@ int64_t fine_lds_bytes(const adt_pcg_plan* p) {
     return lds_bytes_for(p->lds_rows, p->lds_nnz, p->rb, p->L);
 }
 
+// Flexible CG with the multigrid cycle fuses each update with the next finest pre-smoothing
+// when the finest level is staged in LDS.
+bool fused_update(const adt_pcg_plan* p, int mode) {
+    return mode == 1 && p->lds_rows > 0 && !p->amg.empty() && p->args_h.r_alt;
+}
+
 // Restriction from level l to level l + 1 (tiles of the next level).
 template <int L>
 void launch_restrict(const adt_pcg_plan* p, int l, hipStream_t st) {
@@ -690,15 +784,17 @@ void launch_restrict(const adt_pcg_plan* p, int l, hipStream_t st) {
 
 // One V-cycle: z = B r (flexible-CG partials z.r, z.q_old written by the finest post-smoothing).
 // Staged (LDS) kernels wherever the plan built their tables, gather kernels elsewhere.
+// With `presmoothed`, the finest pre-smoothing was done by the previous update
+// (fcg_update_smooth, or the standalone pass after the initial residual).
 template <int L>
-void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity) {
+void launch_cycle(const adt_pcg_plan* p, hipStream_t st, int parity, bool presmoothed) {
     const int m = static_cast<int>(p->amg.size());
     const PcgArgs* pa = p->args_d;
     const auto& f = p->amg[0];
     const size_t lds = p->lds_rows > 0 ? static_cast<size_t>(fine_lds_bytes(p)) : 0;
-    if (p->lds_rows > 0)
+    if (!presmoothed && p->lds_rows > 0)
         hipLaunchKernelGGL(amg_smooth_residual_lds<L>, dim3(p->n_tiles), dim3(256), lds, st, pa, f.args_d);
-    else
+    else if (!presmoothed)
         hipLaunchKernelGGL(amg_smooth_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, f.args_d);
     launch_restrict<L>(p, 0, st);
     for (int l = 1; l < m - 1; ++l) {
@@ -745,9 +841,10 @@ void launch_reduce(const adt_pcg_plan* p, hipStream_t st, void (*kernel)(const P
 template <int L>
 void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mode, hipEvent_t* events) {
     const PcgArgs* pa = p->args_d;
+    const bool fused = fused_update(p, mode);
     if (mode == 1) {
         if (events) (void)hipEventRecord(events[4], st);
-        launch_cycle<L>(p, st, parity);
+        launch_cycle<L>(p, st, parity, fused);
         if (events) (void)hipEventRecord(events[5], st);
         launch_reduce<L>(p, st, fcg_beta<L>, 3);
     }
@@ -767,7 +864,11 @@ void launch_iteration(const adt_pcg_plan* p, hipStream_t st, int parity, int mod
     if (events) (void)hipEventRecord(events[1], st);
     launch_reduce<L>(p, st, cg_alpha<L>, 1);
     if (events) (void)hipEventRecord(events[2], st);
-    hipLaunchKernelGGL(cg_update<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
+    if (fused)
+        hipLaunchKernelGGL(fcg_update_smooth<L>, dim3(p->n_tiles), dim3(256), fine_lds_bytes(p), st, pa,
+                           p->amg[0].args_d, parity);
+    else
+        hipLaunchKernelGGL(cg_update<L>, dim3(p->n_tiles), dim3(256), 0, st, pa, parity);
     if (events) (void)hipEventRecord(events[3], st);
     if (mode == 1)
         launch_reduce<L>(p, st, cg_check<L>, 4);
@@ -856,6 +957,9 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mod
     hipLaunchKernelGGL(cg_init<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     launch_reduce<L>(p, st, cg_init_reduce<L>, 15);
     hipLaunchKernelGGL(cg_zero_solution<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
+    if (fused_update(p, mode))  // the first cycle's finest pre-smoothing (later ones: fcg_update_smooth)
+        hipLaunchKernelGGL(amg_smooth_residual_lds<L>, dim3(p->n_tiles), dim3(256), fine_lds_bytes(p), st, pa,
+                           p->amg[0].args_d);
     ADT_HIP_CHECK(hipGetLastError());
     if (max_iter > 0) {
         int rc = p->profile ? ADT_OK : ensure_graphs<L>(p, chunk, mode);
@@ -1009,7 +1113,7 @@ extern "C" int adt_pcg_create(adt_pcg_plan** out, int n_sys, int r, int lanes, i
             hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
             limit > 0)
             lds_bytes = std::min(lds_bytes, limit - 8192);  // the block reductions keep static LDS
-        // staged tiles keep their own rows' operands in registers: at most kOwnMax rows per thread
+        // staged tiles keep their own rows' operands in registers: at most own_max(L) rows per thread
         for (int rb = std::min(p->rb, max_staged_tile_rows(lanes)); rb >= rpp; rb = (rb / 2 + rpp - 1) / rpp * rpp) {
             ptiles.clear();
             int max_staged = 0, max_nnz = 0;
@@ -1260,7 +1364,7 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
     p->free_amg();
     p->amg.resize(depth);
     const int rp = p->rp, L = p->L;
-    const int rbc = std::max(256 / L, 32);  // coarse-level tile height
+    const int rbc = std::min(std::max(256 / L, 32), max_staged_tile_rows(L));  // coarse-level tile height
     int lds_limit = 0, device = 0;
     if (hipGetDevice(&device) != hipSuccess ||
         hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
@@ -1556,6 +1660,12 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         if (rc) return rc;
         p->amg_buffers.push_back(p->amg[l].args_d);
     }
+    if (p->lds_rows > 0) {  // second residual buffer of the fused update (fused_update)
+        double* r_alt = nullptr;
+        const int rc = zeros(p, p->vec_len, &r_alt);
+        if (rc) return rc;
+        p->args_h.r_alt = r_alt;
+    }
     return ADT_OK;
 }
 
@@ -1713,6 +1823,7 @@ extern "C" int adt_pcg_solve(adt_pcg_plan* p, const int32_t* d_indptr, const int
     a.tol2 = tol * tol;
     const int mode = precond == ADT_PRECOND_JACOBI ? 0 : 1;
     a.stall_limit = mode == 1 ? p->stall_limit : 0;
+    a.r_pingpong = fused_update(p, mode) ? 1 : 0;
     // Kernels read the arguments from device memory so one captured graph serves every solve.
     int rc = upload_args(p, st, precond == ADT_PRECOND_LEARNED ? static_cast<const double*>(precond_data) : nullptr);
     if (rc) return rc;
@@ -1736,6 +1847,7 @@ extern "C" int adt_pcg_solve(adt_pcg_plan* p, const int32_t* d_indptr, const int
             // stop at once (their residual already meets the tolerance).
             p->last_fallback = need;
             a.stall_limit = 0;
+            a.r_pingpong = 0;
             if ((rc = upload_args(p, st, nullptr)) || (rc = solve_dispatch(p, st, max_iter, chunk, 0)) ||
                 (rc = read_report(p, st, rep)))
                 return rc;
