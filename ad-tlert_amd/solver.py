@@ -74,11 +74,15 @@ class BatchedSolver:
     max_iterations: int = 50000
     check_every: int = 16
     rows_per_tile: int | None = None
+    lanes: int | None = None       # lane-group width (None: choose_lanes)
+    lds_bytes: int | None = None   # LDS budget of the staged products (None: native.LDS_BYTES; 0: gather kernels)
     preconditioner: str = "jacobi"
     amg_theta: float = 0.08
     amg_coarse_max: int = 256
     amg_stall_limit: int = 64
     damping_network: object = None  # learned.DampingNetwork for preconditioner="learned"
+    autotune: bool = True          # time the candidate lane widths before the first plan of a shape
+    tuning: dict = field(default_factory=dict)  # shape -> {lanes: seconds per iteration}
     _plans: dict = field(default_factory=dict)
     last: SolveStats | None = None
     last_plan: native.PcgPlan | None = None
@@ -94,13 +98,18 @@ class BatchedSolver:
 
         rows = np.asarray(rows, dtype=np.int32)
         n_sys = int(rows.size)
-        lanes = choose_lanes(r) if lanes is None else int(lanes)
+        if lanes is None:
+            lanes = self.lanes
+        lds = native.LDS_BYTES if self.lds_bytes is None else int(self.lds_bytes)
+        if lanes is None:
+            lanes = self._tuned_lanes(rows, r, pattern_offsets, host_pattern, lds)
+        lanes = int(lanes)
         if rows_per_tile is None:
             rows_per_tile = self.rows_per_tile
         if rows_per_tile is None:
             rows_per_tile = choose_rows_per_tile(n_sys, int(rows.max()), r, lanes)
         bkey = None if b_off is None else tuple(np.asarray(b_off).tolist())
-        full_key = (key, tuple(rows.tolist()), int(r), lanes, int(rows_per_tile), bkey)
+        full_key = (key, tuple(rows.tolist()), int(r), lanes, int(rows_per_tile), bkey, lds)
         plan = self._plans.get(full_key)
         if plan is None:
             if pattern_offsets is None:
@@ -111,9 +120,71 @@ class BatchedSolver:
                 ptr_off, idx_off, val_off = pattern_offsets
             if val_off is None:
                 raise ValueError("val_off is required")
-            plan = native.PcgPlan(rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off, host_pattern)
+            plan = native.PcgPlan(rows, r, lanes, rows_per_tile, ptr_off, idx_off, val_off, b_off, host_pattern,
+                                  lds_bytes=lds)
             self._plans[full_key] = plan
         return plan
+
+    def _tuned_lanes(self, rows, r: int, pattern_offsets, host_pattern, lds: int) -> int:
+        """Lane-group width for a new problem shape.  Every width that keeps the padded source
+        count of :func:`choose_lanes` is timed on the plan's CSR pattern (graph-Laplacian values,
+        random right-hand sides, the solver's preconditioner, a fixed number of iterations) and
+        the fastest per iteration wins (one tuning per shape, whatever the preconditioner).
+        Narrow groups stage longer tiles in LDS (fewer halo rows per row, which pays on 3D
+        stencils); wide groups read the matrix fewer times."""
+
+        default = choose_lanes(r)
+        rp = padded(r, default)
+        candidates = [L for L in LANE_CHOICES if L >= 16 and padded(r, L) == rp]
+        if (not self.autotune or host_pattern is None or pattern_offsets is None or len(candidates) < 2
+                or not torch.cuda.is_available()):
+            return default
+        ptr_off, idx_off, _ = (np.asarray(o, dtype=np.int64) if o is not None else None for o in pattern_offsets)
+        shape = (tuple(rows.tolist()), int(r), tuple(ptr_off.tolist()), tuple(idx_off.tolist()), lds,
+                 self.rows_per_tile)
+        if shape in self.tuning:
+            times = self.tuning[shape]
+            return min(times, key=times.get)
+        indptr_h, indices_h = (np.ascontiguousarray(a, dtype=np.int32) for a in host_pattern)
+        values, val_off = _laplacian(indptr_h, indices_h, rows, ptr_off, idx_off)
+        device = native.device()
+        dev_ptr, dev_idx = (torch.as_tensor(a, device=device) for a in (indptr_h, indices_h))
+        values_d = torch.as_tensor(values, device=device)
+        times = {}
+        for lanes in candidates:
+            rpt = self.rows_per_tile or choose_rows_per_tile(len(rows), int(rows.max()), r, lanes)
+            plan = native.PcgPlan(rows, r, lanes, rpt, ptr_off, idx_off, val_off, None, (indptr_h, indices_h),
+                                  lds_bytes=lds)
+            try:
+                precond = native.PRECOND_JACOBI
+                if self.preconditioner in ("amg", "learned"):
+                    try:
+                        plan.attach_amg(values_d, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
+                                        stall_limit=0)
+                        precond = native.PRECOND_AMG
+                    except ValueError:  # no hierarchy for this pattern: time Jacobi PCG instead
+                        pass
+                gen = torch.Generator(device=device).manual_seed(0)
+                b = torch.randn(plan.vec_len, dtype=torch.float64, device=device, generator=gen)
+                x = torch.zeros_like(b)
+
+                def run(iterations):  # tolerance below reach: exactly `iterations` iterations
+                    x.zero_()
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    rc, _, _ = plan.solve(dev_ptr, dev_idx, values_d, b, x, 1e-300, iterations, precond=precond,
+                                          check_every=8)
+                    torch.cuda.synchronize()
+                    if rc not in (native.ADT_OK, native.ADT_ERR_NOT_CONVERGED):
+                        native.check(rc)
+                    return time.perf_counter() - t0
+
+                run(8)  # graphs and hierarchy values
+                times[lanes] = (run(24) - run(8)) / 16
+            finally:
+                plan.close()
+        self.tuning[shape] = times
+        return min(times, key=times.get)
 
     def _precond(self, plan: native.PcgPlan, values, preconditioner):
         """(native preconditioner id, name used, per-slot damping array or None)."""
@@ -188,6 +259,26 @@ class BatchedSolver:
         for plan in self._plans.values():
             plan.close()
         self._plans.clear()
+
+
+def _laplacian(indptr, indices, rows, ptr_off, idx_off):
+    """Graph-Laplacian values (off-diagonals -1, diagonal = degree + 1e-3) for every system of a
+    batch layout, one block per system: (values, val_off)."""
+
+    blocks, val_off, offset = [], [], 0
+    for s in range(len(rows)):
+        p0, i0, n = int(ptr_off[s]), int(idx_off[s]), int(rows[s])
+        ptr = indptr[p0:p0 + n + 1].astype(np.int64)
+        nnz = int(ptr[-1])
+        cols = indices[i0:i0 + nnz]
+        row_of = np.repeat(np.arange(n), np.diff(ptr))
+        vals = np.where(cols == row_of, 0.0, -1.0)
+        deg = np.bincount(row_of, weights=-vals, minlength=n)
+        vals = np.where(cols == row_of, deg[row_of] + 1e-3, vals)
+        blocks.append(vals)
+        val_off.append(offset)
+        offset += nnz
+    return np.concatenate(blocks), np.asarray(val_off, dtype=np.int64)
 
 
 def interleave(fields: torch.Tensor, rp: int) -> torch.Tensor:

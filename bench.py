@@ -203,6 +203,8 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=32, help="Jacobian rows in the CPU-baseline sample")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--rows-per-tile", type=int, default=None, help="PCG workgroup tile height (default: auto)")
+    ap.add_argument("--lanes", type=int, default=None, help="lane-group width 8/16/32/64 (default: auto)")
+    ap.add_argument("--lds-bytes", type=int, default=None, help="LDS budget of the staged kernels (0: gather kernels)")
     ap.add_argument("--preconditioner", choices=("amg", "jacobi"), default="amg")
     ap.add_argument("--pcg-steps", type=int, default=2, help="timed steps of plain Jacobi PCG on the same batches (0: skip)")
     ap.add_argument("--learned-steps", type=int, default=2,
@@ -228,6 +230,8 @@ def main() -> int:
     op = wl.op
     op._solver.check_every = args.check_every
     op._solver.rows_per_tile = args.rows_per_tile
+    op._solver.lanes = args.lanes
+    op._solver.lds_bytes = args.lds_bytes
     log(f"setup {time.perf_counter() - t0:.1f}s: {args.workload} N={wl.N} C={wl.C} nnz={wl.nnz} D={wl.D} "
         f"rhs/step={wl.rhs_per_step} device={arch} CUs={cus}")
 
@@ -306,7 +310,9 @@ def main() -> int:
     config = {"workload": wl.name, **wl.config, "solver": solver_name, "tolerance": args.tol,
               "lanes": plan.lanes, "rp": plan.rp, "n_tiles": plan.tiles, "rows_per_tile": plan.rows_per_tile,
               "lds_staged_rows": plan.lds_rows, "dof_order": os.environ.get("ADT_DOF_ORDER", "hilbert"),
-              "multigrid_levels": levels}
+              "multigrid_levels": levels,
+              "lane_autotune_ms_per_iteration": [{str(k): round(v * 1e3, 4) for k, v in t.items()}
+                                                 for t in op._solver.tuning.values()]}
     result = {
         "metric": METRIC,
         "value": rhs_per_s,
