@@ -217,6 +217,105 @@ int launch_gram(GramArgs& A, size_t lds, hipStream_t st) {
     return ADT_OK;
 }
 
+// Cross-Gram form for separate receiver and current fields (3D: the receiver fields are solved
+// with their own right-hand sides):  G_c = Phi_R^T (T Phi_C)  (S_r x S_c per cell), then
+//   J[d, c] = -((G[m, a] - G[m, b]) - G[n, a]) + G[n, b])
+// with current indices a, b >= S_c and receiver indices m, n >= S_r reading zero.  No symmetry is
+// needed: the full block is kept in LDS.  Same structure as gram_kernel: one workgroup per cell,
+// 16 x 16 threads of B x B accumulators, staged chunks of wavenumbers.
+struct GramCrossArgs {
+    const double* xr;
+    const double* xc;
+    const int32_t* dofs;
+    const double* stiff;
+    const double* mass;
+    const int32_t* a;
+    const int32_t* b;
+    const int32_t* m;
+    const int32_t* n;
+    double* out;
+    int Sr, Sc, Sp, K, W, rp_r, rp_c, wchunk, C, D;
+    int64_t off_r[kMaxWaves];
+    int64_t off_c[kMaxWaves];
+    double ws[kMaxWaves];
+    double wm[kMaxWaves];
+};
+
+template <int B>
+__global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
+    extern __shared__ double lds[];
+    const int c = adt::xcd_tile(static_cast<int>(blockIdx.x), A.C);
+    const int K = A.K, Sp = A.Sp;
+    const int rows_max = A.wchunk * K;
+    double* phr = lds;                                            // rows_max x Sp (receiver fields)
+    double* phc = lds + static_cast<int64_t>(rows_max) * Sp;      // rows_max x Sp (current fields)
+    double* psi = lds + static_cast<int64_t>(2 * rows_max) * Sp;  // rows_max x Sp (T times current)
+    const int te = static_cast<int>(threadIdx.x) / 16, tf = static_cast<int>(threadIdx.x) % 16;
+    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
+    double acc[B][B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) acc[i][j] = 0.0;
+    for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
+        const int nw = min(A.wchunk, A.W - w0);
+        const int rows = nw * K;
+        __syncthreads();
+        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
+            const int r = t / Sp, e = t % Sp;
+            const int w = w0 + r / K;
+            const int64_t node = dof[r % K];
+            phr[t] = e < A.Sr ? A.xr[A.off_r[w] + node * A.rp_r + e] : 0.0;
+            phc[t] = e < A.Sc ? A.xc[A.off_c[w] + node * A.rp_c + e] : 0.0;
+        }
+        __syncthreads();
+        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
+            const int r = t / Sp, f = t % Sp;
+            const int wl = r / K, i = r % K, w = w0 + wl;
+            double v = 0.0;
+            for (int j = 0; j < K; ++j) {
+                const double tij = Mc ? fma(A.wm[w], Mc[i * K + j], A.ws[w] * Kc[i * K + j]) : A.ws[w] * Kc[i * K + j];
+                v = fma(tij, phc[(wl * K + j) * Sp + f], v);
+            }
+            psi[t] = v;
+        }
+        __syncthreads();
+        for (int r = 0; r < rows; ++r) {
+            double av[B], bv[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) av[i] = phr[r * Sp + te * B + i];
+#pragma unroll
+            for (int j = 0; j < B; ++j) bv[j] = psi[r * Sp + tf * B + j];
+#pragma unroll
+            for (int i = 0; i < B; ++i)
+#pragma unroll
+                for (int j = 0; j < B; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+        }
+    }
+    __syncthreads();
+    double* G = lds;  // Sp x Sp block, receiver-major
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) G[(te * B + i) * Sp + tf * B + j] = acc[i][j];
+    __syncthreads();
+    auto g = [&](int e, int f) { return e < A.Sr && f < A.Sc ? G[e * Sp + f] : 0.0; };
+    for (int d = static_cast<int>(threadIdx.x); d < A.D; d += 256) {
+        const int ia = A.a[d], ib = A.b[d], im = A.m[d], in = A.n[d];
+        const double v = ((g(im, ia) - g(im, ib)) - g(in, ia)) + g(in, ib);
+        A.out[static_cast<int64_t>(d) * A.C + c] = -v;
+    }
+}
+
+template <int B>
+int launch_gram_cross(GramCrossArgs& A, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL(gram_cross_kernel<B>, dim3(A.C), dim3(256), lds, st, A);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
 // ------------------------------------------------------------------------------------------
 // (d) adjoint cell gradient on interleaved (N, Rp) fields: one wavefront per cell, lanes over
 // right-hand sides, every row access a contiguous Rp*8-byte run.
@@ -534,6 +633,66 @@ extern "C" int adt_aggregate_columns(int n_rows, int64_t in_cols, int n_out, con
                        d_col_idx, d_out, n_out);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
+}
+
+extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,
+                                          const double* d_xc, int rp_c, int s_c, const int64_t* h_off_c, int n_meas,
+                                          int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b,
+                                          const int32_t* d_m, const int32_t* d_n, const int32_t* d_cell_dofs,
+                                          const double* d_stiff, const double* d_mass, const double* h_ws,
+                                          const double* h_wm, double* d_out, void* stream) {
+    ADT_REQUIRE(n_waves >= 1 && n_waves <= kMaxWaves, "adt_sensitivity_gram_cross: 1 <= n_waves <= %d", kMaxWaves);
+    ADT_REQUIRE(s_r >= 1 && s_r <= 128 && s_r <= rp_r && s_c >= 1 && s_c <= 128 && s_c <= rp_c,
+                "adt_sensitivity_gram_cross: 1 <= receivers <= min(128, rp_r) and 1 <= currents <= min(128, rp_c)");
+    ADT_REQUIRE(k_local >= 1 && k_local <= 10, "adt_sensitivity_gram_cross: unsupported local size %d", k_local);
+    ADT_REQUIRE(h_off_r && h_off_c && h_ws && (d_mass == nullptr || h_wm),
+                "adt_sensitivity_gram_cross: missing host arrays");
+    if (n_meas == 0 || n_cells == 0) return ADT_OK;
+    ADT_REQUIRE(d_xr && d_xc && d_a && d_b && d_m && d_n && d_cell_dofs && d_stiff && d_out,
+                "adt_sensitivity_gram_cross: null pointer");
+    GramCrossArgs A{};
+    A.xr = d_xr;
+    A.xc = d_xc;
+    A.dofs = d_cell_dofs;
+    A.stiff = d_stiff;
+    A.mass = d_mass;
+    A.a = d_a;
+    A.b = d_b;
+    A.m = d_m;
+    A.n = d_n;
+    A.out = d_out;
+    A.Sr = s_r;
+    A.Sc = s_c;
+    const int B = (std::max(s_r, s_c) + 15) / 16;
+    A.Sp = 16 * B;
+    A.K = k_local;
+    A.W = n_waves;
+    A.rp_r = rp_r;
+    A.rp_c = rp_c;
+    A.C = n_cells;
+    A.D = n_meas;
+    // three staged (rows x Sp) blocks per wavenumber chunk within 24 KB; the G block reuses the LDS
+    A.wchunk = std::max(1, std::min(n_waves, (24576 / (3 * 8 * A.Sp)) / k_local));
+    const size_t staging = static_cast<size_t>(3) * A.wchunk * k_local * A.Sp * 8;
+    const size_t block = static_cast<size_t>(A.Sp) * A.Sp * 8;
+    const size_t lds = std::max(staging, block);
+    for (int w = 0; w < n_waves; ++w) {
+        A.off_r[w] = h_off_r[w];
+        A.off_c[w] = h_off_c[w];
+        A.ws[w] = h_ws[w];
+        A.wm[w] = d_mass ? h_wm[w] : 0.0;
+    }
+    hipStream_t st = adt::as_stream(stream);
+    switch (B) {
+        case 1: return launch_gram_cross<1>(A, lds, st);
+        case 2: return launch_gram_cross<2>(A, lds, st);
+        case 3: return launch_gram_cross<3>(A, lds, st);
+        case 4: return launch_gram_cross<4>(A, lds, st);
+        case 5: return launch_gram_cross<5>(A, lds, st);
+        case 6: return launch_gram_cross<6>(A, lds, st);
+        case 7: return launch_gram_cross<7>(A, lds, st);
+        default: return launch_gram_cross<8>(A, lds, st);
+    }
 }
 
 extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x,

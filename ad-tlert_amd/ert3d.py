@@ -579,6 +579,25 @@ class ERTForward3D:
         sigma = self._sigma(conductivity)
         fields, _ = self._fields_series(sigma[None])
         (receiver,) = self._receivers(fields, sigma[None])
+        templates = columns.select(s.templates if include_robin_boundary_derivative else s.volume_templates, 0)
+        precision = 1 if self.sensitivity_precision == "mixed" else 0
+        if precision == 0 and s.nc <= 128 and s.nr <= 128:
+            # cross-Gram form on the interleaved device-order blocks: per cell G_c = Phi_R^T T Phi_C
+            # (receivers x currents) once, then four reads per row (FP64; the mixed mode keeps the
+            # per-output FP32 kernel)
+            current = fields[0].current.contiguous()
+            rec = receiver.contiguous()
+            dofs_dev = columns.select(s.cell_dofs, 0).contiguous()
+            part = torch.empty((count, dofs_dev.shape[0]), dtype=torch.float64, device=s.device)
+            abmn = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in s.abmn_host]
+            native.sensitivity_gram_cross(rec, [0], rec.shape[-1], s.nr, current, [0], current.shape[-1], s.nc, *abmn,
+                                          dofs_dev, templates.contiguous(), None, [1.0], None, part)
+            result = columns.columns(part)
+            if return_device:
+                return result
+            out = result.to(device="cpu", dtype=FLOAT_DTYPE)
+            check_finite(out, "jacobian")
+            return out
         S = s.nc + s.nr
         phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)  # mesh DOF order
         phi[0, : s.nc] = fields[0].current[s.inv, : s.nc].T
@@ -588,9 +607,7 @@ class ERTForward3D:
         b = np.where(b == s.nc, S, b)
         m, n = m + s.nc, n + s.nc
         idx = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in (a, b, m, n)]
-        templates = columns.select(s.templates if include_robin_boundary_derivative else s.volume_templates, 0)
         dofs = columns.select(s.cell_dofs_mesh, 0).contiguous()
-        precision = 1 if self.sensitivity_precision == "mixed" else 0
         result = torch.empty((count, columns.count), dtype=torch.float64, device=s.device)
         for start in range(0, count, batch_size):
             stop = min(count, start + batch_size)

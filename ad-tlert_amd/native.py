@@ -51,6 +51,8 @@ _SIGNATURES = {
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
                                      P]),
+    "adt_sensitivity_gram_cross": (c_int, [c_int, P, c_int, c_int, P, P, c_int, c_int, P, c_int, c_int, c_int, P, P,
+                                           P, P, P, P, P, P, P, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
     "adt_electrode_potentials": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P]),
@@ -275,6 +277,22 @@ def sensitivity_gram(x, vec_off, rp, s_rows, a, b, m, n, cell_dofs, stiff, mass,
     check(library().adt_sensitivity_gram(W, int(rp), int(s_rows), host(vec_off), ptr(x), int(a.numel()), C, K,
                                          ptr(a), ptr(b), ptr(m), ptr(n), ptr(cell_dofs), ptr(stiff), ptr(mass),
                                          host(ws), host(wm), ptr(out), stream()))
+
+
+def sensitivity_gram_cross(xr, off_r, rp_r, s_r, xc, off_c, rp_c, s_c, a, b, m, n, cell_dofs, stiff, mass, ws, wm,
+                           out):
+    """Jacobian rows from separate receiver / current interleaved fields (``adt_sensitivity_gram_cross``)."""
+
+    W = len(off_r)
+    C, K = cell_dofs.shape
+    off_r = np.ascontiguousarray(off_r, dtype=np.int64)
+    off_c = np.ascontiguousarray(off_c, dtype=np.int64)
+    ws = np.ascontiguousarray(ws, dtype=np.float64)
+    wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
+    check(library().adt_sensitivity_gram_cross(W, ptr(xr), int(rp_r), int(s_r), host(off_r), ptr(xc), int(rp_c),
+                                               int(s_c), host(off_c), int(a.numel()), C, K, ptr(a), ptr(b), ptr(m),
+                                               ptr(n), ptr(cell_dofs), ptr(stiff), ptr(mass), host(ws), host(wm),
+                                               ptr(out), stream()))
 
 
 def cell_gradient(lam, rp_lam, lam_off, phi, rp_phi, phi_off, r, cell_dofs, stiff, mass, ws, wm, out, *,

@@ -179,10 +179,11 @@ class Box3D:
         def run():
             return op.jacobian(sigma, return_device=True, batch_size=self.D)
 
-        K, S = 4, 2 * op._dev.nc
-        bytes_ = 8 * self.D * self.C + 8 * S * self.N + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
-        flops = self.D * self.C * (4 * K * K + 6 * K)
-        return run, bytes_, flops, "sensitivity_kernel<4,double>"
+        # cross-Gram form: per cell G_c = Phi_R^T (T Phi_C) over K rows of nr receivers x nc sources
+        K, nc, nr = 4, op._dev.nc, op._dev.nr
+        bytes_ = 8 * self.D * self.C + 8 * self.N * (nc + nr) + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
+        flops = self.C * (2 * K * K * nc + 2 * K * nr * nc) + 4 * self.D * self.C
+        return run, bytes_, flops, "gram_cross_kernel<4> (cross-Gram sensitivity)"
 
     def cpu(self):
         return None  # a 3D direct factorization of this size does not fit a bounded CPU sample

@@ -193,6 +193,23 @@ int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_o
                          const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff, const double* d_mass,
                          const double* h_ws, const double* h_wm, double* d_out, void* stream);
 
+/* Same Jacobian rows in cross-Gram form, for receiver fields that differ from the current
+ * fields (3D: ERTForward3D.jacobian, forward/ert3d.py:867-927, with the kernel of
+ * forward/kernels.py:132-178):
+ *   G_c[e][f] = sum_w phiR_w[e, dofs_c]^T (ws[w] stiff_c + wm[w] mass_c) phiC_w[f, dofs_c]
+ *   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])
+ * phiR_w / phiC_w are the (N, rp_r) / (N, rp_c) blocks at d_xr + h_off_r[w] / d_xc + h_off_c[w];
+ * their first s_r / s_c (<= 128) columns are receivers / current sources; a, b >= s_c and
+ * m, n >= s_r read zero (remote electrodes).  Templates need not be symmetric.  One workgroup
+ * per cell keeps the s_r x s_c block in LDS (no workspace).  FP64, fixed summation order:
+ * bitwise repeatable.  out is (D, C) row-major. */
+int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,
+                               const double* d_xc, int rp_c, int s_c, const int64_t* h_off_c, int n_meas,
+                               int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
+                               const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff,
+                               const double* d_mass, const double* h_ws, const double* h_wm, double* d_out,
+                               void* stream);
+
 /* ------------------------------------------------------------------------------------------
  * (d) Adjoint cell gradient:  g[c] = - sum_w sum_s lam_w[:, s]^T (ws[w] K_c + wm[w] M_c) phi_w[:, s]
  *     (+ Robin edges:  - sum_w sum_s lam_w[e]^T (balpha[w,b] * bws[w] * Mb_b) phi_w[e]).

@@ -282,6 +282,45 @@ def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
     assert results[native.PRECOND_AMG].max() < 0.5 * results[native.PRECOND_JACOBI].max()
 
 
+@pytest.mark.parametrize("k, Sr, Sc", [(4, 64, 64), (10, 17, 40), (4, 128, 33), (3, 5, 96)])
+def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc):
+    """Cross-Gram Jacobian rows (separate receiver and current fields, as in 3D) equal the
+    per-output formula with non-symmetric templates, remote indices reading zero and padding
+    columns ignored; bitwise repeatable."""
+
+    gen = torch.Generator().manual_seed(100 * k + Sr + Sc)
+    W, N, C, D = 3, 80, 47, 39
+    phr = torch.randn(W, Sr, N, generator=gen, dtype=torch.float64)
+    phc = torch.randn(W, Sc, N, generator=gen, dtype=torch.float64)
+    stiff = torch.randn(C, k, k, generator=gen, dtype=torch.float64)
+    mass = torch.randn(C, k, k, generator=gen, dtype=torch.float64)
+    ws, wm = torch.rand(W, generator=gen, dtype=torch.float64), torch.rand(W, generator=gen, dtype=torch.float64)
+    cells = torch.randint(0, N, (C, k), generator=gen)
+    a, b = (torch.randint(0, Sc + 1, (D,), generator=gen) for _ in range(2))  # Sc = remote
+    m, n = (torch.randint(0, Sr + 1, (D,), generator=gen) for _ in range(2))  # Sr = remote
+    pr = torch.cat((phr, torch.zeros_like(phr[:, :1])), dim=1)
+    pc = torch.cat((phc, torch.zeros_like(phc[:, :1])), dim=1)
+    T = ws[:, None, None, None] * stiff[None] + wm[:, None, None, None] * mass[None]
+    expected = -torch.einsum("wdci,wcij,wdcj->dc", (pr[:, m] - pr[:, n])[..., cells], T, (pc[:, a] - pc[:, b])[..., cells])
+    rp_r, rp_c = -(-Sr // 16) * 16 + 16, -(-Sc // 8) * 8
+    xr = torch.full((W, N, rp_r), 5.0, dtype=torch.float64)  # padding columns must not be read
+    xc = torch.full((W, N, rp_c), -3.0, dtype=torch.float64)
+    xr[:, :, :Sr] = phr.transpose(1, 2)
+    xc[:, :, :Sc] = phc.transpose(1, 2)
+    g = lambda t, dt=None: t.to(dev(), dtype=dt or t.dtype).contiguous()  # noqa: E731
+
+    def run():
+        out = torch.empty((D, C), dtype=torch.float64, device=dev())
+        native.sensitivity_gram_cross(g(xr), np.arange(W) * N * rp_r, rp_r, Sr, g(xc), np.arange(W) * N * rp_c, rp_c,
+                                      Sc, *(g(x, torch.int32) for x in (a, b, m, n)), g(cells, torch.int32), g(stiff),
+                                      g(mass), ws.numpy(), wm.numpy(), out)
+        return out
+
+    first, second = run(), run()
+    assert torch.equal(first, second)
+    assert relmax(first.cpu(), expected) < 1e-12
+
+
 @pytest.mark.parametrize("k, S, rp", [(3, 9, 16), (4, 40, 64), (10, 128, 128), (3, 96, 96)])
 def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp):
     """Gram-form Jacobian rows from interleaved (N, Rp) blocks equal the per-output formula
