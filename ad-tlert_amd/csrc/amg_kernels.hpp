@@ -91,44 +91,74 @@ __global__ __launch_bounds__(256) void amg_cvals(const LevelArgs* __restrict__ l
     adt::gview(a.next_valp)[S.next_val + c] = acc;
 }
 
-// One workgroup per system: dense copy of the coarsest operator inverted in place by
-// Gauss-Jordan elimination without pivoting (the operator is SPD).  A non-positive pivot
-// (a singular coarse operator) zeroes that row and column of the inverse instead.
-// Dynamic LDS: one column (rows doubles).
-__global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __restrict__ la) {
-    extern __shared__ double colk[];
+// One workgroup per system: dense copy of the coarsest operator inverted by Gauss-Jordan
+// elimination without pivoting (the operator is SPD).  A non-positive pivot (a singular coarse
+// operator) zeroes that row and column of the inverse instead.  Each step stages the pivot
+// column and the scaled pivot row in LDS, so the rank-one update reads them from LDS and only
+// touches other rows of the matrix.  With `lds_matrix` the whole matrix lives in LDS and is
+// written out once (operators of up to kDenseLdsRows rows); otherwise it is updated in place
+// in global memory.  Dynamic LDS: 2 rows (+ rows^2 with lds_matrix) doubles.
+constexpr int kDenseLdsRows = 128;
+
+template <typename MP>
+__device__ __forceinline__ void gauss_jordan(MP M, int n, double* colk, double* rowk) {
+    const int tid = static_cast<int>(threadIdx.x);
+    for (int k = 0; k < n; ++k) {
+        const double piv = M[static_cast<int64_t>(k) * n + k];
+        const double inv = piv > 0.0 && std::isfinite(piv) ? 1.0 / piv : 0.0;
+        for (int i = tid; i < n; i += 256) colk[i] = M[static_cast<int64_t>(i) * n + k];
+        for (int j = tid; j < n; j += 256) rowk[j] = (j == k) ? inv : M[static_cast<int64_t>(k) * n + j] * inv;
+        __syncthreads();
+        for (int j = tid; j < n; j += 256) M[static_cast<int64_t>(k) * n + j] = rowk[j];
+        // 16 x 16 thread layout over (row, column), four columns per thread with their loads
+        // issued together
+        for (int i = tid / 16; i < n; i += 16) {
+            if (i == k) continue;
+            const double f = colk[i];
+            const int64_t base = static_cast<int64_t>(i) * n;
+            for (int j0 = tid % 16; j0 < n; j0 += 64) {
+                double m[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) m[u] = M[base + (j0 + 16 * u < n ? j0 + 16 * u : j0)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + 16 * u;
+                    if (j < n) M[base + j] = fma(-f, rowk[j], (j == k) ? 0.0 : m[u]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void amg_dense_inverse(const LevelArgs* __restrict__ la, int lds_matrix) {
+    extern __shared__ double sh[];
     const LevelArgs& a = *la;
     const LvSys S = adt::gview(a.ls)[blockIdx.x];
-    const int n = S.rows;
-    const auto M = adt::gview(a.dense + S.dense);
+    const int n = S.rows, tid = static_cast<int>(threadIdx.x);
+    const auto Mg = adt::gview(a.dense + S.dense);
     const auto ptr = adt::gview(a.ptr + S.ptr);
     const auto idx = adt::gview(a.idx + S.idx);
     const auto V = adt::gview(a.val + S.val);
     const int64_t nn = static_cast<int64_t>(n) * n;
-    for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) M[e] = 0.0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        for (int k = ptr[i]; k < ptr[i + 1]; ++k) M[static_cast<int64_t>(i) * n + idx[k]] = V[k];
-    __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        const auto rowk = adt::gview(M + static_cast<int64_t>(k) * n);
-        const double piv = rowk[k];
-        const double inv = piv > 0.0 && std::isfinite(piv) ? 1.0 / piv : 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = M[static_cast<int64_t>(i) * n + k];
+    double* colk = sh;
+    double* rowk = sh + n;
+    if (lds_matrix) {
+        double* M = sh + 2 * n;
+        for (int64_t e = tid; e < nn; e += 256) M[e] = 0.0;
         __syncthreads();
-        for (int j = threadIdx.x; j < n; j += blockDim.x) rowk[j] = (j == k) ? inv : rowk[j] * inv;
+        for (int i = tid; i < n; i += 256)
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) M[static_cast<int64_t>(i) * n + idx[k]] = V[k];
         __syncthreads();
-        // 16 x 16 thread layout over (row, column)
-        for (int i = static_cast<int>(threadIdx.x) / 16; i < n; i += 16) {
-            if (i == k) continue;
-            const double f = colk[i];
-            const auto Mi = adt::gview(M + static_cast<int64_t>(i) * n);
-            for (int j = static_cast<int>(threadIdx.x) % 16; j < n; j += 16) {
-                const double m = (j == k) ? 0.0 : Mi[j];
-                Mi[j] = fma(-f, rowk[j], m);
-            }
-        }
+        gauss_jordan(M, n, colk, rowk);
+        for (int64_t e = tid; e < nn; e += 256) Mg[e] = M[e];
+    } else {
+        for (int64_t e = tid; e < nn; e += 256) Mg[e] = 0.0;
         __syncthreads();
+        for (int i = tid; i < n; i += 256)
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) Mg[static_cast<int64_t>(i) * n + idx[k]] = V[k];
+        __syncthreads();
+        gauss_jordan(Mg, n, colk, rowk);
     }
 }
 

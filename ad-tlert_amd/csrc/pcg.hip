@@ -944,8 +944,9 @@ int amg_numeric(adt_pcg_plan* p, hipStream_t st) {
     }
     const AmgLevel& C = p->amg[m - 1];
     hipLaunchKernelGGL(amg_pvals, dim3(adt::blocks_for(C.max_rows, 256), p->n_sys), dim3(256), 0, st, C.args_d);
-    hipLaunchKernelGGL(amg_dense_inverse, dim3(p->n_sys), dim3(256), static_cast<size_t>(C.max_rows) * 8, st,
-                       C.args_d);
+    const bool lds_matrix = C.max_rows <= kDenseLdsRows;
+    const size_t lds = (2 + (lds_matrix ? static_cast<size_t>(C.max_rows) : 0)) * C.max_rows * 8;
+    hipLaunchKernelGGL(amg_dense_inverse, dim3(p->n_sys), dim3(256), lds, st, C.args_d, lds_matrix ? 1 : 0);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }

@@ -78,7 +78,7 @@ class BatchedSolver:
     lds_bytes: int | None = None   # LDS budget of the staged products (None: native.LDS_BYTES; 0: gather kernels)
     preconditioner: str = "jacobi"
     amg_theta: float = 0.08
-    amg_coarse_max: int = 256
+    amg_coarse_max: int = 128      # coarsest operators up to 128 rows are inverted in LDS
     amg_stall_limit: int = 64
     damping_network: object = None  # learned.DampingNetwork for preconditioner="learned"
     autotune: bool = True          # time the candidate lane widths before the first plan of a shape

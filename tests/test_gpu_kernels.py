@@ -136,7 +136,8 @@ def random_spd_batch(rng, side=20, S=2):
     return indptr, c.astype(np.int32), values, dense
 
 
-def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2, omega=None, rows_per_tile=32):
+def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2, omega=None, rows_per_tile=32,
+                coarse_max=16):
     rng = np.random.default_rng(seed)
     indptr, indices, values, dense = random_spd_batch(rng, side)
     S, n = values.shape[0], side * side
@@ -147,7 +148,7 @@ def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2
     values_d = torch.as_tensor(values, device=dev())
     data = None
     if precond != native.PRECOND_JACOBI:
-        assert plan.attach_amg(values_d, coarse_max=16, stall_limit=stall_limit) >= 3
+        assert plan.attach_amg(values_d, coarse_max=coarse_max, stall_limit=stall_limit) >= (3 if coarse_max < 64 else 2)
         assert plan.amg_level(1)["rows"] < n
     if precond == native.PRECOND_LEARNED:
         data = torch.full_like(values_d, omega)
@@ -177,6 +178,20 @@ def test_pcg_solves_random_spd_batches_to_tolerance(staged, precond):
         jacobi_iters, _ = solve_batch(native.PRECOND_JACOBI, staged)
         assert stats["levels"] >= 3 and stats["fallback_rhs"] == 0
         assert iters.max() < 0.5 * jacobi_iters.max(), (iters.max(), jacobi_iters.max())
+
+
+def test_coarsest_operators_beyond_the_lds_size_are_inverted_in_global_memory():
+    """A coarsest level of more than 128 rows (900-row grid, coarsening stopped at 400 rows) is
+    inverted in global memory instead of LDS; the solutions still match dense solves."""
+
+    rng = np.random.default_rng(2)
+    indptr, indices, values, _ = random_spd_batch(rng, 30)
+    plan = native.PcgPlan(np.full(2, 900), 13, 16, 32, np.zeros(2), np.zeros(2), np.arange(2) * values.shape[1],
+                          host_pattern=(indptr, indices), lds_bytes=8192)
+    plan.attach_amg(torch.as_tensor(values, device=dev()), coarse_max=400)
+    assert plan.amg_level(plan.amg_levels - 1)["rows"] > 128
+    plan.close()
+    solve_batch(native.PRECOND_AMG, True, side=30, coarse_max=400)
 
 
 def test_staged_and_gather_multigrid_iterate_identically():
