@@ -659,6 +659,7 @@ struct AmgLevel {
     int smooth_lds = 0;   // dynamic LDS of the staged smoothing kernels (0: gather kernels)
     int prolong_lds = 0;  // dynamic LDS of the staged prolongation (0: gather kernel)
     int restrict_lds = 0; // dynamic LDS of the staged restriction to the next level (0: gather kernel)
+    int tile_rows = 0;    // rows per tile on this level
 };
 
 }  // namespace
@@ -1371,6 +1372,26 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         lds_limit = 0;
     lds_limit = std::max(0, std::min(lds_limit - 8192, 65536));  // the block reductions keep static LDS
+    // tile height per level: the finest level's tiles; coarse levels start at rbc rows and halve
+    // (down to one pass of the workgroup) until their staged smoothing tiles fit the LDS budget
+    for (int l = 0; l < depth; ++l) {
+        if (l == 0) {
+            p->amg[0].tile_rows = p->rb;
+            continue;
+        }
+        int th = rbc;
+        for (; th > 256 / L; th = std::max(256 / L, th / 2)) {
+            int staged = 0, nnz = 0;
+            for (int q = 0; q < n_pat; ++q) {
+                const HostLevel& h = H[q][l];
+                const PatternTiles pt = build_pattern_tiles(h.ptr.data(), h.idx.data(), h.n, th);
+                staged = std::max(staged, pt.max_staged);
+                nnz = std::max(nnz, pt.max_nnz);
+            }
+            if (lds_bytes_for(staged, nnz, th, L) <= lds_limit) break;
+        }
+        p->amg[l].tile_rows = th;
+    }
     // pattern-level offsets, per level
     std::vector<std::vector<int64_t>> off_ptr(depth, std::vector<int64_t>(n_pat)), off_idx = off_ptr,
         off_diag = off_ptr, off_pptr = off_ptr, off_pidx = off_ptr, off_pmp = off_ptr, off_pmap = off_ptr,
@@ -1453,20 +1474,21 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         // LDS staging tables: smoothing on levels >= 1, prolongation on every level but the
         // coarsest; per pattern tile (tiles of th rows, numbered pattern by pattern), kept only
         // when the staged tile fits the LDS budget
-        if (l + 1 < depth) {  // restriction: slices of the next level's tiles (rbc coarse rows)
+        if (l + 1 < depth) {  // restriction: slices of the next level's tiles
+            const int thn = p->amg[l + 1].tile_rows;
             int r_nnz = 0;
             for (int q = 0; q < n_pat; ++q) {
                 const HostLevel& h = H[q][l];
-                for (int r0 = 0; r0 < h.nc; r0 += rbc)
-                    r_nnz = std::max(r_nnz, h.rptr[std::min(r0 + rbc, h.nc)] - h.rptr[r0]);
+                for (int r0 = 0; r0 < h.nc; r0 += thn)
+                    r_nnz = std::max(r_nnz, h.rptr[std::min(r0 + thn, h.nc)] - h.rptr[r0]);
             }
-            const int64_t r_lds = static_cast<int64_t>(r_nnz) * 12 + (rbc + 1) * 4;
+            const int64_t r_lds = static_cast<int64_t>(r_nnz) * 12 + (thn + 1) * 4;
             if (r_lds <= lds_limit) {
                 a.r_lds_nnz = r_nnz;
                 A.restrict_lds = static_cast<int>(r_lds);
             }
         }
-        const int th = l == 0 ? p->rb : rbc;
+        const int th = A.tile_rows;
         if (th > max_staged_tile_rows(L)) continue;
         std::vector<uint16_t> lidx_l, plidx_l;
         std::vector<int> hp_l, hr_l, pcp_l, pcr_l;
@@ -1580,11 +1602,12 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
             A.max_t = std::max(A.max_t, S.n_t);
             A.max_next = std::max(A.max_next, S.n_next);
             if (l > 0) {
+                const int th = A.tile_rows;
                 int base = 0;  // pattern tiles of this level before pattern q
-                for (int q2 = 0; q2 < q; ++q2) base += (H[q2][l].n + rbc - 1) / rbc;
+                for (int q2 = 0; q2 < q; ++q2) base += (H[q2][l].n + th - 1) / th;
                 for (int g = 0; g < p->groups; ++g)
-                    for (int r0 = 0; r0 < h.n; r0 += rbc)
-                        A.tiles_h.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + rbc, h.n), base + r0 / rbc, 0});
+                    for (int r0 = 0; r0 < h.n; r0 += th)
+                        A.tiles_h.push_back(Tile{s, s * p->groups + g, r0, std::min(r0 + th, h.n), base + r0 / th, 0});
             }
         }
     }
