@@ -36,13 +36,18 @@ constexpr int SLOTS_PER_BLOCK = 64;  // slots per block of the head kernels (8 p
 
 __device__ __forceinline__ float bf16_to_float(uint16_t v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
 
-// per-system FP32 work layout
+// per-system FP32 work layout; every region starts on a 16-byte boundary (pq: per node, the
+// head's first layer applied to h3 as the row [h3 U[0:H]] and as the column [h3 U[H:2H]] of a
+// slot; the head's hidden units are recomputed from it where the backward needs them)
 struct Work {
-    float *x, *w, *e, *h[4], *m[4], *u, *s, *mean;
+    float *x, *w, *e, *h[4], *m[4], *s, *mean, *pq;
 };
 
+__host__ __device__ inline int64_t round4(int64_t v) { return (v + 3) & ~int64_t(3); }
+
 __host__ __device__ inline int64_t work_floats(int n, int nnz) {
-    return static_cast<int64_t>(n) * (NF + 4 * H + 3 * H) + static_cast<int64_t>(nnz) * (1 + EF + HD + 1) + 4;
+    return static_cast<int64_t>(n) * (NF + 4 * H + 3 * H + 2 * HD) + 2 * round4(nnz) +
+           static_cast<int64_t>(nnz) * EF + 4;
 }
 
 __host__ __device__ inline Work work_at(float* base, int n, int nnz) {
@@ -50,16 +55,16 @@ __host__ __device__ inline Work work_at(float* base, int n, int nnz) {
     const int64_t N = n, Z = nnz;
     W.x = base;
     W.w = W.x + N * NF;
-    W.e = W.w + Z;
+    W.e = W.w + round4(Z);
     W.h[0] = W.e + Z * EF;
     for (int l = 1; l < 4; ++l) W.h[l] = W.h[l - 1] + N * H;
     W.m[0] = nullptr;
     W.m[1] = W.h[3] + N * H;
     W.m[2] = W.m[1] + N * H;
     W.m[3] = W.m[2] + N * H;
-    W.u = W.m[3] + N * H;
-    W.s = W.u + Z * HD;
-    W.mean = W.s + Z;
+    W.s = W.m[3] + N * H;
+    W.mean = W.s + round4(Z);
+    W.pq = W.mean + 4;
     return W;
 }
 
@@ -152,80 +157,201 @@ __global__ __launch_bounds__(256) void gnn_layer0(Graph g, const uint16_t* param
     }
 }
 
-// message layer l: m = Wmsg h_{l-1}, h_l = relu([h_{l-1}, m] W_l + b_l)
-__global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params, float* work, int64_t wstride,
-                                                 int l) {
-    __shared__ uint16_t Wl[2 * H * H];
-    __shared__ float bl[H];
-    __shared__ float in[4][2 * H];
-    const uint16_t* P = params + OFF_L1 + (l - 1) * LAYER_SIZE;
-    for (int t = threadIdx.x; t < 2 * H * H; t += 256) Wl[t] = P[t];
-    if (threadIdx.x < H) bl[threadIdx.x] = bf16_to_float(P[2 * H * H + threadIdx.x]);
-    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
-    const float* hp = W.h[l - 1];
-    const int q = threadIdx.x / H, c = threadIdx.x % H;
-    for (int r = 0; r < NODES_PER_BLOCK; r += 4) {
-        const int i = blockIdx.x * NODES_PER_BLOCK + r + q;
-        __syncthreads();
-        if (i < g.n) {
-            float m = 0.0f;
-            for (int k = g.ptr[i]; k < g.ptr[i + 1]; ++k) m = fmaf(W.w[k], hp[static_cast<int64_t>(g.idx[k]) * H + c], m);
-            in[q][c] = hp[static_cast<int64_t>(i) * H + c];
-            in[q][H + c] = m;
-            W.m[l][static_cast<int64_t>(i) * H + c] = m;
-        }
-        __syncthreads();
-        if (i < g.n) {
-            float acc = bl[c];
-            for (int t = 0; t < 2 * H; ++t) acc = fmaf(in[q][t], bf16_to_float(Wl[t * H + c]), acc);
-            W.h[l][static_cast<int64_t>(i) * H + c] = fmaxf(acc, 0.0f);
-        }
+// Node GEMM of a block of 16 U nodes: out[i][o] = act(bias[o] + sum_t in[i][t] W[t][o]) for 64
+// outputs, inputs in[i][0..K) staged in LDS (row stride K + 1 floats), weights W[K][64] in LDS
+// as FP32.  Thread = U nodes x 4 outputs, so each weight read serves U nodes and each input read
+// four outputs.  Sums in order t = 0..K-1.
+template <int U>
+constexpr int gemm_nodes() { return 16 * U; }
+
+template <int K, int U, bool RELU>
+__device__ __forceinline__ void node_gemm(const float* in, const float* Wt, const float* bias, int i0, int n,
+                                          float* out) {
+    const int nq = static_cast<int>(threadIdx.x) >> 4, oq = static_cast<int>(threadIdx.x) & 15;
+    float acc[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = bias ? bias[oq * 4 + v] : 0.0f;
+#pragma unroll 4
+    for (int t = 0; t < K; ++t) {
+        float a[U], w[4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = in[(nq * U + u) * (K + 1) + t];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[v] = Wt[t * H + oq * 4 + v];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(a[u], w[v], acc[u][v]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = i0 + nq * U + u;
+        if (i < n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                out[static_cast<int64_t>(i) * H + oq * 4 + v] = RELU ? fmaxf(acc[u][v], 0.0f) : acc[u][v];
     }
 }
 
-// slot head: block = 8 slots x 32 hidden units per round
+constexpr int kLayerU = 2;  // 32 nodes per block: weights (32 KB) + inputs (16.5 KB) in LDS
+constexpr int kProjU = 4;   // 64 nodes per block: weights (16 KB) + inputs (16.6 KB) in LDS
+static_assert(2 * HD == H, "the head projection writes rows of H floats");
+
+// messages of layer l: m_i = sum_k w_k h_{l-1}[idx_k] in slot order; 16 threads per node, four
+// channels each (16-byte loads), neighbour rows loaded eight at a time
+constexpr int kMsgNodes = 16;  // nodes per block
+
+__global__ __launch_bounds__(256) void gnn_message(Graph g, float* work, int64_t wstride, int l) {
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const auto hp = adt::gview(reinterpret_cast<const float4*>(W.h[l - 1]));
+    const auto wk = adt::gview(W.w);
+    const auto idx = adt::gview(g.idx);
+    const auto ptr = adt::gview(g.ptr);
+    const int c4 = static_cast<int>(threadIdx.x) % (H / 4);
+    const int i = blockIdx.x * kMsgNodes + static_cast<int>(threadIdx.x) / (H / 4);
+    if (i >= g.n) return;
+    const int k0 = ptr[i], k1 = ptr[i + 1];
+    float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int kb = k0; kb < k1; kb += 8) {
+        int j[8];
+        float w[8];
+        float4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = kb + u < k1 ? kb + u : k0;
+            j[u] = idx[k];
+            w[u] = kb + u < k1 ? wk[k] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = hp[static_cast<int64_t>(j[u]) * (H / 4) + c4];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            m.x = fmaf(w[u], v[u].x, m.x);
+            m.y = fmaf(w[u], v[u].y, m.y);
+            m.z = fmaf(w[u], v[u].z, m.z);
+            m.w = fmaf(w[u], v[u].w, m.w);
+        }
+    }
+    adt::gview(reinterpret_cast<float4*>(W.m[l]))[static_cast<int64_t>(i) * (H / 4) + c4] = m;
+}
+
+// message layer l: h_l = relu([h_{l-1}, m_l] W_l + b_l); block = kLayerBatches x 32 nodes
+constexpr int kLayerBatches = 4;
+
+__global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params, float* work, int64_t wstride,
+                                                 int l) {
+    constexpr int NB = gemm_nodes<kLayerU>();
+    __shared__ float Wl[2 * H * H];
+    __shared__ float bl[H];
+    __shared__ float in[NB * (2 * H + 1)];
+    const uint16_t* P = params + OFF_L1 + (l - 1) * LAYER_SIZE;
+    for (int t = threadIdx.x; t < 2 * H * H; t += 256) Wl[t] = bf16_to_float(P[t]);
+    if (threadIdx.x < H) bl[threadIdx.x] = bf16_to_float(P[2 * H * H + threadIdx.x]);
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const auto hp = adt::gview(W.h[l - 1]);
+    const auto mp = adt::gview(W.m[l]);
+    for (int b = 0; b < kLayerBatches; ++b) {
+        const int i0 = (blockIdx.x * kLayerBatches + b) * NB;
+        if (i0 >= g.n) break;
+        __syncthreads();  // weights stored; the previous batch's inputs read
+        for (int t = threadIdx.x; t < NB * 2 * H; t += 256) {
+            const int r = t / (2 * H), c = t % (2 * H), i = i0 + r;
+            const int ic = i < g.n ? i : g.n - 1;
+            const float v = c < H ? hp[static_cast<int64_t>(ic) * H + c] : mp[static_cast<int64_t>(ic) * H + c - H];
+            in[r * (2 * H + 1) + c] = i < g.n ? v : 0.0f;
+        }
+        __syncthreads();
+        node_gemm<2 * H, kLayerU, true>(in, Wl, bl, i0, g.n, W.h[l]);
+    }
+}
+
+// head, node part: pq[i] = [h3_i U[0:H], h3_i U[H:2H]] (the row and column terms of every slot)
+__global__ __launch_bounds__(256) void gnn_head_proj(Graph g, const uint16_t* params, float* work, int64_t wstride) {
+    __shared__ float Ws[H * 2 * HD];
+    constexpr int NB = gemm_nodes<kProjU>();
+    __shared__ float in[NB * (H + 1)];
+    for (int t = threadIdx.x; t < H * 2 * HD; t += 256) {
+        const int r = t / (2 * HD), o = t % (2 * HD);
+        Ws[t] = bf16_to_float(params[OFF_U + (o < HD ? r : H + r) * HD + (o % HD)]);
+    }
+    Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
+    const int i0 = blockIdx.x * NB;
+    for (int t = threadIdx.x; t < NB * H; t += 256) {
+        const int r = t / H, c = t % H, i = i0 + r;
+        in[r * (H + 1) + c] = i < g.n ? W.h[3][static_cast<int64_t>(i) * H + c] : 0.0f;
+    }
+    __syncthreads();
+    node_gemm<H, kProjU, false>(in, Ws, nullptr, i0, g.n, W.pq);
+}
+
+// hidden units j0..j0+3 of slot k before the ReLU: c + pq_row + pq_col + e_k U[2H:2H+4] (the
+// backward recomputes them with this same sequence)
+__device__ __forceinline__ float4 head_pre(const float* cs, const float* Ue, float4 pr, float4 pc, float4 e, int j0) {
+    float a[4] = {pr.x, pr.y, pr.z, pr.w}, b[4] = {pc.x, pc.y, pc.z, pc.w}, ev[4] = {e.x, e.y, e.z, e.w};
+    float o[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        float acc = cs[j0 + v] + a[v] + b[v];
+#pragma unroll
+        for (int t = 0; t < EF; ++t) acc = fmaf(ev[t], Ue[t * HD + j0 + v], acc);
+        o[v] = acc;
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__device__ __forceinline__ float4 load4(const float* p) {
+    return *adt::gview(reinterpret_cast<const float4*>(p));
+}
+
+// slot head: thread = 4 hidden units of one slot, 32 slots per round;
+//   u = relu(c + pq_row[i] + pq_col[j] + e_k U[2H:2H+4]),  s = u . v + d
+constexpr int kHeadThreadsPerSlot = HD / 4;
+
 __global__ __launch_bounds__(256) void gnn_head(Graph g, const uint16_t* params, float* work, int64_t wstride,
                                                 const double* omega_p, double* omega, int64_t ostride) {
-    __shared__ uint16_t Us[ZIN * HD];
+    __shared__ float Ue[EF * HD];
     __shared__ float cs[HD], vs[HD];
-    __shared__ float z[8][ZIN];
-    for (int t = threadIdx.x; t < ZIN * HD; t += 256) Us[t] = params[OFF_U + t];
+    for (int t = threadIdx.x; t < EF * HD; t += 256) Ue[t] = bf16_to_float(params[OFF_U + 2 * H * HD + t]);
     if (threadIdx.x < HD) {
         cs[threadIdx.x] = bf16_to_float(params[OFF_C + threadIdx.x]);
         vs[threadIdx.x] = bf16_to_float(params[OFF_V + threadIdx.x]);
     }
+    __syncthreads();
     const float d0 = bf16_to_float(params[OFF_D]);
     Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
     const double wp = omega_p[blockIdx.y];
     double* O = omega + blockIdx.y * ostride;
-    const int q = threadIdx.x / HD, j = threadIdx.x % HD;
-    for (int r = 0; r < SLOTS_PER_BLOCK; r += 8) {
-        const int k = blockIdx.x * SLOTS_PER_BLOCK + r + q;
-        __syncthreads();
-        if (k < g.nnz) {
-            const int64_t i = g.rows[k], col = g.idx[k];
-            for (int t = j; t < H; t += HD) {
-                z[q][t] = W.h[3][i * H + t];
-                z[q][H + t] = W.h[3][col * H + t];
-            }
-            if (j < EF) z[q][2 * H + j] = W.e[static_cast<int64_t>(k) * EF + j];
-        }
-        __syncthreads();
-        float part = 0.0f;
-        if (k < g.nnz) {
-            float acc = cs[j];
-            for (int t = 0; t < ZIN; ++t) acc = fmaf(z[q][t], bf16_to_float(Us[t * HD + j]), acc);
-            const float u = fmaxf(acc, 0.0f);
-            W.u[static_cast<int64_t>(k) * HD + j] = u;
-            part = u * vs[j];
-        }
-        // sum over the 32 hidden units (lanes j of one half wavefront), fixed butterfly order
+    constexpr int SPR = 256 / kHeadThreadsPerSlot;  // slots per round
+    const int q = static_cast<int>(threadIdx.x) / kHeadThreadsPerSlot;
+    const int j0 = (static_cast<int>(threadIdx.x) % kHeadThreadsPerSlot) * 4;
+    int64_t ri[SLOTS_PER_BLOCK / SPR], ci[SLOTS_PER_BLOCK / SPR];
 #pragma unroll
-        for (int off = 1; off < HD; off <<= 1) part += __shfl_xor(part, off, 64);
-        if (k < g.nnz && j == 0) {
-            const float s = part + d0;
-            W.s[k] = s;
-            O[k] = wp * (1.0 + 0.5 * tanh(static_cast<double>(s)));
+    for (int r = 0; r < SLOTS_PER_BLOCK / SPR; ++r) {
+        const int k = blockIdx.x * SLOTS_PER_BLOCK + r * SPR + q;
+        const int kc = k < g.nnz ? k : g.nnz - 1;
+        ri[r] = adt::gview(g.rows)[kc];
+        ci[r] = adt::gview(g.idx)[kc];
+    }
+#pragma unroll
+    for (int r = 0; r < SLOTS_PER_BLOCK / SPR; ++r) {
+        const int k = blockIdx.x * SLOTS_PER_BLOCK + r * SPR + q;
+        const int kc = k < g.nnz ? k : g.nnz - 1;
+        const float4 pre = head_pre(cs, Ue, load4(W.pq + ri[r] * (2 * HD) + j0),
+                                    load4(W.pq + ci[r] * (2 * HD) + HD + j0),
+                                    load4(W.e + static_cast<int64_t>(kc) * EF), j0);
+        float part = fmaxf(pre.x, 0.0f) * vs[j0];
+        part = fmaf(fmaxf(pre.y, 0.0f), vs[j0 + 1], part);
+        part = fmaf(fmaxf(pre.z, 0.0f), vs[j0 + 2], part);
+        part = fmaf(fmaxf(pre.w, 0.0f), vs[j0 + 3], part);
+        // sum over the slot's eight threads, fixed butterfly order
+#pragma unroll
+        for (int off = 1; off < kHeadThreadsPerSlot; off <<= 1) part += __shfl_xor(part, off, 64);
+        if (k < g.nnz && j0 == 0) {
+            const float sv = part + d0;
+            W.s[k] = sv;
+            O[k] = wp * (1.0 + 0.5 * tanh(static_cast<double>(sv)));
         }
     }
 }
@@ -269,11 +395,16 @@ __global__ __launch_bounds__(256) void gnn_head_bwd(Graph g, const uint16_t* par
                                                     double omega_p, const double* domega, float* scratch) {
     __shared__ float z[8][ZIN];
     __shared__ float du[8][HD];
+    __shared__ float us[8][HD];
     __shared__ float dsv[8];
     __shared__ uint16_t Us[ZIN * HD];
-    __shared__ float vs[HD];
+    __shared__ float vs[HD], cs[HD], Ue[EF * HD];
     for (int t = threadIdx.x; t < ZIN * HD; t += 256) Us[t] = params[OFF_U + t];
-    if (threadIdx.x < HD) vs[threadIdx.x] = bf16_to_float(params[OFF_V + threadIdx.x]);
+    for (int t = threadIdx.x; t < EF * HD; t += 256) Ue[t] = bf16_to_float(params[OFF_U + 2 * H * HD + t]);
+    if (threadIdx.x < HD) {
+        vs[threadIdx.x] = bf16_to_float(params[OFF_V + threadIdx.x]);
+        cs[threadIdx.x] = bf16_to_float(params[OFF_C + threadIdx.x]);
+    }
     const Work W = work_at(const_cast<float*>(work), g.n, g.nnz);
     Scratch S = scratch_at(scratch, g.n, g.nnz);
     const int q = threadIdx.x / HD, j = threadIdx.x % HD;
@@ -295,12 +426,18 @@ __global__ __launch_bounds__(256) void gnn_head_bwd(Graph g, const uint16_t* par
             const float s = W.s[k];
             const float th = tanhf(s);
             const float ds = static_cast<float>(domega[k] * omega_p * 0.5) * (1.0f - th * th);
-            const float u = W.u[static_cast<int64_t>(k) * HD + j];
+            const int j4 = j & ~3;
+            const float4 pre = head_pre(cs, Ue, load4(W.pq + i * (2 * HD) + j4), load4(W.pq + col * (2 * HD) + HD + j4),
+                                        load4(W.e + static_cast<int64_t>(k) * EF), j4);
+            const float pj = (j & 3) == 0 ? pre.x : (j & 3) == 1 ? pre.y : (j & 3) == 2 ? pre.z : pre.w;
+            const float u = fmaxf(pj, 0.0f);
+            us[q][j] = u;
             du[q][j] = u > 0.0f ? ds * vs[j] : 0.0f;
             if (j == 0) dsv[q] = ds;
         } else {
             for (int t = j; t < ZIN; t += HD) z[q][t] = 0.0f;
             du[q][j] = 0.0f;
+            us[q][j] = 0.0f;
             if (j == 0) dsv[q] = 0.0f;
         }
         __syncthreads();
@@ -324,9 +461,7 @@ __global__ __launch_bounds__(256) void gnn_head_bwd(Graph g, const uint16_t* par
                 } else if (e < ZIN * HD + HD) {
                     term = du[qq][e - ZIN * HD];
                 } else if (e < ZIN * HD + 2 * HD) {
-                    const int jj = e - ZIN * HD - HD;
-                    const int kk = blockIdx.x * SLOTS_PER_BLOCK + r0 + qq;
-                    term = kk < g.nnz ? dsv[qq] * W.u[static_cast<int64_t>(kk) * HD + jj] : 0.0f;
+                    term = dsv[qq] * us[qq][e - ZIN * HD - HD];
                 } else {
                     term = dsv[qq];
                 }
@@ -521,7 +656,14 @@ extern "C" int adt_gnn_forward(int n_sys, int n, int nnz, const int32_t* d_indpt
                        d_work, ws, static_cast<float>(theta));
     const dim3 nodes(node_blocks(n), n_sys);
     hipLaunchKernelGGL(gnn_layer0, nodes, dim3(256), 0, st, g, d_params, d_work, ws);
-    for (int l = 1; l <= 3; ++l) hipLaunchKernelGGL(gnn_layer, nodes, dim3(256), 0, st, g, d_params, d_work, ws, l);
+    const dim3 msg_grid(adt::blocks_for(n, kMsgNodes), n_sys);
+    const dim3 layer_grid(adt::blocks_for(n, kLayerBatches * gemm_nodes<kLayerU>()), n_sys);
+    for (int l = 1; l <= 3; ++l) {
+        hipLaunchKernelGGL(gnn_message, msg_grid, dim3(256), 0, st, g, d_work, ws, l);
+        hipLaunchKernelGGL(gnn_layer, layer_grid, dim3(256), 0, st, g, d_params, d_work, ws, l);
+    }
+    hipLaunchKernelGGL(gnn_head_proj, dim3(adt::blocks_for(n, gemm_nodes<kProjU>()), n_sys), dim3(256), 0, st, g,
+                       d_params, d_work, ws);
     hipLaunchKernelGGL(gnn_head, dim3(head_blocks(nnz), n_sys), dim3(256), 0, st, g, d_params, d_work, ws, d_omega_p,
                        d_omega, val_stride);
     ADT_HIP_CHECK(hipGetLastError());

@@ -40,7 +40,7 @@ _SIGNATURES = {
     "adt_pcg_profile_read": (c_int, [P, P]),
     "adt_pcg_attach_amg": (c_int, [P, P, P, P, P, c_double, c_int, c_int, c_int, P]),
     "adt_pcg_amg_level": (c_int, [P, c_int, c_int, P, P, P]),
-    "adt_pcg_stats": (c_int, [P, P, P]),
+    "adt_pcg_stats": (c_int, [P, P, P, P]),
     "adt_amg_hierarchy_info": (c_int, [c_int, P, P, P, c_double, c_int, c_int, P, P, P]),
     "adt_amg_aggregates": (c_int, [c_int, P, P, P, c_double, P]),
     "adt_gnn_param_count": (c_int, []),
@@ -223,9 +223,10 @@ class PcgPlan:
         return dict(rows=rows.value, nnz=nnz.value, omega=omega.value)
 
     def stats(self) -> dict:
-        levels, fallback = c_int(0), c_int(0)
-        check(self._lib.adt_pcg_stats(self._handle, ctypes.byref(levels), ctypes.byref(fallback)))
-        return dict(levels=levels.value, fallback_rhs=fallback.value)
+        levels, fallback, polished = c_int(0), c_int(0), c_int(0)
+        check(self._lib.adt_pcg_stats(self._handle, ctypes.byref(levels), ctypes.byref(fallback),
+                                      ctypes.byref(polished)))
+        return dict(levels=levels.value, fallback_rhs=fallback.value, polished_rhs=polished.value)
 
     def solve(self, indptr, indices, values, b, x, tol, max_iter, *, precond=PRECOND_JACOBI, precond_data=None,
               check_every=16):

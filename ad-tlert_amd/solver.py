@@ -63,7 +63,8 @@ class SolveStats:
     relative_residual: np.ndarray
     seconds: float
     preconditioner: str = "jacobi"
-    fallback_rhs: int = 0
+    fallback_rhs: int = 0  # finished by Jacobi PCG after the cycle stalled
+    polished_rhs: int = 0  # restarted once because the true residual missed the tolerance
 
 
 @dataclass
@@ -243,8 +244,9 @@ class BatchedSolver:
         t0 = time.perf_counter()
         rc, iters, relres = plan.solve(indptr, indices, values, b, x, tol, max_iter, precond=precond,
                                        precond_data=data, check_every=self.check_every)
-        fallback = plan.stats()["fallback_rhs"] if precond != native.PRECOND_JACOBI else 0
-        self.last = SolveStats(iters, relres, time.perf_counter() - t0, used, fallback)
+        st = plan.stats()
+        fallback = st["fallback_rhs"] if precond != native.PRECOND_JACOBI else 0
+        self.last = SolveStats(iters, relres, time.perf_counter() - t0, used, fallback, st["polished_rhs"])
         self.last_plan = plan
         if raise_on_failure and rc == native.ADT_ERR_NONFINITE:
             raise FloatingPointError("conjugate gradients produced non-finite values")

@@ -13,14 +13,17 @@ Workloads (``--workload``):
   right-hand sides of ``--batch-steps`` conductivity models.
 
 Every step solves to a relative residual of 1e-10 and reduces the fields to apparent
-resistivities.  The main solver (``--preconditioner``, default multigrid-preconditioned
-flexible CG) and plain Jacobi PCG are timed on the same batches.  After the timed steps a
+resistivities.  The main solver (``--preconditioner``, default ``learned``: flexible CG
+preconditioned by the multigrid cycle whose finest prolongator is damped by the trained graph
+network), the same cycle with constant damping (``amg``) and plain Jacobi PCG are timed on the
+same batches.  After the timed steps a
 second timed phase materializes the full normal-sensitivity Jacobian (all quadrupole rows x
 all cells) of one model with the fused kernel.
 
 Prints ONE JSON line.  ``value`` = RHS solves/s of the main solver over the timed steps (wall
-clock between two synchronizations, graph-replayed iterations); ``solver.rhs_solves_per_s_pcg``
-is plain Jacobi PCG; ``jacobian_rows_per_s`` is the second phase.  Kernel rooflines come from
+clock between two synchronizations, graph-replayed iterations, the network's inference
+included); ``solver.rhs_solves_per_s_pcg`` is plain Jacobi PCG and
+``solver.rhs_solves_per_s_amg`` the constant-damping cycle; ``jacobian_rows_per_s`` is the second phase.  Kernel rooflines come from
 one more step of the same workload run with the solver's profiling mode: the same kernels
 launched directly on the same stream with HIP timing events around every multigrid cycle,
 spmv and update launch, so each kernel's average duration is measured live.
@@ -206,10 +209,10 @@ def main() -> int:
     ap.add_argument("--rows-per-tile", type=int, default=None, help="PCG workgroup tile height (default: auto)")
     ap.add_argument("--lanes", type=int, default=None, help="lane-group width 8/16/32/64 (default: auto)")
     ap.add_argument("--lds-bytes", type=int, default=None, help="LDS budget of the staged kernels (0: gather kernels)")
-    ap.add_argument("--preconditioner", choices=("amg", "jacobi"), default="amg")
+    ap.add_argument("--preconditioner", choices=("learned", "amg", "jacobi"), default="learned")
     ap.add_argument("--pcg-steps", type=int, default=2, help="timed steps of plain Jacobi PCG on the same batches (0: skip)")
-    ap.add_argument("--learned-steps", type=int, default=2,
-                    help="timed steps of flexible CG with the trained damping network on the same batches (0: skip)")
+    ap.add_argument("--amg-steps", type=int, default=2,
+                    help="timed steps of flexible CG with the constant-damping cycle on the same batches (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -225,7 +228,7 @@ def main() -> int:
     adt = importlib.import_module("ad-tlert_amd")
     dev = adt.native.device()
     cus, arch = adt.native.device_info(0)
-    models = (max(args.warmup + args.steps, 1 + args.pcg_steps, 1 + args.learned_steps) + 1) * args.batch_steps
+    models = (max(args.warmup + args.steps, 1 + args.pcg_steps, 1 + args.amg_steps) + 1) * args.batch_steps
     t0 = time.perf_counter()
     wl = (Timelapse if args.workload == "timelapse" else Box3D)(adt, torch, args, models)
     op = wl.op
@@ -244,7 +247,7 @@ def main() -> int:
             stats = wl.step(i)
             log(f"[{preconditioner}] warmup {i}: iters mean {stats.iterations.mean():.0f} max "
                 f"{stats.iterations.max()} relres max {stats.relative_residual.max():.2e}")
-        iters, worst, fallback = [], 0.0, 0
+        iters, worst, fallback, polished = [], 0.0, 0, 0
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for i in range(first + warmup, first + warmup + steps):
@@ -252,26 +255,27 @@ def main() -> int:
             iters.append(stats.iterations.mean())
             worst = max(worst, float(stats.relative_residual.max()))
             fallback += stats.fallback_rhs
+            polished += stats.polished_rhs
             log(f"[{preconditioner}] step {i}: iters mean {stats.iterations.mean():.1f} max {stats.iterations.max()} "
-                f"relres max {stats.relative_residual.max():.2e} fallback {stats.fallback_rhs}")
+                f"relres max {stats.relative_residual.max():.2e} fallback {stats.fallback_rhs} "
+                f"polished {stats.polished_rhs}")
         torch.cuda.synchronize()
-        return time.perf_counter() - t_start, float(np.mean(iters)) if iters else None, worst, fallback
+        return time.perf_counter() - t_start, float(np.mean(iters)) if iters else None, worst, fallback, polished
 
     # the same batches (steps 0 .. warmup+steps-1) for both solvers
-    elapsed, mean_iters, worst_relres, fallback = timed(args.preconditioner, 0, args.steps, args.warmup)
+    elapsed, mean_iters, worst_relres, fallback, polished = timed(args.preconditioner, 0, args.steps, args.warmup)
     rhs_per_s = args.steps * wl.rhs_per_step / elapsed
     pcg_per_s = pcg_iters = None
-    pcg_relres = 0.0
+    pcg_relres = pcg_polished = None
     if args.pcg_steps > 0 and args.preconditioner != "jacobi":
-        pcg_s, pcg_iters, pcg_relres, _ = timed("jacobi", 0, args.pcg_steps, 1)
+        pcg_s, pcg_iters, pcg_relres, _, pcg_polished = timed("jacobi", 0, args.pcg_steps, 1)
         pcg_per_s = args.pcg_steps * wl.rhs_per_step / pcg_s
     elif args.preconditioner == "jacobi":
         pcg_per_s, pcg_iters = rhs_per_s, mean_iters
-    learned_per_s = learned_iters = None
-    if args.learned_steps > 0 and args.preconditioner == "amg":
-        learned_s, learned_iters, learned_relres, _ = timed("learned", 0, args.learned_steps, 1)
-        learned_per_s = args.learned_steps * wl.rhs_per_step / learned_s
-        pcg_relres = max(pcg_relres, learned_relres)
+    amg_per_s = amg_iters = amg_relres = None
+    if args.amg_steps > 0 and args.preconditioner == "learned":
+        amg_s, amg_iters, amg_relres, _, _ = timed("amg", 0, args.amg_steps, 1)
+        amg_per_s = args.amg_steps * wl.rhs_per_step / amg_s
     op._solver.preconditioner = args.preconditioner
     # a copy of known size (256 MiB read + 256 MiB written): hardware-counter runs of this command
     # scale the solver kernels' FETCH_SIZE / WRITE_SIZE by what the counters read for it
@@ -300,13 +304,23 @@ def main() -> int:
     N, nnz = wl.N, wl.nnz
     matrix_bytes = 12 * nnz + 4 * (N + 1) + 8 * N  # values+indices, indptr, D^-1 (gathered)
     spmv_bytes = rhs_iters * N * 32 + group_iters * matrix_bytes  # read r (z), p_old; write p_new, q
-    update_bytes = rhs_iters * N * 48 + group_iters * 8 * N  # read x, p, r, q; write x, r
+    # the multigrid FCG fuses its update with the next cycle's fine pre-smoothing: it also writes
+    # the smoothed residual r - A w D^-1 r and reads the fine matrix once per group
+    fused = args.preconditioner != "jacobi" and plan.lds_rows > 0 and plan.amg_levels > 0
+    if fused:
+        update_name = "fcg_update_smooth"
+        update_bytes = rhs_iters * N * 56 + group_iters * matrix_bytes  # read x, p, r, q; write x, r, r_s
+    else:
+        update_name = "cg_update"
+        update_bytes = rhs_iters * N * 48 + group_iters * 8 * N  # read x, p, r, q; write x, r
     spmv_gbs = spmv_bytes / (prof["spmv_ms"] * 1e-3) / 1e9
     update_gbs = update_bytes / (prof["update_ms"] * 1e-3) / 1e9
     dominant = ("cg_spmv", spmv_gbs, spmv_bytes, prof["spmv_ms"]) if prof["spmv_ms"] >= prof["update_ms"] else \
-        ("cg_update", update_gbs, update_bytes, prof["update_ms"])
+        (update_name, update_gbs, update_bytes, prof["update_ms"])
     launches = max(prof["iterations"], 1)
-    solver_name = {"amg": "FP64 flexible CG, filtered smoothed-aggregation V(1,1) cycle, graph-replayed",
+    solver_name = {"learned": "FP64 flexible CG, filtered smoothed-aggregation V(1,1) cycle with the finest "
+                              "prolongator damped per slot by the trained graph network, graph-replayed",
+                   "amg": "FP64 flexible CG, filtered smoothed-aggregation V(1,1) cycle, graph-replayed",
                    "jacobi": "FP64 Jacobi-PCG, graph-replayed"}[args.preconditioner]
     config = {"workload": wl.name, **wl.config, "solver": solver_name, "tolerance": args.tol,
               "lanes": plan.lanes, "rp": plan.rp, "n_tiles": plan.tiles, "rows_per_tile": plan.rows_per_tile,
@@ -332,23 +346,25 @@ def main() -> int:
                    "rhs_solves_per_s_pcg": pcg_per_s,
                    "speedup_vs_pcg": rhs_per_s / pcg_per_s if pcg_per_s else None,
                    "mean_iterations": mean_iters, "mean_iterations_pcg": pcg_iters,
-                   "worst_true_relres": max(worst_relres, pcg_relres), "jacobi_fallback_rhs": fallback,
-                   "rhs_solves_per_s_learned_fcg": learned_per_s, "mean_iterations_learned_fcg": learned_iters,
-                   "speedup_learned_fcg_vs_pcg": learned_per_s / pcg_per_s if learned_per_s and pcg_per_s else None},
+                   "worst_true_relres": worst_relres, "worst_true_relres_pcg": pcg_relres,
+                   "worst_true_relres_amg": amg_relres, "jacobi_fallback_rhs": fallback,
+                   "true_residual_restarts": polished, "true_residual_restarts_pcg": pcg_polished,
+                   "rhs_solves_per_s_amg": amg_per_s, "mean_iterations_amg": amg_iters,
+                   "speedup_amg_vs_pcg": amg_per_s / pcg_per_s if amg_per_s and pcg_per_s else None},
         "roofline": {"bound": "hbm", "kernel": dominant[0], "achieved": dominant[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dominant[1] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": dominant[2] / launches, "avg_launch_ms": dominant[3] / launches,
                      "launches": launches},
         "kernels": {"cg_spmv": {"gbs": spmv_gbs, "avg_ms": prof["spmv_ms"] / launches},
-                    "cg_update": {"gbs": update_gbs, "avg_ms": prof["update_ms"] / launches},
+                    update_name: {"gbs": update_gbs, "avg_ms": prof["update_ms"] / launches},
                     "multigrid_cycle": {"avg_ms": prof["cycle_ms"] / launches}},
         "profiled_step": {"seconds": profiled_s, "rhs_solves_per_s": wl.rhs_per_step / profiled_s,
                           "iterations_profiled": prof["iterations"]},
     }
     # HBM traffic of the dominant kernel from the committed hardware-counter runs of this same
     # configuration (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known size)
-    traffic_file = ROOT / "profiles" / "timelapse_cg_spmv_traffic.json"
-    if args.workload == "timelapse" and dominant[0] == "cg_spmv" and traffic_file.exists():
+    traffic_file = ROOT / "profiles" / f"{args.workload}_{dominant[0]}_traffic.json"
+    if traffic_file.exists():
         t = json.loads(traffic_file.read_text())
         c = t["config"]
         if (c["nodes"], c["nnz"], c["rp"], c["time_steps_per_step"], c["preconditioner"]) == \
@@ -356,9 +372,9 @@ def main() -> int:
             result["roofline"]["traffic"] = t["traffic_bytes_per_dispatch"]
             result["roofline"]["traffic_source"] = f"profiles/{traffic_file.name}"
     log(f"timed: {elapsed:.2f}s, {rhs_per_s:.0f} RHS/s ({args.preconditioner}); plain PCG "
-        f"{pcg_per_s or 0:.0f} RHS/s; learned FCG {learned_per_s or 0:.0f} RHS/s; profiled step {profiled_s:.2f}s; cycle {prof['cycle_ms'] / launches:.3f} ms, "
+        f"{pcg_per_s or 0:.0f} RHS/s; constant-damping FCG {amg_per_s or 0:.0f} RHS/s; profiled step {profiled_s:.2f}s; cycle {prof['cycle_ms'] / launches:.3f} ms, "
         f"spmv {spmv_gbs:.0f} GB/s ({prof['spmv_ms'] / launches:.3f} ms), "
-        f"update {update_gbs:.0f} GB/s ({prof['update_ms'] / launches:.3f} ms)")
+        f"{update_name} {update_gbs:.0f} GB/s ({prof['update_ms'] / launches:.3f} ms)")
 
     if not args.no_jacobian:
         run, sens_bytes, sens_flops, kernel = wl.jacobian()

@@ -95,6 +95,9 @@ int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t*
  * device array laid out like d_values, e.g. the output of the learned graph network).  With
  * either cycle, a right-hand side whose residual stalls or breaks down is finished by Jacobi
  * PCG from its current iterate, to the same tolerance (adt_pcg_stats reports how many).
+ * Every solve ends with the true residual b - A x; right-hand sides whose true residual misses
+ * tol (the recurrence residual drifts by rounding) restart once from their iterate in Jacobi
+ * PCG (adt_pcg_stats reports how many), so the reported residuals are the true ones.
  * h_indptr/h_indices (optional): host copies of the concatenated CSR patterns.  With them and
  * lds_bytes > 0 the plan cuts every pattern into row tiles whose rows and halo (referenced rows
  * outside the tile) fit in lds_bytes of LDS per workgroup, and the sparse product stages each
@@ -140,8 +143,9 @@ int adt_pcg_attach_amg(adt_pcg_plan* plan, const int32_t* h_indptr, const int32_
  * damping of one level (the value a learned per-slot damping array replaces). */
 int adt_pcg_amg_level(const adt_pcg_plan* plan, int sys, int level, int* h_rows, int64_t* h_nnz,
                       double* h_omega);
-/* Levels of the attached hierarchy and right-hand sides finished by Jacobi PCG in the last solve. */
-int adt_pcg_stats(const adt_pcg_plan* plan, int* h_levels, int* h_fallback_rhs);
+/* Levels of the attached hierarchy, right-hand sides finished by Jacobi PCG after a stall and
+ * right-hand sides restarted for their true residual in the last solve (null outputs skipped). */
+int adt_pcg_stats(const adt_pcg_plan* plan, int* h_levels, int* h_fallback_rhs, int* h_polished_rhs);
 /* Host-only: the hierarchy that adt_pcg_attach_amg would build for one matrix (level rows and
  * matrix slots into h_rows / h_nnz, max_levels entries each).  Needs no GPU. */
 int adt_amg_hierarchy_info(int n, const int32_t* h_indptr, const int32_t* h_indices, const double* h_values,

@@ -161,7 +161,7 @@ def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2
                                    values_d, torch.as_tensor(b, device=dev()), x, 1e-10, 5000, precond=precond,
                                    precond_data=data)
     assert rc == 0, native.library().adt_last_error()
-    assert relres.max() <= 1e-10 * 1.5
+    assert relres.max() <= 1e-10  # true residuals (restarted once where the recurrence drifted)
     assert iters[1, 3] == 0 and float(x[1, :, 3].abs().max()) == 0.0
     sol = np.linalg.solve(dense, b[:, :, :R])
     assert relmax(x.cpu().numpy()[:, :, :R], sol) < 1e-7
@@ -287,7 +287,7 @@ def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
         rc, iters, relres = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()),
                                        values_d, torch.as_tensor(b, device=dev()), x, 1e-10, 5000, precond=precond)
         assert rc == 0, native.library().adt_last_error()
-        assert relres.max() <= 1.5e-10
+        assert relres.max() <= 1e-10
         xh = x.cpu().numpy()
         for s in range(4):
             xs = xh[plan.vec_off[s]:plan.vec_off[s] + rows[s] * plan.rp].reshape(rows[s], plan.rp)[:, :R]
