@@ -52,6 +52,14 @@ __device__ __forceinline__ T* gview(T* p) {
 }
 #endif
 
+// s_waitcnt vmcnt(0): every vector memory operation of the wavefront has completed.  The
+// compiler tracks loads whose only uses sit in optional or predicated code (halo batches past
+// the first, stores into LDS for rows in range) as possibly outstanding past that code, and
+// then, wherever their registers are reused, waits for all memory operations issued before
+// the reuse -- the stores of earlier rows included.  Called where nothing useful is left in
+// flight anyway (before the barrier of a staged product), it clears that state.
+__device__ __forceinline__ void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Workgroup b runs on XCD b % 8.  Remap so that each XCD receives a contiguous range of
 // tiles (neighbouring row ranges of one system share its L2).  Bijective for any count.
 __device__ __forceinline__ int xcd_tile(int b, int n) {

@@ -636,6 +636,7 @@ __device__ __forceinline__ void stage_halo_rest(const StageSrc& src, const LdsTi
         halo_ids<L>(src, s, base, jj);
         halo_values<L>(jj, v, value);
         store_halo<L>(s, base, v);
+        adt::wait_vector_memory();  // the batch's loads are not left pending past this optional loop
     }
 }
 
@@ -716,6 +717,7 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_lds(const PcgArgs* __
     }
     store_halo<L>(s, 0, hv);
     stage_halo_rest<L>(src, s, value);
+    adt::wait_vector_memory();  // the product loops below then never wait for the stores of earlier rows
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < own_max(L); ++i) {
@@ -830,6 +832,7 @@ __global__ __launch_bounds__(256) void amg_smooth_residual_st(const LevelArgs* _
     }
     store_halo<L>(s, 0, hv);
     stage_halo_rest<L>(src, s, value);
+    adt::wait_vector_memory();  // the product loops below then never wait for the stores of earlier rows
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < own_max(L); ++i) {

@@ -437,7 +437,10 @@ __global__ __launch_bounds__(256) void fcg_update_smooth(const PcgArgs* __restri
     const auto D = adt::gview(g.dinv + S.diag_off);
     const StageSrc src = fine_src(g, S);
     const LdsTile s = lds_tile(src, t, L, smem, true);
-    const auto updated = [&](double r, double q) { return act ? fma(-alpha, q, r) : r; };
+    // inactive columns keep r (alpha taken as 0): a select between the loaded operands would be
+    // compiled into a branch around the halo loads, with a full wait for memory in each
+    const double alpha_act = act ? alpha : 0.0;
+    const auto updated = [&](double r, double q) { return fma(-alpha_act, q, r); };
     const auto value = [&](int j) {
         const int64_t o = static_cast<int64_t>(j) * rp;
         return w * D[j] * updated(R[o], Q[o]);
@@ -477,6 +480,9 @@ __global__ __launch_bounds__(256) void fcg_update_smooth(const PcgArgs* __restri
     }
     store_halo<L>(s, 0, hv);
     stage_halo_rest<L>(src, s, value);
+    // X and Rn stores complete here once, instead of the product loop of every row waiting for
+    // them (and for the R1 stores of earlier rows)
+    adt::wait_vector_memory();
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < own_max(L); ++i) {
