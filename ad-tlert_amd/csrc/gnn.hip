@@ -41,13 +41,16 @@ __device__ __forceinline__ float bf16_to_float(uint16_t v) { return __uint_as_fl
 // slot; the head's hidden units are recomputed from it where the backward needs them)
 struct Work {
     float *x, *w, *e, *h[4], *m[4], *s, *mean, *pq;
+    double* mean_part;  // per-block partial sums of the diagonal
 };
+
+constexpr int kMeanParts = 64;  // blocks per system of the diagonal sum
 
 __host__ __device__ inline int64_t round4(int64_t v) { return (v + 3) & ~int64_t(3); }
 
 __host__ __device__ inline int64_t work_floats(int n, int nnz) {
     return static_cast<int64_t>(n) * (NF + 4 * H + 3 * H + 2 * HD) + 2 * round4(nnz) +
-           static_cast<int64_t>(nnz) * EF + 4;
+           static_cast<int64_t>(nnz) * EF + 4 + 2 * kMeanParts;
 }
 
 __host__ __device__ inline Work work_at(float* base, int n, int nnz) {
@@ -65,6 +68,7 @@ __host__ __device__ inline Work work_at(float* base, int n, int nnz) {
     W.s = W.m[3] + N * H;
     W.mean = W.s + round4(Z);
     W.pq = W.mean + 4;
+    W.mean_part = reinterpret_cast<double*>(W.pq + N * 2 * HD);
     return W;
 }
 
@@ -87,13 +91,14 @@ __device__ __forceinline__ int diag_slot(const Graph& g, int i) {
     return -1;
 }
 
-// mean of the diagonal per system (one block, fixed order)
-__global__ __launch_bounds__(256) void gnn_mean_diag(Graph g, const double* values, int64_t stride, float* work,
-                                                     int64_t wstride) {
+// mean of the diagonal per system, fixed order: kMeanParts blocks per system sum rows
+// i = b * 256 + t (mod kMeanParts * 256) into block partials, then one block sums the partials
+__global__ __launch_bounds__(256) void gnn_diag_partials(Graph g, const double* values, int64_t stride, float* work,
+                                                         int64_t wstride) {
     __shared__ double sh[256];
-    const double* V = values + blockIdx.x * stride;
+    const double* V = values + blockIdx.y * stride;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < g.n; i += 256) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < g.n; i += kMeanParts * 256) {
         const int k = diag_slot(g, i);
         acc += k >= 0 ? V[k] : 0.0;
     }
@@ -103,7 +108,13 @@ __global__ __launch_bounds__(256) void gnn_mean_diag(Graph g, const double* valu
         if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) work_at(work + blockIdx.x * wstride, g.n, g.nnz).mean[0] = static_cast<float>(sh[0] / g.n);
+    if (threadIdx.x == 0) work_at(work + blockIdx.y * wstride, g.n, g.nnz).mean_part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(64) void gnn_mean_diag(Graph g, float* work, int64_t wstride) {
+    Work W = work_at(work + blockIdx.x * wstride, g.n, g.nnz);
+    const double v = adt::wave_sum(threadIdx.x < kMeanParts ? W.mean_part[threadIdx.x] : 0.0);
+    if (threadIdx.x == 0) W.mean[0] = static_cast<float>(v / g.n);
 }
 
 // node features, message weights and slot features: thread per row
@@ -198,6 +209,7 @@ __device__ __forceinline__ void node_gemm(const float* in, const float* Wt, cons
 constexpr int kLayerU = 2;  // 32 nodes per block: weights (32 KB) + inputs (16.5 KB) in LDS
 constexpr int kProjU = 4;   // 64 nodes per block: weights (16 KB) + inputs (16.6 KB) in LDS
 static_assert(2 * HD == H, "the head projection writes rows of H floats");
+static_assert(kMeanParts == adt::kWave, "one wavefront sums the partials");
 
 // messages of layer l: m_i = sum_k w_k h_{l-1}[idx_k] in slot order; 16 threads per node, four
 // channels each (16-byte loads), neighbour rows loaded eight at a time
@@ -651,7 +663,9 @@ extern "C" int adt_gnn_forward(int n_sys, int n, int nnz, const int32_t* d_indpt
     hipStream_t st = adt::as_stream(stream);
     const Graph g = make_graph(n, nnz, d_indptr, d_indices, d_rows, nullptr, d_flags);
     const int64_t ws = work_floats(n, nnz);
-    hipLaunchKernelGGL(gnn_mean_diag, dim3(n_sys), dim3(256), 0, st, g, d_values, val_stride, d_work, ws);
+    hipLaunchKernelGGL(gnn_diag_partials, dim3(kMeanParts, n_sys), dim3(256), 0, st, g, d_values, val_stride, d_work,
+                       ws);
+    hipLaunchKernelGGL(gnn_mean_diag, dim3(n_sys), dim3(64), 0, st, g, d_work, ws);
     hipLaunchKernelGGL(gnn_features, dim3(adt::blocks_for(n, 256), n_sys), dim3(256), 0, st, g, d_values, val_stride,
                        d_work, ws, static_cast<float>(theta));
     const dim3 nodes(node_blocks(n), n_sys);

@@ -367,7 +367,7 @@ def main() -> int:
     if traffic_file.exists():
         t = json.loads(traffic_file.read_text())
         c = t["config"]
-        if (c["nodes"], c["nnz"], c["rp"], c["time_steps_per_step"], c["preconditioner"]) == \
+        if (c.get("nodes"), c.get("nnz"), c.get("rp"), c.get("batch_steps"), c.get("preconditioner")) == \
                 (N, nnz, plan.rp, args.batch_steps, args.preconditioner):
             result["roofline"]["traffic"] = t["traffic_bytes_per_dispatch"]
             result["roofline"]["traffic_source"] = f"profiles/{traffic_file.name}"
