@@ -60,12 +60,22 @@ __device__ __forceinline__ T* gview(T* p) {
 // flight anyway (before the barrier of a staged product), it clears that state.
 __device__ __forceinline__ void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// Workgroup b runs on XCD b % 8.  Remap so that each XCD receives a contiguous range of
-// tiles (neighbouring row ranges of one system share its L2).  Bijective for any count.
+// Workgroup b runs on XCD b % 8.  Remap so that each XCD receives chunks of kXcdChunk
+// consecutive tiles (neighbouring row ranges of one system share its L2), dealt round-robin:
+// XCD x takes chunks x, x + 8, x + 16, ...  Whole-range blocks per XCD would put each system on
+// one or two XCDs, and late in a solve, when only the slowest systems are still iterating, most
+// XCDs would idle.  The tail past the last full round of chunks is split into contiguous ranges.
+// Bijective for any count.
+constexpr int kXcdChunk = 16;
+
 __device__ __forceinline__ int xcd_tile(int b, int n) {
-    const int q = n / kXcds, rmd = n % kXcds;
+    const int full = n / (kXcds * kXcdChunk) * (kXcds * kXcdChunk);
     const int x = b % kXcds, i = b / kXcds;
-    return x * q + (x < rmd ? x : rmd) + i;
+    if (b < full) return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + (i % kXcdChunk);
+    const int m = n - full, bt = b - full;
+    const int q = m / kXcds, rmd = m % kXcds;
+    const int xt = bt % kXcds, it = bt / kXcds;
+    return full + xt * q + (xt < rmd ? xt : rmd) + it;
 }
 
 // Butterfly sum over the lanes of a wavefront that share (lane % width): lanes whose ids
