@@ -73,7 +73,7 @@ class BatchedSolver:
 
     tolerance: float = 1e-10
     max_iterations: int = 50000
-    check_every: int = 16
+    check_every: int = 4  # iterations per graph replay; the host polls one replay behind
     rows_per_tile: int | None = None
     lanes: int | None = None       # lane-group width (None: choose_lanes)
     lds_bytes: int | None = None   # LDS budget of the staged products (None: native.LDS_BYTES; 0: gather kernels)

@@ -202,7 +202,7 @@ def main() -> int:
                     help="time steps (timelapse, default 4) or models (box3d, default 2) per timed step")
     ap.add_argument("--nodes", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--check-every", type=int, default=16)
+    ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="time steps in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=32, help="Jacobian rows in the CPU-baseline sample")
     ap.add_argument("--no-jacobian", action="store_true")
@@ -367,8 +367,8 @@ def main() -> int:
     if traffic_file.exists():
         t = json.loads(traffic_file.read_text())
         c = t["config"]
-        if (c.get("nodes"), c.get("nnz"), c.get("rp"), c.get("batch_steps"), c.get("preconditioner")) == \
-                (N, nnz, plan.rp, args.batch_steps, args.preconditioner):
+        if (c.get("nodes"), c.get("nnz"), c.get("rp"), c.get("batch_steps"), c.get("preconditioner"),
+                c.get("check_every")) == (N, nnz, plan.rp, args.batch_steps, args.preconditioner, args.check_every):
             result["roofline"]["traffic"] = t["traffic_bytes_per_dispatch"]
             result["roofline"]["traffic_source"] = f"profiles/{traffic_file.name}"
     log(f"timed: {elapsed:.2f}s, {rhs_per_s:.0f} RHS/s ({args.preconditioner}); plain PCG "
