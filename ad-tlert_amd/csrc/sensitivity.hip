@@ -137,6 +137,36 @@ __device__ __forceinline__ int tri_index(int e, int f, int S) {  // e <= f
     return e * S - (e * (e - 1)) / 2 + (f - e);
 }
 
+// Column c of J from the cell's Gram block: J[d, c] = -((g(m,a) - g(m,b)) - g(n,a)) + g(n,b)) for
+// every quadrupole d.  The electrode indices of U quadrupoles per thread are loaded together
+// (clamped, branch-free) before any store, so the loop does not wait for a load round trip per
+// quadrupole.
+template <typename GF>
+__device__ __forceinline__ void write_cell_column(const int32_t* a, const int32_t* b, const int32_t* m,
+                                                  const int32_t* n, int D, int64_t C, int c, double* out, GF g) {
+    constexpr int U = 8;
+    const auto ga = adt::gview(a), gb = adt::gview(b), gm = adt::gview(m), gn = adt::gview(n);
+    const auto o = adt::gview(out);
+    for (int d0 = static_cast<int>(threadIdx.x); d0 < D; d0 += 256 * U) {
+        int ia[U], ib[U], im[U], in[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + u * 256;
+            const int dc = d < D ? d : 0;
+            ia[u] = ga[dc];
+            ib[u] = gb[dc];
+            im[u] = gm[dc];
+            in[u] = gn[dc];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + u * 256;
+            const double v = ((g(im[u], ia[u]) - g(im[u], ib[u])) - g(in[u], ia[u])) + g(in[u], ib[u]);
+            if (d < D) o[static_cast<int64_t>(d) * C + c] = -v;
+        }
+    }
+}
+
 template <int B>
 __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
     extern __shared__ double lds[];
@@ -203,11 +233,7 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
         if (e >= S || f >= S) return 0.0;
         return e <= f ? G[tri_index(e, f, S)] : G[tri_index(f, e, S)];
     };
-    for (int d = static_cast<int>(threadIdx.x); d < A.D; d += 256) {
-        const int ia = A.a[d], ib = A.b[d], im = A.m[d], in = A.n[d];
-        const double v = ((g(im, ia) - g(im, ib)) - g(in, ia)) + g(in, ib);
-        A.out[static_cast<int64_t>(d) * A.C + c] = -v;
-    }
+    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.C, c, A.out, g);
 }
 
 template <int B>
@@ -302,11 +328,120 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
         for (int j = 0; j < B; ++j) G[(te * B + i) * Sp + tf * B + j] = acc[i][j];
     __syncthreads();
     auto g = [&](int e, int f) { return e < A.Sr && f < A.Sc ? G[e * Sp + f] : 0.0; };
-    for (int d = static_cast<int>(threadIdx.x); d < A.D; d += 256) {
-        const int ia = A.a[d], ib = A.b[d], im = A.m[d], in = A.n[d];
-        const double v = ((g(im, ia) - g(im, ib)) - g(in, ia)) + g(in, ib);
-        A.out[static_cast<int64_t>(d) * A.C + c] = -v;
+    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.C, c, A.out, g);
+}
+
+// Cell-block form for separate receiver and current fields (3D): J[d, c] =
+//   -sum_{w,i} (phi_R[w,i,m] - phi_R[w,i,n]) (Psi[w,i,a] - Psi[w,i,b]),  Psi = T_w phi_C,
+// over the cell's K DOFs.  A block takes CB consecutive cells and stages, per cell, the receiver
+// rows phi_R (W K x Sr) and Psi (W K x Sc) in LDS with the cell index fastest and a zero column
+// for the remote electrode.  Its output rows then come out as CB consecutive doubles per
+// quadrupole (full 64-byte segments): the one-cell-per-block Gram kernel wrote one 8-byte entry
+// per quadrupole and cell, and partial-line writes bound it.  Sums over (w, i) in order.
+constexpr int kAbmnChunk = 2048;  // quadrupoles whose indices are staged at a time
+
+template <int CB>
+__global__ __launch_bounds__(256) void cell_block_kernel(GramCrossArgs A) {
+    extern __shared__ double lds[];
+    const int R = A.W * A.K, K = A.K;
+    const int Pr = A.Sr + 1, Pc = A.Sc + 1;  // columns per staged row, the last one zero
+    double* phr = lds;                                         // [R][Pr][CB]
+    double* phc = phr + static_cast<int64_t>(R) * Pr * CB;     // [R][Pc][CB]
+    double* psi = phc + static_cast<int64_t>(R) * Pc * CB;     // [R][Pc][CB]
+    // electrode indices of a chunk of quadrupoles, one packed word each (a | b << 8 | m << 16 | n << 24)
+    uint32_t* abmn = reinterpret_cast<uint32_t*>(psi + static_cast<int64_t>(R) * Pc * CB);
+    const int nblk = (A.C + CB - 1) / CB;
+    const int c0 = adt::xcd_tile(static_cast<int>(blockIdx.x), nblk) * CB;
+    const auto dofs = adt::gview(A.dofs);
+    const auto xr = adt::gview(A.xr);
+    const auto xc = adt::gview(A.xc);
+    const int tid = static_cast<int>(threadIdx.x);
+    // stage phi_R and phi_C: entry (r, e, cb) of cell c0 + cb (cells past C read cell C - 1 and
+    // are never stored)
+    for (int t = tid; t < R * Pr * CB; t += 256) {
+        const int cb = t % CB, e = (t / CB) % Pr, r = t / (CB * Pr);
+        const int w = r / K, i = r % K;
+        const int cell = min(c0 + cb, A.C - 1);
+        const int64_t node = dofs[static_cast<int64_t>(cell) * K + i];
+        const double v = xr[A.off_r[w] + node * A.rp_r + min(e, A.Sr - 1)];
+        phr[t] = e < A.Sr ? v : 0.0;
     }
+    for (int t = tid; t < R * Pc * CB; t += 256) {
+        const int cb = t % CB, f = (t / CB) % Pc, r = t / (CB * Pc);
+        const int w = r / K, i = r % K;
+        const int cell = min(c0 + cb, A.C - 1);
+        const int64_t node = dofs[static_cast<int64_t>(cell) * K + i];
+        const double v = xc[A.off_c[w] + node * A.rp_c + min(f, A.Sc - 1)];
+        phc[t] = f < A.Sc ? v : 0.0;
+    }
+    __syncthreads();
+    // Psi[w,i,f] = sum_j (ws_w K_c[i][j] + wm_w M_c[i][j]) phi_C[w,j,f], j in order
+    const auto stiff = adt::gview(A.stiff);
+    const auto mass = adt::gview(A.mass);
+    for (int t = tid; t < R * Pc * CB; t += 256) {
+        const int cb = t % CB, f = (t / CB) % Pc, r = t / (CB * Pc);
+        const int w = r / K, i = r % K;
+        const int64_t cell = min(c0 + cb, A.C - 1);
+        const int64_t tb = cell * K * K + static_cast<int64_t>(i) * K;
+        double v = 0.0;
+        for (int j = 0; j < K; ++j) {
+            const double tij = A.mass ? fma(A.wm[w], mass[tb + j], A.ws[w] * stiff[tb + j]) : A.ws[w] * stiff[tb + j];
+            v = fma(tij, phc[(static_cast<int64_t>(w * K + j) * Pc + f) * CB + cb], v);
+        }
+        psi[t] = v;
+    }
+    __syncthreads();
+    constexpr int RPP = 256 / CB;  // quadrupoles per pass
+    const int cb = tid % CB, dl = tid / CB;
+    const int cell = c0 + cb;
+    const auto ga = adt::gview(A.a), gb = adt::gview(A.b), gm = adt::gview(A.m), gn = adt::gview(A.n);
+    const auto out = adt::gview(A.out);
+    for (int dc0 = 0; dc0 < A.D; dc0 += kAbmnChunk) {
+        const int nd = min(kAbmnChunk, A.D - dc0);
+        __syncthreads();  // the previous chunk's indices are consumed
+        constexpr int PER = kAbmnChunk / 256;
+        uint32_t word[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {  // all loads of the chunk in flight together
+            const int k = tid + u * 256;
+            const int d = dc0 + (k < nd ? k : 0);
+            // indices >= S name the remote electrode: the zero column
+            const uint32_t ia = min(max(ga[d], 0), A.Sc), ib = min(max(gb[d], 0), A.Sc);
+            const uint32_t im = min(max(gm[d], 0), A.Sr), in = min(max(gn[d], 0), A.Sr);
+            word[u] = ia | ib << 8 | im << 16 | in << 24;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (tid + u * 256 < nd) abmn[tid + u * 256] = word[u];
+        __syncthreads();
+        for (int k = dl; k < nd; k += RPP) {
+            const uint32_t wd = abmn[k];
+            const int ia = wd & 255, ib = (wd >> 8) & 255, im = (wd >> 16) & 255, in = wd >> 24;
+            double v = 0.0;
+            for (int r = 0; r < R; ++r) {
+                const double* pr = phr + static_cast<int64_t>(r) * Pr * CB + cb;
+                const double* pc = psi + static_cast<int64_t>(r) * Pc * CB + cb;
+                v = fma(pr[im * CB] - pr[in * CB], pc[ia * CB] - pc[ib * CB], v);
+            }
+            if (cell < A.C) out[static_cast<int64_t>(dc0 + k) * A.C + cell] = -v;
+        }
+    }
+}
+
+// Cells per block of cell_block_kernel that fit the LDS budget (0: none).
+inline int cell_block_width(const GramCrossArgs& A, size_t budget) {
+    const size_t per_cell = static_cast<size_t>(A.W) * A.K * ((A.Sr + 1) + 2 * (A.Sc + 1)) * 8;
+    for (int cb = 16; cb >= 2; cb /= 2)
+        if (per_cell * cb + kAbmnChunk * 4 <= budget) return cb;
+    return 0;
+}
+
+template <int CB>
+int launch_cell_block(GramCrossArgs& A, hipStream_t st) {
+    const size_t lds = static_cast<size_t>(A.W) * A.K * ((A.Sr + 1) + 2 * (A.Sc + 1)) * 8 * CB + kAbmnChunk * 4;
+    hipLaunchKernelGGL(cell_block_kernel<CB>, dim3((A.C + CB - 1) / CB), dim3(256), lds, st, A);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
 }
 
 template <int B>
@@ -683,6 +818,13 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
         A.wm[w] = d_mass ? h_wm[w] : 0.0;
     }
     hipStream_t st = adt::as_stream(stream);
+    switch (cell_block_width(A, 65536)) {
+        case 16: return launch_cell_block<16>(A, st);
+        case 8: return launch_cell_block<8>(A, st);
+        case 4: return launch_cell_block<4>(A, st);
+        case 2: return launch_cell_block<2>(A, st);
+        default: break;
+    }
     switch (B) {
         case 1: return launch_gram_cross<1>(A, lds, st);
         case 2: return launch_gram_cross<2>(A, lds, st);

@@ -116,7 +116,7 @@ class Timelapse:
         K, S, rp = 3, self.E, self.op._dev.rp
         bytes_ = 8 * self.D * self.C + 8 * self.W * self.N * rp + 8 * 2 * self.C * K * K + 4 * self.C * K + 16 * self.D
         flops = self.C * (2 * self.W * K * S * S + 2 * self.W * K * K * S) + 4 * self.D * self.C
-        return run, bytes_, flops, "gram_kernel<6> (Gram-form sensitivity)"
+        return run, bytes_, flops, "gram_kernel<6> (Gram-form sensitivity)", {"bound": "vector-fp64"}
 
     def cpu(self):
         args = self.args
@@ -182,11 +182,14 @@ class Box3D:
         def run():
             return op.jacobian(sigma, return_device=True, batch_size=self.D)
 
-        # cross-Gram form: per cell G_c = Phi_R^T (T Phi_C) over K rows of nr receivers x nc sources
+        # cell-block form: per cell Psi = T Phi_C (K x nc), then per output a K-term sum of
+        # (receiver difference) x (Psi difference), four LDS reads per term
         K, nc, nr = 4, op._dev.nc, op._dev.nr
         bytes_ = 8 * self.D * self.C + 8 * self.N * (nc + nr) + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
-        flops = self.C * (2 * K * K * nc + 2 * K * nr * nc) + 4 * self.D * self.C
-        return run, bytes_, flops, "gram_cross_kernel<4> (cross-Gram sensitivity)"
+        flops = self.C * 2 * K * K * nc + 4 * K * self.D * self.C
+        lds_bytes = 4 * K * 8 * self.D * self.C
+        return run, bytes_, flops, "cell_block_kernel<8> (cell-block sensitivity)", {
+            "bound": "lds", "lds_bytes_per_launch": lds_bytes}
 
     def cpu(self):
         return None  # a 3D direct factorization of this size does not fit a bounded CPU sample
@@ -377,7 +380,7 @@ def main() -> int:
         f"{update_name} {update_gbs:.0f} GB/s ({prof['update_ms'] / launches:.3f} ms)")
 
     if not args.no_jacobian:
-        run, sens_bytes, sens_flops, kernel = wl.jacobian()
+        run, sens_bytes, sens_flops, kernel, sens_extra = wl.jacobian()
         run()  # warm
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps = 3
@@ -389,10 +392,12 @@ def main() -> int:
         ms = start.elapsed_time(stop) / reps
         result["jacobian_rows_per_s"] = wl.D / (ms * 1e-3)
         result["roofline_sensitivity"] = {
-            "bound": "vector-fp64", "kernel": kernel, "avg_launch_ms": ms,
+            "kernel": kernel, "avg_launch_ms": ms,
             "achieved_gbs": sens_bytes / (ms * 1e-3) / 1e9, "hbm_frac": sens_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "achieved_tflops": sens_flops / (ms * 1e-3) / 1e12, "fp64_vector_peak_tflops": 78.6,
-            "flops_per_launch": sens_flops, "bytes_per_launch": sens_bytes}
+            "flops_per_launch": sens_flops, "bytes_per_launch": sens_bytes, **sens_extra}
+        if "lds_bytes_per_launch" in sens_extra:
+            result["roofline_sensitivity"]["achieved_lds_tbs"] = sens_extra["lds_bytes_per_launch"] / (ms * 1e-3) / 1e12
         log(f"jacobian {wl.D}x{wl.C}: {ms:.1f} ms/launch -> {result['jacobian_rows_per_s']:.0f} rows/s")
         del jac
 

@@ -297,14 +297,18 @@ def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
     assert results[native.PRECOND_AMG].max() < 0.5 * results[native.PRECOND_JACOBI].max()
 
 
-@pytest.mark.parametrize("k, Sr, Sc", [(4, 64, 64), (10, 17, 40), (4, 128, 33), (3, 5, 96)])
-def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc):
-    """Cross-Gram Jacobian rows (separate receiver and current fields, as in 3D) equal the
-    per-output formula with non-symmetric templates, remote indices reading zero and padding
-    columns ignored; bitwise repeatable."""
+@pytest.mark.parametrize("k, Sr, Sc, W", [(4, 64, 64, 1), (4, 64, 64, 3), (10, 17, 40, 3), (4, 128, 33, 3),
+                                          (3, 5, 96, 3), (10, 128, 96, 8)],
+                         ids=["p1-3d", "p1-3w", "p2-3w", "wide", "narrow", "whole-gram"])
+def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W):
+    """Jacobian rows from separate receiver and current fields (as in 3D) equal the per-output
+    formula with non-symmetric templates, remote indices reading zero and padding columns ignored;
+    bitwise repeatable.  Cell blocks whose staged rows fit LDS use the cell-block kernel (16/8/4/2
+    cells per workgroup, a tail block past the last cell); the last case is too large for it and
+    takes the whole-Gram kernel."""
 
-    gen = torch.Generator().manual_seed(100 * k + Sr + Sc)
-    W, N, C, D = 3, 80, 47, 39
+    gen = torch.Generator().manual_seed(100 * k + Sr + Sc + W)
+    N, C, D = 80, 47, 39
     phr = torch.randn(W, Sr, N, generator=gen, dtype=torch.float64)
     phc = torch.randn(W, Sc, N, generator=gen, dtype=torch.float64)
     stiff = torch.randn(C, k, k, generator=gen, dtype=torch.float64)
