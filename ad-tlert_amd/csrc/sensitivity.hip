@@ -873,9 +873,12 @@ extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64
         const int v = e ? std::atoi(e) : 16384;
         return std::min(65536, std::max(4096, v));
     }();
-    A.wchunk = std::max(1, std::min(n_waves, (lds_budget / (2 * 8 * A.Sp)) / k_local));
-    const size_t staging = static_cast<size_t>(2) * A.wchunk * k_local * A.Sp * 8;
+    // The triangle's LDS is allocated anyway, so the staging may fill it: fewer chunks, each
+    // a global-load round trip and two barriers per cell.
     const size_t tri = static_cast<size_t>(s_rows) * (s_rows + 1) / 2 * 8;
+    const int budget = std::max(lds_budget, static_cast<int>(tri));
+    A.wchunk = std::max(1, std::min(n_waves, (budget / (2 * 8 * A.Sp)) / k_local));
+    const size_t staging = static_cast<size_t>(2) * A.wchunk * k_local * A.Sp * 8;
     const size_t lds = std::max(staging, tri);
     for (int w = 0; w < n_waves; ++w) {
         A.vec_off[w] = h_vec_off[w];
