@@ -204,9 +204,12 @@ int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_o
  *   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])
  * phiR_w / phiC_w are the (N, rp_r) / (N, rp_c) blocks at d_xr + h_off_r[w] / d_xc + h_off_c[w];
  * their first s_r / s_c (<= 128) columns are receivers / current sources; a, b >= s_c and
- * m, n >= s_r read zero (remote electrodes).  Templates need not be symmetric.  One workgroup
- * per cell keeps the s_r x s_c block in LDS (no workspace).  FP64, fixed summation order:
- * bitwise repeatable.  out is (D, C) row-major. */
+ * m, n >= s_r read zero (remote electrodes).  Templates need not be symmetric.  When the
+ * staged rows of 2 or more cells fit 64 KB of LDS, G_c is not formed: a workgroup of
+ * consecutive cells evaluates out[d, c] = -sum_{w,i} (phiR[m] - phiR[n])_{w,i} (Psi[a] - Psi[b])_{w,i}
+ * with Psi_w = T_w phiC_w on the cell's DOFs, and writes each output row as one contiguous
+ * segment; otherwise one workgroup per cell keeps G_c in LDS.  No workspace.  FP64, fixed
+ * summation order: bitwise repeatable.  out is (D, C) row-major. */
 int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,
                                const double* d_xc, int rp_c, int s_c, const int64_t* h_off_c, int n_meas,
                                int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
