@@ -607,12 +607,28 @@ __global__ __launch_bounds__(256) void cg_true_residual(const PcgArgs* __restric
     const auto X = adt::gview(a.x + c.S.vec_off + c.col);
     double sr = 0.0;
     if (c.col < a.r_real) {
-        for (int row = c.t.row0 + static_cast<int>(threadIdx.x / L); row < c.t.row1; row += kRowsPerPass(L)) {
-            double acc = 0.0;
-            for (int k = ptr[row]; k < ptr[row + 1]; ++k)
-                acc = fma(val[k], X[static_cast<int64_t>(idx[k]) * a.rp], acc);
-            const double ri = B[static_cast<int64_t>(row) * a.rp] - acc;
-            sr = fma(ri, ri, sr);
+        // rows and slots batched as in cg_init; products in slot order, rows in order
+        constexpr int RB = kGatherRows, RPP = kRowsPerPass(L);
+        const int rows = c.t.row1 - c.t.row0;
+        for (int q0 = static_cast<int>(threadIdx.x / L); q0 < rows; q0 += RB * RPP) {
+            int k0[RB], k1[RB];
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int q = q0 + b * RPP;
+                k0[b] = q < rows ? ptr[c.t.row0 + q] : 0;
+                k1[b] = q < rows ? ptr[c.t.row0 + q + 1] : 0;
+            }
+            double acc[RB] = {};
+            gather_rows<RB, kGatherSlots>(
+                k0, k1, [&](int e) { return idx[e]; }, [&](int e) { return val[e]; },
+                [&](int j) { return X[static_cast<int64_t>(j) * a.rp]; }, acc);
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const int q = q0 + b * RPP;
+                if (q >= rows) continue;
+                const double ri = B[static_cast<int64_t>(c.t.row0 + q) * a.rp] - acc[b];
+                sr = fma(ri, ri, sr);
+            }
         }
     }
     sr = block_sum1<L>(sr);
