@@ -108,13 +108,14 @@ int launch_sensitivity(const SensArgs& a, int precision, hipStream_t st) {
 //   G_c[e][f] = sum_w sum_i phi_w[e, dof_ci] (T_wc phi_w[f, dof_c.])_i,   T_wc = ws_w K_c + wm_w M_c
 //   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])          (row S: remote, zero)
 // which equals the per-output formula of sensitivity_kernel; the element templates are symmetric
-// (stiffness and mass), so G_c is symmetric and only its upper triangle is kept.  One workgroup
-// per cell: 16 x 16 threads each own a B x B block of G_c (S <= 16 B); the cell's field rows
-// Phi[(w, i)][e] (one contiguous interleaved row per node) and Psi = T Phi are staged in LDS in
-// chunks of wavenumbers; then the upper triangle of G_c replaces the staging area in LDS and
-// the workgroup writes column c of every measurement row.  Cells are mapped XCD-contiguously,
-// so neighbouring columns of a row are written by workgroups that share one L2.  FP64 and
-// fixed orders: k = (w, i) then j; bitwise repeatable.
+// (stiffness and mass), so G_c is symmetric and only its upper triangle is formed and kept.  One
+// workgroup per cell: the cell's field rows Phi[(w, i)][e] (one contiguous interleaved row per
+// node) and Psi = T Phi are staged in LDS in chunks of wavenumbers, the 4 waves accumulate the
+// upper-triangle 16 x 16 tiles of G_c on the FP64 matrix cores (S <= 16 Bt), then the upper
+// triangle replaces the staging area in LDS and the workgroup writes column c of every
+// measurement row.  Cells are mapped XCD-contiguously, so neighbouring columns of a row are
+// written by workgroups that share one L2.  FP64 with a fixed order of rows (w, i); bitwise
+// repeatable.
 // ------------------------------------------------------------------------------------------
 
 struct GramArgs {
@@ -127,7 +128,7 @@ struct GramArgs {
     const int32_t* m;
     const int32_t* n;
     double* out;
-    int S, Sp, K, W, rp, wchunk, C, D;
+    int S, Sp, K, W, rp, wchunk, rows_max, C, D;
     int64_t vec_off[kMaxWaves];
     double ws[kMaxWaves];
     double wm[kMaxWaves];
@@ -167,78 +168,150 @@ __device__ __forceinline__ void write_cell_column(const int32_t* a, const int32_
     }
 }
 
-template <int B>
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Upper-triangle 16 x 16 tiles of G_c per wave: Sp = 16 Bt sources give Bt (Bt + 1) / 2 tiles
+// dealt over the workgroup's 4 waves.
+constexpr int gram_tiles_per_wave(int bt) { return (bt * (bt + 1) / 2 + 3) / 4; }
+
+// floor(r / k) for 0 <= r < 2^16 and 1 <= k <= 256 from a float reciprocal: (r + 1/2) / k lies at
+// least 1 / (2k) from an integer, far beyond the float rounding of the product.
+__device__ __forceinline__ int div_small(int r, float inv_k) {
+    return static_cast<int>((static_cast<float>(r) + 0.5f) * inv_k);
+}
+
+// Staged rows per wave whose loads are in flight together.
+constexpr int kGramStageRows = 6;
+
+// G_c on the matrix cores: v_mfma_f64_16x16x4f64 multiplies a 16 x 4 block of Phi^T by a 4 x 16
+// block of Psi; lane l supplies Phi[r0 + l / 16][16 I + l % 16] and Psi[r0 + l / 16][16 J + l % 16]
+// and holds G[16 I + l / 16 + 4 v][16 J + l % 16] in item v of its accumulator.  Only the tiles
+// with I <= J are formed (G_c is symmetric).  The staged rows are padded with zero rows to a
+// multiple of 4.  The wave-to-tile assignment rotates with the block index, so the wave with the
+// extra tile falls on every SIMD in turn.  The source offset of every staged row (w, i) is
+// tabulated once per cell, so staging a row is one LDS read and one global load per element.
+template <int BT>
 __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
+    constexpr int Sp = 16 * BT, TPW = gram_tiles_per_wave(BT), n_tiles = BT * (BT + 1) / 2;
+    constexpr int kCols = (Sp + 63) / 64;  // staged sources per lane and row
     extern __shared__ double lds[];
     const int c = adt::xcd_tile(static_cast<int>(blockIdx.x), A.C);
-    const int K = A.K, Sp = A.Sp, S = A.S;
-    const int rows_max = A.wchunk * K;
+    const int K = A.K, S = A.S, tid = static_cast<int>(threadIdx.x);
+    const int rows_max = A.rows_max;
+    const float inv_k = 1.0f / static_cast<float>(K);
     double* phi = lds;                                        // rows_max x Sp
     double* psi = lds + static_cast<int64_t>(rows_max) * Sp;  // rows_max x Sp
-    const int te = static_cast<int>(threadIdx.x) / 16, tf = static_cast<int>(threadIdx.x) % 16;
+    double* tw = psi + static_cast<int64_t>(rows_max) * Sp;   // wchunk x K x K: T_wc of the chunk
+    int64_t* roff = reinterpret_cast<int64_t*>(tw + A.wchunk * K * K);  // W x K row offsets in x
     const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
     const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
     const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
-    double acc[B][B];
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wave = (wv + static_cast<int>(blockIdx.x)) & 3;
+    int ti[TPW], tj[TPW];
+    int nt = 0;
 #pragma unroll
-    for (int i = 0; i < B; ++i)
+    for (int t = 0; t < TPW; ++t) {
+        int q = wave + 4 * t, I = 0;
+        if (q < n_tiles) ++nt;
+        else q = 0;
+        while (q >= BT - I) q -= BT - I++;
+        ti[t] = I;
+        tj[t] = I + q;
+    }
+    for (int r = tid; r < A.W * K; r += 256) {
+        const int w = div_small(r, inv_k);
+        roff[r] = A.vec_off[w] + static_cast<int64_t>(dof[r - w * K]) * A.rp;
+    }
+    f64x4 acc[TPW];
 #pragma unroll
-        for (int j = 0; j < B; ++j) acc[i][j] = 0.0;
+    for (int t = 0; t < TPW; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
     for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
         const int nw = min(A.wchunk, A.W - w0);
-        const int rows = nw * K;
+        const int rows = nw * K, rows4 = (rows + 3) & ~3;
         __syncthreads();
-        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
-            const int r = t / Sp, e = t % Sp;
-            const int w = w0 + r / K, i = r % K;
-            phi[t] = e < S ? A.x[A.vec_off[w] + static_cast<int64_t>(dof[i]) * A.rp + e] : 0.0;
-        }
-        __syncthreads();
-        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
-            const int r = t / Sp, f = t % Sp;
-            const int wl = r / K, i = r % K, w = w0 + wl;
-            double v = 0.0;
-            for (int j = 0; j < K; ++j) {
-                const double tij = Mc ? fma(A.wm[w], Mc[i * K + j], A.ws[w] * Kc[i * K + j]) : A.ws[w] * Kc[i * K + j];
-                v = fma(tij, phi[(wl * K + j) * Sp + f], v);
+        // Phi: wave-uniform rows (row offset read once per row), lanes along the sources;
+        // kGramStageRows rows per wave in flight
+        for (int r0 = wv; r0 < rows4; r0 += 4 * kGramStageRows) {
+            double v[kGramStageRows][kCols];
+#pragma unroll
+            for (int u = 0; u < kGramStageRows; ++u) {
+                const int r = r0 + 4 * u;
+                const int64_t off = r < rows ? roff[w0 * K + r] : 0;
+#pragma unroll
+                for (int q = 0; q < kCols; ++q) {
+                    const int e = lane + 64 * q;
+                    v[u][q] = r < rows && e < S ? A.x[off + e] : 0.0;
+                }
             }
-            psi[t] = v;
+#pragma unroll
+            for (int u = 0; u < kGramStageRows; ++u) {
+                const int r = r0 + 4 * u;
+#pragma unroll
+                for (int q = 0; q < kCols; ++q) {
+                    const int e = lane + 64 * q;
+                    if (r < rows4 && e < Sp) phi[r * Sp + e] = v[u][q];
+                }
+            }
+        }
+        for (int t = tid; t < nw * K * K; t += 256) {
+            const int wl = div_small(t, 1.0f / static_cast<float>(K * K)), ij = t - wl * K * K, w = w0 + wl;
+            tw[t] = Mc ? fma(A.wm[w], Mc[ij], A.ws[w] * Kc[ij]) : A.ws[w] * Kc[ij];
         }
         __syncthreads();
-        for (int r = 0; r < rows; ++r) {
-            double av[B], bv[B];
+        // Psi = T Phi per (wavenumber, source column): the K staged values of the column give its
+        // K rows of Psi; padding rows are zero
+        for (int p = tid; p < nw * Sp; p += 256) {
+            const int wl = p / Sp, f = p % Sp;
+            const double* pc = phi + wl * K * Sp + f;
+            const double* tr = tw + wl * K * K;
+            for (int i = 0; i < K; ++i) {
+                double v = 0.0;
+                for (int j = 0; j < K; ++j) v = fma(tr[i * K + j], pc[j * Sp], v);
+                psi[(wl * K + i) * Sp + f] = v;
+            }
+        }
+        for (int t = tid; t < (rows4 - rows) * Sp; t += 256) psi[rows * Sp + t] = 0.0;
+        __syncthreads();
+        const double* ph = phi + (lane >> 4) * Sp + (lane & 15);
+        const double* ps = psi + (lane >> 4) * Sp + (lane & 15);
+        for (int r0 = 0; r0 < rows4; r0 += 4) {
 #pragma unroll
-            for (int i = 0; i < B; ++i) av[i] = phi[r * Sp + te * B + i];
-#pragma unroll
-            for (int j = 0; j < B; ++j) bv[j] = psi[r * Sp + tf * B + j];
-#pragma unroll
-            for (int i = 0; i < B; ++i)
-#pragma unroll
-                for (int j = 0; j < B; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+            for (int t = 0; t < TPW; ++t)
+                if (t < nt)
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph[r0 * Sp + 16 * ti[t]], ps[r0 * Sp + 16 * tj[t]],
+                                                                  acc[t], 0, 0, 0);
         }
     }
     __syncthreads();
-    double* G = lds;  // upper triangle of G_c, S (S + 1) / 2 entries
+    // upper triangle of G_c over S + 1 slots: slot S (the remote electrode) is a zero row
+    double* G = lds;
 #pragma unroll
-    for (int i = 0; i < B; ++i) {
-        const int e = te * B + i;
+    for (int t = 0; t < TPW; ++t) {
+        if (t >= nt) continue;
+        const int f = 16 * tj[t] + (lane & 15);
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            const int f = tf * B + j;
-            if (e < S && f < S && e <= f) G[tri_index(e, f, S)] = acc[i][j];
+        for (int v = 0; v < 4; ++v) {
+            const int e = 16 * ti[t] + (lane >> 4) + 4 * v;
+            if (e < S && f < S && e <= f) G[tri_index(e, f, S + 1)] = acc[t][v];
         }
     }
+    for (int e = tid; e <= S; e += 256) G[tri_index(e, S, S + 1)] = 0.0;
     __syncthreads();
+    // branch-free lookup: indices >= S (and negative ones) read the remote slot
+    const unsigned s1 = static_cast<unsigned>(S), two_s1 = 2u * s1 + 1u;
     auto g = [&](int e, int f) {
-        if (e >= S || f >= S) return 0.0;
-        return e <= f ? G[tri_index(e, f, S)] : G[tri_index(f, e, S)];
+        const unsigned ue = min(static_cast<unsigned>(e), s1), uf = min(static_cast<unsigned>(f), s1);
+        const unsigned lo = min(ue, uf), hi = max(ue, uf);
+        return G[(__umul24(lo, two_s1 - lo) >> 1) + hi];
     };
     write_cell_column(A.a, A.b, A.m, A.n, A.D, A.C, c, A.out, g);
 }
 
-template <int B>
+template <int BT>
 int launch_gram(GramArgs& A, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(gram_kernel<B>, dim3(A.C), dim3(256), lds, st, A);
+    hipLaunchKernelGGL(gram_kernel<BT>, dim3(A.C), dim3(256), lds, st, A);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -875,10 +948,13 @@ extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64
     }();
     // The triangle's LDS is allocated anyway, so the staging may fill it: fewer chunks, each
     // a global-load round trip and two barriers per cell.
-    const size_t tri = static_cast<size_t>(s_rows) * (s_rows + 1) / 2 * 8;
+    const size_t tri = static_cast<size_t>(s_rows + 1) * (s_rows + 2) / 2 * 8;  // with the remote slot
     const int budget = std::max(lds_budget, static_cast<int>(tri));
     A.wchunk = std::max(1, std::min(n_waves, (budget / (2 * 8 * A.Sp)) / k_local));
-    const size_t staging = static_cast<size_t>(2) * A.wchunk * k_local * A.Sp * 8;
+    A.rows_max = (A.wchunk * k_local + 3) & ~3;
+    // staging: Phi and Psi blocks, the chunk's K x K templates, the row offsets of all W K rows
+    const size_t staging = (static_cast<size_t>(2) * A.rows_max * A.Sp + static_cast<size_t>(A.wchunk) * k_local * k_local +
+                            static_cast<size_t>(n_waves) * k_local) * 8;
     const size_t lds = std::max(staging, tri);
     for (int w = 0; w < n_waves; ++w) {
         A.vec_off[w] = h_vec_off[w];
