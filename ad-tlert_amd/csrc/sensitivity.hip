@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <cmath>
 
 namespace {
@@ -320,8 +321,15 @@ int launch_gram(GramArgs& A, size_t lds, hipStream_t st) {
 // with their own right-hand sides):  G_c = Phi_R^T (T Phi_C)  (S_r x S_c per cell), then
 //   J[d, c] = -((G[m, a] - G[m, b]) - G[n, a]) + G[n, b])
 // with current indices a, b >= S_c and receiver indices m, n >= S_r reading zero.  No symmetry is
-// needed: the full block is kept in LDS.  Same structure as gram_kernel: one workgroup per cell,
-// 16 x 16 threads of B x B accumulators, staged chunks of wavenumbers.
+// needed: the full block is kept in LDS.  One workgroup per cell, chunks of wavenumbers:
+//   * the chunk's receiver rows Phi_R[(w, i)][e] and current rows Phi_C[(w, i)][f] are staged
+//     row by row (wave-uniform rows: the cell's nodes and the row offsets are scalar, lanes run
+//     along the electrodes), kCrossRows rows per wave in flight;
+//   * Psi = T Phi_C replaces Phi_C in place, one thread per (w, f) column;
+//   * the 4 waves accumulate the 16 x 16 tiles of G_c on the FP64 matrix cores
+//     (v_mfma_f64_16x16x4f64, lane layout as in gram_kernel), rows padded to a multiple of 4;
+// then G_c is stored as (S_r + 1) x (S_c + 1) with a zero row and column for the remote
+// electrode, so that each output is four branch-free lookups.  Sums over (w, i) in order.
 struct GramCrossArgs {
     const double* xr;
     const double* xc;
@@ -334,73 +342,143 @@ struct GramCrossArgs {
     const int32_t* n;
     double* out;
     int Sr, Sc, Sp, K, W, rp_r, rp_c, wchunk, C, D;
+    int btr, btc, rows_max;
     int64_t off_r[kMaxWaves];
     int64_t off_c[kMaxWaves];
     double ws[kMaxWaves];
     double wm[kMaxWaves];
 };
 
-template <int B>
+constexpr int kCrossRows = 4;  // staged rows per wave and array in flight
+constexpr int kMaxLocal = 10;
+
+// Tiles per wave of gram_cross_kernel, rounded up to the instantiated counts.
+inline int cross_tiles_per_wave(int tiles) {
+    const int t = (tiles + 3) / 4;
+    return t <= 1 ? 1 : t <= 2 ? 2 : t <= 4 ? 4 : t <= 8 ? 8 : 16;
+}
+
+template <int TPW>
 __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
     extern __shared__ double lds[];
     const int c = adt::xcd_tile(static_cast<int>(blockIdx.x), A.C);
-    const int K = A.K, Sp = A.Sp;
-    const int rows_max = A.wchunk * K;
-    double* phr = lds;                                            // rows_max x Sp (receiver fields)
-    double* phc = lds + static_cast<int64_t>(rows_max) * Sp;      // rows_max x Sp (current fields)
-    double* psi = lds + static_cast<int64_t>(2 * rows_max) * Sp;  // rows_max x Sp (T times current)
-    const int te = static_cast<int>(threadIdx.x) / 16, tf = static_cast<int>(threadIdx.x) % 16;
+    const int K = A.K, Sr = A.Sr, Sc = A.Sc;
+    const int spr = 16 * A.btr, spc = 16 * A.btc, rows_max = A.rows_max;
+    double* phr = lds;                                          // rows_max x spr
+    double* psi = phr + static_cast<int64_t>(rows_max) * spr;   // rows_max x spc (Phi_C first)
+    double* tw = psi + static_cast<int64_t>(rows_max) * spc;    // wchunk x K x K: T_wc of the chunk
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wave = (wv + static_cast<int>(blockIdx.x)) & 3;
     const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
     const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
     const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
-    double acc[B][B];
+    const int n_tiles = A.btr * A.btc;
+    int ti[TPW], tj[TPW];
+    int nt = 0;
 #pragma unroll
-    for (int i = 0; i < B; ++i)
+    for (int t = 0; t < TPW; ++t) {
+        int q = wave + 4 * t;
+        if (q < n_tiles) ++nt;
+        else q = 0;
+        ti[t] = q / A.btc;
+        tj[t] = q % A.btc;
+    }
+    f64x4 acc[TPW];
 #pragma unroll
-        for (int j = 0; j < B; ++j) acc[i][j] = 0.0;
+    for (int t = 0; t < TPW; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
     for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
         const int nw = min(A.wchunk, A.W - w0);
-        const int rows = nw * K;
+        const int rows = nw * K, rows4 = (rows + 3) & ~3;
         __syncthreads();
-        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
-            const int r = t / Sp, e = t % Sp;
-            const int w = w0 + r / K;
-            const int64_t node = dof[r % K];
-            phr[t] = e < A.Sr ? A.xr[A.off_r[w] + node * A.rp_r + e] : 0.0;
-            phc[t] = e < A.Sc ? A.xc[A.off_c[w] + node * A.rp_c + e] : 0.0;
-        }
-        __syncthreads();
-        for (int t = static_cast<int>(threadIdx.x); t < rows * Sp; t += 256) {
-            const int r = t / Sp, f = t % Sp;
-            const int wl = r / K, i = r % K, w = w0 + wl;
-            double v = 0.0;
-            for (int j = 0; j < K; ++j) {
-                const double tij = Mc ? fma(A.wm[w], Mc[i * K + j], A.ws[w] * Kc[i * K + j]) : A.ws[w] * Kc[i * K + j];
-                v = fma(tij, phc[(wl * K + j) * Sp + f], v);
+        // receiver and current rows of the chunk, wave-uniform; padding rows are zero
+        for (int r0 = wv; r0 < rows4; r0 += 4 * kCrossRows) {
+            double vr[kCrossRows][2], vc[kCrossRows][2];
+#pragma unroll
+            for (int u = 0; u < kCrossRows; ++u) {
+                const int r = r0 + 4 * u;
+                const int rc = min(r, rows - 1);  // rows past the chunk: index math only
+                const int wl = rc / K, i = rc - wl * K;
+                const int64_t node = dof[i];
+                const double* rowr = A.xr + A.off_r[w0 + wl] + node * A.rp_r;
+                const double* rowc = A.xc + A.off_c[w0 + wl] + node * A.rp_c;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {  // sources <= 128
+                    const int e = lane + 64 * h;
+                    vr[u][h] = r < rows && e < Sr ? rowr[e] : 0.0;
+                    vc[u][h] = r < rows && e < Sc ? rowc[e] : 0.0;
+                }
             }
-            psi[t] = v;
+#pragma unroll
+            for (int u = 0; u < kCrossRows; ++u) {
+                const int r = r0 + 4 * u;
+                if (r >= rows4) break;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int e = lane + 64 * h;
+                    if (e < spr) phr[r * spr + e] = vr[u][h];
+                    if (e < spc) psi[r * spc + e] = vc[u][h];
+                }
+            }
+        }
+        for (int t = tid; t < nw * K * K; t += 256) {
+            const int wl = t / (K * K), ij = t - wl * K * K, w = w0 + wl;
+            tw[t] = Mc ? fma(A.wm[w], Mc[ij], A.ws[w] * Kc[ij]) : A.ws[w] * Kc[ij];
         }
         __syncthreads();
-        for (int r = 0; r < rows; ++r) {
-            double av[B], bv[B];
+        // Psi[w,i,f] = sum_j T_w[i][j] Phi_C[w,j,f], j in order, in place per (w, f) column
+        for (int t = tid; t < nw * Sc; t += 256) {
+            const int wl = t / Sc, f = t - wl * Sc;
+            double* col = psi + wl * K * spc + f;
+            const double* tr = tw + wl * K * K;
+            double x[kMaxLocal];
 #pragma unroll
-            for (int i = 0; i < B; ++i) av[i] = phr[r * Sp + te * B + i];
+            for (int jj = 0; jj < kMaxLocal; ++jj)
+                if (jj < K) x[jj] = col[jj * spc];
 #pragma unroll
-            for (int j = 0; j < B; ++j) bv[j] = psi[r * Sp + tf * B + j];
+            for (int ii = 0; ii < kMaxLocal; ++ii) {
+                if (ii >= K) break;
+                double v = 0.0;
 #pragma unroll
-            for (int i = 0; i < B; ++i)
+                for (int jj = 0; jj < kMaxLocal; ++jj) {
+                    if (jj >= K) break;
+                    v = fma(tr[ii * K + jj], x[jj], v);
+                }
+                col[ii * spc] = v;
+            }
+        }
+        __syncthreads();
+        const double* ph = phr + (lane >> 4) * spr + (lane & 15);
+        const double* ps = psi + (lane >> 4) * spc + (lane & 15);
+        for (int r0 = 0; r0 < rows4; r0 += 4) {
 #pragma unroll
-                for (int j = 0; j < B; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+            for (int t = 0; t < TPW; ++t)
+                if (t < nt)
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph[r0 * spr + 16 * ti[t]], ps[r0 * spc + 16 * tj[t]],
+                                                                  acc[t], 0, 0, 0);
         }
     }
     __syncthreads();
-    double* G = lds;  // Sp x Sp block, receiver-major
+    // G_c over (S_r + 1) x (S_c + 1) slots, receiver-major; slot S_r / S_c (remote) is zero
+    const int gs = Sc + 1;
+    double* G = lds;
 #pragma unroll
-    for (int i = 0; i < B; ++i)
+    for (int t = 0; t < TPW; ++t) {
+        if (t >= nt) continue;
+        const int f = 16 * tj[t] + (lane & 15);
 #pragma unroll
-        for (int j = 0; j < B; ++j) G[(te * B + i) * Sp + tf * B + j] = acc[i][j];
+        for (int v = 0; v < 4; ++v) {
+            const int e = 16 * ti[t] + (lane >> 4) + 4 * v;
+            if (e < Sr && f < Sc) G[e * gs + f] = acc[t][v];
+        }
+    }
+    for (int t = tid; t < Sr + gs; t += 256) G[t < Sr ? t * gs + Sc : Sr * gs + (t - Sr)] = 0.0;
     __syncthreads();
-    auto g = [&](int e, int f) { return e < A.Sr && f < A.Sc ? G[e * Sp + f] : 0.0; };
+    const unsigned ur = static_cast<unsigned>(Sr), uc = static_cast<unsigned>(Sc);
+    auto g = [&](int e, int f) {
+        const unsigned ue = min(static_cast<unsigned>(e), ur), uf = min(static_cast<unsigned>(f), uc);
+        return G[__umul24(ue, static_cast<unsigned>(gs)) + uf];
+    };
     write_cell_column(A.a, A.b, A.m, A.n, A.D, A.C, c, A.out, g);
 }
 
@@ -517,9 +595,9 @@ int launch_cell_block(GramCrossArgs& A, hipStream_t st) {
     return ADT_OK;
 }
 
-template <int B>
+template <int TPW>
 int launch_gram_cross(GramCrossArgs& A, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(gram_cross_kernel<B>, dim3(A.C), dim3(256), lds, st, A);
+    hipLaunchKernelGGL(gram_cross_kernel<TPW>, dim3(A.C), dim3(256), lds, st, A);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -871,18 +949,23 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
     A.out = d_out;
     A.Sr = s_r;
     A.Sc = s_c;
-    const int B = (std::max(s_r, s_c) + 15) / 16;
-    A.Sp = 16 * B;
+    A.btr = (s_r + 15) / 16;
+    A.btc = (s_c + 15) / 16;
+    A.Sp = 16 * std::max(A.btr, A.btc);
     A.K = k_local;
     A.W = n_waves;
     A.rp_r = rp_r;
     A.rp_c = rp_c;
     A.C = n_cells;
     A.D = n_meas;
-    // three staged (rows x Sp) blocks per wavenumber chunk within 24 KB; the G block reuses the LDS
-    A.wchunk = std::max(1, std::min(n_waves, (24576 / (3 * 8 * A.Sp)) / k_local));
-    const size_t staging = static_cast<size_t>(3) * A.wchunk * k_local * A.Sp * 8;
-    const size_t block = static_cast<size_t>(A.Sp) * A.Sp * 8;
+    // Gram form: the staged receiver and current rows of a wavenumber chunk fill at most the G
+    // block's LDS (or 16 KB), which the G block reuses afterwards
+    const size_t block = static_cast<size_t>(s_r + 1) * (s_c + 1) * 8;
+    const size_t row_bytes = static_cast<size_t>(16) * (A.btr + A.btc) * 8;
+    const size_t budget = std::max<size_t>(block, 16384);
+    A.wchunk = std::max(1, std::min(n_waves, static_cast<int>(budget / row_bytes) / k_local));
+    A.rows_max = (A.wchunk * k_local + 3) & ~3;
+    const size_t staging = A.rows_max * row_bytes + static_cast<size_t>(A.wchunk) * k_local * k_local * 8;
     const size_t lds = std::max(staging, block);
     for (int w = 0; w < n_waves; ++w) {
         A.off_r[w] = h_off_r[w];
@@ -891,22 +974,27 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
         A.wm[w] = d_mass ? h_wm[w] : 0.0;
     }
     hipStream_t st = adt::as_stream(stream);
-    switch (cell_block_width(A, 65536)) {
-        case 16: return launch_cell_block<16>(A, st);
-        case 8: return launch_cell_block<8>(A, st);
-        case 4: return launch_cell_block<4>(A, st);
-        case 2: return launch_cell_block<2>(A, st);
-        default: break;
+    // Form: the cell block where its staging fits 64 KB (CB cells share the staging and write
+    // CB-wide row segments; 3D box: 12 ms per launch against 19 ms for the Gram block, whose
+    // per-cell staging and 8-byte row writes outweigh its four LDS reads per output); the Gram
+    // block otherwise.  ADT_CROSS_FORM=gram forces the Gram block.
+    const char* form = std::getenv("ADT_CROSS_FORM");
+    const bool force_gram = form && std::strcmp(form, "gram") == 0;
+    if (!force_gram) {
+        switch (cell_block_width(A, 65536)) {
+            case 16: return launch_cell_block<16>(A, st);
+            case 8: return launch_cell_block<8>(A, st);
+            case 4: return launch_cell_block<4>(A, st);
+            case 2: return launch_cell_block<2>(A, st);
+            default: break;
+        }
     }
-    switch (B) {
+    switch (cross_tiles_per_wave(A.btr * A.btc)) {
         case 1: return launch_gram_cross<1>(A, lds, st);
         case 2: return launch_gram_cross<2>(A, lds, st);
-        case 3: return launch_gram_cross<3>(A, lds, st);
         case 4: return launch_gram_cross<4>(A, lds, st);
-        case 5: return launch_gram_cross<5>(A, lds, st);
-        case 6: return launch_gram_cross<6>(A, lds, st);
-        case 7: return launch_gram_cross<7>(A, lds, st);
-        default: return launch_gram_cross<8>(A, lds, st);
+        case 8: return launch_gram_cross<8>(A, lds, st);
+        default: return launch_gram_cross<16>(A, lds, st);
     }
 }
 

@@ -187,9 +187,10 @@ int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_c
  *   G_c[e][f] = sum_w phi_w[e, dofs_c]^T (ws[w] stiff_c + wm[w] mass_c) phi_w[f, dofs_c]
  *   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])      (index s_rows = remote: 0)
  * phi_w is the (N, rp) block at d_x + h_vec_off[w]; its first s_rows (<= 128) columns are the
- * sources; d_cell_dofs index its rows.  The element templates must be symmetric (stiffness and
- * mass templates are), so G_c is symmetric and one workgroup per cell keeps its upper triangle
- * in LDS (no workspace).
+ * sources; d_cell_dofs index its rows; electrode indices >= s_rows (or negative) read zero.  The
+ * element templates must be symmetric (stiffness and mass templates are), so G_c is symmetric:
+ * one workgroup per cell forms its upper-triangle 16 x 16 tiles on the FP64 matrix cores and
+ * keeps the upper triangle, with a zero slot for the remote electrode, in LDS (no workspace).
  * FP64, every sum in a fixed order: bitwise repeatable.  out is (D, C) row-major.  Replaces the
  * same reference kernel as adt_sensitivity. */
 int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x, int n_meas,
@@ -208,8 +209,10 @@ int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_o
  * staged rows of 2 or more cells fit 64 KB of LDS, G_c is not formed: a workgroup of
  * consecutive cells evaluates out[d, c] = -sum_{w,i} (phiR[m] - phiR[n])_{w,i} (Psi[a] - Psi[b])_{w,i}
  * with Psi_w = T_w phiC_w on the cell's DOFs, and writes each output row as one contiguous
- * segment; otherwise one workgroup per cell keeps G_c in LDS.  No workspace.  FP64, fixed
- * summation order: bitwise repeatable.  out is (D, C) row-major. */
+ * segment; otherwise (or with the environment variable ADT_CROSS_FORM=gram) one workgroup per
+ * cell forms G_c on the FP64 matrix cores and keeps it in LDS with a zero row and column for
+ * the remote electrodes.  No workspace.  FP64, fixed summation order: bitwise repeatable.  out
+ * is (D, C) row-major. */
 int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,
                                const double* d_xc, int rp_c, int s_c, const int64_t* h_off_c, int n_meas,
                                int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
