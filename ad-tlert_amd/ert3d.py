@@ -290,6 +290,7 @@ class ERTForward3D:
         m, n = self._receiver_electrode_map[self._slots[:, 2]], self._receiver_electrode_map[self._slots[:, 3]]
         s.abmn_host = (a, b, m, n)
         s.abmn = [torch.as_tensor(x, dtype=torch.long, device=dev) for x in (a, b, m, n)]
+        s.abmn_i32 = [torch.as_tensor(x.astype(np.int32), device=dev) for x in (a, b, m, n)]
         return s
 
     @staticmethod
@@ -589,8 +590,7 @@ class ERTForward3D:
             rec = receiver.contiguous()
             dofs_dev = columns.select(s.cell_dofs, 0).contiguous()
             part = torch.empty((count, dofs_dev.shape[0]), dtype=torch.float64, device=s.device)
-            abmn = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in s.abmn_host]
-            native.sensitivity_gram_cross(rec, [0], rec.shape[-1], s.nr, current, [0], current.shape[-1], s.nc, *abmn,
+            native.sensitivity_gram_cross(rec, [0], rec.shape[-1], s.nr, current, [0], current.shape[-1], s.nc, *s.abmn_i32,
                                           dofs_dev, templates.contiguous(), None, [1.0], None, part)
             result = columns.columns(part)
             if return_device:
