@@ -341,7 +341,7 @@ struct GramCrossArgs {
     const int32_t* m;
     const int32_t* n;
     double* out;
-    int Sr, Sc, Sp, K, W, rp_r, rp_c, wchunk, C, D;
+    int Sr, Sc, K, W, rp_r, rp_c, wchunk, C, D;
     int btr, btc, rows_max;
     int64_t off_r[kMaxWaves];
     int64_t off_c[kMaxWaves];
@@ -951,7 +951,6 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
     A.Sc = s_c;
     A.btr = (s_r + 15) / 16;
     A.btc = (s_c + 15) / 16;
-    A.Sp = 16 * std::max(A.btr, A.btc);
     A.K = k_local;
     A.W = n_waves;
     A.rp_r = rp_r;
