@@ -484,113 +484,188 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
 
 // Cell-block form for separate receiver and current fields (3D): J[d, c] =
 //   -sum_{w,i} (phi_R[w,i,m] - phi_R[w,i,n]) (Psi[w,i,a] - Psi[w,i,b]),  Psi = T_w phi_C,
-// over the cell's K DOFs.  A block takes CB consecutive cells and stages, per cell, the receiver
-// rows phi_R (W K x Sr) and Psi (W K x Sc) in LDS with the cell index fastest and a zero column
-// for the remote electrode.  Its output rows then come out as CB consecutive doubles per
-// quadrupole (full 64-byte segments): the one-cell-per-block Gram kernel wrote one 8-byte entry
-// per quadrupole and cell, and partial-line writes bound it.  Sums over (w, i) in order.
+// over the cell's K DOFs.  A block of NT threads takes CB consecutive cells and stages, per cell,
+// the receiver rows phi_R (W K x Sr) and Psi (W K x Sc) in LDS, electrode-major: the R = W K rows
+// of CB cells of one electrode are R x CB consecutive doubles (cell fastest); a zero electrode
+// slot stands for the remote electrode.  Staging is two rounds of independent loads: the cells'
+// nodes and element templates, then every (cell, w, i) row of both fields (wave-uniform rows,
+// kCellBlockRows rows per wave in flight); Psi then replaces phi_C in place, one thread per
+// (electrode, cell, w) column.  Output rows come out as CB consecutive doubles per quadrupole.
+// With CB = 32 one wave instruction reads two quadrupoles' 256-byte segments, which cover every
+// LDS bank once (no bank conflicts whatever the electrodes); smaller CB pad each electrode slot
+// by 8 doubles so that the 8 or 4 segments of an instruction spread over the banks.  With K fixed
+// at compile time (KT > 0, one wavenumber) the R reads of one electrode are immediate offsets of
+// one address.  Sums over j and over (w, i) in order.
 constexpr int kAbmnChunk = 2048;  // quadrupoles whose indices are staged at a time
+constexpr int kCellBlockRows = 8;
 
-template <int CB>
-__global__ __launch_bounds__(256) void cell_block_kernel(GramCrossArgs A) {
+constexpr int cell_block_pad(int cb) { return cb >= 32 ? 0 : 8; }
+
+template <int CB, int NT, int KT>
+__global__ __launch_bounds__(NT) void cell_block_kernel(GramCrossArgs A) {
     extern __shared__ double lds[];
-    const int R = A.W * A.K, K = A.K;
-    const int Pr = A.Sr + 1, Pc = A.Sc + 1;  // columns per staged row, the last one zero
-    double* phr = lds;                                         // [R][Pr][CB]
-    double* phc = phr + static_cast<int64_t>(R) * Pr * CB;     // [R][Pc][CB]
-    double* psi = phc + static_cast<int64_t>(R) * Pc * CB;     // [R][Pc][CB]
+    constexpr int NW = NT / 64;  // waves per block
+    const int K = KT > 0 ? KT : A.K;
+    const int W = KT > 0 ? 1 : A.W;
+    const int R = W * K;
+    const int ES = R * CB + cell_block_pad(CB);  // doubles per electrode slot
+    const int Pr = A.Sr + 1, Pc = A.Sc + 1;       // electrode slots, the last one zero
+    double* phr = lds;                                      // [Pr][R][CB]
+    double* psi = phr + static_cast<int64_t>(Pr) * ES;      // [Pc][R][CB] (phi_C first)
+    double* tk = psi + static_cast<int64_t>(Pc) * ES;       // [CB][K][K] stiffness templates
+    double* tm = tk + CB * K * K;                           // [CB][K][K] mass templates
+    int32_t* nodes = reinterpret_cast<int32_t*>(tm + CB * K * K);  // [CB][K]
     // electrode indices of a chunk of quadrupoles, one packed word each (a | b << 8 | m << 16 | n << 24)
-    uint32_t* abmn = reinterpret_cast<uint32_t*>(psi + static_cast<int64_t>(R) * Pc * CB);
+    uint32_t* abmn = reinterpret_cast<uint32_t*>(nodes + ((CB * K + 1) & ~1));
     const int nblk = (A.C + CB - 1) / CB;
     const int c0 = adt::xcd_tile(static_cast<int>(blockIdx.x), nblk) * CB;
-    const auto dofs = adt::gview(A.dofs);
-    const auto xr = adt::gview(A.xr);
-    const auto xc = adt::gview(A.xc);
     const int tid = static_cast<int>(threadIdx.x);
-    // stage phi_R and phi_C: entry (r, e, cb) of cell c0 + cb (cells past C read cell C - 1 and
-    // are never stored)
-    for (int t = tid; t < R * Pr * CB; t += 256) {
-        const int cb = t % CB, e = (t / CB) % Pr, r = t / (CB * Pr);
-        const int w = r / K, i = r % K;
-        const int cell = min(c0 + cb, A.C - 1);
-        const int64_t node = dofs[static_cast<int64_t>(cell) * K + i];
-        const double v = xr[A.off_r[w] + node * A.rp_r + min(e, A.Sr - 1)];
-        phr[t] = e < A.Sr ? v : 0.0;
-    }
-    for (int t = tid; t < R * Pc * CB; t += 256) {
-        const int cb = t % CB, f = (t / CB) % Pc, r = t / (CB * Pc);
-        const int w = r / K, i = r % K;
-        const int cell = min(c0 + cb, A.C - 1);
-        const int64_t node = dofs[static_cast<int64_t>(cell) * K + i];
-        const double v = xc[A.off_c[w] + node * A.rp_c + min(f, A.Sc - 1)];
-        phc[t] = f < A.Sc ? v : 0.0;
-    }
-    __syncthreads();
-    // Psi[w,i,f] = sum_j (ws_w K_c[i][j] + wm_w M_c[i][j]) phi_C[w,j,f], j in order
-    const auto stiff = adt::gview(A.stiff);
-    const auto mass = adt::gview(A.mass);
-    for (int t = tid; t < R * Pc * CB; t += 256) {
-        const int cb = t % CB, f = (t / CB) % Pc, r = t / (CB * Pc);
-        const int w = r / K, i = r % K;
-        const int64_t cell = min(c0 + cb, A.C - 1);
-        const int64_t tb = cell * K * K + static_cast<int64_t>(i) * K;
-        double v = 0.0;
-        for (int j = 0; j < K; ++j) {
-            const double tij = A.mass ? fma(A.wm[w], mass[tb + j], A.ws[w] * stiff[tb + j]) : A.ws[w] * stiff[tb + j];
-            v = fma(tij, phc[(static_cast<int64_t>(w * K + j) * Pc + f) * CB + cb], v);
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // round 1: nodes and templates of the block's cells (cells past C repeat cell C - 1 and are
+    // never stored)
+    {
+        const int64_t first = static_cast<int64_t>(c0) * K, last = static_cast<int64_t>(A.C) * K - 1;
+        for (int t = tid; t < CB * K; t += NT) nodes[t] = A.dofs[min(first + t, last - (K - 1) + t % K)];
+        const int64_t tfirst = static_cast<int64_t>(c0) * K * K, tlast = static_cast<int64_t>(A.C) * K * K;
+        for (int t = tid; t < CB * K * K; t += NT) {
+            const int64_t q = tfirst + t < tlast ? tfirst + t : tlast - K * K + t % (K * K);
+            tk[t] = A.stiff[q];
+            tm[t] = A.mass ? A.mass[q] : 0.0;
         }
-        psi[t] = v;
     }
     __syncthreads();
-    constexpr int RPP = 256 / CB;  // quadrupoles per pass
+    // round 2: row (cb, w, i) of phi_R (rows 0 .. CB R - 1) and of phi_C (rows CB R .. 2 CB R - 1),
+    // lanes along the electrodes (S <= 128)
+    const int n_rows = 2 * CB * R;
+    for (int q0 = wv; q0 < n_rows; q0 += NW * kCellBlockRows) {
+        double v[kCellBlockRows][2];
+#pragma unroll
+        for (int u = 0; u < kCellBlockRows; ++u) {
+            const int q = min(q0 + NW * u, n_rows - 1);  // rows past the end: index math only
+            const bool recv = q < CB * R;
+            const int qq = recv ? q : q - CB * R;
+            const int cb = qq % CB, r = qq / CB, w = r / K, i = r - w * K;
+            const int S = recv ? A.Sr : A.Sc;
+            const double* row = recv ? A.xr + A.off_r[w] + static_cast<int64_t>(nodes[cb * K + i]) * A.rp_r
+                                     : A.xc + A.off_c[w] + static_cast<int64_t>(nodes[cb * K + i]) * A.rp_c;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = lane + 64 * h;
+                v[u][h] = q0 + NW * u < n_rows && e < S ? row[e] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCellBlockRows; ++u) {
+            const int q = q0 + NW * u;
+            if (q >= n_rows) break;
+            const bool recv = q < CB * R;
+            const int qq = recv ? q : q - CB * R;
+            const int cb = qq % CB, r = qq / CB;
+            double* dst = (recv ? phr : psi) + r * CB + cb;
+            const int S = recv ? A.Sr : A.Sc;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = lane + 64 * h;
+                if (e < S) dst[e * ES] = v[u][h];
+            }
+        }
+    }
+    for (int t = tid; t < R * CB; t += NT) {  // the remote electrode's zero slots
+        phr[A.Sr * ES + t] = 0.0;
+        psi[A.Sc * ES + t] = 0.0;
+    }
+    __syncthreads();
+    // Psi[w,i,f] = sum_j (ws_w K_c[i][j] + wm_w M_c[i][j]) phi_C[w,j,f], j in order, in place:
+    // each thread reads its column's K values before writing the K results
+    for (int t = tid; t < A.Sc * CB * W; t += NT) {
+        const int f = t % A.Sc, wc = t / A.Sc, cb = wc % CB, w = wc / CB;
+        double* col = psi + f * ES + (w * K) * CB + cb;
+        const double* kc = tk + cb * K * K;
+        const double* mc = tm + cb * K * K;
+        const double ws = A.ws[w], wm = A.wm[w];
+        double x[KT > 0 ? KT : kMaxLocal];
+#pragma unroll
+        for (int jj = 0; jj < (KT > 0 ? KT : kMaxLocal); ++jj)
+            if (jj < K) x[jj] = col[jj * CB];
+#pragma unroll
+        for (int ii = 0; ii < (KT > 0 ? KT : kMaxLocal); ++ii) {
+            if (ii >= K) break;
+            double v = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < (KT > 0 ? KT : kMaxLocal); ++jj) {
+                if (jj >= K) break;
+                const double tij = A.mass ? fma(wm, mc[ii * K + jj], ws * kc[ii * K + jj]) : ws * kc[ii * K + jj];
+                v = fma(tij, x[jj], v);
+            }
+            col[ii * CB] = v;
+        }
+    }
+    constexpr int RPP = NT / CB;  // quadrupoles per pass
     const int cb = tid % CB, dl = tid / CB;
     const int cell = c0 + cb;
     const auto ga = adt::gview(A.a), gb = adt::gview(A.b), gm = adt::gview(A.m), gn = adt::gview(A.n);
     const auto out = adt::gview(A.out);
     for (int dc0 = 0; dc0 < A.D; dc0 += kAbmnChunk) {
         const int nd = min(kAbmnChunk, A.D - dc0);
-        __syncthreads();  // the previous chunk's indices are consumed
-        constexpr int PER = kAbmnChunk / 256;
-        uint32_t word[PER];
+        __syncthreads();  // the previous chunk's indices are consumed (first chunk: Psi done)
+        constexpr int PER = (kAbmnChunk + NT - 1) / NT;
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {  // all loads of the chunk in flight together
-            const int k = tid + u * 256;
-            const int d = dc0 + (k < nd ? k : 0);
-            // indices >= S name the remote electrode: the zero column
-            const uint32_t ia = min(max(ga[d], 0), A.Sc), ib = min(max(gb[d], 0), A.Sc);
-            const uint32_t im = min(max(gm[d], 0), A.Sr), in = min(max(gn[d], 0), A.Sr);
-            word[u] = ia | ib << 8 | im << 16 | in << 24;
+        for (int u = 0; u < PER; ++u) {
+            const int k = tid + u * NT;
+            if (k < nd) {
+                const int d = dc0 + k;
+                // indices >= S name the remote electrode: the zero slot
+                const uint32_t ia = min(max(ga[d], 0), A.Sc), ib = min(max(gb[d], 0), A.Sc);
+                const uint32_t im = min(max(gm[d], 0), A.Sr), in = min(max(gn[d], 0), A.Sr);
+                abmn[k] = ia | ib << 8 | im << 16 | in << 24;
+            }
         }
-#pragma unroll
-        for (int u = 0; u < PER; ++u)
-            if (tid + u * 256 < nd) abmn[tid + u * 256] = word[u];
         __syncthreads();
         for (int k = dl; k < nd; k += RPP) {
             const uint32_t wd = abmn[k];
-            const int ia = wd & 255, ib = (wd >> 8) & 255, im = (wd >> 16) & 255, in = wd >> 24;
-            double v = 0.0;
-            for (int r = 0; r < R; ++r) {
-                const double* pr = phr + static_cast<int64_t>(r) * Pr * CB + cb;
-                const double* pc = psi + static_cast<int64_t>(r) * Pc * CB + cb;
-                v = fma(pr[im * CB] - pr[in * CB], pc[ia * CB] - pc[ib * CB], v);
+            const double* pm = phr + ((wd >> 16) & 255) * ES + cb;
+            const double* pn = phr + (wd >> 24) * ES + cb;
+            const double* pa = psi + (wd & 255) * ES + cb;
+            const double* pb = psi + ((wd >> 8) & 255) * ES + cb;
+            double acc = 0.0;
+            if constexpr (KT > 0) {
+#pragma unroll
+                for (int r = 0; r < KT; ++r) acc = fma(pm[r * CB] - pn[r * CB], pa[r * CB] - pb[r * CB], acc);
+            } else {
+                for (int r = 0; r < R; ++r) acc = fma(pm[r * CB] - pn[r * CB], pa[r * CB] - pb[r * CB], acc);
             }
-            if (cell < A.C) out[static_cast<int64_t>(dc0 + k) * A.C + cell] = -v;
+            if (cell < A.C) out[static_cast<int64_t>(dc0 + k) * A.C + cell] = -acc;
         }
     }
 }
 
-// Cells per block of cell_block_kernel that fit the LDS budget (0: none).
-inline int cell_block_width(const GramCrossArgs& A, size_t budget) {
-    const size_t per_cell = static_cast<size_t>(A.W) * A.K * ((A.Sr + 1) + 2 * (A.Sc + 1)) * 8;
+// LDS bytes of cell_block_kernel for CB cells per block.
+inline size_t cell_block_lds(const GramCrossArgs& A, int cb) {
+    const size_t es = static_cast<size_t>(A.W) * A.K * cb + cell_block_pad(cb);
+    const size_t templates = static_cast<size_t>(2) * cb * A.K * A.K * 8;
+    const size_t nodes = static_cast<size_t>((cb * A.K + 1) & ~1) * 4;
+    return (static_cast<size_t>(A.Sr + 1) + (A.Sc + 1)) * es * 8 + templates + nodes + kAbmnChunk * 4;
+}
+
+// Cells per block of cell_block_kernel: 32 (1024 threads, conflict-free reads) where its LDS fits
+// one block per CU, else 16/8/4/2 (256 threads) within 64 KB; 0: none.  ADT_CELL_BLOCK=8 caps it
+// at the 64 KB widths.
+inline int cell_block_width(const GramCrossArgs& A) {
+    const char* cap = std::getenv("ADT_CELL_BLOCK");
+    if (!(cap && std::atoi(cap) < 32) && cell_block_lds(A, 32) <= 160 * 1024) return 32;
     for (int cb = 16; cb >= 2; cb /= 2)
-        if (per_cell * cb + kAbmnChunk * 4 <= budget) return cb;
+        if (cell_block_lds(A, cb) <= 65536) return cb;
     return 0;
 }
 
-template <int CB>
+template <int CB, int NT>
 int launch_cell_block(GramCrossArgs& A, hipStream_t st) {
-    const size_t lds = static_cast<size_t>(A.W) * A.K * ((A.Sr + 1) + 2 * (A.Sc + 1)) * 8 * CB + kAbmnChunk * 4;
-    hipLaunchKernelGGL(cell_block_kernel<CB>, dim3((A.C + CB - 1) / CB), dim3(256), lds, st, A);
+    const size_t lds = cell_block_lds(A, CB);
+    const dim3 grid((A.C + CB - 1) / CB);
+    if (A.W == 1 && A.K == 4)  // 3D P1, one field set
+        hipLaunchKernelGGL((cell_block_kernel<CB, NT, 4>), grid, dim3(NT), lds, st, A);
+    else
+        hipLaunchKernelGGL((cell_block_kernel<CB, NT, 0>), grid, dim3(NT), lds, st, A);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -980,11 +1055,12 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
     const char* form = std::getenv("ADT_CROSS_FORM");
     const bool force_gram = form && std::strcmp(form, "gram") == 0;
     if (!force_gram) {
-        switch (cell_block_width(A, 65536)) {
-            case 16: return launch_cell_block<16>(A, st);
-            case 8: return launch_cell_block<8>(A, st);
-            case 4: return launch_cell_block<4>(A, st);
-            case 2: return launch_cell_block<2>(A, st);
+        switch (cell_block_width(A)) {
+            case 32: return launch_cell_block<32, 1024>(A, st);
+            case 16: return launch_cell_block<16, 256>(A, st);
+            case 8: return launch_cell_block<8, 256>(A, st);
+            case 4: return launch_cell_block<4, 256>(A, st);
+            case 2: return launch_cell_block<2, 256>(A, st);
             default: break;
         }
     }

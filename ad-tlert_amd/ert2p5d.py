@@ -15,7 +15,6 @@ part of this path and raise ``NotImplementedError``.
 
 from __future__ import annotations
 
-import hashlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -25,7 +24,7 @@ from scipy.special import k0 as bessel_k0
 from . import fem, native
 from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
-from .field_cache import FieldCache, newest_batches_first
+from .field_cache import FieldCache, digest16, newest_batches_first
 from .inputs import (cell_vector, check_finite, conductivity_model, conductivity_series, cotangent_series,
                      measurement_vector, option, parameter_targets)
 from .mesh import Mesh
@@ -616,7 +615,8 @@ class ERTForward2p5D:
     def _columns(self, cell_parameter_ids=None, parameter_count=None) -> _Columns:
         ids, count = parameter_targets(cell_parameter_ids, parameter_count, self.mesh.cell_count)
         targets = ids[self.discretization.parent_cell_ids.numpy()]
-        key = ("columns", hashlib.blake2b(targets.tobytes(), digest_size=16).digest(), count)
+        # the default (every cell its own column) needs no digest of its cell_count ids
+        key = ("columns", None if cell_parameter_ids is None else digest16(targets.tobytes()), count)
         if key not in self._cache:
             self._cache[key] = _Columns(targets, count, self._dev.device)
         return self._cache[key]

@@ -10,7 +10,6 @@ native gfx950 library.
 
 from __future__ import annotations
 
-import hashlib
 from dataclasses import dataclass
 
 import numpy as np
@@ -20,7 +19,7 @@ from . import fem, native
 from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
 from .ert2p5d import _Columns, electrode_stencil, slot_map
-from .field_cache import FieldCache, newest_batches_first, step_batches
+from .field_cache import FieldCache, digest16, newest_batches_first, step_batches
 from .inputs import (cell_vector, check_finite, conductivity_model, conductivity_series, cotangent_series,
                      measurement_vector, option, parameter_targets)
 from .mesh import Mesh3D
@@ -458,7 +457,8 @@ class ERTForward3D:
 
     def _columns(self, cell_parameter_ids=None, parameter_count=None) -> _Columns:
         targets, count = parameter_targets(cell_parameter_ids, parameter_count, self.mesh.cell_count)
-        key = ("columns", hashlib.blake2b(targets.tobytes(), digest_size=16).digest(), count)
+        # the default (every cell its own column) needs no digest of its cell_count ids
+        key = ("columns", None if cell_parameter_ids is None else digest16(targets.tobytes()), count)
         if key not in self._cache:
             self._cache[key] = _Columns(targets, count, self._dev.device)
         return self._cache[key]

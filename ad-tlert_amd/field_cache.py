@@ -1,7 +1,10 @@
 """LRU cache of solved fields keyed by a hash of the conductivity model.
 
 Same semantics as ``adtlert/forward/field_cache.py:17-110`` (keys, hit/miss counters,
-``step_batches`` splitting, newest-batches-first adjoint order).
+``step_batches`` splitting, newest-batches-first adjoint order).  The key is a 16-byte digest
+of the model's bytes plus its dtype, as there (``field_cache.py:54``, blake2b); the digest is
+xxh3-128 here where the xxhash package is present, which takes 0.7 ms instead of 14 ms for an
+8 MB model.
 """
 
 from __future__ import annotations
@@ -14,7 +17,20 @@ from typing import Generic, TypeVar
 import numpy as np
 import torch
 
+try:  # 128-bit xxh3: ~12 GB/s on one core, against ~0.6 GB/s for blake2b
+    import xxhash
+except ImportError:  # pragma: no cover - the image ships xxhash; blake2b keeps the same contract
+    xxhash = None
+
 Fields = TypeVar("Fields")
+
+
+def digest16(data) -> bytes:
+    """16-byte digest of a contiguous buffer (xxh3-128, or blake2b where xxhash is absent)."""
+
+    if xxhash is not None:
+        return xxhash.xxh3_128_digest(data)
+    return hashlib.blake2b(data, digest_size=16).digest()
 ModelKey = bytes | None
 
 
@@ -43,7 +59,10 @@ class FieldCache(Generic[Fields]):
         if self.max_entries < 1:
             return None
         host = np.ascontiguousarray(model.detach().cpu().numpy())
-        return hashlib.blake2b(host.view(np.uint8), digest_size=16).digest() + host.dtype.str.encode()
+        data = host.view(np.uint8)
+        # a 16-byte digest of the model's bytes plus its dtype; the 1.01M-cell 3D model (8 MB)
+        # hashes in 0.7 ms with xxh3-128 and 14 ms with blake2b, once per jacobian/vjp call
+        return digest16(data) + host.dtype.str.encode()
 
     def _lookup(self, key: ModelKey):
         if key is None:

@@ -331,16 +331,19 @@ def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
 @pytest.mark.parametrize("k, Sr, Sc, W", [(4, 64, 64, 1), (4, 64, 64, 3), (10, 17, 40, 3), (4, 128, 33, 3),
                                           (3, 5, 96, 3), (10, 128, 96, 8)],
                          ids=["p1-3d", "p1-3w", "p2-3w", "wide", "narrow", "large"])
-@pytest.mark.parametrize("form", ["gram", "default"])
+@pytest.mark.parametrize("form", ["gram", "default", "block8"])
 def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W, form, monkeypatch):
     """Jacobian rows from separate receiver and current fields (as in 3D) equal the per-output
     formula with non-symmetric templates, remote indices reading zero and padding columns ignored;
     bitwise repeatable.  Both forms: the per-cell Gram block (matrix cores, four lookups per
-    output; forced by ADT_CROSS_FORM=gram) and the default cell-block kernel (16/8/4/2 cells per
-    workgroup, a tail block past the last cell); the last case is too large for the cell block and
+    output; forced by ADT_CROSS_FORM=gram) and the default cell-block kernel (32 cells and 1024
+    threads where its LDS fits, else 16/8/4/2 cells; ADT_CELL_BLOCK=8 caps it at the narrower
+    widths; a tail block past the last cell); the last case is too large for the cell block and
     takes the Gram form either way."""
 
     monkeypatch.setenv("ADT_CROSS_FORM", form)
+    if form == "block8":
+        monkeypatch.setenv("ADT_CELL_BLOCK", "8")
 
     gen = torch.Generator().manual_seed(100 * k + Sr + Sc + W)
     N, C, D = 80, 47, 39
