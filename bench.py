@@ -184,13 +184,14 @@ class Box3D:
             return op.jacobian(sigma, return_device=True, batch_size=self.D)
 
         # cell-block form: per cell Psi = T Phi_C (K x nc), then per output a K-term sum of
-        # (receiver difference) x (Psi difference), four LDS reads per term
+        # (receiver difference) x (Psi difference), four LDS reads per term; the 16 GB output
+        # writes bound it (a variant with the stores alone takes ~80% of its time)
         K, nc, nr = 4, op._dev.nc, op._dev.nr
         bytes_ = 8 * self.D * self.C + 8 * self.N * (nc + nr) + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
         flops = self.C * 2 * K * K * nc + 4 * K * self.D * self.C
         lds_bytes = 4 * K * 8 * self.D * self.C
-        return run, bytes_, flops, "cell_block_kernel<8> (cell-block sensitivity)", {
-            "bound": "lds", "lds_bytes_per_launch": lds_bytes}
+        return run, bytes_, flops, "cell_block_kernel<32, 1024, 4> (cell-block sensitivity)", {
+            "bound": "hbm", "lds_bytes_per_launch": lds_bytes}
 
     def cpu(self):
         return None  # a 3D direct factorization of this size does not fit a bounded CPU sample
