@@ -491,9 +491,11 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
 // nodes and element templates, then every (cell, w, i) row of both fields (wave-uniform rows,
 // kCellBlockRows rows per wave in flight); Psi then replaces phi_C in place, one thread per
 // (electrode, cell, w) column.  Output rows come out as CB consecutive doubles per quadrupole.
-// With CB = 32 one wave instruction reads two quadrupoles' 256-byte segments, which cover every
-// LDS bank once (no bank conflicts whatever the electrodes); smaller CB pad each electrode slot
-// by 8 doubles so that the 8 or 4 segments of an instruction spread over the banks.  With K fixed
+// With CB = 32 each thread evaluates two neighbouring cells of a quadrupole (16-byte LDS reads and
+// 16-byte stores): one wave instruction reads four quadrupoles' 256-byte segments, each of which
+// covers every LDS bank once (no bank conflicts whatever the electrodes).  Smaller CB (one cell
+// per thread) pad each electrode slot by 8 doubles so that the 8 or 4 segments of an instruction
+// spread over the banks.  With K fixed
 // at compile time (KT > 0, one wavenumber) the R reads of one electrode are immediate offsets of
 // one address.  Sums over j and over (w, i) in order.
 constexpr int kAbmnChunk = 2048;  // quadrupoles whose indices are staged at a time
@@ -621,20 +623,57 @@ __global__ __launch_bounds__(NT) void cell_block_kernel(GramCrossArgs A) {
             }
         }
         __syncthreads();
-        for (int k = dl; k < nd; k += RPP) {
-            const uint32_t wd = abmn[k];
-            const double* pm = phr + ((wd >> 16) & 255) * ES + cb;
-            const double* pn = phr + (wd >> 24) * ES + cb;
-            const double* pa = psi + (wd & 255) * ES + cb;
-            const double* pb = psi + ((wd >> 8) & 255) * ES + cb;
-            double acc = 0.0;
-            if constexpr (KT > 0) {
+        if constexpr (CB >= 32) {
+            // two neighbouring cells per thread: 16-byte LDS reads and stores
+            constexpr int HP = CB / 2, RPP2 = NT / HP;
+            const int cp = tid % HP, dl2 = tid / HP;
+            const int cell0 = c0 + 2 * cp;
+            const bool even = (A.C & 1) == 0 && (reinterpret_cast<uintptr_t>(A.out) & 15) == 0;
+            for (int k = dl2; k < nd; k += RPP2) {
+                const uint32_t wd = abmn[k];
+                const double2* pm = reinterpret_cast<const double2*>(phr + ((wd >> 16) & 255) * ES) + cp;
+                const double2* pn = reinterpret_cast<const double2*>(phr + (wd >> 24) * ES) + cp;
+                const double2* pa = reinterpret_cast<const double2*>(psi + (wd & 255) * ES) + cp;
+                const double2* pb = reinterpret_cast<const double2*>(psi + ((wd >> 8) & 255) * ES) + cp;
+                double a0 = 0.0, a1 = 0.0;
+                if constexpr (KT > 0) {
 #pragma unroll
-                for (int r = 0; r < KT; ++r) acc = fma(pm[r * CB] - pn[r * CB], pa[r * CB] - pb[r * CB], acc);
-            } else {
-                for (int r = 0; r < R; ++r) acc = fma(pm[r * CB] - pn[r * CB], pa[r * CB] - pb[r * CB], acc);
+                    for (int r = 0; r < KT; ++r) {
+                        const double2 m2 = pm[r * HP], n2 = pn[r * HP], x2 = pa[r * HP], y2 = pb[r * HP];
+                        a0 = fma(m2.x - n2.x, x2.x - y2.x, a0);
+                        a1 = fma(m2.y - n2.y, x2.y - y2.y, a1);
+                    }
+                } else {
+                    for (int r = 0; r < R; ++r) {
+                        const double2 m2 = pm[r * HP], n2 = pn[r * HP], x2 = pa[r * HP], y2 = pb[r * HP];
+                        a0 = fma(m2.x - n2.x, x2.x - y2.x, a0);
+                        a1 = fma(m2.y - n2.y, x2.y - y2.y, a1);
+                    }
+                }
+                double* o = A.out + static_cast<int64_t>(dc0 + k) * A.C + cell0;
+                if (even && cell0 + 1 < A.C) {
+                    *reinterpret_cast<double2*>(o) = make_double2(-a0, -a1);
+                } else {
+                    if (cell0 < A.C) o[0] = -a0;
+                    if (cell0 + 1 < A.C) o[1] = -a1;
+                }
             }
-            if (cell < A.C) out[static_cast<int64_t>(dc0 + k) * A.C + cell] = -acc;
+        } else {
+            for (int k = dl; k < nd; k += RPP) {
+                const uint32_t wd = abmn[k];
+                const double* pm = phr + ((wd >> 16) & 255) * ES + cb;
+                const double* pn = phr + (wd >> 24) * ES + cb;
+                const double* pa = psi + (wd & 255) * ES + cb;
+                const double* pb = psi + ((wd >> 8) & 255) * ES + cb;
+                double acc = 0.0;
+                if constexpr (KT > 0) {
+#pragma unroll
+                    for (int r = 0; r < KT; ++r) acc = fma(pm[r * CB] - pn[r * CB], pa[r * CB] - pb[r * CB], acc);
+                } else {
+                    for (int r = 0; r < R; ++r) acc = fma(pm[r * CB] - pn[r * CB], pa[r * CB] - pb[r * CB], acc);
+                }
+                if (cell < A.C) out[static_cast<int64_t>(dc0 + k) * A.C + cell] = -acc;
+            }
         }
     }
 }
