@@ -495,9 +495,8 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
 // 16-byte stores): one wave instruction reads four quadrupoles' 256-byte segments, each of which
 // covers every LDS bank once (no bank conflicts whatever the electrodes).  Smaller CB (one cell
 // per thread) pad each electrode slot by 8 doubles so that the 8 or 4 segments of an instruction
-// spread over the banks.  With K fixed
-// at compile time (KT > 0, one wavenumber) the R reads of one electrode are immediate offsets of
-// one address.  Sums over j and over (w, i) in order.
+// spread over the banks.  With K fixed at compile time (KT > 0, one wavenumber) the R reads of one
+// electrode are immediate offsets of one address.  Sums over j and over (w, i) in order.
 constexpr int kAbmnChunk = 2048;  // quadrupoles whose indices are staged at a time
 constexpr int kCellBlockRows = 8;
 
@@ -687,8 +686,8 @@ inline size_t cell_block_lds(const GramCrossArgs& A, int cb) {
 }
 
 // Cells per block of cell_block_kernel: 32 (1024 threads, conflict-free reads) where its LDS fits
-// one block per CU, else 16/8/4/2 (256 threads) within 64 KB; 0: none.  ADT_CELL_BLOCK=8 caps it
-// at the 64 KB widths.
+// one block per CU, else 16/8/4/2 (256 threads) within 64 KB; 0: none.  ADT_CELL_BLOCK set below
+// 32 (e.g. 8) leaves out the 32-cell width.
 inline int cell_block_width(const GramCrossArgs& A) {
     const char* cap = std::getenv("ADT_CELL_BLOCK");
     if (!(cap && std::atoi(cap) < 32) && cell_block_lds(A, 32) <= 160 * 1024) return 32;
