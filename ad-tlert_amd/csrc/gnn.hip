@@ -169,45 +169,50 @@ __global__ __launch_bounds__(256) void gnn_layer0(Graph g, const uint16_t* param
 }
 
 // Node GEMM of a block of 16 U nodes: out[i][o] = act(bias[o] + sum_t in[i][t] W[t][o]) for 64
-// outputs, inputs in[i][0..K) staged in LDS (row stride K + 1 floats), weights W[K][64] in LDS
-// as FP32.  Thread = U nodes x 4 outputs, so each weight read serves U nodes and each input read
-// four outputs.  Sums in order t = 0..K-1.
+// outputs, inputs in[i][0..K) staged in LDS (row stride K + kInPad floats), weights W[K][64] in
+// LDS as FP32, on the matrix cores: v_mfma_f32_16x16x4f32 multiplies a 16 x 4 block of inputs by a
+// 4 x 16 block of weights; lane l supplies in[16 u + l % 16][t0 + l / 16] and W[t0 + l / 16][16 w +
+// l % 16] and holds out[16 u + 4 (l / 16) + v][16 w + l % 16] in item v of its accumulator.  Wave w
+// owns outputs 16 w .. 16 w + 15 for all U node tiles, so each weight fragment serves U products.
+// The input stride K + 4 puts the 64 lanes of an input fragment on distinct LDS banks.  FP32 sums
+// in a fixed order (the matrix core's, 4 terms per step): repeatable.
+constexpr int kInPad = 4;
+
 template <int U>
 constexpr int gemm_nodes() { return 16 * U; }
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int K, int U, bool RELU>
 __device__ __forceinline__ void node_gemm(const float* in, const float* Wt, const float* bias, int i0, int n,
                                           float* out) {
-    const int nq = static_cast<int>(threadIdx.x) >> 4, oq = static_cast<int>(threadIdx.x) & 15;
-    float acc[U][4];
+    static_assert(K % 4 == 0 && H == 64, "16 x 16 x 4 tiles over 4 waves");
+    constexpr int KS = K + kInPad;
+    const int lane = static_cast<int>(threadIdx.x) & 63, w = static_cast<int>(threadIdx.x) >> 6;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    const int o = 16 * w + r16;
+    const float b0 = bias ? bias[o] : 0.0f;
+    f32x4 acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = f32x4{b0, b0, b0, b0};
+#pragma unroll 4
+    for (int t0 = 0; t0 < K; t0 += 4) {
+        const float bv = Wt[(t0 + k4) * H + o];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(in[(16 * u + r16) * KS + t0 + k4], bv, acc[u], 0, 0, 0);
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = bias ? bias[oq * 4 + v] : 0.0f;
-#pragma unroll 4
-    for (int t = 0; t < K; ++t) {
-        float a[U], w[4];
-#pragma unroll
-        for (int u = 0; u < U; ++u) a[u] = in[(nq * U + u) * (K + 1) + t];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) w[v] = Wt[t * H + oq * 4 + v];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(a[u], w[v], acc[u][v]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int i = i0 + nq * U + u;
-        if (i < n)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                out[static_cast<int64_t>(i) * H + oq * 4 + v] = RELU ? fmaxf(acc[u][v], 0.0f) : acc[u][v];
-    }
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + 16 * u + 4 * k4 + v;
+            if (i < n) out[static_cast<int64_t>(i) * H + o] = RELU ? fmaxf(acc[u][v], 0.0f) : acc[u][v];
+        }
 }
 
-constexpr int kLayerU = 2;  // 32 nodes per block: weights (32 KB) + inputs (16.5 KB) in LDS
-constexpr int kProjU = 4;   // 64 nodes per block: weights (16 KB) + inputs (16.6 KB) in LDS
+constexpr int kLayerU = 2;  // 32 nodes per block: weights (32 KB) + inputs (16.9 KB) in LDS
+constexpr int kProjU = 4;   // 64 nodes per block: weights (16 KB) + inputs (17 KB) in LDS
 static_assert(2 * HD == H, "the head projection writes rows of H floats");
 static_assert(kMeanParts == adt::kWave, "one wavefront sums the partials");
 
@@ -257,7 +262,7 @@ __global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params
     constexpr int NB = gemm_nodes<kLayerU>();
     __shared__ float Wl[2 * H * H];
     __shared__ float bl[H];
-    __shared__ float in[NB * (2 * H + 1)];
+    __shared__ float in[NB * (2 * H + kInPad)];
     const uint16_t* P = params + OFF_L1 + (l - 1) * LAYER_SIZE;
     for (int t = threadIdx.x; t < 2 * H * H; t += 256) Wl[t] = bf16_to_float(P[t]);
     if (threadIdx.x < H) bl[threadIdx.x] = bf16_to_float(P[2 * H * H + threadIdx.x]);
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params
             const int r = t / (2 * H), c = t % (2 * H), i = i0 + r;
             const int ic = i < g.n ? i : g.n - 1;
             const float v = c < H ? hp[static_cast<int64_t>(ic) * H + c] : mp[static_cast<int64_t>(ic) * H + c - H];
-            in[r * (2 * H + 1) + c] = i < g.n ? v : 0.0f;
+            in[r * (2 * H + kInPad) + c] = i < g.n ? v : 0.0f;
         }
         __syncthreads();
         node_gemm<2 * H, kLayerU, true>(in, Wl, bl, i0, g.n, W.h[l]);
@@ -283,7 +288,7 @@ __global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params
 __global__ __launch_bounds__(256) void gnn_head_proj(Graph g, const uint16_t* params, float* work, int64_t wstride) {
     __shared__ float Ws[H * 2 * HD];
     constexpr int NB = gemm_nodes<kProjU>();
-    __shared__ float in[NB * (H + 1)];
+    __shared__ float in[NB * (H + kInPad)];
     for (int t = threadIdx.x; t < H * 2 * HD; t += 256) {
         const int r = t / (2 * HD), o = t % (2 * HD);
         Ws[t] = bf16_to_float(params[OFF_U + (o < HD ? r : H + r) * HD + (o % HD)]);
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(256) void gnn_head_proj(Graph g, const uint16_t* pa
     const int i0 = blockIdx.x * NB;
     for (int t = threadIdx.x; t < NB * H; t += 256) {
         const int r = t / H, c = t % H, i = i0 + r;
-        in[r * (H + 1) + c] = i < g.n ? W.h[3][static_cast<int64_t>(i) * H + c] : 0.0f;
+        in[r * (H + kInPad) + c] = i < g.n ? W.h[3][static_cast<int64_t>(i) * H + c] : 0.0f;
     }
     __syncthreads();
     node_gemm<H, kProjU, false>(in, Ws, nullptr, i0, g.n, W.pq);
