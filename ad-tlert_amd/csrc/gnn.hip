@@ -269,17 +269,33 @@ __global__ __launch_bounds__(256) void gnn_layer(Graph g, const uint16_t* params
     Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
     const auto hp = adt::gview(W.h[l - 1]);
     const auto mp = adt::gview(W.m[l]);
-    for (int b = 0; b < kLayerBatches; ++b) {
-        const int i0 = (blockIdx.x * kLayerBatches + b) * NB;
-        if (i0 >= g.n) break;
-        __syncthreads();  // weights stored; the previous batch's inputs read
-        for (int t = threadIdx.x; t < NB * 2 * H; t += 256) {
+    // each thread's share of a batch's [h, m] rows, loaded together into registers; the next
+    // batch's loads are issued before this batch's products
+    constexpr int PER = NB * 2 * H / 256;
+    float v[PER];
+    const auto load = [&](int i0) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int t = static_cast<int>(threadIdx.x) + q * 256;
             const int r = t / (2 * H), c = t % (2 * H), i = i0 + r;
             const int ic = i < g.n ? i : g.n - 1;
-            const float v = c < H ? hp[static_cast<int64_t>(ic) * H + c] : mp[static_cast<int64_t>(ic) * H + c - H];
-            in[r * (2 * H + kInPad) + c] = i < g.n ? v : 0.0f;
+            v[q] = c < H ? hp[static_cast<int64_t>(ic) * H + c] : mp[static_cast<int64_t>(ic) * H + c - H];
+        }
+    };
+    const int first = blockIdx.x * kLayerBatches * NB;
+    if (first < g.n) load(first);
+    for (int b = 0; b < kLayerBatches; ++b) {
+        const int i0 = first + b * NB;
+        if (i0 >= g.n) break;
+        __syncthreads();  // weights stored; the previous batch's inputs read
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int t = static_cast<int>(threadIdx.x) + q * 256;
+            const int r = t / (2 * H), c = t % (2 * H);
+            in[r * (2 * H + kInPad) + c] = i0 + r < g.n ? v[q] : 0.0f;
         }
         __syncthreads();
+        if (b + 1 < kLayerBatches && i0 + NB < g.n) load(i0 + NB);
         node_gemm<2 * H, kLayerU, true>(in, Wl, bl, i0, g.n, W.h[l]);
     }
 }
@@ -295,9 +311,19 @@ __global__ __launch_bounds__(256) void gnn_head_proj(Graph g, const uint16_t* pa
     }
     Work W = work_at(work + blockIdx.y * wstride, g.n, g.nnz);
     const int i0 = blockIdx.x * NB;
-    for (int t = threadIdx.x; t < NB * H; t += 256) {
-        const int r = t / H, c = t % H, i = i0 + r;
-        in[r * (H + kInPad) + c] = i < g.n ? W.h[3][static_cast<int64_t>(i) * H + c] : 0.0f;
+    constexpr int PER = NB * H / 256;  // loaded together, then stored
+    float v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int t = static_cast<int>(threadIdx.x) + q * 256;
+        const int r = t / H, c = t % H, i = min(i0 + r, g.n - 1);
+        v[q] = W.h[3][static_cast<int64_t>(i) * H + c];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int t = static_cast<int>(threadIdx.x) + q * 256;
+        const int r = t / H, c = t % H;
+        in[r * (H + kInPad) + c] = i0 + r < g.n ? v[q] : 0.0f;
     }
     __syncthreads();
     node_gemm<H, kProjU, false>(in, Ws, nullptr, i0, g.n, W.pq);
