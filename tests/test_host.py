@@ -106,6 +106,28 @@ def test_field_cache_semantics():
         fc.FieldCache(-1)
 
 
+def test_field_cache_key_digests(monkeypatch):
+    """Keys are a 16-byte digest of the model's bytes plus its dtype: equal models share a key,
+    a one-ulp change or another dtype does not; the blake2b fallback (no xxhash) keeps that."""
+
+    fc = importlib_cache()
+    model = torch.linspace(0.5, 2.0, 1001, dtype=torch.float64)
+    nudged = model.clone()
+    nudged[500] = np.nextafter(nudged[500].item(), 3.0)
+    for backend in ("default", "blake2b"):
+        if backend == "blake2b":
+            monkeypatch.setattr(fc, "xxhash", None)
+        cache = fc.FieldCache(4)
+        key = cache.key(model)
+        assert isinstance(key, bytes) and len(key) == 16 + len(b"<f8")
+        assert cache.key(model.clone()) == key
+        assert cache.key(nudged) != key
+        assert cache.key(model.float()) != key
+    import hashlib
+
+    assert fc.digest16(b"abc") == hashlib.blake2b(b"abc", digest_size=16).digest()
+
+
 def importlib_cache():
     import importlib
 
