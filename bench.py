@@ -104,6 +104,9 @@ class Timelapse:
         op._geometric_factors().to(self.sig.device, self.torch.float64) * op._combine(normal, reciprocal)
         return stats
 
+    def sensitivity_config(self):
+        return {"measurements": self.D, "cells": self.C, "sources": self.E, "wavenumbers": self.W}
+
     def jacobian(self):
         op = self.op
         fields = op._solve_fields(self.sig[:1])
@@ -174,6 +177,9 @@ class Box3D:
         current = self.torch.stack([f.current for f in fields])
         op._geometric_factors().to(self.sig.device, self.torch.float64) * op._measurement_resistances(current)
         return stats
+
+    def sensitivity_config(self):
+        return {"measurements": self.D, "cells": self.C, "receivers": self.op._dev.nr, "currents": self.op._dev.nc}
 
     def jacobian(self):
         op = self.op
@@ -402,6 +408,14 @@ def main() -> int:
             "flops_per_launch": sens_flops, "bytes_per_launch": sens_bytes, **sens_extra}
         if "lds_bytes_per_launch" in sens_extra:
             result["roofline_sensitivity"]["achieved_lds_tbs"] = sens_extra["lds_bytes_per_launch"] / (ms * 1e-3) / 1e12
+        # measured HBM traffic of the Jacobian kernel (committed FETCH_SIZE / WRITE_SIZE runs of this
+        # same configuration, scaled by a copy of known size)
+        sens_file = ROOT / "profiles" / f"{args.workload}_{kernel.split('<')[0]}_traffic.json"
+        if sens_file.exists():
+            t = json.loads(sens_file.read_text())
+            if t["kernel"] == kernel.split(" (")[0] and t["config"] == wl.sensitivity_config():
+                result["roofline_sensitivity"]["traffic"] = t["traffic_bytes_per_dispatch"]
+                result["roofline_sensitivity"]["traffic_source"] = f"profiles/{sens_file.name}"
         log(f"jacobian {wl.D}x{wl.C}: {ms:.1f} ms/launch -> {result['jacobian_rows_per_s']:.0f} rows/s")
         del jac
 
