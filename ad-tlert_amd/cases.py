@@ -65,15 +65,28 @@ def wenner_schlumberger(count: int, limit=None) -> np.ndarray:
     return rows if limit is None else rows[:limit]
 
 
-def random_quadrupoles(count: int, rows: int, rng: np.random.Generator) -> np.ndarray:
-    """``rows`` distinct random ABMN quadruples of four distinct electrodes."""
+def random_quadrupoles(count: int, rows: int, rng: np.random.Generator, positions=None,
+                       max_factor: float = 2.0e4) -> np.ndarray:
+    """``rows`` distinct random ABMN quadruples of four distinct electrodes.
+
+    With electrode ``positions``, quadruples whose half-space geometric factor
+    ``2 pi / (1/AM - 1/AN - 1/BM + 1/BN)`` exceeds ``max_factor`` metres in magnitude are drawn
+    again: on a regular grid some arrangements are null couplings (the denominator vanishes and
+    the factor is infinite), which no survey would record."""
 
     seen, out = set(), []
     while len(out) < rows:
         quad = tuple(int(x) for x in rng.choice(count, 4, replace=False))
-        if quad not in seen:
-            seen.add(quad)
-            out.append(quad)
+        if quad in seen:
+            continue
+        seen.add(quad)
+        if positions is not None:
+            a, b, m, n = (np.asarray(positions[i], dtype=float) for i in quad)
+            coupling = (1 / np.linalg.norm(a - m) - 1 / np.linalg.norm(a - n) - 1 / np.linalg.norm(b - m)
+                        + 1 / np.linalg.norm(b - n))
+            if abs(coupling) * max_factor < 2 * np.pi:
+                continue
+        out.append(quad)
     return np.asarray(out, dtype=np.int32)
 
 
@@ -317,9 +330,9 @@ def timelapse_ws_20k(seed=2, steps=365, node_target=20000):
 
 
 def strip_dd_50k(seed=3):
-    """~50k nodes, 64 electrodes at 1 m, 2000 dipole-dipole rows."""
+    """~50k nodes (49 776 nodes, 98 550 triangles), 64 electrodes at 1 m, 2000 dipole-dipole rows."""
 
-    return config_2p5d(50000, 64, 1.0, seed, "dd", 2000, 0.25, dd_spacings=(1, 2, 3, 4, 5, 6, 7))[0]
+    return config_2p5d(50000, 64, 1.0, seed, "dd", 2000, 0.2, dd_spacings=(1, 2, 3, 4, 5, 6, 7))[0]
 
 
 def grf_3d(points, rng, mean=-2.0, std=0.5, length=20.0, grid=2.0):
@@ -337,8 +350,8 @@ def grf_3d(points, rng, mean=-2.0, std=0.5, length=20.0, grid=2.0):
     return 10.0 ** (mean + std * field[idx[:, 0], idx[:, 1], idx[:, 2]])
 
 
-def box_p1_200k(seed=4, fine=1.25, growth=1.15, surface_step=0.5, depth_growth=1.06, rows=2000):
-    """200 x 200 x 100 m box, 8 x 8 electrodes at 5 m, ~200k nodes / ~1.1M tets, random ABMN rows."""
+def box_p1_200k(seed=4, fine=1.25, growth=1.13, surface_step=0.5, depth_growth=1.06, rows=2000):
+    """200 x 200 x 100 m box, 8 x 8 electrodes at 5 m, 202 005 nodes / 1 149 984 tets, random ABMN rows."""
 
     rng = np.random.default_rng(seed)
     grid = 5.0 * (np.arange(8) - 3.5)
@@ -347,7 +360,7 @@ def box_p1_200k(seed=4, fine=1.25, growth=1.15, surface_step=0.5, depth_growth=1
     nodes, tets = hex_tets(xs, xs, zs)
     gx, gy = np.meshgrid(grid, grid, indexing="ij")
     positions = np.column_stack((gx.ravel(), gy.ravel(), np.zeros(64)))
-    quads = random_quadrupoles(64, rows, rng)
+    quads = random_quadrupoles(64, rows, rng, positions)
     centers = nodes[tets].mean(axis=1)
     return Case3D(nodes, tets, positions, quads, grf_3d(centers, rng))
 
@@ -378,6 +391,6 @@ def terrain_p2_150k(seed=5, fine=2.5, growth=1.25, surface_step=1.0, depth_growt
     nodes, tets = hex_tets(xs, xs, zs, bend)
     gx, gy = np.meshgrid(grid, grid, indexing="ij")
     positions = np.column_stack((gx.ravel(), gy.ravel(), surface(gx.ravel(), gy.ravel())))
-    quads = random_quadrupoles(64, rows, rng)
+    quads = random_quadrupoles(64, rows, rng, positions)
     centers = nodes[tets].mean(axis=1)
     return Case3D(nodes, tets, positions, quads, grf_3d(centers, rng))

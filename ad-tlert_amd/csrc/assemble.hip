@@ -1,7 +1,7 @@
 // (a) Deterministic finite-element assembly by CSR-slot gather.
 //
 // The reference materialises the (W, C, K, K) local blocks and folds them into CSR slots
-// with padded gather levels (forward/kernels.py:79-134).  Here every CSR slot owns the
+// with padded gather levels (forward/kernels.py:30-85).  Here every CSR slot owns the
 // ordered list of local entries that land on it (a precomputed scatter map), and one
 // thread per slot sums them directly from the unit templates, so no per-wavenumber
 // element blocks are ever written.  Each thread handles one slot for a run of systems:

@@ -1888,7 +1888,7 @@ extern "C" int adt_pcg_solve(adt_pcg_plan* p, const int32_t* d_indptr, const int
             for (int c = 0; c < p->r; ++c) {
                 const int64_t i = static_cast<int64_t>(s) * p->rp + c;
                 p->last_fcg_iters_max = std::max(p->last_fcg_iters_max, rep.iters[i]);
-                need += rep.flags[i] == -1 || rep.flags[i] == -2 || rep.flags[i] == 1;
+                need += rep.flags[i] == -1 || rep.flags[i] == -2;
             }
         if (need > 0 && max_iter > 0) {
             // Jacobi PCG finishes every right-hand side from its current iterate; converged ones

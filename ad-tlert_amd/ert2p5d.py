@@ -198,6 +198,43 @@ def pair_scatter(slots: np.ndarray, count: int, normal: np.ndarray, reciprocal: 
     return pairs.reshape(q, width, width)[:, :count, :count]
 
 
+JACOBIAN_BLOCK_BYTES = 32 << 30
+"""Device bytes of one sensitivity launch's (rows, cells) FP64 output by default (an eighth of
+the 288 GB of HBM)."""
+
+
+def jacobian_rows_per_launch(rows: int, cells: int) -> int:
+    """Every row in one launch unless the FP64 block would exceed ``JACOBIAN_BLOCK_BYTES``."""
+
+    return max(1, min(rows, JACOBIAN_BLOCK_BYTES // max(8 * cells, 1)))
+
+
+def chunked_rows(count: int, cells: int, batch_size: int, launch, columns, device, return_device: bool) -> Tensor:
+    """Run ``launch(start, stop, part)`` for consecutive chunks of ``batch_size`` measurement
+    rows into a device ``(rows, cells)`` block, aggregate its columns, and gather the rows on the
+    host in ``FLOAT_DTYPE`` (or on the device with ``return_device``)."""
+
+    if count <= batch_size or return_device:
+        result = torch.empty((count, columns.count), dtype=torch.float64, device=device)
+        for start in range(0, count, batch_size):
+            stop = min(count, start + batch_size)
+            part = torch.empty((stop - start, cells), dtype=torch.float64, device=device)
+            launch(start, stop, part)
+            result[start:stop] = columns.columns(part)
+        if return_device:
+            return result
+        out = result.to(device="cpu", dtype=FLOAT_DTYPE)
+    else:
+        out = torch.empty((count, columns.count), dtype=FLOAT_DTYPE)
+        part = torch.empty((batch_size, cells), dtype=torch.float64, device=device)
+        for start in range(0, count, batch_size):
+            stop = min(count, start + batch_size)
+            launch(start, stop, part[: stop - start])
+            out[start:stop] = columns.columns(part[: stop - start]).to(device="cpu", dtype=FLOAT_DTYPE)
+    check_finite(out, "jacobian")
+    return out
+
+
 @dataclass
 class _Fields:
     """Assembled values ``(W, nnz)`` and interleaved total fields ``(W, N, Rp)`` of one model,
@@ -225,7 +262,7 @@ class _Fields:
 
 
 class _Columns:
-    """Cell -> output-column aggregation (``kernels.py:137-162``) on the device."""
+    """Cell -> output-column aggregation (``kernels.py:88-113``) on the device."""
 
     def __init__(self, targets: np.ndarray, count: int, device):
         self.count = int(count)
@@ -752,16 +789,24 @@ class ERTForward2p5D:
         return out
 
     def _batch_size(self, batch_size, direct: bool) -> int:
+        """Measurement rows per sensitivity launch (``ert2p5d.py:971-982`` semantics for an
+        explicit value).  The reference's default of 64 rows on large meshes bounds a consumer
+        GPU's memory; on the 288 GB device the default takes every row in one launch unless the
+        (rows, cells) FP64 block would exceed ``JACOBIAN_BLOCK_BYTES``.  The adjoint path keeps
+        the reference's 8."""
+
         if batch_size is None:
             if not direct:
                 return 8
-            return min(self.survey.measurement_count, 64) if self.mesh.cell_count >= 4096 else self.survey.measurement_count
+            return jacobian_rows_per_launch(self.survey.measurement_count, self.mesh.cell_count)
         if batch_size < 1:
             raise ValueError("batch_size must be >= 1")
         return int(batch_size)
 
     def _normal_jacobian(self, fields: _Fields, batch_size: int, columns: _Columns, *, return_device=False) -> Tensor:
-        """``-sum_w w_w (u_M - u_N)^T (K + k_w^2 M) (u_A - u_B)`` by the fused kernel (``ert2p5d.py:984-1010``)."""
+        """``-sum_w w_w (u_M - u_N)^T (K + k_w^2 M) (u_A - u_B)`` by the fused kernel
+        (``ert2p5d.py:984-1010``), ``batch_size`` measurement rows per launch: the device holds
+        one launch's rows at a time unless ``return_device`` asks for the whole block there."""
 
         dev = self._dev
         stiff = columns.select(dev.stiff, 0).contiguous()
@@ -769,29 +814,24 @@ class ERTForward2p5D:
         ws, wm = dev.q_host, dev.q_host * dev.k2_host
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         D = self.survey.measurement_count
-        cols = columns.count
         if dev.E <= 128 and precision == 0:
-            # Gram form on the interleaved device-order blocks: per cell G_c = Phi^T T Phi once,
-            # then four gathers per row (FP64; the mixed mode keeps the per-output FP32 kernel)
+            # Gram form on the interleaved device-order blocks: per cell G_c = Phi^T T Phi once per
+            # launch, then four gathers per row (FP64; the mixed mode keeps the per-output FP32 kernel)
             dofs = columns.select(dev.cell_dofs, 0).contiguous()
             blocks = fields.blocks.contiguous()
-            n_cells = dofs.shape[0]
-            part = torch.empty((D, n_cells), dtype=torch.float64, device=dev.device)
             vec_off = np.arange(dev.W, dtype=np.int64) * dev.N * blocks.shape[-1]
-            native.sensitivity_gram(blocks, vec_off, blocks.shape[-1], dev.E, dev.a, dev.b, dev.m, dev.n, dofs, stiff,
-                                    mass, ws, wm, part)
-            result = columns.columns(part)
-            return result if return_device else result.cpu().to(FLOAT_DTYPE)
-        phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
-        dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
-        result = torch.empty((D, cols), dtype=torch.float64, device=dev.device)
-        for start in range(0, D, batch_size):
-            stop = min(D, start + batch_size)
-            part = torch.empty((stop - start, dofs.shape[0]), dtype=torch.float64, device=dev.device)
-            native.sensitivity(phi, dev.a[start:stop], dev.b[start:stop], dev.m[start:stop], dev.n[start:stop], dofs,
-                               stiff, mass, ws, wm, part, precision)
-            result[start:stop] = columns.columns(part)
-        return result if return_device else result.cpu().to(FLOAT_DTYPE)
+
+            def launch(start, stop, part):
+                native.sensitivity_gram(blocks, vec_off, blocks.shape[-1], dev.E, dev.a[start:stop], dev.b[start:stop],
+                                        dev.m[start:stop], dev.n[start:stop], dofs, stiff, mass, ws, wm, part)
+        else:
+            phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
+            dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
+
+            def launch(start, stop, part):
+                native.sensitivity(phi, dev.a[start:stop], dev.b[start:stop], dev.m[start:stop], dev.n[start:stop],
+                                   dofs, stiff, mass, ws, wm, part, precision)
+        return chunked_rows(D, dofs.shape[0], batch_size, launch, columns, dev.device, return_device)
 
     def _adjoint_jacobian(self, fields: _Fields, normal, reciprocal, batch_size: int, robin: bool,
                           normal_sensitivity: bool) -> Tensor:

@@ -18,7 +18,7 @@ import torch
 from . import fem, native
 from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
-from .ert2p5d import _Columns, electrode_stencil, slot_map
+from .ert2p5d import _Columns, chunked_rows, electrode_stencil, jacobian_rows_per_launch, slot_map
 from .field_cache import FieldCache, digest16, newest_batches_first, step_batches
 from .inputs import (cell_vector, check_finite, conductivity_model, conductivity_series, cotangent_series,
                      measurement_vector, option, parameter_targets)
@@ -568,12 +568,18 @@ class ERTForward3D:
     def jacobian(self, conductivity, *, batch_size=None, include_robin_boundary_derivative: bool = False,
                  normal_sensitivity: bool = True, jacobian_cell_parameter_ids=None, jacobian_parameter_count=None,
                  return_device: bool = False) -> Tensor:
-        """``-(u_M - u_N)^T T_c (u_A - u_B)`` by the fused kernel (``ert3d.py:867-927``)."""
+        """``-(u_M - u_N)^T T_c (u_A - u_B)`` by the fused kernel (``ert3d.py:867-927``),
+        ``batch_size`` measurement rows per launch.  An explicit ``batch_size`` bounds the device
+        block as in the reference; by default every row goes in one launch unless the (rows,
+        cells) FP64 block would exceed ``JACOBIAN_BLOCK_BYTES`` (the 288 GB device holds the
+        17 GB block of 2000 rows x 1.1M cells)."""
 
         del normal_sensitivity
         s = self._dev
         count = self.survey.measurement_count
-        batch_size = min(max(count, 1), 64) if batch_size is None else int(batch_size)
+        if batch_size is None:
+            batch_size = jacobian_rows_per_launch(count, self.mesh.cell_count)
+        batch_size = int(batch_size)
         if batch_size < 1:
             raise ValueError("batch_size must be >= 1")
         columns = self._columns(jacobian_cell_parameter_ids, jacobian_parameter_count)
@@ -581,45 +587,37 @@ class ERTForward3D:
         fields, _ = self._fields_series(sigma[None])
         (receiver,) = self._receivers(fields, sigma[None])
         templates = columns.select(s.templates if include_robin_boundary_derivative else s.volume_templates, 0)
+        templates = templates.contiguous()
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         if precision == 0 and s.nc <= 128 and s.nr <= 128:
             # cross-Gram form on the interleaved device-order blocks: per cell G_c = Phi_R^T T Phi_C
-            # (receivers x currents) once, then four reads per row (FP64; the mixed mode keeps the
-            # per-output FP32 kernel)
+            # (receivers x currents) evaluated per launch, then four reads per row (FP64; the mixed
+            # mode keeps the per-output FP32 kernel)
             current = fields[0].current.contiguous()
             rec = receiver.contiguous()
-            dofs_dev = columns.select(s.cell_dofs, 0).contiguous()
-            part = torch.empty((count, dofs_dev.shape[0]), dtype=torch.float64, device=s.device)
-            native.sensitivity_gram_cross(rec, [0], rec.shape[-1], s.nr, current, [0], current.shape[-1], s.nc, *s.abmn_i32,
-                                          dofs_dev, templates.contiguous(), None, [1.0], None, part)
-            result = columns.columns(part)
-            if return_device:
-                return result
-            out = result.to(device="cpu", dtype=FLOAT_DTYPE)
-            check_finite(out, "jacobian")
-            return out
-        S = s.nc + s.nr
-        phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)  # mesh DOF order
-        phi[0, : s.nc] = fields[0].current[s.inv, : s.nc].T
-        phi[0, s.nc:] = receiver[s.inv, : s.nr].T
-        a, b, m, n = (x.copy() for x in s.abmn_host)
-        a = np.where(a == s.nc, S, a)
-        b = np.where(b == s.nc, S, b)
-        m, n = m + s.nc, n + s.nc
-        idx = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in (a, b, m, n)]
-        dofs = columns.select(s.cell_dofs_mesh, 0).contiguous()
-        result = torch.empty((count, columns.count), dtype=torch.float64, device=s.device)
-        for start in range(0, count, batch_size):
-            stop = min(count, start + batch_size)
-            part = torch.empty((stop - start, dofs.shape[0]), dtype=torch.float64, device=s.device)
-            native.sensitivity(phi, *(x[start:stop] for x in idx), dofs, templates.contiguous(), None, [1.0], None, part,
-                               precision)
-            result[start:stop] = columns.columns(part)
-        if return_device:
-            return result
-        out = result.to(device="cpu", dtype=FLOAT_DTYPE)
-        check_finite(out, "jacobian")
-        return out
+            dofs = columns.select(s.cell_dofs, 0).contiguous()
+            a, b, m, n = s.abmn_i32
+
+            def launch(start, stop, part):
+                native.sensitivity_gram_cross(rec, [0], rec.shape[-1], s.nr, current, [0], current.shape[-1], s.nc,
+                                              a[start:stop], b[start:stop], m[start:stop], n[start:stop], dofs,
+                                              templates, None, [1.0], None, part)
+        else:
+            S = s.nc + s.nr
+            phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)  # mesh DOF order
+            phi[0, : s.nc] = fields[0].current[s.inv, : s.nc].T
+            phi[0, s.nc:] = receiver[s.inv, : s.nr].T
+            a, b, m, n = (x.copy() for x in s.abmn_host)
+            a = np.where(a == s.nc, S, a)
+            b = np.where(b == s.nc, S, b)
+            m, n = m + s.nc, n + s.nc
+            idx = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in (a, b, m, n)]
+            dofs = columns.select(s.cell_dofs_mesh, 0).contiguous()
+
+            def launch(start, stop, part):
+                native.sensitivity(phi, *(x[start:stop] for x in idx), dofs, templates, None, [1.0], None, part,
+                                   precision)
+        return chunked_rows(count, dofs.shape[0], batch_size, launch, columns, s.device, return_device)
 
     def jacobian_columnwise(self, conductivity) -> Tensor:
         basis = torch.eye(self.mesh.cell_count, dtype=FLOAT_DTYPE)

@@ -1,7 +1,7 @@
 """Batched iterative SPD solves on the GPU (replaces the cuDSS ``BatchedSolver``).
 
 The reference factorizes every matrix with cuDSS and back-substitutes all right-hand sides
-(``forward/cudss.py:182-247``).  Here each system is solved in the native library by FP64
+(``forward/cudss.py:63-122``).  Here each system is solved in the native library by FP64
 conjugate gradients: Jacobi-preconditioned (``preconditioner="jacobi"``), or flexible CG
 preconditioned by one smoothed-aggregation multigrid V-cycle per iteration (``"amg"``), whose
 hierarchy is built once per plan from the first matrices solved with it and whose numeric

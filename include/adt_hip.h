@@ -48,7 +48,7 @@ int adt_device_info(int device, int* h_cus, char* h_name, int name_len);
  * (a) Deterministic finite-element assembly.
  *
  * Replaces the gather-levels GroupSum of ERTForward2p5D._assemble
- * (forward/ert2p5d.py:275-314, forward/kernels.py:79-134) and ERTForward3D._assemble_batch
+ * (forward/ert2p5d.py:275-314, forward/kernels.py:30-85) and ERTForward3D._assemble_batch
  * (forward/ert3d.py:436-461).  For system s = model*n_waves + w and CSR slot j:
  *
  *   values[s, j] = sum_{e in slot_entry[slot_ptr[j] : slot_ptr[j+1]]} coef(s, e)
@@ -83,7 +83,7 @@ int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t*
 /* ------------------------------------------------------------------------------------------
  * (b) Batched FP64 preconditioned conjugate gradients on SPD systems.
  *
- * Replaces BatchedSolver.solve (forward/cudss.py:182-247), the cuDSS direct solve that
+ * Replaces BatchedSolver.solve (forward/cudss.py:63-122), the cuDSS direct solve that
  * ERTForward2p5D._solve (forward/ert2p5d.py:328-347) and ERTForward3D._solve
  * (forward/ert3d.py:463-481) call.  A plan fixes the batch layout (systems, rows, CSR
  * offsets, lane width) and owns the Krylov workspace; solves then take the matrix values,
@@ -94,7 +94,9 @@ int adt_spmm(int n_sys, int lanes, int rp, const int32_t* h_rows, const int64_t*
  * (the same cycle whose finest prolongator damping comes per matrix slot from precond_data, a
  * device array laid out like d_values, e.g. the output of the learned graph network).  With
  * either cycle, a right-hand side whose residual stalls or breaks down is finished by Jacobi
- * PCG from its current iterate, to the same tolerance (adt_pcg_stats reports how many).
+ * PCG from its current iterate, to the same tolerance (adt_pcg_stats reports how many); one
+ * that only reaches max_iter is reported as not converged after exactly max_iter cycle
+ * iterations.
  * Every solve ends with the true residual b - A x; right-hand sides whose true residual misses
  * tol (the recurrence residual drifts by rounding) restart once from their iterate in Jacobi
  * PCG (adt_pcg_stats reports how many), so the reported residuals are the true ones.
@@ -259,7 +261,7 @@ int adt_electrode_potentials(int n_models, int n_waves, int rp, int r, int n_rec
 int adt_deinterleave(int n_sys, int rp, int r, int64_t n, const int64_t* h_vec_off, const double* d_x,
                      double* d_out, int64_t out_stride, void* stream);
 
-/* Deterministic column aggregation (CellAggregation, forward/kernels.py:137-162):
+/* Deterministic column aggregation (CellAggregation, forward/kernels.py:88-113):
  * out[d, p] = sum_{i in col_ptr[p] .. col_ptr[p+1]} in[d, col_idx[i]] in CSR order. */
 int adt_aggregate_columns(int n_rows, int64_t in_cols, int n_out, const double* d_in,
                           const int32_t* d_col_ptr, const int32_t* d_col_idx, double* d_out,

@@ -194,7 +194,7 @@ class Forward2p5D:
         return ids, int(ids.max()) + 1 if count is None else int(count)
 
     def aggregate(self, per_cell, ids=None, count=None):
-        """Sum cell columns into parameters, dropping id -1 (``kernels.py:137-162``)."""
+        """Sum cell columns into parameters, dropping id -1 (``kernels.py:88-113``)."""
 
         targets, count = self.columns(ids, count)
         keep = targets >= 0
@@ -203,15 +203,18 @@ class Forward2p5D:
             out[..., p] = per_cell[..., keep & (targets == p)].sum(axis=-1)
         return out
 
-    def jacobian(self, sigma, ids=None, count=None, chunk=64):
-        """Normal sensitivity ``-sum_w q_w (u_M-u_N)^T (K+k^2M) (u_A-u_B)`` (``ert2p5d.py:984-1010``, ``kernels.py:132-178``)."""
+    def jacobian(self, sigma, ids=None, count=None, chunk=64, rows=None):
+        """Normal sensitivity ``-sum_w q_w (u_M-u_N)^T (K+k^2M) (u_A-u_B)`` (``ert2p5d.py:984-1010``, ``kernels.py:132-178``).
+
+        ``rows`` (optional) selects measurement rows; the result then has ``len(rows)`` rows."""
 
         phi = np.concatenate((self.fields(sigma)[0], np.zeros((self.W, 1, self.N))), axis=1)
         T = self.q[:, None, None, None] * self.templates()
         cells = self.mesh["cells"]
-        a, b, m, n = self.slots.T
+        slots = self.slots if rows is None else self.slots[np.asarray(rows)]
+        a, b, m, n = slots.T
         rows = []
-        for s in range(0, self.D, chunk):
+        for s in range(0, slots.shape[0], chunk):
             sl = slice(s, s + chunk)
             cur = (phi[:, a[sl]] - phi[:, b[sl]])[..., cells]
             rec = (phi[:, m[sl]] - phi[:, n[sl]])[..., cells]

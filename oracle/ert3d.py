@@ -14,6 +14,13 @@ from . import fem, mesh as meshlib, survey as surveylib
 RADIUS_FRACTION = 0.5  # ert3d.py:47
 
 
+def factorize(matrix):
+    """Sparse LU of a symmetric matrix with a minimum-degree ordering of ``A^T + A`` (SuperLU's
+    column ordering for symmetric patterns: about half the fill of COLAMD on tetrahedral meshes)."""
+
+    return spla.splu(matrix.tocsc(), permc_spec="MMD_AT_PLUS_A")
+
+
 def electrode_rows(slots, count):
     """Electrodes used by ``slots`` and the survey-slot -> field-row map (``ert3d.py:93-106``)."""
 
@@ -66,13 +73,10 @@ class Forward3D:
         """First-order far-field Robin faces added into cell templates (``ert3d.py:224-256``)."""
 
         m = self.mesh
-        center = self.positions.mean(axis=0)
-        for f in np.flatnonzero(~m["surface_face_mask"]):
-            radial = m["boundary_face_centers"][f] - center
-            r2 = float(radial @ radial)
-            if r2 <= 0.0:
-                continue
-            alpha = max(float(radial @ m["boundary_face_normals"][f]) / r2, 0.0)
+        far = np.flatnonzero(~m["surface_face_mask"])
+        coefficients = fem.robin_coefficient_3d(m["boundary_face_centers"][far], m["boundary_face_normals"][far],
+                                                self.positions.mean(axis=0))
+        for f, alpha in zip(far, coefficients):
             if alpha == 0.0:
                 continue
             c = int(m["boundary_face_cells"][f])
@@ -172,7 +176,7 @@ class Forward3D:
         sigma = np.asarray(sigma, float)
         key = sigma.tobytes()
         if key not in self._cache:
-            lu = spla.splu(self._assemble_matrix(sigma).tocsc())
+            lu = factorize(self._assemble_matrix(sigma))
             current = lu.solve(self.source_rhs.T).T[: self.curr_ids.size]
             self._cache = {key: [current, None, lu]}
         return self._cache[key]
@@ -200,7 +204,7 @@ class Forward3D:
         numerical = self.gf_mode == "numerical" or (self.gf_mode == "auto" and not self.mesh["flat"])
         if not numerical:
             return surveylib.geometric_factors(self.positions, self.quads)
-        unit = spla.splu(self.unit.tocsc()).solve(self.source_rhs.T).T[: self.curr_ids.size]
+        unit = factorize(self.unit).solve(self.source_rhs.T).T[: self.curr_ids.size]
         return 1.0 / self.measurement(unit)
 
     def apparent_resistivity(self, sigma, currents=1.0):
@@ -216,19 +220,22 @@ class Forward3D:
             out[..., p] = per_cell[..., ids == p].sum(axis=-1)
         return out
 
-    def jacobian(self, sigma, robin=False, ids=None, count=None, chunk=64):
-        """``-(u_M-u_N)^T T_c (u_A-u_B)`` on current+receiver rows (``ert3d.py:867-927``)."""
+    def jacobian(self, sigma, robin=False, ids=None, count=None, chunk=64, rows=None):
+        """``-(u_M-u_N)^T T_c (u_A-u_B)`` on current+receiver rows (``ert3d.py:867-927``).
+
+        ``rows`` (optional) selects measurement rows; the result then has ``len(rows)`` rows."""
 
         current, receiver = self.fields(sigma)[0], self.receivers(sigma)
         phi = np.concatenate((current, receiver, np.zeros((1, self.N))))
         nc = current.shape[0]
-        a, b = self.curr_map[self.slots[:, 0]], self.curr_map[self.slots[:, 1]]
+        slots = self.slots if rows is None else self.slots[np.asarray(rows)]
+        a, b = self.curr_map[slots[:, 0]], self.curr_map[slots[:, 1]]
         a, b = np.where(a == nc, phi.shape[0] - 1, a), np.where(b == nc, phi.shape[0] - 1, b)
-        m, n = self.recv_map[self.slots[:, 2]] + nc, self.recv_map[self.slots[:, 3]] + nc
+        m, n = self.recv_map[slots[:, 2]] + nc, self.recv_map[slots[:, 3]] + nc
         k = self.conn.shape[1]
         T = (self.flat_t if robin else self.flat_vt).reshape(self.C, k, k)
         rows = []
-        for s in range(0, self.D, chunk):
+        for s in range(0, slots.shape[0], chunk):
             sl = slice(s, s + chunk)
             cur = (phi[a[sl]] - phi[b[sl]])[:, self.conn]
             rec = (phi[m[sl]] - phi[n[sl]])[:, self.conn]

@@ -153,4 +153,17 @@ def p2_face_mass(area):
     return float(area) * np.einsum("q,qi,qj->ij", w, shape, shape)
 
 
+def robin_coefficient_3d(centers, normals, center):
+    """First-order far-field coefficient ``alpha = max(r . n / |r|^2, 0)`` with ``r`` from the
+    survey centre to each face centre (``ert3d.py:239-256``); faces at the centre get 0."""
+
+    radial = np.asarray(centers, float) - np.asarray(center, float)
+    r2 = np.einsum("fd,fd->f", radial, radial)
+    dot = np.einsum("fd,fd->f", radial, np.asarray(normals, float))
+    out = np.zeros(radial.shape[0])
+    ok = r2 > 0.0
+    out[ok] = np.maximum(dot[ok] / r2[ok], 0.0)
+    return out
+
+
 P1_FACE_MASS = np.array([[2.0, 1.0, 1.0], [1.0, 2.0, 1.0], [1.0, 1.0, 2.0]]) / 12.0  # ert3d.py:236-238

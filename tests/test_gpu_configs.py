@@ -1,0 +1,320 @@
+"""Parity at the production tolerance on the five throughput configurations, at full size.
+
+Every operator here is built with the shipped default ``solver_tolerance`` (relative residual
+1e-10), not a tightened one, and compared with the FP64 CPU oracle (sparse direct solves):
+
+* C1 ``strip_dd_4k`` (3993 nodes, 1000 dipole-dipole rows, 17 wavenumbers): apparent
+  resistivity within 1e-6 for every preconditioner, the full normal Jacobian within 1e-5;
+* C2 ``timelapse_ws_20k`` (19 800 nodes, 96 electrodes, 3000 rows, 19 wavenumbers): two time
+  steps of apparent resistivity through the series API with the learned preconditioner, and
+  the Jacobian of one step against 48 oracle rows;
+* C3 ``strip_dd_50k`` (49 776 nodes, 64 electrodes, 2000 rows): apparent resistivity, the
+  gradient of the L2 misfit of log apparent resistivity (autograd bridge -> vjp kernel) against
+  the oracle vjp, jvp on three seeded directions against the oracle jvp, central finite
+  differences with h = 1e-6 and dot tests;
+* C4 ``box_p1_200k`` (202 005 nodes, 1.15M tets) and C5 ``terrain_p2_150k`` (195 201 P2 DOFs,
+  terrain, numerical geometric factors, learned preconditioner): the oracle's factorizations
+  take minutes there, so its results are read from the committed fixtures that
+  ``tests/golden/make_golden.py`` writes (apparent resistivity, jvp, sampled vjp and Jacobian
+  rows), plus homogeneous half-space exactness, dot tests, ``J d`` against ``normal_jvp`` and
+  finite differences.
+
+Bars (max-norm relative, the reference suite's ``assert_close`` measure): 1e-6 on apparent
+resistivity, 1e-5 on Jacobian, vjp and jvp entries, 1e-6 on dot tests.  Every measured error is
+printed.
+"""
+
+from __future__ import annotations
+
+import hashlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ert2p5d as oracle2p5d
+from tests.conftest import adt, relmax
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+RHOA_BAR, DERIV_BAR, DOT_BAR = 1e-6, 1e-5, 1e-6
+
+
+def operator_2p5d(case, **options):
+    mesh = adt.Mesh.from_arrays(case.nodes, case.cells)
+    survey = adt.Survey.from_arrays(case.electrodes, case.measurements)
+    return adt.ERTForward2p5D.from_mesh_survey(mesh, survey, **options)
+
+
+def oracle_2p5d(case):
+    return oracle2p5d.Forward2p5D(case.nodes, case.cells, case.electrodes, case.measurements)
+
+
+def report(name: str, value: float, bar: float) -> float:
+    print(f"[{name}] {value:.3e} (bar {bar:.0e})", flush=True)
+    return value
+
+
+def assert_default_tolerance(forward):
+    assert forward._solver.tolerance == 1e-10
+    stats = forward._solver.last
+    assert stats is not None and float(stats.relative_residual.max()) <= 1e-10
+
+
+# --------------------------------------------------------------------------------------------
+# C1: 2.5D forward + full Jacobian
+# --------------------------------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def c1():
+    case = adt.cases.strip_dd_4k()
+    return case, oracle_2p5d(case)
+
+
+@pytest.mark.parametrize("preconditioner", ["amg", "learned", "jacobi"])
+def test_c1_apparent_resistivity_at_the_default_tolerance(c1, preconditioner):
+    case, ref = c1
+    forward = operator_2p5d(case, preconditioner=preconditioner)
+    rhoa = forward.apparent_resistivity_values(torch.as_tensor(case.conductivity)).numpy()
+    assert_default_tolerance(forward)
+    err = report(f"C1 {preconditioner} rhoa", relmax(rhoa, ref.apparent_resistivity(case.conductivity)), RHOA_BAR)
+    assert err < RHOA_BAR
+    forward.close()
+
+
+def test_c1_full_jacobian_at_the_default_tolerance(c1):
+    case, ref = c1
+    forward = operator_2p5d(case)
+    sigma = torch.as_tensor(case.conductivity)
+    jac = forward.jacobian(sigma).numpy()
+    assert jac.shape == (ref.D, ref.C)
+    err = report("C1 jacobian", relmax(jac, ref.jacobian(case.conductivity)), DERIV_BAR)
+    assert err < DERIV_BAR
+    # rows of a chunked call are the same launches on fewer rows: bitwise equal
+    assert np.array_equal(forward.jacobian(sigma, batch_size=333).numpy(), jac)
+
+
+# --------------------------------------------------------------------------------------------
+# C2: time-lapse mesh, two steps through the series API
+# --------------------------------------------------------------------------------------------
+
+
+def test_c2_series_and_jacobian_at_the_default_tolerance():
+    case, series = adt.cases.timelapse_ws_20k(steps=2)
+    ref = oracle_2p5d(case)
+    forward = operator_2p5d(case, preconditioner="learned")
+    rhoa = forward.apparent_resistivity_series(torch.as_tensor(series)).numpy()
+    assert_default_tolerance(forward)
+    assert forward._solver.last.preconditioner == "learned"
+    for t in range(2):
+        err = report(f"C2 step {t} rhoa", relmax(rhoa[t], ref.apparent_resistivity(series[t])), RHOA_BAR)
+        assert err < RHOA_BAR
+    jac = forward.jacobian(torch.as_tensor(series[1])).numpy()
+    rows = np.sort(np.random.default_rng(20).choice(ref.D, 48, replace=False))
+    expected = ref.jacobian(series[1], rows=rows, chunk=16)
+    err = report("C2 jacobian (48 oracle rows)", relmax(jac[rows], expected), DERIV_BAR)
+    assert err < DERIV_BAR
+    forward.close()
+
+
+# --------------------------------------------------------------------------------------------
+# C3: differentiation on the 50k-node strip
+# --------------------------------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def c3():
+    case = adt.cases.strip_dd_50k()
+    ref = oracle_2p5d(case)
+    forward = operator_2p5d(case)
+    rng = np.random.default_rng(33)
+    centers = case.nodes[case.cells].mean(axis=1)
+    smooth = adt.cases.smooth_fields_2d(centers, rng, 1)[0]
+    truth = case.conductivity * np.exp(0.1 * smooth)
+    observed = np.log(ref.apparent_resistivity(truth))
+    directions = rng.standard_normal((3, ref.C)) * case.conductivity[None]
+    yield case, ref, forward, observed, directions
+    forward.close()
+
+
+def test_c3_apparent_resistivity_at_the_default_tolerance(c3):
+    case, ref, forward, _, _ = c3
+    rhoa = forward.apparent_resistivity_values(torch.as_tensor(case.conductivity)).numpy()
+    assert_default_tolerance(forward)
+    err = report("C3 rhoa", relmax(rhoa, ref.apparent_resistivity(case.conductivity)), RHOA_BAR)
+    assert err < RHOA_BAR
+
+
+def misfit_and_gradient(forward, sigma: np.ndarray, observed: np.ndarray):
+    s = torch.as_tensor(sigma).clone().requires_grad_()
+    rhoa = adt.apparent_resistivity_autograd(s, forward)
+    loss = 0.5 * ((torch.log(rhoa) - torch.as_tensor(observed)) ** 2).sum()
+    loss.backward()
+    return float(loss), s.grad.numpy(), rhoa.detach().numpy()
+
+
+def test_c3_misfit_gradient_matches_the_oracle_vjp(c3):
+    case, ref, forward, observed, _ = c3
+    _, grad, rhoa = misfit_and_gradient(forward, case.conductivity, observed)
+    k = ref.geometric_factors()
+    cotangent = k * (np.log(rhoa) - observed) / rhoa
+    expected = ref.vjp(case.conductivity, cotangent)
+    err = report("C3 misfit gradient vs oracle vjp", relmax(grad, expected), DERIV_BAR)
+    assert err < DERIV_BAR
+
+
+def test_c3_jvp_matches_the_oracle_and_dot_tests(c3):
+    case, ref, forward, _, directions = c3
+    sigma = torch.as_tensor(case.conductivity)
+    g = np.random.default_rng(34).standard_normal(ref.D)
+    vjp = forward.vjp(sigma, torch.as_tensor(g)).numpy()
+    nvjp = forward.normal_vjp(sigma, torch.as_tensor(g)).numpy()
+    for i, d in enumerate(directions):
+        jvp = forward.jvp(sigma, torch.as_tensor(d)).numpy()
+        err = report(f"C3 jvp direction {i}", relmax(jvp, ref.jvp(case.conductivity, d)), DERIV_BAR)
+        assert err < DERIV_BAR
+        lhs, rhs = float(jvp @ g), float(d @ vjp)
+        assert report(f"C3 dot test {i}", abs(lhs - rhs) / abs(rhs), DOT_BAR) < DOT_BAR
+        lhs, rhs = float(forward.normal_jvp(sigma, torch.as_tensor(d)).numpy() @ g), float(d @ nvjp)
+        assert report(f"C3 normal dot test {i}", abs(lhs - rhs) / abs(rhs), DOT_BAR) < DOT_BAR
+
+
+def test_c3_central_differences_of_the_misfit(c3):
+    """Central differences with h = 1e-6 along three relative directions.
+
+    The difference quotient divides the solver's error by h: at the default tolerance a
+    relative residual of 1e-10 leaves ~1e-11 of relative noise in the misfit, i.e. ~1e-5 of
+    the quotient, so the bar there is 1e-4; an operator solving to 1e-13 checks the
+    derivative at the 1e-5 bar."""
+
+    case, ref, forward, observed, directions = c3
+    sigma = case.conductivity
+    tight = operator_2p5d(case, solver_tolerance=1e-13)
+    h = 1e-6
+
+    def misfit(op, s):
+        rhoa = op.apparent_resistivity_values(torch.as_tensor(s)).numpy()
+        return 0.5 * float(((np.log(rhoa) - observed) ** 2).sum())
+
+    for op, name, bar in ((tight, "tol 1e-13", DERIV_BAR), (forward, "tol 1e-10", 1e-4)):
+        _, grad, _ = misfit_and_gradient(op, sigma, observed)
+        for i, d in enumerate(directions):
+            numeric = (misfit(op, sigma + h * d) - misfit(op, sigma - h * d)) / (2 * h)
+            analytic = float(grad @ d)
+            err = report(f"C3 FD {name} direction {i}", abs(numeric - analytic) / abs(analytic), bar)
+            assert err < bar
+    tight.close()
+
+
+# --------------------------------------------------------------------------------------------
+# C4 / C5: 3D at full size against the committed oracle fixtures
+# --------------------------------------------------------------------------------------------
+
+
+def input_digest(case) -> str:
+    """Same digest as ``tests/golden/make_golden.py``."""
+
+    h = hashlib.blake2b(digest_size=16)
+    for a in (case.nodes, case.cells, case.electrodes, case.measurements, case.conductivity):
+        a = np.ascontiguousarray(a)
+        h.update(str(a.dtype).encode() + str(a.shape).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def golden(name: str, case):
+    path = GOLDEN / f"{name}.npz"
+    if not path.exists():
+        pytest.fail(f"missing oracle fixture {path.name}: run tests/golden/make_golden.py")
+    data = dict(np.load(path, allow_pickle=False))
+    assert str(data["digest"]) == input_digest(case), f"{path.name} was written for other inputs"
+    return data
+
+
+def three_d(name: str, case, order: int, preconditioner: str):
+    mesh = adt.Mesh3D.from_arrays(case.nodes, case.cells)
+    survey = adt.Survey.from_arrays(case.electrodes, case.measurements)
+    forward = adt.ERTForward3D.from_mesh_survey(mesh, survey, element_order=order, preconditioner=preconditioner)
+    return forward, golden(name, case)
+
+
+@pytest.fixture(scope="module")
+def c4():
+    case = adt.cases.box_p1_200k()
+    forward, data = three_d("box_p1_200k", case, 1, "amg")
+    yield case, forward, data
+    forward.close()
+
+
+@pytest.fixture(scope="module")
+def c5():
+    case = adt.cases.terrain_p2_150k()
+    forward, data = three_d("terrain_p2_150k", case, 2, "learned")
+    yield case, forward, data
+    forward.close()
+
+
+def golden_vectors(label, case, forward, data):
+    """Apparent resistivity, jvp, sampled vjp and sampled Jacobian rows against the fixture."""
+
+    from tests.golden.make_golden import jvp_direction, vjp_cotangent
+
+    sigma = torch.as_tensor(case.conductivity)
+    rhoa = forward.apparent_resistivity_values(sigma).numpy()
+    assert_default_tolerance(forward)
+    assert report(f"{label} rhoa", relmax(rhoa, data["rhoa"]), RHOA_BAR) < RHOA_BAR
+    gf = forward._geometric_factors().double().numpy()
+    assert report(f"{label} geometric factors", relmax(gf, data["geometric_factors"]), RHOA_BAR) < RHOA_BAR
+    jvp = forward.jvp(sigma, torch.as_tensor(jvp_direction(case))).numpy()
+    assert report(f"{label} jvp", relmax(jvp, data["jvp"]), DERIV_BAR) < DERIV_BAR
+    vjp = forward.vjp(sigma, torch.as_tensor(vjp_cotangent(case))).numpy()
+    assert report(f"{label} vjp (sampled cells)", relmax(vjp[data["vjp_cells"]], data["vjp"]), DERIV_BAR) < DERIV_BAR
+    rows = data["jacobian_rows"]
+    jac = forward.jacobian(sigma, return_device=True)
+    sampled = jac[torch.as_tensor(rows, dtype=torch.long, device=jac.device)][
+        :, torch.as_tensor(data["jacobian_cells"], dtype=torch.long, device=jac.device)].cpu().numpy()
+    err = relmax(sampled, data["jacobian"])
+    assert report(f"{label} jacobian (sampled rows and cells)", err, DERIV_BAR) < DERIV_BAR
+    return jac
+
+
+def properties(label, case, forward, jac):
+    """Homogeneous exactness, dot tests, J d against normal_jvp and central differences."""
+
+    C, D = case.cells.shape[0], case.measurements.shape[0]
+    homogeneous = forward.apparent_resistivity_values(torch.full((C,), 0.01, dtype=torch.float64)).numpy()
+    assert report(f"{label} homogeneous half space", relmax(homogeneous, 100.0), RHOA_BAR) < RHOA_BAR
+    rng = np.random.default_rng(45)
+    sigma = torch.as_tensor(case.conductivity)
+    d = torch.as_tensor(rng.standard_normal(C) * case.conductivity)
+    g = torch.as_tensor(rng.standard_normal(D))
+    jvp, vjp = forward.jvp(sigma, d), forward.vjp(sigma, g)
+    err = abs(float(jvp @ g) - float(d @ vjp)) / abs(float(d @ vjp))
+    assert report(f"{label} dot test", err, DOT_BAR) < DOT_BAR
+    njvp, nvjp = forward.normal_jvp(sigma, d), forward.normal_vjp(sigma, g)
+    err = abs(float(njvp @ g) - float(d @ nvjp)) / abs(float(d @ nvjp))
+    assert report(f"{label} normal dot test", err, DOT_BAR) < DOT_BAR
+    dev = jac.device
+    jd = (jac @ d.to(dev)).cpu().numpy()
+    assert report(f"{label} J d vs normal_jvp", relmax(jd, njvp.numpy()), DERIV_BAR) < DERIV_BAR
+    jtg = (g.to(dev) @ jac).cpu().numpy()
+    assert report(f"{label} J^T g vs normal_vjp", relmax(jtg, nvjp.numpy()), DERIV_BAR) < DERIV_BAR
+    h = 1e-6
+    numeric = ((forward.resistance(sigma + h * d) - forward.resistance(sigma - h * d)) / (2 * h)).numpy()
+    assert report(f"{label} jvp vs central differences", relmax(jvp.numpy(), numeric), 1e-4) < 1e-4
+
+
+def test_c4_box_matches_the_oracle_fixture_and_properties(c4):
+    case, forward, data = c4
+    jac = golden_vectors("C4", case, forward, data)
+    properties("C4", case, forward, jac)
+
+
+def test_c5_terrain_p2_learned_matches_the_oracle_fixture_and_properties(c5):
+    case, forward, data = c5
+    assert forward.element_order == 2 and not forward.mesh.is_flat_surface
+    jac = golden_vectors("C5", case, forward, data)
+    assert forward._solver.last.preconditioner in ("learned",)
+    properties("C5", case, forward, jac)

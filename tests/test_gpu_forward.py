@@ -106,7 +106,7 @@ def test_homogeneous_half_space_and_scaling(quad):
 
 
 @pytest.mark.parametrize("name", ["quad", "tri"])
-def test_normal_jacobian_matches_the_oracle(name, quad, tri):
+def test_normal_jacobian_matches_the_oracle(name, quad, tri, monkeypatch):
     if name == "quad":
         case, forward, ref = quad
         sigma = case.conductivity
@@ -116,8 +116,14 @@ def test_normal_jacobian_matches_the_oracle(name, quad, tri):
     expected = ref.jacobian(sigma)
     print(f"[{name}] jacobian max rel err {relmax(jac, expected):.3e}")
     assert relmax(jac, expected) < 1e-5
+    # an explicit batch size launches the Gram kernel once per chunk of rows (the per-cell Gram
+    # is recomputed per launch in the same order, so the rows are bitwise equal)
+    launches = []
+    gram = adt.native.sensitivity_gram
+    monkeypatch.setattr(adt.native, "sensitivity_gram", lambda *a: launches.append(a[4].numel()) or gram(*a))
     chunked = forward.jacobian(torch.as_tensor(sigma), batch_size=7).numpy()
-    assert relmax(chunked, jac) < 1e-13
+    assert launches == [7] * (ref.D // 7) + ([ref.D % 7] if ref.D % 7 else [])
+    assert np.array_equal(chunked, jac)
 
 
 def test_vjp_jvp_and_normal_products_match_the_oracle(quad):
@@ -166,6 +172,8 @@ def test_3d_matches_the_oracle(order, boundary):
     assert relmax(rhoa, ref.apparent_resistivity(sigma)) < 1e-6
     jac = forward.jacobian(torch.as_tensor(sigma), include_robin_boundary_derivative=True).numpy()
     assert relmax(jac, ref.jacobian(sigma, robin=True)) < 1e-5
+    chunked = forward.jacobian(torch.as_tensor(sigma), include_robin_boundary_derivative=True, batch_size=2).numpy()
+    assert np.array_equal(chunked, jac)  # one launch per pair of rows, same per-row arithmetic
     rng = np.random.default_rng(order * 10 + (boundary == "dirichlet"))
     g, d = rng.standard_normal(ref.D), rng.standard_normal(ref.C) * 1e-3
     assert relmax(forward.vjp(torch.as_tensor(sigma), torch.as_tensor(g)).numpy(), ref.vjp(sigma, g)) < 1e-5

@@ -170,6 +170,32 @@ def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2
     return iters, stats
 
 
+def test_iteration_cap_under_the_cycle_runs_exactly_that_many_cycle_iterations():
+    """The lane-width autotuner times a fixed number of preconditioned iterations by solving at
+    an unreachable tolerance.  A right-hand side that merely reaches max_iter under the cycle
+    is reported as not converged; it is not handed to Jacobi PCG (only stalled or broken-down
+    ones are), so every right-hand side runs exactly max_iter cycle iterations."""
+
+    rng = np.random.default_rng(5)
+    side, R, cap = 20, 13, 12
+    indptr, indices, values, _ = random_spd_batch(rng, side)
+    S, n = values.shape[0], side * side
+    plan = native.PcgPlan(np.full(S, n), R, 16, 32, np.zeros(S), np.zeros(S), np.arange(S) * values.shape[1],
+                          host_pattern=(indptr, indices), lds_bytes=8192)
+    values_d = torch.as_tensor(values, device=dev())
+    plan.attach_amg(values_d, coarse_max=16, stall_limit=64)
+    b = torch.as_tensor(rng.standard_normal((S, n, plan.rp)), device=dev())
+    b[:, :, R:] = 0.0
+    x = torch.zeros_like(b)
+    rc, iters, _ = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()), values_d,
+                              b, x, 1e-300, cap, precond=native.PRECOND_AMG, check_every=4)
+    stats = plan.stats()
+    plan.close()
+    assert rc == native.ADT_ERR_NOT_CONVERGED
+    assert stats["fallback_rhs"] == 0 and stats["polished_rhs"] == 0
+    np.testing.assert_array_equal(iters, np.full_like(iters, cap))
+
+
 def test_true_residual_restart_brings_every_rhs_to_the_tolerance():
     """At a tight tolerance the CG recurrence residual of some right-hand sides drifts below tol
     while their true residual b - A x is still above it (here 14 of 128, seeded).  Those restart

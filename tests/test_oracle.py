@@ -206,6 +206,80 @@ def test_robin_2p5d_coefficient_is_the_log_derivative_of_the_image_field():
     np.testing.assert_allclose(alpha[far], -dudn / u(c), rtol=1e-6)
 
 
+def test_robin_3d_coefficient_is_the_log_derivative_of_the_point_source_field():
+    """``alpha = max(r . n / r^2, 0)`` is what makes ``u = 1/|x - c|`` satisfy
+    ``du/dn + alpha u = 0`` on the far faces: it equals ``-d log u / dn``, checked by central
+    differences of the point-source field along each face normal (a sign or radius mistake
+    fails).  Faces whose outward normal points back towards the centre are clipped to 0."""
+
+    nodes, tets = adt.cases.box_mesh(relief=1.0)
+    m = omesh.mesh3d(nodes, tets)
+    center = np.array([5.5, 6.0, m["boundary_face_centers"][m["surface_face_mask"], 2].max()])
+    far = ~m["surface_face_mask"]
+    c, n = m["boundary_face_centers"][far], m["boundary_face_normals"][far]
+    alpha = fem.robin_coefficient_3d(c, n, center)
+
+    def u(x):
+        return 1.0 / np.linalg.norm(x - center, axis=-1)
+
+    h = 1e-6
+    dudn = (u(c + h * n) - u(c - h * n)) / (2 * h)
+    expected = np.maximum(-dudn / u(c), 0.0)
+    np.testing.assert_allclose(alpha, expected, rtol=1e-6, atol=1e-12)
+    assert np.all(alpha >= 0) and np.count_nonzero(alpha) > 0.9 * alpha.size
+    # a face centre inward of the centre (normal towards it) gets no Robin term
+    assert fem.robin_coefficient_3d(np.array([[0.0, 0.0, -1.0]]), np.array([[0.0, 0.0, 1.0]]), np.zeros(3))[0] == 0.0
+    assert fem.robin_coefficient_3d(np.zeros((1, 3)), np.array([[0.0, 0.0, 1.0]]), np.zeros(3))[0] == 0.0
+
+
+def test_p2_face_mass_matches_an_independent_rule():
+    pts, wts = _duffy_triangle()
+    bary = np.column_stack((1.0 - pts.sum(axis=1), pts))
+    edges = np.stack([4.0 * bary[:, i] * bary[:, j] for i, j in ((0, 1), (1, 2), (2, 0))], -1)
+    shape = np.concatenate((bary * (2.0 * bary - 1.0), edges), axis=1)
+    area = 0.37
+    expected = 2.0 * area * np.einsum("q,qi,qj->ij", wts, shape, shape)  # reference triangle has area 1/2
+    assert relmax(fem.p2_face_mass(area), expected) < 1e-13
+    p1 = 2.0 * area * np.einsum("q,qi,qj->ij", wts, bary, bary)
+    assert relmax(area * fem.P1_FACE_MASS, p1) < 1e-14
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_robin_3d_faces_enter_the_templates_as_a_surface_integral(order):
+    """Every cell's Robin term equals ``sum_f alpha_f int_f phi_i phi_j dS`` over its far faces,
+    with the tetrahedron's own shape functions evaluated at points of the face (barycentric
+    coordinates solved from the vertex coordinates, a collapsed Gauss rule on the face): this
+    pins the face mass, its area scaling and the face-to-cell local DOF mapping together."""
+
+    nodes, tets = adt.cases.box_mesh(relief=1.0)
+    quads = osurvey.wenner_alpha(5)
+    line = np.flatnonzero(np.isclose(nodes[:, 1], 6.0))
+    top = [line[np.isclose(nodes[line, 0], x)][np.argmax(nodes[line[np.isclose(nodes[line, 0], x)], 2])]
+           for x in np.unique(nodes[line, 0])[1:6]]  # the surface node of five columns
+    f = ert3d.Forward3D(nodes, tets, nodes[top], quads, element_order=order)
+    m = f.mesh
+    far = np.flatnonzero(~m["surface_face_mask"])
+    alpha = fem.robin_coefficient_3d(m["boundary_face_centers"][far], m["boundary_face_normals"][far],
+                                     f.positions.mean(axis=0))
+    pts, wts = _duffy_triangle(6)
+    tri = np.column_stack((1.0 - pts.sum(axis=1), pts))
+    expected = np.zeros_like(f.cell_t)
+    for face, a in zip(far, alpha):
+        cell = int(m["boundary_face_cells"][face])
+        corners = m["nodes"][m["boundary_faces"][face]]
+        x = tri @ corners  # points on the face
+        verts = m["nodes"][m["cells"][cell]]
+        jac = np.column_stack([verts[i] - verts[0] for i in (1, 2, 3)])
+        lam = np.linalg.solve(jac, (x - verts[0]).T).T
+        bary = np.column_stack((1.0 - lam.sum(axis=1), lam))
+        shape = fem.tet_p2_shape(bary) if order == 2 else bary
+        area = 0.5 * np.linalg.norm(np.cross(corners[1] - corners[0], corners[2] - corners[0]))
+        expected[cell] += a * 2.0 * area * np.einsum("q,qi,qj->ij", wts, shape, shape)
+    robin = f.cell_t - f.volume_t
+    assert relmax(robin, expected) < 1e-12
+    assert np.count_nonzero(np.abs(robin).sum(axis=(1, 2))) == np.unique(m["boundary_face_cells"][far[alpha > 0]]).size
+
+
 # ---------------------------------------------------------------- operators
 
 
