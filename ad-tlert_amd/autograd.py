@@ -1,9 +1,18 @@
 """PyTorch autograd bridge for the 2.5D and 3D operators.
 
-Same contract as ``adtlert/forward/autograd.py:17-207``: forward returns apparent resistivity
-of one model ``(n_cells,)`` or a series ``(n_steps, n_cells)``; reverse mode calls
-``vjp_series`` (hand-written adjoint kernels), forward mode calls ``jvp``; gradients flow to
-conductivity and to scalar / per-datum currents; second derivatives are not supported.
+Contract of ``adtlert/forward/autograd.py:17-207``:
+
+* ``apparent_resistivity_autograd(conductivity, forward_operator, currents=1.0)`` maps one model
+  ``(n_cells,)`` or a series ``(n_steps, n_cells)`` to apparent resistivities ``(n_data,)`` or
+  ``(n_steps, n_data)`` on the conductivity's device and dtype;
+* reverse mode goes through the operator's ``vjp_series`` (the native adjoint kernels), forward
+  mode through ``jvp`` per step; conductivity and currents (a scalar, a length-one tensor or one
+  value per datum) both receive derivatives; the solve is opaque, so there are no second
+  derivatives;
+* inputs are validated with the reference's exception types and messages.
+
+The work runs on the host copies in ``FLOAT_DTYPE`` that the operators take.  With
+``rho_a = k R(sigma) / I``: ``d rho_a = k dR / I - rho_a dI / I``.
 """
 
 from __future__ import annotations
@@ -17,84 +26,93 @@ from .dtypes import FLOAT_DTYPE
 from .ert2p5d import ERTForward2p5D
 from .ert3d import ERTForward3D
 
+_OPERATORS = (ERTForward2p5D, ERTForward3D)
 
-def _validate_inputs(conductivity, currents, forward_operator) -> None:
-    if not isinstance(forward_operator, (ERTForward2p5D, ERTForward3D)):
+
+def _host(t: torch.Tensor) -> torch.Tensor:
+    return t.detach().to(device="cpu", dtype=FLOAT_DTYPE)
+
+
+def _check(conductivity, currents, forward_operator) -> None:
+    """Raise the reference's error for the first invalid argument."""
+
+    if not isinstance(forward_operator, _OPERATORS):
         raise TypeError("forward_operator must be an ERTForward2p5D or ERTForward3D instance")
-    if not isinstance(conductivity, torch.Tensor) or not conductivity.is_floating_point():
+    floating = lambda t: isinstance(t, torch.Tensor) and t.is_floating_point()  # noqa: E731
+    if not floating(conductivity):
         raise TypeError("conductivity must be a floating-point torch.Tensor")
     cells = forward_operator.mesh.cell_count
-    if conductivity.ndim not in (1, 2) or conductivity.shape[-1] != cells:
-        raise ValueError(f"conductivity must have shape ({cells},) or (n_steps, {cells}), got {tuple(conductivity.shape)}")
-    if not bool(torch.all(torch.isfinite(conductivity))) or bool(torch.any(conductivity <= 0.0)):
+    if conductivity.dim() not in (1, 2) or int(conductivity.shape[-1]) != cells:
+        raise ValueError(f"conductivity must have shape ({cells},) or (n_steps, {cells}), "
+                         f"got {tuple(conductivity.shape)}")
+    if not bool((torch.isfinite(conductivity) & (conductivity > 0)).all()):
         raise ValueError("conductivity must contain finite positive values")
-    if not isinstance(currents, torch.Tensor) or not currents.is_floating_point():
+    if not floating(currents):
         raise TypeError("currents must be a floating-point torch.Tensor")
-    count = forward_operator.survey.measurement_count
-    if tuple(currents.shape) not in {(), (1,), (count,)}:
-        raise ValueError(f"currents must be scalar or have shape ({count},), got {tuple(currents.shape)}")
-    if not bool(torch.all(torch.isfinite(currents))) or bool(torch.any(currents == 0.0)):
+    data = forward_operator.survey.measurement_count
+    if tuple(currents.shape) not in ((), (1,), (data,)):
+        raise ValueError(f"currents must be scalar or have shape ({data},), got {tuple(currents.shape)}")
+    if not bool((torch.isfinite(currents) & (currents != 0)).all()):
         raise ValueError("currents must contain finite non-zero values")
 
 
-def _reduce_current_gradient(gradient: torch.Tensor, shape: torch.Size) -> torch.Tensor:
-    if len(shape) == 0:
-        return gradient.sum()
-    if tuple(shape) == (1,):
-        return gradient.sum().reshape(shape)
-    return gradient
+def _steps(t: torch.Tensor, width: int) -> torch.Tensor:
+    """A model or a series as ``(n_steps, width)`` rows."""
+
+    return t.reshape(-1, width)
 
 
 class ERT2p5DApparentResistivityFunction(torch.autograd.Function):
-    """Matrix-free 2.5D/3D apparent resistivity as a differentiable torch operation."""
+    """Differentiable apparent resistivity of a 2.5D or 3D operator (one model or a series)."""
 
     @staticmethod
     def forward(ctx: Any, conductivity, currents, forward_operator):
-        _validate_inputs(conductivity, currents, forward_operator)
-        sigma = conductivity.detach().to(device="cpu", dtype=FLOAT_DTYPE)
-        cur = currents.detach().to(device="cpu", dtype=FLOAT_DTYPE)
-        scale = forward_operator._geometric_factors().detach()
-        resistance = forward_operator.resistance_series(sigma.reshape(-1, sigma.shape[-1])).reshape(
-            *sigma.shape[:-1], -1)
-        rhoa = scale * resistance / cur
-        ctx.forward_operator = forward_operator
-        ctx.conductivity_device, ctx.conductivity_dtype = conductivity.device, conductivity.dtype
-        ctx.currents_device, ctx.currents_dtype, ctx.currents_shape = currents.device, currents.dtype, currents.shape
-        saved = (sigma, cur, rhoa, scale)
-        ctx.save_for_backward(*saved)
-        ctx.save_for_forward(*saved)
+        _check(conductivity, currents, forward_operator)
+        cells = conductivity.shape[-1]
+        sigma, current = _host(conductivity), _host(currents)
+        k = forward_operator._geometric_factors().detach()
+        resistance = forward_operator.resistance_series(_steps(sigma, cells))
+        rhoa = (k * resistance / current).reshape(*sigma.shape[:-1], -1)
+        ctx.op = forward_operator
+        ctx.sigma_meta = (conductivity.device, conductivity.dtype)
+        ctx.current_meta = (currents.device, currents.dtype, tuple(currents.shape))
+        ctx.save_for_backward(sigma, current, rhoa, k)
+        ctx.save_for_forward(sigma, current, rhoa, k)
         return rhoa.to(device=conductivity.device, dtype=conductivity.dtype)
 
     @staticmethod
     @once_differentiable
     def backward(ctx: Any, grad_output):
-        sigma, cur, rhoa, scale = ctx.saved_tensors
-        g = grad_output.detach().to(device="cpu", dtype=FLOAT_DTYPE)
-        grad_sigma = grad_cur = None
+        sigma, current, rhoa, k = ctx.saved_tensors
+        g = _host(grad_output)
+        cells = sigma.shape[-1]
+        grad_sigma = grad_current = None
         if ctx.needs_input_grad[0]:
-            cot = g * scale / cur
-            grad_sigma = ctx.forward_operator.vjp_series(sigma.reshape(-1, sigma.shape[-1]),
-                                                         cot.reshape(-1, cot.shape[-1])).reshape(sigma.shape)
-            grad_sigma = grad_sigma.to(device=ctx.conductivity_device, dtype=ctx.conductivity_dtype)
+            cotangents = _steps(g * k / current, rhoa.shape[-1])
+            grad_sigma = ctx.op.vjp_series(_steps(sigma, cells), cotangents).reshape(sigma.shape)
+            device, dtype = ctx.sigma_meta
+            grad_sigma = grad_sigma.to(device=device, dtype=dtype)
         if ctx.needs_input_grad[1]:
-            expanded = -g * rhoa / cur
-            grad_cur = _reduce_current_gradient(expanded.reshape(-1, expanded.shape[-1]).sum(0), ctx.currents_shape)
-            grad_cur = grad_cur.to(device=ctx.currents_device, dtype=ctx.currents_dtype)
-        return grad_sigma, grad_cur, None
+            device, dtype, shape = ctx.current_meta
+            per_datum = _steps(-g * rhoa / current, rhoa.shape[-1]).sum(dim=0)  # summed over steps
+            grad_current = per_datum.sum().reshape(shape) if shape in ((), (1,)) else per_datum
+            grad_current = grad_current.to(device=device, dtype=dtype)
+        return grad_sigma, grad_current, None
 
     @staticmethod
     def jvp(ctx: Any, conductivity_tangent, currents_tangent, forward_operator_tangent):
         del forward_operator_tangent
-        sigma, cur, rhoa, scale = ctx.saved_tensors
+        sigma, current, rhoa, k = ctx.saved_tensors
+        cells = sigma.shape[-1]
         tangent = torch.zeros_like(rhoa)
         if conductivity_tangent is not None:
-            direction = conductivity_tangent.detach().to(device="cpu", dtype=FLOAT_DTYPE)
-            rt = torch.stack([ctx.forward_operator.jvp(m, d) for m, d in
-                              zip(sigma.reshape(-1, sigma.shape[-1]), direction.reshape(-1, sigma.shape[-1]))])
-            tangent = tangent + scale * rt.reshape(rhoa.shape) / cur
+            models, directions = _steps(sigma, cells), _steps(_host(conductivity_tangent), cells)
+            dR = torch.stack([ctx.op.jvp(m, d) for m, d in zip(models, directions)]).reshape(rhoa.shape)
+            tangent += k * dR / current
         if currents_tangent is not None:
-            tangent = tangent - rhoa * currents_tangent.detach().to(device="cpu", dtype=FLOAT_DTYPE) / cur
-        return tangent.to(device=ctx.conductivity_device, dtype=ctx.conductivity_dtype)
+            tangent -= rhoa * _host(currents_tangent) / current
+        device, dtype = ctx.sigma_meta
+        return tangent.to(device=device, dtype=dtype)
 
 
 ERT3DApparentResistivityFunction = ERT2p5DApparentResistivityFunction
@@ -102,7 +120,8 @@ ERTApparentResistivityFunction = ERT2p5DApparentResistivityFunction
 
 
 def apparent_resistivity_autograd(conductivity: torch.Tensor, forward_operator, currents=1.0) -> torch.Tensor:
-    """Differentiable apparent resistivity (``forward/autograd.py:188-207``)."""
+    """Differentiable apparent resistivity (``forward/autograd.py:188-207``); ``currents`` is
+    converted to the conductivity's device and dtype."""
 
     if not isinstance(conductivity, torch.Tensor):
         raise TypeError("conductivity must be a torch.Tensor")

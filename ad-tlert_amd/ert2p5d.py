@@ -212,21 +212,31 @@ def jacobian_rows_per_launch(rows: int, cells: int) -> int:
 def chunked_rows(count: int, cells: int, batch_size: int, launch, columns, device, return_device: bool) -> Tensor:
     """Run ``launch(start, stop, part)`` for consecutive chunks of ``batch_size`` measurement
     rows into a device ``(rows, cells)`` block, aggregate its columns, and gather the rows on the
-    host in ``FLOAT_DTYPE`` (or on the device with ``return_device``)."""
+    host in ``FLOAT_DTYPE`` (or on the device with ``return_device``).  Without aggregation the
+    kernel writes straight into the result block."""
 
-    if count <= batch_size or return_device:
-        result = torch.empty((count, columns.count), dtype=torch.float64, device=device)
-        for start in range(0, count, batch_size):
-            stop = min(count, start + batch_size)
-            part = torch.empty((stop - start, cells), dtype=torch.float64, device=device)
-            launch(start, stop, part)
-            result[start:stop] = columns.columns(part)
+    f64 = dict(dtype=torch.float64, device=device)
+    if count <= batch_size:
+        part = torch.empty((count, cells), **f64)
+        launch(0, count, part)
+        result = columns.columns(part)
         if return_device:
             return result
         out = result.to(device="cpu", dtype=FLOAT_DTYPE)
+    elif return_device:
+        result = torch.empty((count, columns.count), **f64)
+        scratch = None if columns.identity else torch.empty((batch_size, cells), **f64)
+        for start in range(0, count, batch_size):
+            stop = min(count, start + batch_size)
+            if scratch is None:
+                launch(start, stop, result[start:stop])
+            else:
+                launch(start, stop, scratch[: stop - start])
+                result[start:stop] = columns.columns(scratch[: stop - start])
+        return result
     else:
         out = torch.empty((count, columns.count), dtype=FLOAT_DTYPE)
-        part = torch.empty((batch_size, cells), dtype=torch.float64, device=device)
+        part = torch.empty((batch_size, cells), **f64)
         for start in range(0, count, batch_size):
             stop = min(count, start + batch_size)
             launch(start, stop, part[: stop - start])

@@ -1,6 +1,13 @@
-"""ABMN survey: electrode positions, measurement slots with a remote electrode, geometric factors.
+"""ABMN surveys: electrode positions, measurement slots with a remote electrode, geometric factors.
 
-Same interface and error behaviour as ``adtlert/survey/core.py:12-120``.
+Interface of ``adtlert/survey/core.py:12-120``: ``Survey.from_arrays`` raises ``ValueError``
+with the reference's messages (callers and tests match on them), ``measurement_slots`` maps
+the remote electrode to one slot past the last electrode, and ``geometric_factors`` is the
+analytic half-space factor ``2 pi / (1/AM - 1/AN - 1/BM + 1/BN)`` with remote terms dropped.
+
+Geometric factors are formed here from per-slot positions in which the remote slot holds NaN:
+every inverse distance involving it is NaN and is replaced by zero, so the four terms need no
+case analysis.
 """
 
 from __future__ import annotations
@@ -17,14 +24,44 @@ REMOTE_ELECTRODE = -1
 
 
 def with_remote_slot(potentials: torch.Tensor) -> torch.Tensor:
-    """Append the zero row and column of the remote slot to ``(..., sources, receivers)``."""
+    """``(..., sources, receivers)`` potentials with a zero last row and column: the remote slot."""
 
     return torch.nn.functional.pad(potentials, (0, 1, 0, 1))
 
 
+def _validation_error(positions: torch.Tensor, quads: torch.Tensor) -> str | None:
+    """First rule of ``Survey.from_arrays`` that the arrays break, or None."""
+
+    if positions.dim() != 2 or int(positions.shape[-1]) not in (2, 3):
+        return "electrode_positions must have shape (num_electrodes, 2 or 3)"
+    if quads.dim() != 2 or int(quads.shape[-1]) != 4:
+        return "measurements must have shape (num_measurements, 4)"
+    if quads.numel() == 0:
+        return None
+    if int(quads.min()) < REMOTE_ELECTRODE:
+        return f"measurements contain negative electrode indices other than {REMOTE_ELECTRODE} (remote)"
+    if int(quads.max()) >= int(positions.shape[0]):
+        return "measurements reference electrodes outside the survey"
+    remote = quads == REMOTE_ELECTRODE
+    current_at_infinity = remote[:, 0] & remote[:, 1]
+    potential_at_infinity = remote[:, 2] & remote[:, 3]
+    if bool((current_at_infinity | potential_at_infinity).any()):
+        return "each current (AB) and potential (MN) pair needs an electrode in the survey"
+    # remote entries become distinct negative labels, so only real electrodes can collide
+    labels = torch.where(remote, -2 - torch.arange(4, dtype=quads.dtype), quads)
+    collide = torch.zeros(quads.shape[0], dtype=torch.bool)
+    for i in range(4):
+        for j in range(i + 1, 4):
+            collide |= labels[:, i] == labels[:, j]
+    if bool(collide.any()):
+        return "each ABMN measurement must use four distinct electrodes"
+    return None
+
+
 @dataclass(frozen=True)
 class Survey:
-    """Electrode geometry and ABMN measurement indexing (``survey/core.py:23-120``)."""
+    """Electrode coordinates ``(E, 2 or 3)`` and ABMN rows ``(D, 4)``; ``REMOTE_ELECTRODE``
+    places B and/or N at infinity (pole-dipole, dipole-pole, pole-pole)."""
 
     electrode_positions: torch.Tensor
     measurements: torch.Tensor
@@ -33,20 +70,9 @@ class Survey:
     def from_arrays(cls, electrode_positions, measurements) -> "Survey":
         positions = torch.as_tensor(electrode_positions, dtype=FLOAT_DTYPE)
         quads = torch.as_tensor(measurements, dtype=INT_DTYPE)
-        if positions.ndim != 2 or positions.shape[1] not in (2, 3):
-            raise ValueError("electrode_positions must have shape (num_electrodes, 2 or 3)")
-        if quads.ndim != 2 or quads.shape[1] != 4:
-            raise ValueError("measurements must have shape (num_measurements, 4)")
-        if bool(torch.any(quads < REMOTE_ELECTRODE)):
-            raise ValueError(f"measurements contain negative electrode indices other than {REMOTE_ELECTRODE} (remote)")
-        if quads.numel() and bool(torch.any(quads >= positions.shape[0])):
-            raise ValueError("measurements reference electrodes outside the survey")
-        remote = quads == REMOTE_ELECTRODE
-        if bool(torch.any(remote[:, :2].all(dim=1) | remote[:, 2:].all(dim=1))):
-            raise ValueError("each current (AB) and potential (MN) pair needs an electrode in the survey")
-        distinct = torch.where(remote, -1 - torch.arange(4, dtype=quads.dtype), quads)
-        if bool(torch.any(torch.diff(torch.sort(distinct, dim=1).values, dim=1) == 0)):
-            raise ValueError("each ABMN measurement must use four distinct electrodes")
+        problem = _validation_error(positions, quads)
+        if problem is not None:
+            raise ValueError(problem)
         return cls(electrode_positions=positions, measurements=quads)
 
     @property
@@ -62,25 +88,24 @@ class Survey:
         return int(self.electrode_positions.shape[1])
 
     def measurement_slots(self) -> torch.Tensor:
-        """ABMN with the remote electrode mapped to slot ``electrode_count``."""
+        """``(D, 4)`` int64 slots: electrode indices, the remote electrode as ``electrode_count``."""
 
-        quads = self.measurements.long()
-        return torch.where(quads == REMOTE_ELECTRODE, self.electrode_count, quads)
+        slots = self.measurements.to(torch.int64)
+        return slots.masked_fill(slots == REMOTE_ELECTRODE, self.electrode_count)
 
     def geometric_factors(self) -> torch.Tensor:
-        """``2 pi / (1/AM - 1/AN - 1/BM + 1/BN)``; terms with a remote electrode vanish."""
+        """Analytic half-space factor ``2 pi / (1/AM - 1/AN - 1/BM + 1/BN)``, remote terms zero."""
 
-        quads = self.measurements.long()
-        points = self.electrode_positions[quads.clamp_min(0)]
-        real = quads != REMOTE_ELECTRODE
-
-        def inverse_distance(p: int, q: int) -> torch.Tensor:
-            distance = torch.linalg.norm(points[:, p] - points[:, q], dim=-1)
-            return torch.where(real[:, p] & real[:, q], 1.0 / distance, 0.0)
-
-        return 2.0 * math.pi / (inverse_distance(0, 2) - inverse_distance(0, 3) - inverse_distance(1, 2)
-                                + inverse_distance(1, 3))
+        remote_row = torch.full((1, self.dimension), float("nan"), dtype=self.electrode_positions.dtype)
+        table = torch.cat((self.electrode_positions, remote_row))  # slot E: the electrode at infinity
+        at = table[self.measurement_slots()]  # (D, 4, dim): A, B, M, N
+        inverse = 1.0 / torch.linalg.norm(at[:, :2, None, :] - at[:, None, 2:, :], dim=-1)  # (D, [A, B], [M, N])
+        inverse = torch.where(torch.isnan(inverse), torch.zeros_like(inverse), inverse)
+        coupling = (inverse[:, 0, 0] - inverse[:, 0, 1]) - (inverse[:, 1, 0] - inverse[:, 1, 1])
+        return 2.0 * math.pi / coupling
 
     def apparent_resistivity(self, voltages, currents=1.0) -> torch.Tensor:
-        voltages = torch.as_tensor(voltages, dtype=FLOAT_DTYPE)
-        return self.geometric_factors() * voltages / torch.as_tensor(currents, dtype=FLOAT_DTYPE)
+        """``k * V / I``."""
+
+        k = self.geometric_factors()
+        return k * torch.as_tensor(voltages, dtype=FLOAT_DTYPE) / torch.as_tensor(currents, dtype=FLOAT_DTYPE)
