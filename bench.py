@@ -1,14 +1,15 @@
 #!/usr/bin/env python
 """Throughput of the MI355X ERT forward-and-sensitivity path on one GPU.
 
-Workloads (``--workload``):
+Workloads (``--workload``; by default both, the time-lapse batch as the top-level result and the
+3D box under ``box3d``):
 
-* ``timelapse`` (default): a ~20k-node graded P1 triangle mesh, 96 surface electrodes at 1 m,
-  ~3000 Wenner-Schlumberger quadrupoles, the reference wavenumber rule (19 wavenumbers for
-  this survey) and a seeded 365-step log-conductivity series.  One timed step solves every
+* ``timelapse``: a ~20k-node graded P1 triangle mesh, 96 surface electrodes at 1 m, ~3000
+  Wenner-Schlumberger quadrupoles, the reference wavenumber rule (19 wavenumbers for this
+  survey) and a seeded 365-step log-conductivity series.  One timed step solves every
   right-hand side (wavenumber x source electrode) of ``--batch-steps`` consecutive time steps.
-* ``box3d``: a 200 m x 200 m x 100 m box of ~179k nodes / ~1.0M P1 tetrahedra refined under
-  an 8 x 8 electrode grid at 5 m, 2000 random quadrupoles, mixed boundary, analytic half-space
+* ``box3d``: a 200 m x 200 m x 100 m box of 202 005 nodes / 1.15M P1 tetrahedra refined under an
+  8 x 8 electrode grid at 5 m, 2000 random quadrupoles, mixed boundary, analytic half-space
   singularity removal, log-normal conductivity fields.  One timed step solves the 64 source
   right-hand sides of ``--batch-steps`` conductivity models.
 
@@ -16,17 +17,19 @@ Every step solves to a relative residual of 1e-10 and reduces the fields to appa
 resistivities.  The main solver (``--preconditioner``, default ``learned``: flexible CG
 preconditioned by the multigrid cycle whose finest prolongator is damped by the trained graph
 network), the same cycle with constant damping (``amg``) and plain Jacobi PCG are timed on the
-same batches.  After the timed steps a
-second timed phase materializes the full normal-sensitivity Jacobian (all quadrupole rows x
-all cells) of one model with the fused kernel.
+same batches.  After the timed steps a second timed phase materializes the full
+normal-sensitivity Jacobian (all quadrupole rows x all cells) of one model with the fused kernel.
 
-Prints ONE JSON line.  ``value`` = RHS solves/s of the main solver over the timed steps (wall
-clock between two synchronizations, graph-replayed iterations, the network's inference
-included); ``solver.rhs_solves_per_s_pcg`` is plain Jacobi PCG and
-``solver.rhs_solves_per_s_amg`` the constant-damping cycle; ``jacobian_rows_per_s`` is the second phase.  Kernel rooflines come from
-one more step of the same workload run with the solver's profiling mode: the same kernels
-launched directly on the same stream with HIP timing events around every multigrid cycle,
-spmv and update launch, so each kernel's average duration is measured live.
+Prints ONE JSON line.  ``value`` = RHS solves/s of the main solver over the timed steps of the
+time-lapse batch (wall clock between two synchronizations, graph-replayed iterations, the
+network's inference included); ``solver.rhs_solves_per_s_pcg`` is plain Jacobi PCG and
+``solver.rhs_solves_per_s_amg`` the constant-damping cycle; ``jacobian_rows_per_s`` is the
+second phase.  Kernel rooflines come from one more step of the same workload run with the
+solver's profiling mode: the same kernels launched directly on the same stream with HIP timing
+events around every multigrid cycle, spmv and update launch, so each kernel's average duration
+is measured live.  ``oracle`` gives the maximum relative error of apparent resistivities and
+Jacobian rows against the FP64 CPU oracle; ``cpu_baseline`` times the oracle (threaded sparse
+LU per (time step, wavenumber) in 2.5D; threaded block Jacobi-PCG in 3D) on the host's cores.
 """
 
 from __future__ import annotations
@@ -52,24 +55,98 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, series, wavenumbers, weights, sample_steps: int, rows: int):
-    """FP64 CPU oracle (scipy splu direct solves, the restatement of the reference path) on a
-    bounded sample: ``sample_steps`` time steps (all wavenumbers x sources) and ``rows`` Jacobian rows."""
+def cpu_threads() -> int:
+    """Host cores for the CPU baseline: this process's affinity set, at most 16 (a one-GPU share)."""
+
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def relmax(actual, expected) -> float:
+    import numpy as np
+
+    actual, expected = np.asarray(actual, dtype=float), np.asarray(expected, dtype=float)
+    return float(np.abs(actual - expected).max() / max(np.abs(expected).max(), 1e-300))
+
+
+_ORACLE = {}
+
+
+def _oracle_worker_init(case, wavenumbers, weights, models) -> None:
+    """Per-process oracle of the CPU baseline's worker pool (built once per worker)."""
+
+    import numpy as np
+    from threadpoolctl import threadpool_limits
 
     from oracle import ert2p5d as oracle2p5d
 
+    # one core per worker: BLAS/OpenMP pools of their own would oversubscribe the host
+    _ORACLE["limits"] = threadpool_limits(1)
     f = oracle2p5d.Forward2p5D(case.nodes, case.cells, case.electrodes, case.measurements, None, wavenumbers, weights)
-    t0 = time.perf_counter()
+    _ORACLE.update(f=f, models=models, rho=[f.source_resistivities(m) for m in models])
+
+
+def _oracle_worker_ready(_) -> bool:
+    return "f" in _ORACLE
+
+
+def _oracle_worker_system(task):
+    """One (time step, wavenumber) system: sparse LU of ``A_w(sigma_t)``, every source solved by
+    the secondary-field split, reduced to the weighted ``(E, E)`` electrode potentials."""
+
+    import numpy as np
+    import scipy.sparse.linalg as spla
+
+    t, w = task
+    f, rho = _ORACLE["f"], _ORACLE["rho"][t]
+    a = f.matrix(_ORACLE["models"][t], w).tocsc()
+    x0 = f.primary[w] * rho[:, None]
+    rhs = f.b[w] - (a @ x0.T).T
+    total = spla.splu(a).solve(np.asfortranarray(rhs.T)).T + x0
+    return t, f.q[w] * (total @ f.electrodes.T)
+
+
+def timelapse_cpu(case, series, wavenumbers, weights, sample_steps: int, rows, workers: int):
+    """FP64 CPU oracle on a bounded sample: every (time step, wavenumber) system of
+    ``sample_steps`` steps factorized by scipy ``splu`` and solved for all sources (the
+    restatement of the reference's direct solve), one task per system on a pool of ``workers``
+    processes (each builds the oracle once before the timed region), reduced to apparent
+    resistivities; then ``rows`` Jacobian rows of step 0 by the oracle's einsum (one core)."""
+
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+
+    import numpy as np
+
+    from oracle import ert2p5d as oracle2p5d
+
+    models = np.ascontiguousarray(series[:sample_steps])
+    f = oracle2p5d.Forward2p5D(case.nodes, case.cells, case.electrodes, case.measurements, None, wavenumbers,
+                               weights)
+    tasks = [(t, w) for t in range(sample_steps) for w in range(f.W)]
+    potentials = np.zeros((sample_steps, f.E, f.E))
+    ctx = multiprocessing.get_context("spawn")  # the parent holds a GPU context: no fork
+    with ProcessPoolExecutor(workers, mp_context=ctx, initializer=_oracle_worker_init,
+                             initargs=(case, wavenumbers, weights, models)) as pool:
+        assert all(pool.map(_oracle_worker_ready, range(4 * workers)))  # every worker has built its oracle
+        t0 = time.perf_counter()
+        for t, part in pool.map(_oracle_worker_system, tasks):
+            potentials[t] += part
+        solve_s = time.perf_counter() - t0
+    k = f.geometric_factors()
+    rhoa = []
     for t in range(sample_steps):
-        f.fields(series[t])
-    solve_s = time.perf_counter() - t0
-    rhs = sample_steps * f.W * f.E
-    f.slots = f.slots[:rows]
-    f.D = rows
+        p = np.zeros((f.E + 1, f.E + 1))
+        p[: f.E, : f.E] = potentials[t]
+        a, b, m, n = f.slots.T
+        normal = (p[a, m] - p[a, n]) - (p[b, m] - p[b, n])
+        reciprocal = (p[m, a] - p[m, b]) - (p[n, a] - p[n, b])
+        rhoa.append(k * np.sign(normal) * np.sqrt(np.abs(normal * reciprocal)))
+    f.fields(series[0])  # the Jacobian's fields (not timed)
     t0 = time.perf_counter()
-    f.jacobian(series[sample_steps - 1])
+    jac = f.jacobian(series[0], rows=rows, chunk=16)
     jac_s = time.perf_counter() - t0
-    return rhs / solve_s, rows / jac_s, solve_s, jac_s
+    return dict(rhs_per_s=len(tasks) * f.E / solve_s, solve_s=solve_s, rows_per_s=len(rows) / jac_s, jac_s=jac_s,
+                rhoa=np.stack(rhoa), jacobian=jac, systems=len(tasks), sources=f.E)
 
 
 class Timelapse:
@@ -122,24 +199,43 @@ class Timelapse:
         flops = self.C * (self.W * K * S * (S + 1) + 2 * self.W * K * K * S) + 3 * self.D * self.C
         return run, bytes_, flops, "gram_kernel<6> (Gram-form sensitivity)", {"bound": "fp64"}
 
-    def cpu(self):
-        args = self.args
+    def cpu_and_oracle(self, jac_device):
+        """CPU baseline (oracle on the host cores) and the GPU's error against it on the same
+        models: apparent resistivities of the sampled steps and Jacobian rows of step 0."""
+
+        import numpy as np
+
+        args, op, torch = self.args, self.op, self.torch
         if args.cpu_steps <= 0:
-            return None
-        op = self.op
+            return None, None
+        rng = np.random.default_rng(7)
+        rows = np.sort(rng.choice(self.D, min(args.cpu_rows, self.D), replace=False))
+        threads = cpu_threads()
         wn, wt = op.wavenumbers.double().numpy(), op.weights.double().numpy()
-        rhs, rows, solve_s, jac_s = cpu_baseline(self.case, self.series, wn, wt, args.cpu_steps, args.cpu_rows)
-        return {"value": rhs, "unit": "RHS solves/s", "cores": 1, "kind": "port",
-                "sample": f"{args.cpu_steps} time steps x {self.W} wavenumbers x {self.E} sources, scipy splu FP64 "
-                          f"({solve_s:.1f}s); Jacobian {args.cpu_rows} rows x {self.C} cells einsum ({jac_s:.1f}s)",
-                "jacobian_rows_per_s": rows, "cpu": platform.processor() or platform.machine(),
-                "host_cores_available": len(os.sched_getaffinity(0))}
+        cpu = timelapse_cpu(self.case, self.series, wn, wt, args.cpu_steps, rows, threads)
+        op._field_cache.clear()
+        gpu_rhoa = op.apparent_resistivity_series(torch.as_tensor(self.series[: args.cpu_steps])).double().numpy()
+        oracle = {"rhoa_max_rel_err": relmax(gpu_rhoa, cpu["rhoa"]),
+                  "rhoa_sample": f"{args.cpu_steps} time steps x {self.D} quadrupoles"}
+        if jac_device is not None:
+            idx = torch.as_tensor(rows, dtype=torch.long, device=jac_device.device)
+            oracle["jacobian_max_rel_err"] = relmax(jac_device[idx].cpu().numpy(), cpu["jacobian"])
+            oracle["jacobian_sample"] = f"{len(rows)} seeded rows x {self.C} cells of time step 0"
+        baseline = {"value": cpu["rhs_per_s"], "unit": "RHS solves/s", "cores": threads, "kind": "port",
+                    "sample": f"{args.cpu_steps} time steps x {self.W} wavenumbers x {self.E} sources: "
+                              f"{cpu['systems']} scipy splu FP64 factorizations + solves on a pool of {threads} "
+                              f"processes ({cpu['solve_s']:.1f}s); Jacobian {len(rows)} rows x {self.C} cells, "
+                              f"oracle einsum on one core ({cpu['jac_s']:.1f}s)",
+                    "jacobian_rows_per_s": cpu["rows_per_s"], "jacobian_cores": 1,
+                    "cpu": platform.processor() or platform.machine(),
+                    "host_cores_available": len(os.sched_getaffinity(0))}
+        return baseline, oracle
 
 
 class Box3D:
     """3D P1 box workload: batches of conductivity models, 64 source right-hand sides each."""
 
-    name = "3D P1 box: ~179k nodes / ~1.0M tetrahedra, 8x8 electrodes at 5 m, 2000 random quadrupoles"
+    name = "3D P1 box: 202 005 nodes / 1.15M tetrahedra, 8x8 electrodes at 5 m, 2000 random quadrupoles"
     data = "synthetic (seeded graded tetrahedral box, random ABMN survey, log-normal GRF conductivity models)"
 
     def __init__(self, adt, torch, args, models: int):
@@ -164,7 +260,8 @@ class Box3D:
         g2 = np.log10(adt.cases.grf_3d(centers, rng, mean=0.0, std=1.0))
         t = np.arange(max(models, 2))[:, None]
         logs = np.log10(self.case.conductivity)[None] + 0.2 * (np.sin(0.7 * t) * g1[None] + np.cos(0.3 * t) * g2[None])
-        self.sig = torch.as_tensor(10.0 ** logs, dtype=torch.float64, device=adt.native.device())
+        self.series_host = 10.0 ** logs
+        self.sig = torch.as_tensor(self.series_host, dtype=torch.float64, device=adt.native.device())
         self.config = {"nodes": self.N, "cells": self.C, "nnz": self.nnz, "electrodes": self.E,
                        "quadrupoles": self.D, "source_rhs_per_model": self.op._dev.nc,
                        "models_per_step": args.batch_steps, "rhs_per_step": self.rhs_per_step,
@@ -199,8 +296,49 @@ class Box3D:
         return run, bytes_, flops, "cell_block_kernel<32, 1024, 4> (cell-block sensitivity)", {
             "bound": "hbm", "lds_bytes_per_launch": lds_bytes}
 
-    def cpu(self):
-        return None  # a 3D direct factorization of this size does not fit a bounded CPU sample
+    def cpu_and_oracle(self, jac_device):
+        """CPU baseline: the oracle's matrix solved for ``--cpu-rhs-3d`` of the 64 source
+        right-hand sides of model 0 by block Jacobi-PCG on the host cores (the oracle's direct
+        factorization takes minutes on this mesh).  Oracle errors: the GPU's apparent
+        resistivities and sampled Jacobian rows of the seeded model against the committed
+        fixture the oracle's direct solves wrote (``tests/golden/box_p1_200k.npz``)."""
+
+        import numpy as np
+
+        from oracle import cpu_pcg, ert3d
+
+        args, op, torch = self.args, self.op, self.torch
+        oracle = None
+        fixture = ROOT / "tests" / "golden" / "box_p1_200k.npz"
+        if fixture.exists():
+            data = np.load(fixture, allow_pickle=False)
+            sigma = torch.as_tensor(self.case.conductivity)
+            rhoa = op.apparent_resistivity_values(sigma).double().numpy()
+            jac = op.jacobian(sigma, return_device=True)
+            rows = torch.as_tensor(data["jacobian_rows"], dtype=torch.long, device=jac.device)
+            cells = torch.as_tensor(data["jacobian_cells"], dtype=torch.long, device=jac.device)
+            oracle = {"rhoa_max_rel_err": relmax(rhoa, data["rhoa"]),
+                      "rhoa_sample": f"{self.D} quadrupoles of the seeded model (oracle: sparse LU, committed fixture)",
+                      "jacobian_max_rel_err": relmax(jac[rows][:, cells].cpu().numpy(), data["jacobian"]),
+                      "jacobian_sample": f"{len(data['jacobian_rows'])} rows x {len(data['jacobian_cells'])} cells "
+                                         "(the largest entries plus seeded cells)"}
+            del jac
+        if args.cpu_rhs_3d <= 0:
+            return None, oracle
+        threads = cpu_threads()
+        ref = ert3d.Forward3D(self.case.nodes, self.case.cells, self.case.electrodes, self.case.measurements)
+        matrix = ref._assemble_matrix(self.series_host[0]).tocsr()
+        rhs = np.ascontiguousarray(ref.source_rhs[: args.cpu_rhs_3d].T)
+        t0 = time.perf_counter()
+        _, iterations = cpu_pcg.jacobi_pcg(matrix, rhs, tol=args.tol, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        baseline = {"value": args.cpu_rhs_3d / cpu_s, "unit": "RHS solves/s", "cores": threads, "kind": "port",
+                    "sample": f"{args.cpu_rhs_3d} of the 64 source right-hand sides of model 0, block Jacobi-PCG "
+                              f"FP64 to {args.tol:g} ({int(iterations.mean())} iterations; torch CPU sparse products "
+                              f"on {threads} threads, {cpu_s:.1f}s)",
+                    "cpu": platform.processor() or platform.machine(),
+                    "host_cores_available": len(os.sched_getaffinity(0))}
+        return baseline, oracle
 
 
 def main() -> int:
@@ -208,14 +346,18 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("timelapse", "box3d"), default="timelapse")
+    ap.add_argument("--workload", choices=("all", "timelapse", "box3d"), default="all",
+                    help="all: the time-lapse batch as the top-level result and the 3D box under 'box3d'")
     ap.add_argument("--batch-steps", type=int, default=None,
                     help="time steps (timelapse, default 4) or models (box3d, default 2) per timed step")
     ap.add_argument("--nodes", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check-every", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="time steps in the CPU-baseline sample (0: skip)")
-    ap.add_argument("--cpu-rows", type=int, default=32, help="Jacobian rows in the CPU-baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="time steps in the 2.5D CPU-baseline sample and its oracle comparison (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=32, help="Jacobian rows in the 2.5D CPU-baseline sample")
+    ap.add_argument("--cpu-rhs-3d", type=int, default=16,
+                    help="source right-hand sides in the 3D CPU-baseline sample (0: skip)")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--rows-per-tile", type=int, default=None, help="PCG workgroup tile height (default: auto)")
     ap.add_argument("--lanes", type=int, default=None, help="lane-group width 8/16/32/64 (default: auto)")
@@ -230,13 +372,31 @@ def main() -> int:
               "(the path shards by wavenumber/time step with one tiny all-reduce; see the design notes)",
               file=sys.stderr)
         return 2
-    if args.batch_steps is None:
-        args.batch_steps = 4 if args.workload == "timelapse" else 2
-
-    import numpy as np
     import torch
 
     adt = importlib.import_module("ad-tlert_amd")
+    names = ["timelapse", "box3d"] if args.workload == "all" else [args.workload]
+    results = []
+    for name in names:
+        results.append(measure(adt, torch, args, name))
+        torch.cuda.empty_cache()
+    result = results[0]
+    if len(results) > 1:
+        nested = {k: v for k, v in results[1].items() if k not in ("n_gpus", "steps", "warmup", "higher_is_better",
+                                                                  "scaling", "vs_baseline")}
+        result["box3d"] = nested
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def measure(adt, torch, args, workload: str) -> dict:
+    """One workload's measured result (the JSON object of the bench line)."""
+
+    import numpy as np
+
+    batch_steps = args.batch_steps or (4 if workload == "timelapse" else 2)
+    args = argparse.Namespace(**{**vars(args), "batch_steps": batch_steps, "workload": workload})
+
     dev = adt.native.device()
     cus, arch = adt.native.device_info(0)
     models = (max(args.warmup + args.steps, 1 + args.pcg_steps, 1 + args.amg_steps) + 1) * args.batch_steps
@@ -387,6 +547,7 @@ def main() -> int:
         f"spmv {spmv_gbs:.0f} GB/s ({prof['spmv_ms'] / launches:.3f} ms), "
         f"{update_name} {update_gbs:.0f} GB/s ({prof['update_ms'] / launches:.3f} ms)")
 
+    jac = None
     if not args.no_jacobian:
         run, sens_bytes, sens_flops, kernel, sens_extra = wl.jacobian()
         run()  # warm (the output block then comes from torch's cache in every timed call)
@@ -417,14 +578,20 @@ def main() -> int:
                 result["roofline_sensitivity"]["traffic"] = t["traffic_bytes_per_dispatch"]
                 result["roofline_sensitivity"]["traffic_source"] = f"profiles/{sens_file.name}"
         log(f"jacobian {wl.D}x{wl.C}: {ms:.1f} ms/launch -> {result['jacobian_rows_per_s']:.0f} rows/s")
-        del jac
 
-    baseline = wl.cpu()
+    jac_device = jac if workload == "timelapse" else None
+    jac = None  # the 3D comparison computes its own block; release this one first
+    baseline, oracle = wl.cpu_and_oracle(jac_device)
+    jac_device = None
+    if oracle is not None:
+        result["oracle"] = oracle
+        log(f"oracle: rhoa max rel err {oracle['rhoa_max_rel_err']:.2e}, "
+            f"jacobian max rel err {oracle.get('jacobian_max_rel_err', float('nan')):.2e}")
     if baseline is not None:
         result["cpu_baseline"] = baseline
-        log(f"cpu baseline: {baseline['value']:.0f} RHS/s, {baseline['jacobian_rows_per_s']:.1f} rows/s")
-    print(json.dumps(result), flush=True)
-    return 0
+        log(f"cpu baseline: {baseline['value']:.1f} RHS/s on {baseline['cores']} cores")
+    op.close()
+    return result
 
 
 if __name__ == "__main__":
