@@ -14,6 +14,7 @@
 // partial sums of every outer product, reduced over blocks in a fixed order.  Transposed
 // message and head contributions are gathered through the mirror slot (j, i) of each slot
 // (i, j), so no atomics are used anywhere.
+#include <mutex>
 #include "adt_common.hpp"
 
 #include <cmath>
@@ -715,14 +716,13 @@ extern "C" int adt_gnn_forward(int n_sys, int n, int nnz, const int32_t* d_indpt
     return ADT_OK;
 }
 
-extern "C" int adt_gnn_backward(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
-                                const int32_t* d_rows, const int32_t* d_mirror, double omega_p,
-                                const uint16_t* d_params, const float* d_work, const double* d_domega,
-                                float* d_scratch, float* d_grad, void* stream) {
-    ADT_REQUIRE(n >= 1 && nnz >= n, "adt_gnn_backward: invalid sizes");
-    ADT_REQUIRE(d_indptr && d_indices && d_rows && d_mirror && d_params && d_work && d_domega && d_scratch && d_grad,
-                "adt_gnn_backward: null pointer");
-    hipStream_t st = adt::as_stream(stream);
+namespace {
+
+// every scratch entry the backward kernels read is written earlier in the same call (per-slot
+// and per-node gradients, per-block partials), so the scratch never has to start at zero
+int backward_launch(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices, const int32_t* d_rows,
+                    const int32_t* d_mirror, double omega_p, const uint16_t* d_params, const float* d_work,
+                    const double* d_domega, float* d_scratch, float* d_grad, hipStream_t st) {
     const Graph g = make_graph(n, nnz, d_indptr, d_indices, d_rows, d_mirror, nullptr);
     Scratch S = scratch_at(d_scratch, n, nnz);
     const int hb = head_blocks(nnz), nb = node_blocks(n);
@@ -742,4 +742,67 @@ extern "C" int adt_gnn_backward(int n, int nnz, const int32_t* d_indptr, const i
                        LAYER0_PARAMS, d_grad + OFF_W0);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
+}
+
+}  // namespace
+
+extern "C" int adt_gnn_backward(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
+                                const int32_t* d_rows, const int32_t* d_mirror, double omega_p,
+                                const uint16_t* d_params, const float* d_work, const double* d_domega,
+                                float* d_scratch, float* d_grad, void* stream) {
+    ADT_REQUIRE(n >= 1 && nnz >= n, "adt_gnn_backward: invalid sizes");
+    ADT_REQUIRE(d_indptr && d_indices && d_rows && d_mirror && d_params && d_work && d_domega && d_scratch && d_grad,
+                "adt_gnn_backward: null pointer");
+    return backward_launch(n, nnz, d_indptr, d_indices, d_rows, d_mirror, omega_p, d_params, d_work, d_domega,
+                           d_scratch, d_grad, adt::as_stream(stream));
+}
+
+// Scratch of adt_gnn_backward_static: one library-owned device array per GPU (each device that
+// runs the library holds its own copy until the process ends).  In host code the array's name
+// is only a host-side stand-in; the device address comes from hipGetSymbolAddress, resolved
+// once per device with that device current, and only that address is handed to kernels.
+static __device__ __align__(256) float static_backward_scratch[ADT_GNN_STATIC_SCRATCH_FLOATS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_scratch_mutex;
+float* g_static_scratch[kMaxDevices] = {};
+
+int static_scratch_address(float** out) {
+    int device = -1;
+    ADT_HIP_CHECK(hipGetDevice(&device));
+    ADT_REQUIRE(device >= 0 && device < kMaxDevices, "adt_gnn_backward_static: device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_static_scratch_mutex);
+    if (!g_static_scratch[device]) {
+        void* address = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&address, HIP_SYMBOL(static_backward_scratch));
+        if (err != hipSuccess || !address)
+            return adt::set_error(ADT_ERR_HIP, "adt_gnn_backward_static: hipGetSymbolAddress failed on device %d: %s",
+                                  device, hipGetErrorString(err));
+        g_static_scratch[device] = static_cast<float*>(address);
+    }
+    *out = g_static_scratch[device];
+    return ADT_OK;
+}
+
+}  // namespace
+
+extern "C" int adt_gnn_backward_static(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
+                                       const int32_t* d_rows, const int32_t* d_mirror, double omega_p,
+                                       const uint16_t* d_params, const float* d_work, const double* d_domega,
+                                       float* d_grad, void* stream) {
+    ADT_REQUIRE(n >= 1 && nnz >= n, "adt_gnn_backward_static: invalid sizes");
+    ADT_REQUIRE(d_indptr && d_indices && d_rows && d_mirror && d_params && d_work && d_domega && d_grad,
+                "adt_gnn_backward_static: null pointer");
+    const int64_t need = scratch_floats(n, nnz);
+    ADT_REQUIRE(need <= static_cast<int64_t>(ADT_GNN_STATIC_SCRATCH_FLOATS),
+                "adt_gnn_backward_static: n=%d, nnz=%d need %lld scratch floats, above the static limit of %lld "
+                "(use adt_gnn_backward with a workspace)",
+                n, nnz, static_cast<long long>(need), static_cast<long long>(ADT_GNN_STATIC_SCRATCH_FLOATS));
+    float* scratch = nullptr;
+    const int rc = static_scratch_address(&scratch);
+    if (rc) return rc;
+    return backward_launch(n, nnz, d_indptr, d_indices, d_rows, d_mirror, omega_p, d_params, d_work, d_domega,
+                           scratch, d_grad, adt::as_stream(stream));
 }

@@ -14,7 +14,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().parent / "_lib" / "libadt_hip.so"))
 
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
@@ -48,6 +48,7 @@ _SIGNATURES = {
     "adt_gnn_scratch_floats": (c_i64, [c_int, c_int]),
     "adt_gnn_forward": (c_int, [c_int, c_int, c_int, P, P, P, P, P, c_i64, c_double, P, P, P, P, P]),
     "adt_gnn_backward": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P, P]),
+    "adt_gnn_backward_static": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
                                      P]),
@@ -375,3 +376,16 @@ def gnn_backward(graph, omega_p, params, work, domega, scratch, grad):
     check(library().adt_gnn_backward(graph.n, graph.nnz, ptr(graph.indptr), ptr(graph.indices), ptr(graph.rows),
                                      ptr(graph.mirror), float(omega_p), ptr(params), ptr(work), ptr(domega),
                                      ptr(scratch), ptr(grad), stream()))
+
+
+GNN_STATIC_SCRATCH_FLOATS = 12 * 1024 * 1024
+"""Floats of the library-owned scratch array of ``adt_gnn_backward_static`` (per GPU)."""
+
+
+def gnn_backward_static(graph, omega_p, params, work, domega, grad):
+    """``gnn_backward`` without a workspace: the scratch is the library's static device array
+    (nothing is allocated; calls that overlap on one GPU race)."""
+
+    check(library().adt_gnn_backward_static(graph.n, graph.nnz, ptr(graph.indptr), ptr(graph.indices),
+                                            ptr(graph.rows), ptr(graph.mirror), float(omega_p), ptr(params),
+                                            ptr(work), ptr(domega), ptr(grad), stream()))

@@ -302,6 +302,20 @@ int adt_gnn_backward(int n, int nnz, const int32_t* d_indptr, const int32_t* d_i
                      const int32_t* d_mirror, double omega_p, const uint16_t* d_params, const float* d_work,
                      const double* d_domega, float* d_scratch, float* d_grad, void* stream);
 
+/* No-allocation variant of adt_gnn_backward.  It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory (training steps captured in a HIP graph, loops that
+ * forbid hipMalloc).  Same arguments and results as adt_gnn_backward without d_scratch: the
+ * scratch is a library-owned device array of ADT_GNN_STATIC_SCRATCH_FLOATS floats (48 MiB) per
+ * GPU, held until the process ends; neither the call nor the library allocates device memory.
+ * Sizes whose adt_gnn_scratch_floats(n, nnz) exceeds the array are rejected with ADT_ERR_INVALID.
+ * Every call on one GPU shares that GPU's array (it keeps the last call's contents; nothing in
+ * it has to start at zero): calls that overlap on one GPU, e.g. on two streams, race. */
+#define ADT_GNN_STATIC_SCRATCH_FLOATS (12LL * 1024 * 1024)
+int adt_gnn_backward_static(int n, int nnz, const int32_t* d_indptr, const int32_t* d_indices,
+                            const int32_t* d_rows, const int32_t* d_mirror, double omega_p,
+                            const uint16_t* d_params, const float* d_work, const double* d_domega, float* d_grad,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif
