@@ -21,7 +21,7 @@ import numpy as np
 import torch
 from scipy.special import k0 as bessel_k0
 
-from . import fem, native
+from . import fem, formats, native
 from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
 from .field_cache import FieldCache, digest16, newest_batches_first
@@ -360,7 +360,7 @@ class _Device:
         self.lanes = native_lanes = op._solver_lanes(self.E)
         self.rp = -(-self.E // native_lanes) * native_lanes
         # unit primary (W, N, Rp) and the reference right-hand side A(1) u_p
-        primary = op._analytic_primary()
+        primary = op._unit_primary()
         self.primary = torch.zeros((self.W, self.N, self.rp), **f64)
         self.primary[:, :, : self.E] = torch.as_tensor(primary[:, :, perm], **f64).transpose(1, 2)
         ones = torch.ones((1, d.cell_dofs.shape[0]), **f64)
@@ -398,7 +398,7 @@ class ERTForward2p5D:
         self.source_cell_ids, self.source_node_ids = source_cell_ids, source_node_ids
         self.numerical_h2_refined, self.numerical_p2_refined = numerical_h2_refined, numerical_p2_refined
         self.topographic_geometric_factor_mode = topographic_geometric_factor_mode
-        self.terrain_cache_dir = terrain_cache_dir
+        self.terrain_cache_dir = formats.terrain_cache_dir(terrain_cache_dir)
         self.normal_field_cache_max_entries = int(normal_field_cache_max_entries)
         self.series_batch_steps = int(series_batch_steps)
         self.sensitivity_precision = option(sensitivity_precision, ("float64", "mixed"), "sensitivity_precision")
@@ -472,6 +472,18 @@ class ERTForward2p5D:
         return choose_lanes(r)
 
     # -- host precomputation ---------------------------------------------------------------------
+
+    def _unit_primary(self) -> np.ndarray:
+        """``(W, E, N)`` unit primary field, read from the operator's disk cache when
+        ``terrain_cache_dir`` (or ``$ADTLERT_TERRAIN_CACHE_DIR``) names one (``ert2p5d.py:349-402``)."""
+
+        d = self.discretization
+        shape = (int(self.wavenumbers.shape[0]), self.survey.electrode_count, d.dof_count)
+        arrays = (self.mesh.nodes, self.mesh.cells, self.survey.electrode_positions, self.survey.measurements,
+                  self.wavenumbers, self.weights, self.source_node_ids, d.dof_nodes, d.cell_dofs)
+        options = (self.numerical_h2_refined, self.numerical_p2_refined, self.topographic_geometric_factor_mode)
+        cache = formats.UnitPotentialCache(self.terrain_cache_dir)
+        return cache.get("native_analytic_primary", "float64", options, arrays, shape, self._analytic_primary)
 
     def _analytic_primary(self) -> np.ndarray:
         """``(W, E, N)`` unit primary with the electrode-node value (``ert2p5d.py:438-468``)."""

@@ -219,3 +219,19 @@ def test_validation_errors_and_close():
         adt.ERTForward3D.from_mesh_survey(mesh, adt.Survey.from_arrays(el, q), element_order=3)
     with pytest.raises(ValueError, match="three-coordinate"):
         adt.ERTForward3D.from_mesh_survey(mesh, adt.Survey.from_arrays(el[:, :2], q))
+
+
+def test_terrain_cache_dir_stores_and_reuses_the_unit_primary(tmp_path, monkeypatch):
+    """With ``terrain_cache_dir`` the host-computed unit primary goes to ``<sha256>.npz`` and a
+    second operator on the same inputs reads it back instead of recomputing it."""
+
+    case, forward = flat_operator(terrain_cache_dir=tmp_path)
+    sigma = torch.as_tensor(case.conductivity)
+    first = forward.apparent_resistivity_values(sigma).numpy()
+    files = list(tmp_path.glob("*.npz"))
+    assert len(files) == 1
+    _, again = flat_operator(terrain_cache_dir=tmp_path)
+    monkeypatch.setattr(again, "_analytic_primary", lambda: pytest.fail("the cached primary was recomputed"))
+    assert np.array_equal(again.apparent_resistivity_values(sigma).numpy(), first)
+    forward.close()
+    again.close()
