@@ -129,11 +129,25 @@ struct GramArgs {
     const int32_t* m;
     const int32_t* n;
     double* out;
+    const int32_t* col_ptr;    // optional column aggregation: column p sums cells col_cells[col_ptr[p] ..]
+    const int32_t* col_cells;
     int S, Sp, K, W, rp, wchunk, rows_max, C, D;
+    int P;                     // output columns (= workgroups): C, or the aggregated column count
     int64_t vec_off[kMaxWaves];
     double ws[kMaxWaves];
     double wm[kMaxWaves];
 };
+
+// Cells of output column `col`: itself, or the aggregation list [c0, c1) into col_cells.
+__device__ __forceinline__ void column_cells(const int32_t* col_ptr, int col, int& c0, int& c1) {
+    if (col_ptr) {
+        c0 = adt::gview(col_ptr)[col];
+        c1 = adt::gview(col_ptr)[col + 1];
+    } else {
+        c0 = col;
+        c1 = col + 1;
+    }
+}
 
 __device__ __forceinline__ int tri_index(int e, int f, int S) {  // e <= f
     return e * S - (e * (e - 1)) / 2 + (f - e);
@@ -196,7 +210,9 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
     constexpr int Sp = 16 * BT, TPW = gram_tiles_per_wave(BT), n_tiles = BT * (BT + 1) / 2;
     constexpr int kCols = (Sp + 63) / 64;  // staged sources per lane and row
     extern __shared__ double lds[];
-    const int c = adt::xcd_tile(static_cast<int>(blockIdx.x), A.C);
+    const int col = adt::xcd_tile(static_cast<int>(blockIdx.x), A.P);
+    int c_begin, c_end;
+    column_cells(A.col_ptr, col, c_begin, c_end);
     const int K = A.K, S = A.S, tid = static_cast<int>(threadIdx.x);
     const int rows_max = A.rows_max;
     const float inv_k = 1.0f / static_cast<float>(K);
@@ -204,9 +220,6 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
     double* psi = lds + static_cast<int64_t>(rows_max) * Sp;  // rows_max x Sp
     double* tw = psi + static_cast<int64_t>(rows_max) * Sp;   // wchunk x K x K: T_wc of the chunk
     int64_t* roff = reinterpret_cast<int64_t*>(tw + A.wchunk * K * K);  // W x K row offsets in x
-    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
-    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
-    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
     const int lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wave = (wv + static_cast<int>(blockIdx.x)) & 3;
@@ -221,13 +234,21 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
         ti[t] = I;
         tj[t] = I + q;
     }
+    f64x4 acc[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+    // G of the column = the sum of its cells' G_c (J is linear in G): the accumulators run on over
+    // the cells of an aggregated column, in list order
+    for (int ci = c_begin; ci < c_end; ++ci) {
+    const int c = A.col_ptr ? adt::gview(A.col_cells)[ci] : ci;
+    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
+    __syncthreads();  // the previous cell's staging has read its row offsets
     for (int r = tid; r < A.W * K; r += 256) {
         const int w = div_small(r, inv_k);
         roff[r] = A.vec_off[w] + static_cast<int64_t>(dof[r - w * K]) * A.rp;
     }
-    f64x4 acc[TPW];
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
     for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
         const int nw = min(A.wchunk, A.W - w0);
         const int rows = nw * K, rows4 = (rows + 3) & ~3;
@@ -285,8 +306,9 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
                                                                   acc[t], 0, 0, 0);
         }
     }
+    }  // cells of the column
     __syncthreads();
-    // upper triangle of G_c over S + 1 slots: slot S (the remote electrode) is a zero row
+    // upper triangle of G over S + 1 slots: slot S (the remote electrode) is a zero row
     double* G = lds;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
@@ -307,12 +329,12 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
         const unsigned lo = min(ue, uf), hi = max(ue, uf);
         return G[(__umul24(lo, two_s1 - lo) >> 1) + hi];
     };
-    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.C, c, A.out, g);
+    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.P, col, A.out, g);
 }
 
 template <int BT>
 int launch_gram(GramArgs& A, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(gram_kernel<BT>, dim3(A.C), dim3(256), lds, st, A);
+    hipLaunchKernelGGL(gram_kernel<BT>, dim3(A.P), dim3(256), lds, st, A);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -341,8 +363,11 @@ struct GramCrossArgs {
     const int32_t* m;
     const int32_t* n;
     double* out;
+    const int32_t* col_ptr;    // optional column aggregation, as in GramArgs
+    const int32_t* col_cells;
     int Sr, Sc, K, W, rp_r, rp_c, wchunk, C, D;
     int btr, btc, rows_max;
+    int P;                     // output columns (= workgroups)
     int64_t off_r[kMaxWaves];
     int64_t off_c[kMaxWaves];
     double ws[kMaxWaves];
@@ -361,7 +386,9 @@ inline int cross_tiles_per_wave(int tiles) {
 template <int TPW>
 __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
     extern __shared__ double lds[];
-    const int c = adt::xcd_tile(static_cast<int>(blockIdx.x), A.C);
+    const int col = adt::xcd_tile(static_cast<int>(blockIdx.x), A.P);
+    int c_begin, c_end;
+    column_cells(A.col_ptr, col, c_begin, c_end);
     const int K = A.K, Sr = A.Sr, Sc = A.Sc;
     const int spr = 16 * A.btr, spc = 16 * A.btc, rows_max = A.rows_max;
     double* phr = lds;                                          // rows_max x spr
@@ -370,9 +397,6 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wave = (wv + static_cast<int>(blockIdx.x)) & 3;
-    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
-    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
-    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
     const int n_tiles = A.btr * A.btc;
     int ti[TPW], tj[TPW];
     int nt = 0;
@@ -387,6 +411,11 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
     f64x4 acc[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int ci = c_begin; ci < c_end; ++ci) {  // G of the column: its cells' G_c summed in list order
+    const int c = A.col_ptr ? adt::gview(A.col_cells)[ci] : ci;
+    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
     for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
         const int nw = min(A.wchunk, A.W - w0);
         const int rows = nw * K, rows4 = (rows + 3) & ~3;
@@ -458,8 +487,9 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
                                                                   acc[t], 0, 0, 0);
         }
     }
+    }  // cells of the column
     __syncthreads();
-    // G_c over (S_r + 1) x (S_c + 1) slots, receiver-major; slot S_r / S_c (remote) is zero
+    // G over (S_r + 1) x (S_c + 1) slots, receiver-major; slot S_r / S_c (remote) is zero
     const int gs = Sc + 1;
     double* G = lds;
 #pragma unroll
@@ -479,7 +509,7 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
         const unsigned ue = min(static_cast<unsigned>(e), ur), uf = min(static_cast<unsigned>(f), uc);
         return G[__umul24(ue, static_cast<unsigned>(gs)) + uf];
     };
-    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.C, c, A.out, g);
+    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.P, col, A.out, g);
 }
 
 // Cell-block form for separate receiver and current fields (3D): J[d, c] =
@@ -710,7 +740,7 @@ int launch_cell_block(GramCrossArgs& A, hipStream_t st) {
 
 template <int TPW>
 int launch_gram_cross(GramCrossArgs& A, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(gram_cross_kernel<TPW>, dim3(A.C), dim3(256), lds, st, A);
+    hipLaunchKernelGGL(gram_cross_kernel<TPW>, dim3(A.P), dim3(256), lds, st, A);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
 }
@@ -1039,17 +1069,24 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
                                           int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b,
                                           const int32_t* d_m, const int32_t* d_n, const int32_t* d_cell_dofs,
                                           const double* d_stiff, const double* d_mass, const double* h_ws,
-                                          const double* h_wm, double* d_out, void* stream) {
+                                          const double* h_wm, const int32_t* d_col_ptr, const int32_t* d_col_cells,
+                                          int n_cols, double* d_out, void* stream) {
     ADT_REQUIRE(n_waves >= 1 && n_waves <= kMaxWaves, "adt_sensitivity_gram_cross: 1 <= n_waves <= %d", kMaxWaves);
+    ADT_REQUIRE(!d_col_ptr || (d_col_cells && n_cols >= 0),
+                "adt_sensitivity_gram_cross: column map needs its cell list");
     ADT_REQUIRE(s_r >= 1 && s_r <= 128 && s_r <= rp_r && s_c >= 1 && s_c <= 128 && s_c <= rp_c,
                 "adt_sensitivity_gram_cross: 1 <= receivers <= min(128, rp_r) and 1 <= currents <= min(128, rp_c)");
     ADT_REQUIRE(k_local >= 1 && k_local <= 10, "adt_sensitivity_gram_cross: unsupported local size %d", k_local);
     ADT_REQUIRE(h_off_r && h_off_c && h_ws && (d_mass == nullptr || h_wm),
                 "adt_sensitivity_gram_cross: missing host arrays");
-    if (n_meas == 0 || n_cells == 0) return ADT_OK;
+    const int cols = d_col_ptr ? n_cols : n_cells;
+    if (n_meas == 0 || n_cells == 0 || cols == 0) return ADT_OK;
     ADT_REQUIRE(d_xr && d_xc && d_a && d_b && d_m && d_n && d_cell_dofs && d_stiff && d_out,
                 "adt_sensitivity_gram_cross: null pointer");
     GramCrossArgs A{};
+    A.col_ptr = d_col_ptr;
+    A.col_cells = d_col_cells;
+    A.P = cols;
     A.xr = d_xr;
     A.xc = d_xc;
     A.dofs = d_cell_dofs;
@@ -1091,7 +1128,7 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
     // per-cell staging and 8-byte row writes outweigh its four LDS reads per output); the Gram
     // block otherwise.  ADT_CROSS_FORM=gram forces the Gram block.
     const char* form = std::getenv("ADT_CROSS_FORM");
-    const bool force_gram = form && std::strcmp(form, "gram") == 0;
+    const bool force_gram = (form && std::strcmp(form, "gram") == 0) || d_col_ptr;  // columns of many cells
     if (!force_gram) {
         switch (cell_block_width(A)) {
             case 32: return launch_cell_block<32, 1024>(A, st);
@@ -1115,14 +1152,20 @@ extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64
                                     int n_meas, int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b,
                                     const int32_t* d_m, const int32_t* d_n, const int32_t* d_cell_dofs,
                                     const double* d_stiff, const double* d_mass, const double* h_ws,
-                                    const double* h_wm, double* d_out, void* stream) {
+                                    const double* h_wm, const int32_t* d_col_ptr, const int32_t* d_col_cells,
+                                    int n_cols, double* d_out, void* stream) {
     ADT_REQUIRE(n_waves >= 1 && n_waves <= kMaxWaves, "adt_sensitivity_gram: 1 <= n_waves <= %d", kMaxWaves);
+    ADT_REQUIRE(!d_col_ptr || (d_col_cells && n_cols >= 0), "adt_sensitivity_gram: column map needs its cell list");
     ADT_REQUIRE(s_rows >= 1 && s_rows <= 128 && s_rows <= rp, "adt_sensitivity_gram: 1 <= sources <= min(128, rp)");
     ADT_REQUIRE(k_local >= 1 && k_local <= 10, "adt_sensitivity_gram: unsupported local size %d", k_local);
     ADT_REQUIRE(h_vec_off && h_ws && (d_mass == nullptr || h_wm), "adt_sensitivity_gram: missing host arrays");
-    if (n_meas == 0 || n_cells == 0) return ADT_OK;
+    const int cols = d_col_ptr ? n_cols : n_cells;
+    if (n_meas == 0 || n_cells == 0 || cols == 0) return ADT_OK;
     ADT_REQUIRE(d_x && d_a && d_b && d_m && d_n && d_cell_dofs && d_stiff && d_out, "adt_sensitivity_gram: null pointer");
     GramArgs A{};
+    A.col_ptr = d_col_ptr;
+    A.col_cells = d_col_cells;
+    A.P = cols;
     A.x = d_x;
     A.dofs = d_cell_dofs;
     A.stiff = d_stiff;

@@ -209,38 +209,41 @@ def jacobian_rows_per_launch(rows: int, cells: int) -> int:
     return max(1, min(rows, JACOBIAN_BLOCK_BYTES // max(8 * cells, 1)))
 
 
-def chunked_rows(count: int, cells: int, batch_size: int, launch, columns, device, return_device: bool) -> Tensor:
+def chunked_rows(count: int, width: int, batch_size: int, launch, finish, columns: int, device,
+                 return_device: bool) -> Tensor:
     """Run ``launch(start, stop, part)`` for consecutive chunks of ``batch_size`` measurement
-    rows into a device ``(rows, cells)`` block, aggregate its columns, and gather the rows on the
-    host in ``FLOAT_DTYPE`` (or on the device with ``return_device``).  Without aggregation the
-    kernel writes straight into the result block."""
+    rows into a device ``(rows, width)`` block, map each block to its ``columns`` output columns
+    with ``finish`` (None: the launch already writes them), and gather the rows on the host in
+    ``FLOAT_DTYPE`` (or on the device with ``return_device``).  Without ``finish`` the kernel
+    writes straight into the result block."""
 
     f64 = dict(dtype=torch.float64, device=device)
+    done = (lambda part: part) if finish is None else finish
     if count <= batch_size:
-        part = torch.empty((count, cells), **f64)
+        part = torch.empty((count, width), **f64)
         launch(0, count, part)
-        result = columns.columns(part)
+        result = done(part)
         if return_device:
             return result
         out = result.to(device="cpu", dtype=FLOAT_DTYPE)
     elif return_device:
-        result = torch.empty((count, columns.count), **f64)
-        scratch = None if columns.identity else torch.empty((batch_size, cells), **f64)
+        result = torch.empty((count, columns), **f64)
+        scratch = None if finish is None else torch.empty((batch_size, width), **f64)
         for start in range(0, count, batch_size):
             stop = min(count, start + batch_size)
             if scratch is None:
                 launch(start, stop, result[start:stop])
             else:
                 launch(start, stop, scratch[: stop - start])
-                result[start:stop] = columns.columns(scratch[: stop - start])
+                result[start:stop] = done(scratch[: stop - start])
         return result
     else:
-        out = torch.empty((count, columns.count), dtype=FLOAT_DTYPE)
-        part = torch.empty((batch_size, cells), **f64)
+        out = torch.empty((count, columns), dtype=FLOAT_DTYPE)
+        part = torch.empty((batch_size, width), **f64)
         for start in range(0, count, batch_size):
             stop = min(count, start + batch_size)
             launch(start, stop, part[: stop - start])
-            out[start:stop] = columns.columns(part[: stop - start]).to(device="cpu", dtype=FLOAT_DTYPE)
+            out[start:stop] = done(part[: stop - start]).to(device="cpu", dtype=FLOAT_DTYPE)
     check_finite(out, "jacobian")
     return out
 
@@ -283,6 +286,19 @@ class _Columns:
         ptr = np.concatenate(([0], np.cumsum(np.bincount(targets[active], minlength=count)))).astype(np.int32)
         self.ptr = torch.as_tensor(ptr, device=device)
         self.idx = torch.as_tensor(order.astype(np.int32), device=device)
+        # the sensitivity kernels' fused form: the original cell ids of every column, in order
+        self.cells = torch.as_tensor(active[order].astype(np.int32), device=device)
+
+    @property
+    def kernel_map(self):
+        """``(col_ptr, col_cells)`` for the Gram kernels' fused aggregation; None for identity."""
+
+        return None if self.identity else (self.ptr, self.cells)
+
+    def finish(self):
+        """Host-side step after a per-cell launch: None when nothing is left to aggregate."""
+
+        return None if self.identity else self.columns
 
     def select(self, values: Tensor, dim: int) -> Tensor:
         return values if self.identity else values.index_select(dim, self.active)
@@ -828,32 +844,37 @@ class ERTForward2p5D:
     def _normal_jacobian(self, fields: _Fields, batch_size: int, columns: _Columns, *, return_device=False) -> Tensor:
         """``-sum_w w_w (u_M - u_N)^T (K + k_w^2 M) (u_A - u_B)`` by the fused kernel
         (``ert2p5d.py:984-1010``), ``batch_size`` measurement rows per launch: the device holds
-        one launch's rows at a time unless ``return_device`` asks for the whole block there."""
+        one launch's rows at a time unless ``return_device`` asks for the whole block there.  In
+        the Gram form the cell-to-parameter aggregation (``kernels.py:88-113``) runs inside the
+        kernel: each parameter column sums its cells' Gram blocks before the output phase."""
 
         dev = self._dev
-        stiff = columns.select(dev.stiff, 0).contiguous()
-        mass = columns.select(dev.mass, 0).contiguous()
         ws, wm = dev.q_host, dev.q_host * dev.k2_host
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         D = self.survey.measurement_count
         if dev.E <= 128 and precision == 0:
-            # Gram form on the interleaved device-order blocks: per cell G_c = Phi^T T Phi once per
-            # launch, then four gathers per row (FP64; the mixed mode keeps the per-output FP32 kernel)
-            dofs = columns.select(dev.cell_dofs, 0).contiguous()
+            # Gram form on the interleaved device-order blocks: per column G = sum_c Phi^T T_c Phi
+            # once per launch, then four gathers per row (FP64; the mixed mode keeps the per-output
+            # FP32 kernel)
             blocks = fields.blocks.contiguous()
             vec_off = np.arange(dev.W, dtype=np.int64) * dev.N * blocks.shape[-1]
+            fused = columns.kernel_map
 
             def launch(start, stop, part):
                 native.sensitivity_gram(blocks, vec_off, blocks.shape[-1], dev.E, dev.a[start:stop], dev.b[start:stop],
-                                        dev.m[start:stop], dev.n[start:stop], dofs, stiff, mass, ws, wm, part)
-        else:
-            phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
-            dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
+                                        dev.m[start:stop], dev.n[start:stop], dev.cell_dofs, dev.stiff, dev.mass, ws,
+                                        wm, part, columns=fused)
+            return chunked_rows(D, columns.count, batch_size, launch, None, columns.count, dev.device, return_device)
+        phi = fields.total  # (W, E, N), mesh node order (cells are walked in mesh order)
+        dofs = columns.select(dev.cell_dofs_mesh, 0).contiguous()
+        stiff = columns.select(dev.stiff, 0).contiguous()
+        mass = columns.select(dev.mass, 0).contiguous()
 
-            def launch(start, stop, part):
-                native.sensitivity(phi, dev.a[start:stop], dev.b[start:stop], dev.m[start:stop], dev.n[start:stop],
-                                   dofs, stiff, mass, ws, wm, part, precision)
-        return chunked_rows(D, dofs.shape[0], batch_size, launch, columns, dev.device, return_device)
+        def launch(start, stop, part):
+            native.sensitivity(phi, dev.a[start:stop], dev.b[start:stop], dev.m[start:stop], dev.n[start:stop],
+                               dofs, stiff, mass, ws, wm, part, precision)
+        return chunked_rows(D, dofs.shape[0], batch_size, launch, columns.finish(), columns.count, dev.device,
+                            return_device)
 
     def _adjoint_jacobian(self, fields: _Fields, normal, reciprocal, batch_size: int, robin: bool,
                           normal_sensitivity: bool) -> Tensor:

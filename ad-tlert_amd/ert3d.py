@@ -586,38 +586,41 @@ class ERTForward3D:
         sigma = self._sigma(conductivity)
         fields, _ = self._fields_series(sigma[None])
         (receiver,) = self._receivers(fields, sigma[None])
-        templates = columns.select(s.templates if include_robin_boundary_derivative else s.volume_templates, 0)
-        templates = templates.contiguous()
+        all_templates = (s.templates if include_robin_boundary_derivative else s.volume_templates).contiguous()
         precision = 1 if self.sensitivity_precision == "mixed" else 0
         if precision == 0 and s.nc <= 128 and s.nr <= 128:
-            # cross-Gram form on the interleaved device-order blocks: per cell G_c = Phi_R^T T Phi_C
-            # (receivers x currents) evaluated per launch, then four reads per row (FP64; the mixed
-            # mode keeps the per-output FP32 kernel)
+            # cross-Gram form on the interleaved device-order blocks: per column G = sum_c Phi_R^T
+            # T_c Phi_C (receivers x currents; aggregated parameter columns sum their cells inside
+            # the kernel), then four reads per row (FP64; the mixed mode keeps the per-output FP32
+            # kernel)
             current = fields[0].current.contiguous()
             rec = receiver.contiguous()
-            dofs = columns.select(s.cell_dofs, 0).contiguous()
             a, b, m, n = s.abmn_i32
+            fused = columns.kernel_map
 
             def launch(start, stop, part):
                 native.sensitivity_gram_cross(rec, [0], rec.shape[-1], s.nr, current, [0], current.shape[-1], s.nc,
-                                              a[start:stop], b[start:stop], m[start:stop], n[start:stop], dofs,
-                                              templates, None, [1.0], None, part)
-        else:
-            S = s.nc + s.nr
-            phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)  # mesh DOF order
-            phi[0, : s.nc] = fields[0].current[s.inv, : s.nc].T
-            phi[0, s.nc:] = receiver[s.inv, : s.nr].T
-            a, b, m, n = (x.copy() for x in s.abmn_host)
-            a = np.where(a == s.nc, S, a)
-            b = np.where(b == s.nc, S, b)
-            m, n = m + s.nc, n + s.nc
-            idx = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in (a, b, m, n)]
-            dofs = columns.select(s.cell_dofs_mesh, 0).contiguous()
+                                              a[start:stop], b[start:stop], m[start:stop], n[start:stop], s.cell_dofs,
+                                              all_templates, None, [1.0], None, part, columns=fused)
+            return chunked_rows(count, columns.count, batch_size, launch, None, columns.count, s.device,
+                                return_device)
+        templates = columns.select(all_templates, 0).contiguous()
+        S = s.nc + s.nr
+        phi = torch.empty((1, S, self.N), dtype=torch.float64, device=s.device)  # mesh DOF order
+        phi[0, : s.nc] = fields[0].current[s.inv, : s.nc].T
+        phi[0, s.nc:] = receiver[s.inv, : s.nr].T
+        a, b, m, n = (x.copy() for x in s.abmn_host)
+        a = np.where(a == s.nc, S, a)
+        b = np.where(b == s.nc, S, b)
+        m, n = m + s.nc, n + s.nc
+        idx = [torch.as_tensor(x.astype(np.int32), device=s.device) for x in (a, b, m, n)]
+        dofs = columns.select(s.cell_dofs_mesh, 0).contiguous()
 
-            def launch(start, stop, part):
-                native.sensitivity(phi, *(x[start:stop] for x in idx), dofs, templates, None, [1.0], None, part,
-                                   precision)
-        return chunked_rows(count, dofs.shape[0], batch_size, launch, columns, s.device, return_device)
+        def launch(start, stop, part):
+            native.sensitivity(phi, *(x[start:stop] for x in idx), dofs, templates, None, [1.0], None, part,
+                               precision)
+        return chunked_rows(count, dofs.shape[0], batch_size, launch, columns.finish(), columns.count, s.device,
+                            return_device)
 
     def jacobian_columnwise(self, conductivity) -> Tensor:
         basis = torch.eye(self.mesh.cell_count, dtype=FLOAT_DTYPE)

@@ -14,7 +14,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().parent / "_lib" / "libadt_hip.so"))
 
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
@@ -51,9 +51,9 @@ _SIGNATURES = {
     "adt_gnn_backward_static": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
-                                     P]),
+                                     P, c_int, P, P]),
     "adt_sensitivity_gram_cross": (c_int, [c_int, P, c_int, c_int, P, P, c_int, c_int, P, c_int, c_int, c_int, P, P,
-                                           P, P, P, P, P, P, P, P, P]),
+                                           P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
     "adt_electrode_potentials": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P]),
@@ -268,22 +268,35 @@ def sensitivity(phi, a, b, m, n, cell_dofs, stiff, mass, ws, wm, out, precision=
                                     ptr(stiff), ptr(mass), host(ws), host(wm), int(precision), ptr(out), stream()))
 
 
-def sensitivity_gram(x, vec_off, rp, s_rows, a, b, m, n, cell_dofs, stiff, mass, ws, wm, out):
-    """Jacobian rows from interleaved fields in Gram form (see ``adt_sensitivity_gram``)."""
+def _column_map(columns):
+    """``(col_ptr, col_cells, n_cols)`` device arguments of the Gram kernels' column aggregation."""
+
+    if columns is None:
+        return None, None, 0
+    col_ptr, col_cells = columns
+    return col_ptr, col_cells, int(col_ptr.numel()) - 1
+
+
+def sensitivity_gram(x, vec_off, rp, s_rows, a, b, m, n, cell_dofs, stiff, mass, ws, wm, out, columns=None):
+    """Jacobian rows from interleaved fields in Gram form (see ``adt_sensitivity_gram``); with
+    ``columns = (col_ptr, col_cells)`` (int32 device tensors) column p sums the cells of its list."""
 
     W = len(vec_off)
     C, K = cell_dofs.shape
     vec_off = np.ascontiguousarray(vec_off, dtype=np.int64)
     ws = np.ascontiguousarray(ws, dtype=np.float64)
     wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
+    col_ptr, col_cells, n_cols = _column_map(columns)
     check(library().adt_sensitivity_gram(W, int(rp), int(s_rows), host(vec_off), ptr(x), int(a.numel()), C, K,
                                          ptr(a), ptr(b), ptr(m), ptr(n), ptr(cell_dofs), ptr(stiff), ptr(mass),
-                                         host(ws), host(wm), ptr(out), stream()))
+                                         host(ws), host(wm), ptr(col_ptr), ptr(col_cells), n_cols, ptr(out),
+                                         stream()))
 
 
 def sensitivity_gram_cross(xr, off_r, rp_r, s_r, xc, off_c, rp_c, s_c, a, b, m, n, cell_dofs, stiff, mass, ws, wm,
-                           out):
-    """Jacobian rows from separate receiver / current interleaved fields (``adt_sensitivity_gram_cross``)."""
+                           out, columns=None):
+    """Jacobian rows from separate receiver / current interleaved fields (``adt_sensitivity_gram_cross``);
+    ``columns`` as in :func:`sensitivity_gram`."""
 
     W = len(off_r)
     C, K = cell_dofs.shape
@@ -291,10 +304,11 @@ def sensitivity_gram_cross(xr, off_r, rp_r, s_r, xc, off_c, rp_c, s_c, a, b, m, 
     off_c = np.ascontiguousarray(off_c, dtype=np.int64)
     ws = np.ascontiguousarray(ws, dtype=np.float64)
     wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
+    col_ptr, col_cells, n_cols = _column_map(columns)
     check(library().adt_sensitivity_gram_cross(W, ptr(xr), int(rp_r), int(s_r), host(off_r), ptr(xc), int(rp_c),
                                                int(s_c), host(off_c), int(a.numel()), C, K, ptr(a), ptr(b), ptr(m),
                                                ptr(n), ptr(cell_dofs), ptr(stiff), ptr(mass), host(ws), host(wm),
-                                               ptr(out), stream()))
+                                               ptr(col_ptr), ptr(col_cells), n_cols, ptr(out), stream()))
 
 
 def cell_gradient(lam, rp_lam, lam_off, phi, rp_phi, phi_off, r, cell_dofs, stiff, mass, ws, wm, out, *,

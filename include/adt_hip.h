@@ -194,11 +194,17 @@ int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_c
  * one workgroup per cell forms its upper-triangle 16 x 16 tiles on the FP64 matrix cores and
  * keeps the upper triangle, with a zero slot for the remote electrode, in LDS (no workspace).
  * FP64, every sum in a fixed order: bitwise repeatable.  out is (D, C) row-major.  Replaces the
- * same reference kernel as adt_sensitivity. */
+ * same reference kernel as adt_sensitivity.
+ * Column aggregation (CellAggregation, forward/kernels.py:88-113, fused): with d_col_ptr (n_cols + 1
+ * device ints) and d_col_cells, output column p is the sum of the Jacobian columns of cells
+ * d_col_cells[d_col_ptr[p] .. d_col_ptr[p+1]); J is linear in G_c, so one workgroup per column sums
+ * the G_c of its cells in list order on the matrix cores and writes one column: out is then
+ * (D, n_cols) and cells in no list are dropped.  d_col_ptr = NULL: one column per cell. */
 int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x, int n_meas,
                          int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
                          const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff, const double* d_mass,
-                         const double* h_ws, const double* h_wm, double* d_out, void* stream);
+                         const double* h_ws, const double* h_wm, const int32_t* d_col_ptr,
+                         const int32_t* d_col_cells, int n_cols, double* d_out, void* stream);
 
 /* Same Jacobian rows in cross-Gram form, for receiver fields that differ from the current
  * fields (3D: ERTForward3D.jacobian, forward/ert3d.py:867-927, with the kernel of
@@ -215,12 +221,14 @@ int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_o
  * segment; otherwise (or with the environment variable ADT_CROSS_FORM=gram) one workgroup per
  * cell forms G_c on the FP64 matrix cores and keeps it in LDS with a zero row and column for
  * the remote electrodes.  No workspace.  FP64, fixed summation order: bitwise repeatable.  out
- * is (D, C) row-major. */
+ * is (D, C) row-major.  Column aggregation as in adt_sensitivity_gram (d_col_ptr, d_col_cells,
+ * n_cols; out (D, n_cols)); it always takes the whole-Gram form, summing its cells' G_c. */
 int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,
                                const double* d_xc, int rp_c, int s_c, const int64_t* h_off_c, int n_meas,
                                int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
                                const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff,
-                               const double* d_mass, const double* h_ws, const double* h_wm, double* d_out,
+                               const double* d_mass, const double* h_ws, const double* h_wm,
+                               const int32_t* d_col_ptr, const int32_t* d_col_cells, int n_cols, double* d_out,
                                void* stream);
 
 /* ------------------------------------------------------------------------------------------

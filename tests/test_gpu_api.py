@@ -155,23 +155,32 @@ def test_polarity_swap_and_reciprocity():
 
 @pytest.mark.parametrize("dim", ["2p5d", "3d"])
 def test_parameter_aggregation_sums_cells_and_ignores_inactive(dim):
+    """Aggregated Jacobian columns (summed inside the Gram kernels) and the aggregated normal vjp
+    against the oracle's aggregation of its own per-cell Jacobian (``kernels.py:88-113``):
+    inactive cells (-1) are dropped and a column with no cell (``parameter_count`` beyond the
+    largest id) is zero."""
+
     if dim == "2p5d":
         case, forward = flat_operator()
-        sigma = torch.as_tensor(case.conductivity)
+        sigma = case.conductivity
+        ref = oracle2p5d.Forward2p5D(case.nodes, case.cells, case.electrodes, case.measurements,
+                                     case.surface_node_ids)
     else:
-        (_, _, _, _, s), forward = box_operator()
-        sigma = torch.as_tensor(s)
+        (nodes, tets, el, quads, sigma), forward = box_operator()
+        ref = oracle3d.Forward3D(nodes, tets, el, quads)
     cells = forward.mesh.cell_count
     ids = np.full(cells, -1)
     ids[: cells // 2] = np.arange(cells // 2) // 3
-    parameters = int(ids.max()) + 1
-    jac = forward.jacobian(sigma).numpy()
-    expected = np.stack([jac[:, ids == p].sum(axis=1) for p in range(parameters)], axis=1)
-    _, aggregated = forward.solve_with_jacobian(sigma, jacobian_cell_parameter_ids=ids,
+    parameters = int(ids.max()) + 2  # the last column has no cell
+    expected = ref.jacobian(sigma, ids=ids, count=parameters)
+    _, aggregated = forward.solve_with_jacobian(torch.as_tensor(sigma), jacobian_cell_parameter_ids=ids,
                                                 jacobian_parameter_count=parameters)
-    assert relmax(aggregated, expected) < 1e-10
+    print(f"[{dim}] aggregated jacobian vs oracle {relmax(aggregated, expected):.2e}")
+    assert aggregated.shape == (forward.survey.measurement_count, parameters)
+    assert relmax(aggregated, expected) < 1e-8 and np.all(aggregated.numpy()[:, -1] == 0)
     cot = np.random.default_rng(4).standard_normal(forward.survey.measurement_count)
-    vjp = forward.normal_vjp(sigma, torch.as_tensor(cot), cell_parameter_ids=ids, parameter_count=parameters)
+    vjp = forward.normal_vjp(torch.as_tensor(sigma), torch.as_tensor(cot), cell_parameter_ids=ids,
+                             parameter_count=parameters)
     assert relmax(vjp, expected.T @ cot) < 1e-8
 
 

@@ -120,7 +120,8 @@ def test_normal_jacobian_matches_the_oracle(name, quad, tri, monkeypatch):
     # is recomputed per launch in the same order, so the rows are bitwise equal)
     launches = []
     gram = adt.native.sensitivity_gram
-    monkeypatch.setattr(adt.native, "sensitivity_gram", lambda *a: launches.append(a[4].numel()) or gram(*a))
+    monkeypatch.setattr(adt.native, "sensitivity_gram",
+                        lambda *a, **k: launches.append(a[4].numel()) or gram(*a, **k))
     chunked = forward.jacobian(torch.as_tensor(sigma), batch_size=7).numpy()
     assert launches == [7] * (ref.D // 7) + ([ref.D % 7] if ref.D % 7 else [])
     assert np.array_equal(chunked, jac)

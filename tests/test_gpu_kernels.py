@@ -429,3 +429,68 @@ def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp):
     out = run()
     assert relmax(out.cpu(), expected) < 1e-12
     assert torch.equal(run(), out)
+
+
+def column_map(C: int, P: int, seed: int):
+    """Seeded cell -> parameter map with dropped cells (-1) and one empty column; the kernels'
+    (col_ptr, col_cells) lists and the host reference matrix (C, P)."""
+
+    rng = np.random.default_rng(seed)
+    targets = rng.integers(-1, P - 1, C)  # column P - 1 stays empty
+    active = np.flatnonzero(targets >= 0)
+    order = active[np.argsort(targets[active], kind="stable")]
+    ptr = np.concatenate(([0], np.cumsum(np.bincount(targets[active], minlength=P)))).astype(np.int32)
+    matrix = np.zeros((C, P))
+    matrix[active, targets[active]] = 1.0
+    return torch.as_tensor(ptr, device=dev()), torch.as_tensor(order.astype(np.int32), device=dev()), matrix
+
+
+@pytest.mark.parametrize("k, S", [(3, 19), (10, 40)])
+def test_gram_sensitivity_with_fused_column_aggregation(k, S):
+    """Aggregated columns (CellAggregation fused into the Gram kernel: each column sums its cells'
+    Gram blocks) equal the per-cell einsum reference summed over each column's cells; dropped cells
+    and empty columns give nothing; bitwise repeatable."""
+
+    phi, a, b, m, n, cells, stiff, mass, ws, wm = random_problem(k, W=5, S=S, N=80, C=61, D=37, seed=k)
+    stiff, mass = 0.5 * (stiff + stiff.transpose(1, 2)), 0.5 * (mass + mass.transpose(1, 2))
+    expected_cells = einsum_reference(phi, a, b, m, n, cells, stiff, mass, ws, wm).numpy()
+    col_ptr, col_cells, matrix = column_map(cells.shape[0], 9, seed=k + 1)
+    expected = expected_cells @ matrix
+    W, _, N = phi.shape
+    rp = 16 * ((S + 15) // 16)
+    blocks = torch.zeros((W, N, rp), dtype=torch.float64)
+    blocks[:, :, :S] = phi.transpose(1, 2)
+    g = lambda t, dt=None: t.to(dev(), dtype=dt or t.dtype).contiguous()  # noqa: E731
+
+    def run():
+        out = torch.full((a.numel(), 9), float("nan"), dtype=torch.float64, device=dev())
+        native.sensitivity_gram(g(blocks), np.arange(W) * N * rp, rp, S, *(g(x, torch.int32) for x in (a, b, m, n)),
+                                g(cells, torch.int32), g(stiff), g(mass), ws.numpy(), wm.numpy(), out,
+                                columns=(col_ptr, col_cells))
+        return out
+
+    out = run()
+    assert relmax(out.cpu(), expected) < 1e-12
+    assert torch.all(out[:, 8] == 0)
+    assert torch.equal(run(), out)
+
+
+def test_cross_gram_sensitivity_with_fused_column_aggregation():
+    rng = np.random.default_rng(9)
+    W, Sr, Sc, N, C, D, K = 1, 21, 13, 70, 47, 29, 4
+    xr, xc = rng.standard_normal((N, 32)), rng.standard_normal((N, 16))
+    cells = rng.integers(0, N, (C, K))
+    T = rng.standard_normal((C, K, K))
+    a, b = rng.integers(0, Sc + 1, D), rng.integers(0, Sc + 1, D)  # Sc: remote
+    m, n = rng.integers(0, Sr + 1, D), rng.integers(0, Sr + 1, D)
+    pr = np.vstack((xr[:, :Sr].T, np.zeros((1, N))))
+    pc = np.vstack((xc[:, :Sc].T, np.zeros((1, N))))
+    rec = (pr[m] - pr[n])[:, cells]
+    cur = (pc[a] - pc[b])[:, cells]
+    per_cell = -np.einsum("dci,cij,dcj->dc", rec, T, cur)
+    col_ptr, col_cells, matrix = column_map(C, 7, seed=3)
+    out = torch.empty((D, 7), dtype=torch.float64, device=dev())
+    g = lambda x, dt=torch.float64: torch.as_tensor(x, dtype=dt, device=dev()).contiguous()  # noqa: E731
+    native.sensitivity_gram_cross(g(xr), [0], 32, Sr, g(xc), [0], 16, Sc, *(g(x, torch.int32) for x in (a, b, m, n)),
+                                  g(cells, torch.int32), g(T), None, [1.0], None, out, columns=(col_ptr, col_cells))
+    assert relmax(out.cpu().numpy(), per_cell @ matrix) < 1e-12
