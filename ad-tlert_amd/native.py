@@ -53,7 +53,7 @@ _SIGNATURES = {
     "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
                                      P, c_int, P, P]),
     "adt_sensitivity_gram_cross": (c_int, [c_int, P, c_int, c_int, P, P, c_int, c_int, P, c_int, c_int, c_int, P, P,
-                                           P, P, P, P, P, P, P, P, c_int, P, P]),
+                                           P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
                                   P, P, P, P, P, P, P]),
     "adt_electrode_potentials": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P]),

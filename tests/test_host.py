@@ -389,3 +389,22 @@ def test_mirror_slots_of_symmetric_patterns_and_rejection_of_others():
     B = sp.csr_matrix(np.triu(np.ones((5, 5))))
     with pytest.raises(ValueError, match="symmetric"):
         learned.mirror_slots(B.indptr, B.indices)
+
+
+def test_ctypes_signatures_match_the_header_argument_counts():
+    """Every function the binding declares takes as many arguments as ``include/adt_hip.h``
+    declares for it (a mismatch would shift every later argument)."""
+
+    import re
+
+    header = (ROOT / "include" / "adt_hip.h").read_text()
+    native = adt.native
+    checked = 0
+    for name, (_, argtypes) in native._SIGNATURES.items():
+        found = re.search(r"^\s*(?:int|int64_t|const char\*)\s+" + name + r"\s*\(([^;]*?)\)\s*;", header,
+                          flags=re.S | re.M)
+        assert found, f"{name} is not declared in the header"
+        params = [p for p in found.group(1).split(",") if p.strip() and p.strip() != "void"]
+        assert len(params) == len(argtypes), f"{name}: header {len(params)} arguments, binding {len(argtypes)}"
+        checked += 1
+    assert checked >= 25
