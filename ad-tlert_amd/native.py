@@ -49,6 +49,8 @@ _SIGNATURES = {
     "adt_gnn_forward": (c_int, [c_int, c_int, c_int, P, P, P, P, P, c_i64, c_double, P, P, P, P, P]),
     "adt_gnn_backward": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P, P]),
     "adt_gnn_backward_static": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P]),
+    "adt_cgls_work_doubles": (c_i64, [c_int, c_int]),
+    "adt_cgls": (c_int, [c_int, c_int, P, P, P, P, P, P, c_i64, P, P, c_int, c_double, c_int, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
                                      P, c_int, P, P]),
@@ -403,3 +405,21 @@ def gnn_backward_static(graph, omega_p, params, work, domega, grad):
     check(library().adt_gnn_backward_static(graph.n, graph.nnz, ptr(graph.indptr), ptr(graph.indices),
                                             ptr(graph.rows), ptr(graph.mirror), float(omega_p), ptr(params),
                                             ptr(work), ptr(domega), ptr(grad), stream()))
+
+
+CGLS_STATUS = {0: "iteration limit", 1: "converged", 2: "zero search direction (q.q <= 0)", 3: "nothing to solve"}
+
+
+def cgls(a, at, b, x, max_iter: int, tol: float, check_every: int = 8) -> tuple[int, int]:
+    """``min ||A x - b||`` by CGLS on the device (``adt_cgls``): ``a`` and ``at`` are
+    ``(indptr, indices, values)`` int32/int32/float64 device tensors of A and A^T; ``x`` receives
+    the solution.  Returns ``(iterations, status)`` (``CGLS_STATUS``)."""
+
+    m, n = int(a[0].numel()) - 1, int(at[0].numel()) - 1
+    lib = library()
+    work = torch.empty(int(lib.adt_cgls_work_doubles(m, n)), dtype=torch.float64, device=b.device)
+    iters, status = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
+    check(lib.adt_cgls(m, n, ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(at[0]), ptr(at[1]), ptr(at[2]), int(a[1].numel()),
+                       ptr(b), ptr(x), int(max_iter), float(tol), int(check_every), ptr(work), host(iters),
+                       host(status), stream()))
+    return int(iters[0]), int(status[0])

@@ -281,6 +281,24 @@ int adt_amg_aggregates(int n, const int32_t* h_indptr, const int32_t* h_indices,
                        double theta, int32_t* h_agg);
 
 /* ------------------------------------------------------------------------------------------
+ * Gauss-Newton linear step: CGLS for min ||A x - b||_2 on the device.
+ *
+ * Replaces the GPU CGLS of the inversion's linearized step (inversion/optimizers.py:270-359,
+ * _gpu_cgls at 304-340): x = 0, r = b, s = A^T r, p = s, gamma = gamma0 = s.s; per iteration
+ * q = A p, alpha = gamma / q.q, x += alpha p, r -= alpha q, s = A^T r, gamma' = s.s, stop when
+ * gamma' <= 0 or gamma' / gamma0 < tol, else p = s + (gamma' / gamma) p.  A (m x n) and A^T
+ * (n x m) are CSR with nnz entries each (int32 indices); d_work holds adt_cgls_work_doubles(m, n)
+ * doubles (vectors, per-workgroup partial sums, scalars).  The scalars stay on the device; the
+ * host checks the stop flag once per check_every iterations.  Every sum runs in a fixed order:
+ * bitwise repeatable.  h_iters receives the number of completed iterations, h_status 0
+ * (max_iter reached), 1 (converged), 2 (q.q <= 0) or 3 (A^T b = 0 or b = 0: x = 0).
+ * ---------------------------------------------------------------------------------------- */
+int64_t adt_cgls_work_doubles(int m, int n);
+int adt_cgls(int m, int n, const int32_t* d_ptr, const int32_t* d_idx, const double* d_val, const int32_t* d_tptr,
+             const int32_t* d_tidx, const double* d_tval, int64_t nnz, const double* d_b, double* d_x, int max_iter,
+             double tol, int check_every, double* d_work, int* h_iters, int* h_status, void* stream);
+
+/* ------------------------------------------------------------------------------------------
  * Learned multigrid damping (no reference counterpart; it feeds ADT_PRECOND_LEARNED).
  *
  * A 3-layer graph message-passing network on the matrix graph (64 channels, bfloat16 weights,

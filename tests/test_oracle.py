@@ -446,3 +446,30 @@ def test_cpu_jacobi_pcg_matches_the_direct_solve():
     assert res.max() <= 1e-12 * 1.01
     direct = spla.splu(a.tocsc()).solve(b)
     assert relmax(x, direct) < 1e-9
+
+
+def test_cgls_oracle_reaches_least_squares_and_damped_normal_equations():
+    """At a tiny tolerance CGLS is the least-squares solution (numpy lstsq) of a full-rank
+    overdetermined system; with sqrt(lambda) I rows stacked it is the closed form
+    (A^T A + lambda I)^-1 A^T b; b = 0 gives x = 0 after no iteration."""
+
+    import scipy.sparse as sp
+
+    from oracle import cgls as ocgls
+
+    rng = np.random.default_rng(8)
+    a = sp.random(60, 25, density=0.3, random_state=8, format="csr") + sp.eye(60, 25, format="csr")
+    b = rng.standard_normal(60)
+    x, iters = ocgls.cgls(a, b, 500, 1e-28)
+    assert relmax(x, np.linalg.lstsq(a.toarray(), b, rcond=None)[0]) < 1e-10 and 0 < iters <= 500
+    lam = 0.3
+    stacked = sp.vstack((a, np.sqrt(lam) * sp.eye(25, format="csr")), format="csr")
+    x2, _ = ocgls.cgls(stacked, np.concatenate((b, np.zeros(25))), 500, 1e-28)
+    dense = a.toarray()
+    assert relmax(x2, np.linalg.solve(dense.T @ dense + lam * np.eye(25), dense.T @ b)) < 1e-10
+    x0, it0 = ocgls.cgls(a, np.zeros(60), 10, 1e-8)
+    assert it0 == 0 and np.all(x0 == 0)
+    # a loose tolerance stops early, with ||A^T r||^2 below tol ||A^T b||^2
+    xl, il = ocgls.cgls(a, b, 500, 1e-2)
+    s0, s = a.T @ b, a.T @ (b - a @ xl)
+    assert il < iters and float(s @ s) < 1e-2 * float(s0 @ s0)
