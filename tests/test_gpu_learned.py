@@ -129,9 +129,12 @@ def test_training_on_the_gpu_reduces_the_two_grid_loss():
 
 
 def test_trained_network_against_the_constant_damping():
-    """What the committed, trained network gains over the constant damping on conductivity
-    fields it never saw (seed 123; training used seed 7): the two-grid energy ratio it was
-    trained to lower, and the iterations of the full multigrid-preconditioned solves."""
+    """What the committed, trained network changes against the constant damping on conductivity
+    fields it never saw (seed 123; training used seed 7).  Measured: it lowers the two-grid
+    energy ratio it was trained on by a few percent, and the full multigrid-preconditioned
+    flexible CG needs the same number of iterations to within 2% (21.26 against 21.24 when the
+    weights were committed): the cycle, not the network, makes the solver fast.  The test checks
+    exactly that: a lower training objective and an iteration count equal within 2%."""
 
     net = learned.DampingNetwork.load()
     case = adt.cases.strip_dd_4k()
@@ -163,5 +166,5 @@ def test_trained_network_against_the_constant_damping():
     print(f"two-grid energy ratio trained/constant: {np.round(ratios, 3).tolist()}; mean FCG iterations "
           f"constant {np.mean(iters['amg']):.2f}, learned {np.mean(iters['learned']):.2f}")
     assert np.mean(ratios) < 1.0
-    assert np.mean(iters["learned"]) <= 1.05 * np.mean(iters["amg"])
+    assert abs(np.mean(iters["learned"]) - np.mean(iters["amg"])) <= 0.02 * np.mean(iters["amg"])
     op.close()
