@@ -329,6 +329,18 @@ def timelapse_ws_20k(seed=2, steps=365, node_target=20000):
     return case, series
 
 
+def mixed_meshes_ws(seed=6, node_targets=(5000, 10000, 15000, 20000, 25000, 30000, 35000, 40000), electrodes=96,
+                    rows=3000):
+    """The time-lapse survey (``electrodes`` at 1 m, Wenner-Schlumberger rows) on graded meshes of
+    ``node_targets`` nodes, one seeded conductivity model each (a mixed batch of uneven meshes)."""
+
+    out = []
+    for k, target in enumerate(node_targets):
+        step = 0.5 * np.sqrt(20000.0 / target)  # finer surface steps on the larger meshes
+        out.append(config_2p5d(target, electrodes, 1.0, seed + 100 * k, "ws", rows, step)[0])
+    return out
+
+
 def strip_dd_50k(seed=3):
     """~50k nodes (49 776 nodes, 98 550 triangles), 64 electrodes at 1 m, 2000 dipole-dipole rows."""
 

@@ -209,30 +209,65 @@ class BatchedSolver:
         return native.PRECOND_LEARNED, "learned", self._learned_damping(plan, values)
 
     def _learned_damping(self, plan: native.PcgPlan, values):
+        """Per-slot finest prolongation damping from the trained network, laid out like ``values``
+        (the network runs once per distinct CSR pattern of the plan, on that pattern's systems)."""
+
         from . import learned
 
         if self.damping_network is None:
             if not learned.DEFAULT_WEIGHTS.exists():
                 raise RuntimeError(f"no trained damping network at {learned.DEFAULT_WEIGHTS}")
             self.damping_network = learned.DampingNetwork.load()
-        graph = getattr(plan, "gnn_graph", None)
-        if graph is None:
-            same = np.all(plan.ptr_off == plan.ptr_off[0]) and np.all(plan.idx_off == plan.idx_off[0]) and \
-                np.all(plan.rows == plan.rows[0])
-            nnz = int(plan.host_pattern[0][int(plan.ptr_off[0]) + int(plan.rows[0])])
-            if not same or not np.array_equal(plan.val_off, plan.val_off[0] + np.arange(plan.n_sys) * nnz):
-                raise ValueError("the learned preconditioner needs systems that share one CSR pattern")
-            p0, i0, n = int(plan.ptr_off[0]), int(plan.idx_off[0]), int(plan.rows[0])
-            indptr = plan.host_pattern[0][p0:p0 + n + 1] - plan.host_pattern[0][p0]
+        groups = getattr(plan, "gnn_groups", None)
+        if groups is None:
+            groups = self._damping_groups(plan, values.device, self.amg_theta)
+            plan.gnn_groups = groups
+        flat = values.reshape(-1)
+        if len(groups) == 1 and groups[0]["contiguous"]:
+            g = groups[0]
+            v0 = int(plan.val_off[g["systems"][0]])
+            omega = self.damping_network.omega(g["graph"], flat[v0:], g["omega_p"], val_stride=g["graph"].nnz)
+            if v0 == 0:
+                return omega
+            out = torch.empty_like(flat)
+            out[v0:v0 + omega.numel()] = omega.reshape(-1)
+            return out
+        out = torch.empty_like(flat)
+        for g in groups:
+            nnz = g["graph"].nnz
+            offs = [int(plan.val_off[s]) for s in g["systems"]]
+            stacked = torch.stack([flat[o:o + nnz] for o in offs]).reshape(-1)
+            omega = self.damping_network.omega(g["graph"], stacked, g["omega_p"], val_stride=nnz).reshape(len(offs), nnz)
+            for j, o in enumerate(offs):
+                out[o:o + nnz] = omega[j]
+        return out
+
+    @staticmethod
+    def _damping_groups(plan: native.PcgPlan, device, theta: float) -> list[dict]:
+        """The plan's systems grouped by CSR pattern, each with its network graph (aggregation of
+        its representative matrix) and the cycle's constant finest damping of every system."""
+
+        from . import learned
+
+        keyed: dict[tuple[int, int, int], list[int]] = {}
+        for s in range(plan.n_sys):
+            keyed.setdefault((int(plan.ptr_off[s]), int(plan.idx_off[s]), int(plan.rows[s])), []).append(s)
+        rep_values, rep_off = plan.amg_representative
+        groups = []
+        for (p0, i0, n), systems in keyed.items():
+            indptr = plan.host_pattern[0][p0:p0 + n + 1].astype(np.int64)
+            nnz = int(indptr[-1] - indptr[0])
+            indptr = (indptr - indptr[0]).astype(np.int32)
             indices = plan.host_pattern[1][i0:i0 + nnz]
-            rep_values, rep_off = plan.amg_representative
-            agg = native.amg_aggregates(indptr, indices, rep_values[rep_off[0]:rep_off[0] + nnz], theta=self.amg_theta)
-            graph = learned.GnnGraph.from_pattern(indptr, indices, agg, values.device)
-            plan.gnn_graph = graph
-            plan.gnn_omega_p = torch.as_tensor([plan.amg_level(0, s)["omega"] for s in range(plan.n_sys)],
-                                               dtype=torch.float64, device=values.device)
-        flat = values.reshape(-1)[int(plan.val_off[0]):]
-        return self.damping_network.omega(graph, flat, plan.gnn_omega_p, val_stride=graph.nnz)
+            r0 = int(rep_off[systems[0]])
+            agg = native.amg_aggregates(indptr, indices, rep_values[r0:r0 + nnz], theta=theta)
+            graph = learned.GnnGraph.from_pattern(indptr, indices, agg, device)
+            omega_p = torch.as_tensor([plan.amg_level(0, s)["omega"] for s in systems], dtype=torch.float64,
+                                      device=device)
+            offs = np.asarray([int(plan.val_off[s]) for s in systems])
+            contiguous = bool(np.array_equal(offs, offs[0] + np.arange(len(systems)) * nnz))
+            groups.append(dict(graph=graph, systems=systems, omega_p=omega_p, contiguous=contiguous))
+        return groups
 
     def solve(self, plan: native.PcgPlan, indptr, indices, values, b, x, *, tol=None, max_iter=None,
               preconditioner=None, raise_on_failure=True):
