@@ -232,6 +232,61 @@ class Timelapse:
         return baseline, oracle
 
 
+def mixed_batch(adt, torch, args) -> dict:
+    """C2's mixed batch: the 96-electrode Wenner-Schlumberger survey on 8 graded meshes of
+    5k-40k nodes, one model each, solved in one plan (``MixedMeshBatch``) against the same
+    operators solved one after another; apparent resistivity of the smallest mesh against the
+    oracle."""
+
+    import numpy as np
+
+    from oracle import ert2p5d as oracle2p5d
+
+    batch = importlib.import_module("ad-tlert_amd.batch")
+    cases = adt.cases.mixed_meshes_ws()
+    ops = []
+    t0 = time.perf_counter()
+    for c in cases:
+        mesh = adt.Mesh.from_arrays(c.nodes, c.cells)
+        survey = adt.Survey.from_arrays(c.electrodes, c.measurements)
+        ops.append(adt.ERTForward2p5D.from_mesh_survey(mesh, survey, solver_tolerance=args.tol,
+                                                       preconditioner=args.preconditioner))
+    mixed = batch.MixedMeshBatch(ops)
+    rng = np.random.default_rng(61)
+    models = [[torch.as_tensor(c.conductivity * np.exp(0.1 * rng.standard_normal(c.cells.shape[0])))
+               for c in cases] for _ in range(args.mixed_steps + 1)]
+    rhoa = mixed.apparent_resistivity_values(models[0])  # plans, hierarchies, tuning
+    for op, sig in zip(ops, models[0]):
+        op.apparent_resistivity_values(sig)
+    setup_s = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(1, args.mixed_steps + 1):
+        mixed.apparent_resistivity_values(models[i])
+    torch.cuda.synchronize()
+    together_s = (time.perf_counter() - t0) / args.mixed_steps
+    worst = float(mixed.solver.last.relative_residual.max())
+    t0 = time.perf_counter()
+    for i in range(1, args.mixed_steps + 1):
+        for op, sig in zip(ops, models[i]):
+            op.apparent_resistivity_values(sig)
+    torch.cuda.synchronize()
+    alone_s = (time.perf_counter() - t0) / args.mixed_steps
+    c = cases[0]
+    ref = oracle2p5d.Forward2p5D(c.nodes, c.cells, c.electrodes, c.measurements)
+    err = relmax(rhoa[0].double().numpy(), ref.apparent_resistivity(models[0][0].double().numpy()))
+    rhs = mixed.rhs_per_solve
+    for op in ops:
+        op.close()
+    mixed.close()
+    log(f"mixed batch of {len(cases)} meshes: {rhs / together_s:.0f} RHS/s together, {rhs / alone_s:.0f} one by one")
+    return {"meshes_nodes": [int(c.nodes.shape[0]) for c in cases], "rhs_per_solve": rhs,
+            "rhs_solves_per_s": rhs / together_s, "rhs_solves_per_s_one_by_one": rhs / alone_s,
+            "speedup_vs_one_by_one": alone_s / together_s, "worst_true_relres": worst,
+            "oracle_rhoa_max_rel_err_smallest_mesh": err, "timed_solves": args.mixed_steps,
+            "setup_s": setup_s, "preconditioner": args.preconditioner}
+
+
 class Box3D:
     """3D P1 box workload: batches of conductivity models, 64 source right-hand sides each."""
 
@@ -356,6 +411,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=8,
                     help="time steps in the 2.5D CPU-baseline sample and its oracle comparison (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=32, help="Jacobian rows in the 2.5D CPU-baseline sample")
+    ap.add_argument("--mixed-steps", type=int, default=3,
+                    help="timed solves of the 8-mesh mixed batch with the time-lapse workload (0: skip)")
     ap.add_argument("--cpu-rhs-3d", type=int, default=16,
                     help="source right-hand sides in the 3D CPU-baseline sample (0: skip)")
     ap.add_argument("--no-jacobian", action="store_true")
@@ -579,6 +636,8 @@ def measure(adt, torch, args, workload: str) -> dict:
                 result["roofline_sensitivity"]["traffic_source"] = f"profiles/{sens_file.name}"
         log(f"jacobian {wl.D}x{wl.C}: {ms:.1f} ms/launch -> {result['jacobian_rows_per_s']:.0f} rows/s")
 
+    if workload == "timelapse" and args.mixed_steps > 0:
+        result["mixed_batch"] = mixed_batch(adt, torch, args)
     jac_device = jac if workload == "timelapse" else None
     jac = None  # the 3D comparison computes its own block; release this one first
     baseline, oracle = wl.cpu_and_oracle(jac_device)
