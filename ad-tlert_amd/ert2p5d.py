@@ -419,7 +419,7 @@ class ERTForward2p5D:
         self.series_batch_steps = int(series_batch_steps)
         self.sensitivity_precision = option(sensitivity_precision, ("float64", "mixed"), "sensitivity_precision")
         self._solver = BatchedSolver(tolerance=float(solver_tolerance), preconditioner=option(
-            preconditioner, ("amg", "jacobi", "learned"), "preconditioner"))
+            preconditioner, ("amg", "jacobi", "learned", "ic0"), "preconditioner"))
         self._cache: dict = {}
         self._dev_state: _Device | None = None
         self._field_cache_obj = FieldCache(self.normal_field_cache_max_entries)

@@ -99,7 +99,7 @@ class ERTForward3D:
         self.normal_field_cache_max_entries = self._field_cache.max_entries
         self.sensitivity_precision = option(sensitivity_precision, ("float64", "mixed"), "sensitivity_precision")
         self._solver = BatchedSolver(tolerance=float(solver_tolerance),
-                                     preconditioner=option(preconditioner, ("amg", "jacobi", "learned"), "preconditioner"))
+                                     preconditioner=option(preconditioner, ("amg", "jacobi", "learned", "ic0"), "preconditioner"))
         self._cache: dict = {}
         electrodes = np.asarray(survey.electrode_positions, dtype=float)
         quads = survey.measurement_slots().numpy()

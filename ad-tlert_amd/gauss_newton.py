@@ -1,6 +1,6 @@
 """Gauss-Newton / Levenberg-Marquardt linear step of the inversion loop on the device.
 
-Follows ``inversion/optimizers.py:128-142, 245-340``: the model increment of the stacked
+Follows ``inversion/optimizers.py:128-142, 245-359``: the model increment of the stacked
 linearized system ``A dm ~= b`` (weighted log-data Jacobian rows over regularization rows, as
 the inversion assembles it) is the least-squares solution by CGLS, with Levenberg-Marquardt
 damping appended as ``sqrt(lambda) I`` rows, and is scaled so that no entry exceeds
@@ -58,7 +58,7 @@ def _device_csr(matrix: sp.csr_matrix, device):
 
 
 def gpu_cgls(matrix, rhs, *, max_iterations: int, tolerance: float, return_info: bool = False):
-    """``argmin ||A x - b||`` by the native CGLS (``optimizers.py:304-340``); A is any scipy
+    """``argmin ||A x - b||`` by the native CGLS (``optimizers.py:314-359``); A is any scipy
     sparse matrix (or dense array), uploaded with its transpose as CSR."""
 
     a = sp.csr_matrix(matrix, dtype=np.float64)
@@ -79,7 +79,7 @@ def gpu_cgls(matrix, rhs, *, max_iterations: int, tolerance: float, return_info:
 
 
 def solve_linearized(matrix, rhs, config) -> np.ndarray:
-    """Least-squares solution of ``A x ~= b`` with the configured solver (``optimizers.py:270-302``)."""
+    """Least-squares solution of ``A x ~= b`` with the configured solver (``optimizers.py:270-311``)."""
 
     if config.linearized_solver != "gpu_cgls":
         raise NotImplementedError(f"linearized_solver={config.linearized_solver!r}: only the device CGLS "

@@ -50,6 +50,9 @@ _SIGNATURES = {
     "adt_gnn_backward": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P, P]),
     "adt_gnn_backward_static": (c_int, [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P]),
     "adt_cgls_work_doubles": (c_i64, [c_int, c_int]),
+    "adt_ic0_work_doubles": (c_i64, [c_int, c_int, c_int, c_int, c_int]),
+    "adt_ic0_pcg": (c_int, [c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, c_int, P, P, P, P, P, c_int, P, c_double,
+                            c_int, c_int, c_int, c_int, P, P, P, P]),
     "adt_cgls": (c_int, [c_int, c_int, P, P, P, P, P, P, c_i64, P, P, c_int, c_double, c_int, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
@@ -423,3 +426,44 @@ def cgls(a, at, b, x, max_iter: int, tol: float, check_every: int = 8) -> tuple[
                        ptr(b), ptr(x), int(max_iter), float(tol), int(check_every), ptr(work), host(iters),
                        host(status), stream()))
     return int(iters[0]), int(status[0])
+
+
+def ic0_patterns(indptr: np.ndarray, indices: np.ndarray):
+    """Host-built patterns of an IC(0) factor of a symmetric CSR pattern: the lower triangle
+    (``lptr``, ``lidx`` ascending with the diagonal last, ``lsrc`` = position in A) and its
+    transpose by rows (``uptr``, ``uidx``, ``upos`` = position in the lower pattern)."""
+
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    n = indptr.size - 1
+    rows = np.repeat(np.arange(n), np.diff(indptr))
+    lower = np.flatnonzero(indices <= rows)  # CSR rows are sorted: per row ascending, diagonal last
+    lrows, lcols = rows[lower], indices[lower]
+    counts = np.bincount(lrows, minlength=n)
+    last = np.cumsum(counts) - 1
+    if np.any(counts == 0) or not np.array_equal(lcols[last], np.arange(n)):
+        raise ValueError("IC(0) needs a stored diagonal in every row")
+    lptr = np.concatenate(([0], np.cumsum(np.bincount(lrows, minlength=n)))).astype(np.int32)
+    order = np.lexsort((lrows, lcols))  # by column, then row: the rows of L^T
+    uptr = np.concatenate(([0], np.cumsum(np.bincount(lcols, minlength=n)))).astype(np.int32)
+    return dict(lptr=lptr, lidx=lcols.astype(np.int32), lsrc=lower.astype(np.int32), uptr=uptr,
+                uidx=lrows[order].astype(np.int32), upos=order.astype(np.int32), lnnz=int(lower.size))
+
+
+def ic0_pcg(n_sys, n, r, rp, indptr, indices, patterns, values, b, n_b_blocks, x, tol, max_iter, *, factor_sweeps=3,
+            solve_sweeps=2, check_every=8):
+    """IC(0)-preconditioned CG of ``n_sys`` systems sharing one pattern (``adt_ic0_pcg``);
+    ``patterns`` are the device tensors of :func:`ic0_patterns`.  Returns ``(rc, iters, relres)``."""
+
+    lib = library()
+    nnz = int(indices.numel())
+    lnnz = int(patterns["lidx"].numel())
+    work = torch.empty(int(lib.adt_ic0_work_doubles(n_sys, n, nnz, lnnz, rp)), dtype=torch.float64, device=b.device)
+    iters = np.zeros((n_sys, r), dtype=np.int32)
+    relres = np.zeros((n_sys, r), dtype=np.float64)
+    rc = lib.adt_ic0_pcg(int(n_sys), int(n), nnz, int(r), int(rp), ptr(indptr), ptr(indices), ptr(patterns["lptr"]),
+                         ptr(patterns["lidx"]), ptr(patterns["lsrc"]), lnnz, ptr(patterns["uptr"]),
+                         ptr(patterns["uidx"]), ptr(patterns["upos"]), ptr(values), ptr(b), int(n_b_blocks), ptr(x),
+                         float(tol), int(max_iter), int(factor_sweeps), int(solve_sweeps), int(check_every),
+                         ptr(work), host(iters), host(relres), stream())
+    return rc, iters, relres

@@ -54,7 +54,7 @@ def choose_rows_per_tile(n_sys: int, rows: int, r: int, lanes: int, target_tiles
     return -(-rb // per_pass) * per_pass
 
 
-PRECONDITIONERS = ("jacobi", "amg", "learned")
+PRECONDITIONERS = ("jacobi", "amg", "learned", "ic0")
 
 
 @dataclass
@@ -82,6 +82,8 @@ class BatchedSolver:
     amg_coarse_max: int = 128      # coarsest operators up to 128 rows are inverted in LDS
     amg_stall_limit: int = 64
     damping_network: object = None  # learned.DampingNetwork for preconditioner="learned"
+    ic0_factor_sweeps: int = 3     # fixed-point sweeps of the IC(0) factor
+    ic0_solve_sweeps: int = 2      # Jacobi sweeps per triangular solve of the IC(0) preconditioner
     autotune: bool = True          # time the candidate lane widths before the first plan of a shape
     tuning: dict = field(default_factory=dict)  # shape -> {lanes: seconds per iteration}
     _plans: dict = field(default_factory=dict)
@@ -193,6 +195,8 @@ class BatchedSolver:
         name = self.preconditioner if preconditioner is None else preconditioner
         if name == "jacobi":
             return native.PRECOND_JACOBI, "jacobi", None
+        if name == "ic0":
+            return None, "ic0", None
         if name not in ("amg", "learned"):
             raise ValueError(f"preconditioner must be one of {PRECONDITIONERS}, got {name!r}")
         if plan.amg_levels == 0 and not getattr(plan, "amg_failed", False):
@@ -276,12 +280,58 @@ class BatchedSolver:
         tol = self.tolerance if tol is None else tol
         max_iter = self.max_iterations if max_iter is None else max_iter
         precond, used, data = self._precond(plan, values, preconditioner)
+        if used == "ic0":
+            return self._solve_ic0(plan, indptr, indices, values, b, x, tol, max_iter, raise_on_failure)
         t0 = time.perf_counter()
         rc, iters, relres = plan.solve(indptr, indices, values, b, x, tol, max_iter, precond=precond,
                                        precond_data=data, check_every=self.check_every)
         st = plan.stats()
         fallback = st["fallback_rhs"] if precond != native.PRECOND_JACOBI else 0
         self.last = SolveStats(iters, relres, time.perf_counter() - t0, used, fallback, st["polished_rhs"])
+        self.last_plan = plan
+        if raise_on_failure and rc == native.ADT_ERR_NONFINITE:
+            raise FloatingPointError("conjugate gradients produced non-finite values")
+        if raise_on_failure and rc == native.ADT_ERR_NOT_CONVERGED:
+            raise RuntimeError(f"conjugate gradients did not reach tol={tol} in {max_iter} iterations "
+                               f"(worst relative residual {float(relres.max()):.3e})")
+        if rc not in (native.ADT_OK, native.ADT_ERR_NONFINITE, native.ADT_ERR_NOT_CONVERGED):
+            native.check(rc)
+        return self.last
+
+    def _solve_ic0(self, plan: native.PcgPlan, indptr, indices, values, b, x, tol, max_iter, raise_on_failure):
+        """IC(0)-preconditioned CG (``adt_ic0_pcg``): the systems must share one CSR pattern, with
+        values ``nnz`` apart and interleaved blocks one after another; a right-hand side shared by
+        several systems (``b_off`` cycling through blocks) is kept shared."""
+
+        n = int(plan.rows[0])
+        nnz = int(plan.host_pattern[0][n]) if plan.host_pattern is not None else int(indices.numel())
+        same = (np.all(plan.rows == n) and np.all(plan.ptr_off == 0) and np.all(plan.idx_off == 0)
+                and np.array_equal(plan.val_off, np.arange(plan.n_sys) * nnz)
+                and np.array_equal(plan.vec_off, np.arange(plan.n_sys) * n * plan.rp)
+                and int(indptr.numel()) == n + 1 and int(indices.numel()) == nnz)
+        if not same:
+            raise ValueError("the IC(0) preconditioner needs systems that share one CSR pattern")
+        block = n * plan.rp
+        if plan.b_off is None:
+            n_b = plan.n_sys
+        else:
+            n_b = int(plan.b_off.max() // block) + 1
+            if not np.array_equal(plan.b_off, (np.arange(plan.n_sys) % n_b) * block) or plan.n_sys % n_b:
+                raise ValueError("the IC(0) preconditioner needs right-hand sides cycling through whole blocks")
+        patterns = getattr(plan, "ic0_device", None)
+        if patterns is None:
+            host_ptr = plan.host_pattern[0][: n + 1] if plan.host_pattern is not None else indptr[: n + 1].cpu().numpy()
+            host_idx = plan.host_pattern[1][:nnz] if plan.host_pattern is not None else indices[:nnz].cpu().numpy()
+            host = native.ic0_patterns(host_ptr, host_idx)
+            patterns = {k: (torch.as_tensor(v, device=values.device) if isinstance(v, np.ndarray) else v)
+                        for k, v in host.items()}
+            plan.ic0_device = patterns
+        t0 = time.perf_counter()
+        rc, iters, relres = native.ic0_pcg(plan.n_sys, n, plan.r, plan.rp, indptr, indices, patterns,
+                                           values.contiguous(), b.contiguous(), n_b, x, tol, max_iter,
+                                           factor_sweeps=self.ic0_factor_sweeps, solve_sweeps=self.ic0_solve_sweeps,
+                                           check_every=8)
+        self.last = SolveStats(iters, relres, time.perf_counter() - t0, "ic0", 0, 0)
         self.last_plan = plan
         if raise_on_failure and rc == native.ADT_ERR_NONFINITE:
             raise FloatingPointError("conjugate gradients produced non-finite values")

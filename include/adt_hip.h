@@ -284,7 +284,7 @@ int adt_amg_aggregates(int n, const int32_t* h_indptr, const int32_t* h_indices,
  * Gauss-Newton linear step: CGLS for min ||A x - b||_2 on the device.
  *
  * Replaces the GPU CGLS of the inversion's linearized step (inversion/optimizers.py:270-359,
- * _gpu_cgls at 304-340): x = 0, r = b, s = A^T r, p = s, gamma = gamma0 = s.s; per iteration
+ * _gpu_cgls at 314-359): x = 0, r = b, s = A^T r, p = s, gamma = gamma0 = s.s; per iteration
  * q = A p, alpha = gamma / q.q, x += alpha p, r -= alpha q, s = A^T r, gamma' = s.s, stop when
  * gamma' <= 0 or gamma' / gamma0 < tol, else p = s + (gamma' / gamma) p.  A (m x n) and A^T
  * (n x m) are CSR with nnz entries each (int32 indices); d_work holds adt_cgls_work_doubles(m, n)
@@ -297,6 +297,33 @@ int64_t adt_cgls_work_doubles(int m, int n);
 int adt_cgls(int m, int n, const int32_t* d_ptr, const int32_t* d_idx, const double* d_val, const int32_t* d_tptr,
              const int32_t* d_tidx, const double* d_tval, int64_t nnz, const double* d_b, double* d_x, int max_iter,
              double tol, int check_every, double* d_work, int* h_iters, int* h_status, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * IC(0)-preconditioned batched CG: the second preconditioner that replaces the batched sparse
+ * Cholesky of the wavenumber systems (forward/cudss.py:63-122, called from
+ * forward/ert2p5d.py:532-553 and forward/ert3d.py:463-481), next to ADT_PRECOND_JACOBI.
+ *
+ * n_sys SPD systems share one CSR pattern (d_indptr, d_indices: n rows, nnz entries); their
+ * values are nnz apart in d_values.  Each system's iterate is an interleaved (n, rp) block of
+ * d_x (r used columns, the rest untouched) and uses right-hand-side block s % n_b_blocks of d_b.
+ * The incomplete factor L (lower pattern of A) comes from factor_sweeps fine-grained fixed-point
+ * sweeps started at l_ij = a_ij / sqrt(a_jj); z = M^-1 r applies L y = r and L^T z = y by
+ * 1 + solve_sweeps Jacobi sweeps each, so M^-1 is a fixed SPD operator and CG is plain CG.
+ * The lower pattern comes from the host: d_lptr / d_lidx (rows, columns ascending, diagonal
+ * last), d_lsrc (position in A of each of the lnnz lower entries), and its transpose by rows
+ * d_uptr / d_uidx / d_upos (position in L).  Stop per column at ||b - A x|| <= tol ||b||, checked
+ * again on the true residual (at most two restarts).  d_work holds
+ * adt_ic0_work_doubles(n_sys, n, nnz, lnnz, rp) doubles.  h_iters / h_relres (n_sys x r) receive
+ * iterations and relative residuals.  Column sums run in a fixed order: bitwise repeatable.
+ * Returns ADT_ERR_NOT_CONVERGED at max_iter, ADT_ERR_NONFINITE on a breakdown.
+ * ---------------------------------------------------------------------------------------- */
+int64_t adt_ic0_work_doubles(int n_sys, int n, int nnz, int lnnz, int rp);
+int adt_ic0_pcg(int n_sys, int n, int nnz, int r, int rp, const int32_t* d_indptr, const int32_t* d_indices,
+                const int32_t* d_lptr, const int32_t* d_lidx, const int32_t* d_lsrc, int lnnz,
+                const int32_t* d_uptr, const int32_t* d_uidx, const int32_t* d_upos, const double* d_values,
+                const double* d_b, int n_b_blocks, double* d_x, double tol, int max_iter, int factor_sweeps,
+                int solve_sweeps, int check_every, double* d_work, int32_t* h_iters, double* h_relres,
+                void* stream);
 
 /* ------------------------------------------------------------------------------------------
  * Learned multigrid damping (no reference counterpart; it feeds ADT_PRECOND_LEARNED).

@@ -1,6 +1,6 @@
 """CGLS for ``min ||A x - b||_2`` in FP64 numpy (TEST INFRASTRUCTURE ONLY).
 
-Restates the reference's GPU CGLS (``inversion/optimizers.py:304-340``): the same iteration
+Restates the reference's GPU CGLS (``inversion/optimizers.py:314-359``): the same iteration
 (Hestenes-Stiefel on the normal equations without forming ``A^T A``) and the same stop rule,
 ``gamma' <= 0`` or ``gamma' / gamma0 < tol`` with ``gamma = ||A^T r||^2``; also stops when
 ``||A p||^2 <= 0``.  Returns the solution and the number of completed iterations.  Pinned in

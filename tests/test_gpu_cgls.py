@@ -1,6 +1,6 @@
 """Device CGLS (the Gauss-Newton linear step) against the FP64 CGLS oracle, least squares and the
 damped normal equations; the Levenberg-Marquardt step on the C1 strip's log Jacobian stacked
-over a first-order smoothness operator.  Mirrors ``inversion/optimizers.py:245-340``."""
+over a first-order smoothness operator.  Mirrors ``inversion/optimizers.py:245-359``."""
 
 from __future__ import annotations
 
