@@ -19,7 +19,7 @@ int set_error(int code, const char* fmt, ...) {
 
 extern "C" const char* adt_last_error(void) { return adt::g_error; }
 
-extern "C" int adt_abi_version(void) { return 6; }
+extern "C" int adt_abi_version(void) { return 7; }
 
 extern "C" int adt_device_info(int device, int* h_cus, char* h_name, int name_len) {
     hipDeviceProp_t prop;

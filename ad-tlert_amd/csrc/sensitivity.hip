@@ -108,15 +108,19 @@ int launch_sensitivity(const SensArgs& a, int precision, hipStream_t st) {
 // (c') Gram form of the normal sensitivity on interleaved (N, Rp) fields, for many wavenumbers:
 //   G_c[e][f] = sum_w sum_i phi_w[e, dof_ci] (T_wc phi_w[f, dof_c.])_i,   T_wc = ws_w K_c + wm_w M_c
 //   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])          (row S: remote, zero)
-// which equals the per-output formula of sensitivity_kernel; the element templates are symmetric
-// (stiffness and mass), so G_c is symmetric and only its upper triangle is formed and kept.  One
-// workgroup per cell: the cell's field rows Phi[(w, i)][e] (one contiguous interleaved row per
-// node) and Psi = T Phi are staged in LDS in chunks of wavenumbers, the 4 waves accumulate the
-// upper-triangle 16 x 16 tiles of G_c on the FP64 matrix cores (S <= 16 Bt), then the upper
-// triangle replaces the staging area in LDS and the workgroup writes column c of every
-// measurement row.  Cells are mapped XCD-contiguously, so neighbouring columns of a row are
-// written by workgroups that share one L2.  FP64 with a fixed order of rows (w, i); bitwise
-// repeatable.
+// which equals the per-output formula of sensitivity_kernel.  The element templates are symmetric
+// positive semidefinite, so T_wc = R^T R with R = D^1/2 L^T from T = L D L^T (L unit lower, the
+// pivots of a semidefinite template that vanish to rounding are dropped), and
+//   G_c = Y^T Y,   Y = [R_w Phi_w]_w   (rows (w, i), one column per source)
+// is a symmetric rank-(W K) product: only its upper-triangle 16 x 16 tiles are formed.  One
+// workgroup per cell: the W factors R_w are computed into LDS (one thread per wavenumber), every
+// wave gathers whole field rows of its wavenumbers (the cell's K nodes, lanes along the sources,
+// all of them in flight together) and stores Y = R Phi to LDS; the 4 waves then accumulate the
+// tiles on the FP64 matrix cores, the upper triangle replaces Y in LDS, and the workgroup writes
+// column c of every measurement row.  All wavenumbers are staged at once when Y fits the LDS
+// budget (one global round trip per cell), else in chunks.  Cells are mapped XCD-contiguously, so
+// neighbouring columns of a row are written by workgroups that share one L2.  FP64 with a fixed
+// order of rows (w, i); bitwise repeatable.
 // ------------------------------------------------------------------------------------------
 
 struct GramArgs {
@@ -124,15 +128,14 @@ struct GramArgs {
     const int32_t* dofs;
     const double* stiff;
     const double* mass;
-    const int32_t* a;
-    const int32_t* b;
-    const int32_t* m;
-    const int32_t* n;
+    const uint4* quad;         // per quadrupole: LDS byte offsets of G(m,a), G(m,b), G(n,a), G(n,b)
     double* out;
     const int32_t* col_ptr;    // optional column aggregation: column p sums cells col_cells[col_ptr[p] ..]
     const int32_t* col_cells;
     int S, Sp, K, W, rp, wchunk, rows_max, C, D;
-    int P;                     // output columns (= workgroups): C, or the aggregated column count
+    int P;                     // output columns: C, or the aggregated column count
+    int group_doubles;         // LDS doubles of one column group (factors and staged rows)
+    int factor_doubles;        // LDS doubles of the factors R_w of all wavenumbers
     int64_t vec_off[kMaxWaves];
     double ws[kMaxWaves];
     double wm[kMaxWaves];
@@ -183,46 +186,109 @@ __device__ __forceinline__ void write_cell_column(const int32_t* a, const int32_
     }
 }
 
+// Byte offsets in the Gram triangle of gram_kernel (upper triangle over S + 1 slots, slot S the
+// remote electrode) of the four G entries of every quadrupole: electrode indices >= S or < 0 are
+// the remote slot.
+__global__ void quad_offsets_kernel(int D, int S, const int32_t* a, const int32_t* b, const int32_t* m,
+                                    const int32_t* n, uint4* quad) {
+    const int d = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (d >= D) return;
+    auto slot = [S](int e) { return e < 0 || e > S ? S : e; };
+    auto off = [S](int e, int f) {
+        const int lo = min(e, f), hi = max(e, f);
+        return static_cast<uint32_t>(tri_index(lo, hi, S + 1)) * 8u;
+    };
+    const int ia = slot(adt::gview(a)[d]), ib = slot(adt::gview(b)[d]);
+    const int im = slot(adt::gview(m)[d]), in = slot(adt::gview(n)[d]);
+    adt::gview(quad)[d] = make_uint4(off(im, ia), off(im, ib), off(in, ia), off(in, ib));
+}
+
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // Upper-triangle 16 x 16 tiles of G_c per wave: Sp = 16 Bt sources give Bt (Bt + 1) / 2 tiles
 // dealt over the workgroup's 4 waves.
 constexpr int gram_tiles_per_wave(int bt) { return (bt * (bt + 1) / 2 + 3) / 4; }
 
-// floor(r / k) for 0 <= r < 2^16 and 1 <= k <= 256 from a float reciprocal: (r + 1/2) / k lies at
-// least 1 / (2k) from an integer, far beyond the float rounding of the product.
-__device__ __forceinline__ int div_small(int r, float inv_k) {
-    return static_cast<int>((static_cast<float>(r) + 0.5f) * inv_k);
+// In place: the upper triangle of the symmetric positive semidefinite K x K matrix in R (row-major)
+// becomes R = D^1/2 L^T of T = L D L^T, i.e. T = R^T R (the upper Cholesky factor), by outer
+// products row by row.  A pivot at or below 1e-13 of the largest diagonal entry is rounding left
+// by a semidefinite template: its row of R is zero.  Entries below the diagonal are zeroed.
+__device__ void template_root(double* R, int K) {
+    double tmax = 0.0;
+    for (int i = 0; i < K; ++i) tmax = fmax(tmax, R[i * K + i]);
+    const double tiny = 1e-13 * tmax;
+    for (int i = 0; i < K; ++i) {
+        const double p = R[i * K + i];
+        if (p > tiny) {
+            const double s = sqrt(p), inv = 1.0 / s;
+            R[i * K + i] = s;
+            for (int j = i + 1; j < K; ++j) R[i * K + j] *= inv;
+            for (int j = i + 1; j < K; ++j)
+                for (int k = j; k < K; ++k) R[j * K + k] = fma(-R[i * K + j], R[i * K + k], R[j * K + k]);
+        } else {
+            for (int j = i; j < K; ++j) R[i * K + j] = 0.0;
+        }
+        for (int j = 0; j < i; ++j) R[i * K + j] = 0.0;
+    }
 }
 
-// Staged rows per wave whose loads are in flight together.
-constexpr int kGramStageRows = 6;
+// Output columns per workgroup of gram_kernel.  One: four independent workgroups per CU (40 KB of
+// LDS each) overlap their load, product and store phases.  Blocks of 2 or 4 adjacent columns
+// (16- or 32-byte row segments per store) were measured slower on the 20k-node time-lapse mesh
+// (1.36 and 1.38 ms against 1.17 ms): their triangles take the CU's whole LDS, leaving one
+// workgroup per CU whose phases run one after another.
+constexpr int kGramColumns = 1;
 
-// G_c on the matrix cores: v_mfma_f64_16x16x4f64 multiplies a 16 x 4 block of Phi^T by a 4 x 16
-// block of Psi; lane l supplies Phi[r0 + l / 16][16 I + l % 16] and Psi[r0 + l / 16][16 J + l % 16]
-// and holds G[16 I + l / 16 + 4 v][16 J + l % 16] in item v of its accumulator.  Only the tiles
-// with I <= J are formed (G_c is symmetric).  The staged rows are padded with zero rows to a
-// multiple of 4.  The wave-to-tile assignment rotates with the block index, so the wave with the
-// extra tile falls on every SIMD in turn.  The source offset of every staged row (w, i) is
-// tabulated once per cell, so staging a row is one LDS read and one global load per element.
+// Y in LDS is stored in 4-row x 16-source bricks of 64 doubles, brick (r / 4, e / 16) at
+// ((r / 4) * BT + e / 16) * 64 and entry (r % 4, e % 16) inside it, so that the operand of one
+// matrix-core step (4 rows x 16 sources) is 512 contiguous bytes: conflict-free LDS reads.
 template <int BT>
-__global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
+__device__ __forceinline__ int brick_index(int r, int e) {
+    return (((r >> 2) * BT + (e >> 4)) << 6) + ((r & 3) << 4) + (e & 15);
+}
+
+// G_c on the matrix cores: v_mfma_f64_16x16x4f64 multiplies a 16 x 4 block of Y^T by a 4 x 16
+// block of Y; lane l supplies Y[r0 + l / 16][16 I + l % 16] and Y[r0 + l / 16][16 J + l % 16] and
+// holds G[16 I + l / 16 + 4 v][16 J + l % 16] in item v of its accumulator.  Only the tiles with
+// I <= J are formed.  The staged rows are padded with zero rows to a multiple of 4.
+//
+// A workgroup of CB x 256 threads owns CB adjacent output columns: group g (waves 4g .. 4g + 3)
+// forms the G of column g with its own LDS region (factors R_w and the staged Y rows); wave w of
+// every group runs on SIMD w, and the tile sets of the groups' waves rotate so that each SIMD gets
+// the same number of tiles.  Every group runs the same number of steps (the longest column's cell
+// count) so that all threads meet at every barrier.  Then the CB triangles are stored slot-major
+// (G[slot * CB + g]) and the workgroup writes J[d, c0 .. c0 + CB) for every quadrupole d: adjacent
+// lanes write adjacent columns of one row, so a store instruction writes 64 / CB row segments of
+// CB x 8 contiguous bytes instead of 64 scattered doubles.  KM bounds the local size K.
+template <int BT, int KM, int CB>
+__global__ __launch_bounds__(256 * CB) void gram_kernel(GramArgs A) {
     constexpr int Sp = 16 * BT, TPW = gram_tiles_per_wave(BT), n_tiles = BT * (BT + 1) / 2;
-    constexpr int kCols = (Sp + 63) / 64;  // staged sources per lane and row
+    constexpr int kCols = (Sp + 63) / 64;    // staged sources per lane and row
+    constexpr int kGroup = KM <= 4 ? 2 : 1;  // wavenumbers per wave whose rows are in flight together
+    constexpr int NT = 256 * CB;
     extern __shared__ double lds[];
-    const int col = adt::xcd_tile(static_cast<int>(blockIdx.x), A.P);
-    int c_begin, c_end;
-    column_cells(A.col_ptr, col, c_begin, c_end);
-    const int K = A.K, S = A.S, tid = static_cast<int>(threadIdx.x);
-    const int rows_max = A.rows_max;
-    const float inv_k = 1.0f / static_cast<float>(K);
-    double* phi = lds;                                        // rows_max x Sp
-    double* psi = lds + static_cast<int64_t>(rows_max) * Sp;  // rows_max x Sp
-    double* tw = psi + static_cast<int64_t>(rows_max) * Sp;   // wchunk x K x K: T_wc of the chunk
-    int64_t* roff = reinterpret_cast<int64_t*>(tw + A.wchunk * K * K);  // W x K row offsets in x
-    const int lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wave = (wv + static_cast<int>(blockIdx.x)) & 3;
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63;
+    const int wave_id = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int grp = wave_id >> 2, wv = wave_id & 3;
+    const int n_blocks = (A.P + CB - 1) / CB;
+    const int blk = adt::xcd_tile(static_cast<int>(blockIdx.x), n_blocks);
+    const int c0 = blk * CB;
+    const int K = A.K, S = A.S;
+    double* rl = lds + static_cast<int64_t>(grp) * A.group_doubles;  // W x K x K: R_w of the cell
+    double* Y = rl + A.factor_doubles;                               // bricks of the staged rows
+    // cells of every column of the block: all groups step together through the longest list
+    int c_begin = 0, c_end = 0, steps = 0;
+#pragma unroll
+    for (int k = 0; k < CB; ++k) {
+        int b0 = 0, b1 = 0;
+        if (c0 + k < A.P) column_cells(A.col_ptr, c0 + k, b0, b1);
+        steps = max(steps, b1 - b0);
+        if (k == grp) {
+            c_begin = b0;
+            c_end = b1;
+        }
+    }
+    const int wave = (wv + grp + static_cast<int>(blockIdx.x)) & 3;
     int ti[TPW], tj[TPW];
     int nt = 0;
 #pragma unroll
@@ -237,78 +303,91 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
     f64x4 acc[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
-    // G of the column = the sum of its cells' G_c (J is linear in G): the accumulators run on over
+    const auto x = adt::gview(A.x);
+    const int gtid = tid & 255;  // thread index within the group
+    // G of a column = the sum of its cells' G_c (J is linear in G): the accumulators run on over
     // the cells of an aggregated column, in list order
-    for (int ci = c_begin; ci < c_end; ++ci) {
-    const int c = A.col_ptr ? adt::gview(A.col_cells)[ci] : ci;
-    const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
-    const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
-    const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
-    __syncthreads();  // the previous cell's staging has read its row offsets
-    for (int r = tid; r < A.W * K; r += 256) {
-        const int w = div_small(r, inv_k);
-        roff[r] = A.vec_off[w] + static_cast<int64_t>(dof[r - w * K]) * A.rp;
-    }
-    for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
-        const int nw = min(A.wchunk, A.W - w0);
-        const int rows = nw * K, rows4 = (rows + 3) & ~3;
+    for (int step = 0; step < steps; ++step) {
+        const bool active = c_begin + step < c_end;
+        const int c = active ? __builtin_amdgcn_readfirstlane(A.col_ptr ? adt::gview(A.col_cells)[c_begin + step]
+                                                                         : c_begin + step)
+                             : 0;
+        const double* Kc = A.stiff + static_cast<int64_t>(c) * K * K;
+        const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
+        const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
+        __syncthreads();  // the previous step's factors and rows are consumed
+        if (active && gtid < A.W) {
+            double* R = rl + gtid * K * K;
+            const double ws = A.ws[gtid], wm = A.wm[gtid];
+            for (int e = 0; e < K * K; ++e) R[e] = Mc ? fma(wm, Mc[e], ws * Kc[e]) : ws * Kc[e];
+            template_root(R, K);
+        }
+        int64_t node[KM];  // wave-uniform row offsets of the cell's nodes
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
+            node[k] = (active && k < K) ? static_cast<int64_t>(__builtin_amdgcn_readfirstlane(dof[k])) * A.rp : 0;
         __syncthreads();
-        // Phi: wave-uniform rows (row offset read once per row), lanes along the sources;
-        // kGramStageRows rows per wave in flight
-        for (int r0 = wv; r0 < rows4; r0 += 4 * kGramStageRows) {
-            double v[kGramStageRows][kCols];
+        for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
+            const int nw = min(A.wchunk, A.W - w0);
+            const int rows = nw * K, rows4 = (rows + 3) & ~3;
+            if (w0 > 0) __syncthreads();  // the previous chunk's products have read Y
+            if (active) {
+                // Y rows of the wave's wavenumbers: the K field rows of each, lanes along the sources
+                for (int wl0 = wv; wl0 < nw; wl0 += 4 * kGroup) {
+                    double v[kGroup][KM][kCols];
 #pragma unroll
-            for (int u = 0; u < kGramStageRows; ++u) {
-                const int r = r0 + 4 * u;
-                const int64_t off = r < rows ? roff[w0 * K + r] : 0;
+                    for (int g = 0; g < kGroup; ++g) {
+                        const int wl = wl0 + 4 * g;
+                        const int64_t base = wl < nw ? A.vec_off[w0 + wl] : 0;
 #pragma unroll
-                for (int q = 0; q < kCols; ++q) {
-                    const int e = lane + 64 * q;
-                    v[u][q] = r < rows && e < S ? A.x[off + e] : 0.0;
+                        for (int k = 0; k < KM; ++k) {
+                            const auto row = x + (base + node[k]);  // wave-uniform: lanes add their source
+#pragma unroll
+                            for (int q = 0; q < kCols; ++q) {
+                                const int e = lane + 64 * q;
+                                v[g][k][q] = (wl < nw && k < K && e < S) ? row[e] : 0.0;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int g = 0; g < kGroup; ++g) {
+                        const int wl = wl0 + 4 * g;
+                        if (wl >= nw) continue;
+                        const double* R = rl + (w0 + wl) * K * K;
+#pragma unroll
+                        for (int i = 0; i < KM; ++i) {
+                            if (i >= K) continue;
+#pragma unroll
+                            for (int q = 0; q < kCols; ++q) {
+                                const int e = lane + 64 * q;
+                                double y = 0.0;
+#pragma unroll
+                                for (int j = i; j < KM; ++j)
+                                    if (j < K) y = fma(R[i * K + j], v[g][j][q], y);
+                                if (e < Sp) Y[brick_index<BT>(wl * K + i, e)] = y;
+                            }
+                        }
+                    }
+                }
+                for (int t = gtid; t < (rows4 - rows) * Sp; t += 256) Y[brick_index<BT>(rows + t / Sp, t % Sp)] = 0.0;
+            }
+            __syncthreads();
+            if (active) {
+                const double* yl = Y + lane;
+                for (int r0 = 0; r0 < rows4; r0 += 4) {
+                    const double* yr = yl + (r0 >> 2) * BT * 64;
+#pragma unroll
+                    for (int t = 0; t < TPW; ++t)
+                        if (t < nt)
+                            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(yr[ti[t] * 64], yr[tj[t] * 64], acc[t], 0,
+                                                                          0, 0);
                 }
             }
-#pragma unroll
-            for (int u = 0; u < kGramStageRows; ++u) {
-                const int r = r0 + 4 * u;
-#pragma unroll
-                for (int q = 0; q < kCols; ++q) {
-                    const int e = lane + 64 * q;
-                    if (r < rows4 && e < Sp) phi[r * Sp + e] = v[u][q];
-                }
-            }
         }
-        for (int t = tid; t < nw * K * K; t += 256) {
-            const int wl = div_small(t, 1.0f / static_cast<float>(K * K)), ij = t - wl * K * K, w = w0 + wl;
-            tw[t] = Mc ? fma(A.wm[w], Mc[ij], A.ws[w] * Kc[ij]) : A.ws[w] * Kc[ij];
-        }
-        __syncthreads();
-        // Psi = T Phi per (wavenumber, source column): the K staged values of the column give its
-        // K rows of Psi; padding rows are zero
-        for (int p = tid; p < nw * Sp; p += 256) {
-            const int wl = p / Sp, f = p % Sp;
-            const double* pc = phi + wl * K * Sp + f;
-            const double* tr = tw + wl * K * K;
-            for (int i = 0; i < K; ++i) {
-                double v = 0.0;
-                for (int j = 0; j < K; ++j) v = fma(tr[i * K + j], pc[j * Sp], v);
-                psi[(wl * K + i) * Sp + f] = v;
-            }
-        }
-        for (int t = tid; t < (rows4 - rows) * Sp; t += 256) psi[rows * Sp + t] = 0.0;
-        __syncthreads();
-        const double* ph = phi + (lane >> 4) * Sp + (lane & 15);
-        const double* ps = psi + (lane >> 4) * Sp + (lane & 15);
-        for (int r0 = 0; r0 < rows4; r0 += 4) {
-#pragma unroll
-            for (int t = 0; t < TPW; ++t)
-                if (t < nt)
-                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph[r0 * Sp + 16 * ti[t]], ps[r0 * Sp + 16 * tj[t]],
-                                                                  acc[t], 0, 0, 0);
-        }
-    }
-    }  // cells of the column
+    }  // steps
     __syncthreads();
-    // upper triangle of G over S + 1 slots: slot S (the remote electrode) is a zero row
+    // upper triangles over S + 1 slots, slot-major over the CB columns (slot S: the remote
+    // electrode, a zero row)
     double* G = lds;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
@@ -317,26 +396,49 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs A) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int e = 16 * ti[t] + (lane >> 4) + 4 * v;
-            if (e < S && f < S && e <= f) G[tri_index(e, f, S + 1)] = acc[t][v];
+            if (e < S && f < S && e <= f) G[tri_index(e, f, S + 1) * CB + grp] = acc[t][v];
         }
     }
-    for (int e = tid; e <= S; e += 256) G[tri_index(e, S, S + 1)] = 0.0;
+    for (int e = gtid; e <= S; e += 256) G[tri_index(e, S, S + 1) * CB + grp] = 0.0;
     __syncthreads();
-    // branch-free lookup: indices >= S (and negative ones) read the remote slot
-    const unsigned s1 = static_cast<unsigned>(S), two_s1 = 2u * s1 + 1u;
-    auto g = [&](int e, int f) {
-        const unsigned ue = min(static_cast<unsigned>(e), s1), uf = min(static_cast<unsigned>(f), s1);
-        const unsigned lo = min(ue, uf), hi = max(ue, uf);
-        return G[(__umul24(lo, two_s1 - lo) >> 1) + hi];
-    };
-    write_cell_column(A.a, A.b, A.m, A.n, A.D, A.P, col, A.out, g);
+    // J[d, c0 + k] for (d, k) = (i / CB, i % CB): four slot reads of column k, three adds
+    constexpr int U = 4;
+    const auto q = adt::gview(A.quad);
+    const auto o = adt::gview(A.out);
+    const char* gb = reinterpret_cast<const char*>(G);
+    const int D = A.D, total = D * CB;
+    for (int i0 = tid; i0 < total; i0 += NT * U) {
+        uint4 s[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * NT;
+            s[u] = q[i < total ? i / CB : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * NT, d = i / CB, k = i % CB;
+            const uint32_t kb = static_cast<uint32_t>(k) * 8u;
+            auto at = [&](uint32_t byte) { return *reinterpret_cast<const double*>(gb + byte * CB + kb); };
+            const double v = ((at(s[u].x) - at(s[u].y)) - at(s[u].z)) + at(s[u].w);
+            if (i < total && c0 + k < A.P) o[static_cast<int64_t>(d) * A.P + c0 + k] = -v;
+        }
+    }
+}
+
+template <int BT, int CB>
+int launch_gram_cb(GramArgs& A, size_t lds, hipStream_t st) {
+    const dim3 grid((A.P + CB - 1) / CB), block(256 * CB);
+    if (A.K <= 4)
+        hipLaunchKernelGGL((gram_kernel<BT, 4, CB>), grid, block, lds, st, A);
+    else
+        hipLaunchKernelGGL((gram_kernel<BT, 10, CB>), grid, block, lds, st, A);
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
 }
 
 template <int BT>
 int launch_gram(GramArgs& A, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(gram_kernel<BT>, dim3(A.P), dim3(256), lds, st, A);
-    ADT_HIP_CHECK(hipGetLastError());
-    return ADT_OK;
+    return launch_gram_cb<BT, kGramColumns>(A, lds, st);
 }
 
 // Cross-Gram form for separate receiver and current fields (3D: the receiver fields are solved
@@ -1148,9 +1250,20 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
     }
 }
 
+extern "C" int adt_gram_quad_offsets(int n_meas, int s_rows, const int32_t* d_a, const int32_t* d_b,
+                                     const int32_t* d_m, const int32_t* d_n, uint32_t* d_quad, void* stream) {
+    ADT_REQUIRE(n_meas >= 0 && s_rows >= 1 && s_rows <= 128, "adt_gram_quad_offsets: invalid sizes");
+    if (n_meas == 0) return ADT_OK;
+    ADT_REQUIRE(d_a && d_b && d_m && d_n && d_quad, "adt_gram_quad_offsets: null pointer");
+    ADT_REQUIRE(reinterpret_cast<uintptr_t>(d_quad) % 16 == 0, "adt_gram_quad_offsets: d_quad must be 16-byte aligned");
+    hipLaunchKernelGGL(quad_offsets_kernel, dim3(adt::blocks_for(n_meas, 256)), dim3(256), 0, adt::as_stream(stream),
+                       n_meas, s_rows, d_a, d_b, d_m, d_n, reinterpret_cast<uint4*>(d_quad));
+    ADT_HIP_CHECK(hipGetLastError());
+    return ADT_OK;
+}
+
 extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x,
-                                    int n_meas, int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b,
-                                    const int32_t* d_m, const int32_t* d_n, const int32_t* d_cell_dofs,
+                                    int n_meas, int n_cells, int k_local, const uint32_t* d_quad, const int32_t* d_cell_dofs,
                                     const double* d_stiff, const double* d_mass, const double* h_ws,
                                     const double* h_wm, const int32_t* d_col_ptr, const int32_t* d_col_cells,
                                     int n_cols, double* d_out, void* stream) {
@@ -1159,9 +1272,13 @@ extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64
     ADT_REQUIRE(s_rows >= 1 && s_rows <= 128 && s_rows <= rp, "adt_sensitivity_gram: 1 <= sources <= min(128, rp)");
     ADT_REQUIRE(k_local >= 1 && k_local <= 10, "adt_sensitivity_gram: unsupported local size %d", k_local);
     ADT_REQUIRE(h_vec_off && h_ws && (d_mass == nullptr || h_wm), "adt_sensitivity_gram: missing host arrays");
+    for (int w = 0; w < n_waves; ++w)
+        ADT_REQUIRE(h_ws[w] >= 0.0 && (d_mass == nullptr || h_wm[w] >= 0.0),
+                    "adt_sensitivity_gram: the weights must be non-negative (the Gram form factors T = R^T R)");
     const int cols = d_col_ptr ? n_cols : n_cells;
     if (n_meas == 0 || n_cells == 0 || cols == 0) return ADT_OK;
-    ADT_REQUIRE(d_x && d_a && d_b && d_m && d_n && d_cell_dofs && d_stiff && d_out, "adt_sensitivity_gram: null pointer");
+    ADT_REQUIRE(d_x && d_quad && d_cell_dofs && d_stiff && d_out, "adt_sensitivity_gram: null pointer");
+    ADT_REQUIRE(reinterpret_cast<uintptr_t>(d_quad) % 16 == 0, "adt_sensitivity_gram: d_quad must be 16-byte aligned");
     GramArgs A{};
     A.col_ptr = d_col_ptr;
     A.col_cells = d_col_cells;
@@ -1170,10 +1287,7 @@ extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64
     A.dofs = d_cell_dofs;
     A.stiff = d_stiff;
     A.mass = d_mass;
-    A.a = d_a;
-    A.b = d_b;
-    A.m = d_m;
-    A.n = d_n;
+    A.quad = reinterpret_cast<const uint4*>(d_quad);
     A.out = d_out;
     A.S = s_rows;
     const int B = (s_rows + 15) / 16;
@@ -1183,29 +1297,27 @@ extern "C" int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64
     A.rp = rp;
     A.C = n_cells;
     A.D = n_meas;
-    // wavenumbers per LDS chunk: the two staged (rows x Sp) blocks fit the budget (default 16 KB,
-    // several workgroups per CU); the Gram triangle reuses the same LDS afterwards
-    static const int lds_budget = [] {
-        const char* e = std::getenv("ADT_GRAM_LDS_BYTES");
-        const int v = e ? std::atoi(e) : 16384;
-        return std::min(65536, std::max(4096, v));
-    }();
-    // The triangle's LDS is allocated anyway, so the staging may fill it: fewer chunks, each
-    // a global-load round trip and two barriers per cell.
-    const size_t tri = static_cast<size_t>(s_rows + 1) * (s_rows + 2) / 2 * 8;  // with the remote slot
-    const int budget = std::max(lds_budget, static_cast<int>(tri));
-    A.wchunk = std::max(1, std::min(n_waves, (budget / (2 * 8 * A.Sp)) / k_local));
-    A.rows_max = (A.wchunk * k_local + 3) & ~3;
-    // staging: Phi and Psi blocks, the chunk's K x K templates, the row offsets of all W K rows
-    const size_t staging = (static_cast<size_t>(2) * A.rows_max * A.Sp + static_cast<size_t>(A.wchunk) * k_local * k_local +
-                            static_cast<size_t>(n_waves) * k_local) * 8;
-    const size_t lds = std::max(staging, tri);
     for (int w = 0; w < n_waves; ++w) {
         A.vec_off[w] = h_vec_off[w];
         A.ws[w] = h_ws[w];
         A.wm[w] = d_mass ? h_wm[w] : 0.0;
     }
     hipStream_t st = adt::as_stream(stream);
+    // Each column group of 4 waves has a 40 KB share of LDS (four groups per CU) for the factors R_w
+    // and the Y rows of a wavenumber chunk (all wavenumbers at once where they fit); the
+    // triangles reuse the area afterwards (a larger share where one triangle needs more).
+    const size_t tri = static_cast<size_t>(s_rows + 1) * (s_rows + 2) / 2;  // slots, with the remote one
+    const size_t factors = static_cast<size_t>(n_waves) * k_local * k_local;
+    A.factor_doubles = static_cast<int>(factors);
+    const size_t share = std::max<size_t>(40960 / 8, tri);  // doubles per group
+    ADT_REQUIRE(factors + static_cast<size_t>((k_local + 3) & ~3) * A.Sp <= share,
+                "adt_sensitivity_gram: %d wavenumbers x local size %d do not fit the LDS", n_waves, k_local);
+    A.wchunk = std::min(n_waves, static_cast<int>((share - factors) / (static_cast<size_t>(k_local) * A.Sp)));
+    while (A.wchunk > 1 && factors + static_cast<size_t>((A.wchunk * k_local + 3) & ~3) * A.Sp > share) --A.wchunk;
+    constexpr int cb = kGramColumns;
+    A.rows_max = (A.wchunk * k_local + 3) & ~3;
+    A.group_doubles = static_cast<int>(factors + static_cast<size_t>(A.rows_max) * A.Sp);
+    const size_t lds = static_cast<size_t>(cb) * std::max(static_cast<size_t>(A.group_doubles), tri) * 8;
     switch (B) {
         case 1: return launch_gram<1>(A, lds, st);
         case 2: return launch_gram<2>(A, lds, st);

@@ -14,7 +14,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().parent / "_lib" / "libadt_hip.so"))
 
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
@@ -55,8 +55,9 @@ _SIGNATURES = {
                             c_int, c_int, c_int, c_int, P, P, P, P]),
     "adt_cgls": (c_int, [c_int, c_int, P, P, P, P, P, P, c_i64, P, P, c_int, c_double, c_int, P, P, P, P]),
     "adt_sensitivity": (c_int, [c_int, c_int, c_i64, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, P]),
-    "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P,
-                                     P, c_int, P, P]),
+    "adt_gram_quad_offsets": (c_int, [c_int, c_int, P, P, P, P, P, P]),
+    "adt_sensitivity_gram": (c_int, [c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P, P, P, P, P, P, P,
+                                     c_int, P, P]),
     "adt_sensitivity_gram_cross": (c_int, [c_int, P, c_int, c_int, P, P, c_int, c_int, P, c_int, c_int, c_int, P, P,
                                            P, P, P, P, P, P, P, P, P, c_int, P, P]),
     "adt_cell_gradient": (c_int, [c_int, c_int, c_int, c_int, P, c_int, P, P, c_int, P, P, P, P, P, P, c_int, c_int,
@@ -292,10 +293,20 @@ def sensitivity_gram(x, vec_off, rp, s_rows, a, b, m, n, cell_dofs, stiff, mass,
     ws = np.ascontiguousarray(ws, dtype=np.float64)
     wm = np.ascontiguousarray(wm if wm is not None else np.zeros(W), dtype=np.float64)
     col_ptr, col_cells, n_cols = _column_map(columns)
+    quad = gram_quad_offsets(a, b, m, n, s_rows)
     check(library().adt_sensitivity_gram(W, int(rp), int(s_rows), host(vec_off), ptr(x), int(a.numel()), C, K,
-                                         ptr(a), ptr(b), ptr(m), ptr(n), ptr(cell_dofs), ptr(stiff), ptr(mass),
-                                         host(ws), host(wm), ptr(col_ptr), ptr(col_cells), n_cols, ptr(out),
-                                         stream()))
+                                         ptr(quad), ptr(cell_dofs), ptr(stiff), ptr(mass), host(ws), host(wm),
+                                         ptr(col_ptr), ptr(col_cells), n_cols, ptr(out), stream()))
+
+
+def gram_quad_offsets(a, b, m, n, s_rows):
+    """``(D, 4)`` int32 device table of the Gram kernel's LDS byte offsets of G(m,a), G(m,b), G(n,a),
+    G(n,b) per quadrupole (``adt_gram_quad_offsets``)."""
+
+    quad = torch.empty((int(a.numel()), 4), dtype=torch.int32, device=a.device)
+    check(library().adt_gram_quad_offsets(int(a.numel()), int(s_rows), ptr(a), ptr(b), ptr(m), ptr(n), ptr(quad),
+                                          stream()))
+    return quad
 
 
 def sensitivity_gram_cross(xr, off_r, rp_r, s_r, xc, off_c, rp_c, s_c, a, b, m, n, cell_dofs, stiff, mass, ws, wm,

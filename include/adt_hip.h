@@ -189,10 +189,15 @@ int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_c
  *   G_c[e][f] = sum_w phi_w[e, dofs_c]^T (ws[w] stiff_c + wm[w] mass_c) phi_w[f, dofs_c]
  *   out[d, c] = -(G_c[m][a] - G_c[m][b] - G_c[n][a] + G_c[n][b])      (index s_rows = remote: 0)
  * phi_w is the (N, rp) block at d_x + h_vec_off[w]; its first s_rows (<= 128) columns are the
- * sources; d_cell_dofs index its rows; electrode indices >= s_rows (or negative) read zero.  The
- * element templates must be symmetric (stiffness and mass templates are), so G_c is symmetric:
- * one workgroup per cell forms its upper-triangle 16 x 16 tiles on the FP64 matrix cores and
- * keeps the upper triangle, with a zero slot for the remote electrode, in LDS (no workspace).
+ * sources; d_cell_dofs index its rows.  The quadrupoles come as d_quad (n_meas x 4 uint32, 16-byte
+ * aligned), built once per survey by adt_gram_quad_offsets from the electrode indices (indices
+ * >= s_rows or negative are the remote electrode and read zero).  The
+ * element templates must be symmetric positive semidefinite and the weights ws, wm >= 0 (as for
+ * every stiffness and mass template and quadrature rule): T = ws stiff + wm mass = R^T R (upper
+ * Cholesky factor, pivots that vanish to rounding dropped) and G_c = Y^T Y with Y = [R phi_w]_w.
+ * One workgroup per cell forms the upper-triangle 16 x 16 tiles of G_c on the FP64 matrix cores
+ * and keeps the upper triangle, with a zero slot for the remote electrode, in LDS (no workspace).
+ * d_mass may be NULL (no mass term).  Negative weights are rejected with ADT_ERR_INVALID.
  * FP64, every sum in a fixed order: bitwise repeatable.  out is (D, C) row-major.  Replaces the
  * same reference kernel as adt_sensitivity.
  * Column aggregation (CellAggregation, forward/kernels.py:88-113, fused): with d_col_ptr (n_cols + 1
@@ -200,9 +205,14 @@ int adt_sensitivity(int n_waves, int s_rows, int64_t n_dofs, int n_meas, int n_c
  * d_col_cells[d_col_ptr[p] .. d_col_ptr[p+1]); J is linear in G_c, so one workgroup per column sums
  * the G_c of its cells in list order on the matrix cores and writes one column: out is then
  * (D, n_cols) and cells in no list are dropped.  d_col_ptr = NULL: one column per cell. */
+/* The quadrupole table of adt_sensitivity_gram: for row d the LDS byte offsets of G(m,a), G(m,b),
+ * G(n,a), G(n,b) in the kernel's Gram triangle for s_rows sources (a pure function of the indices
+ * and s_rows; one small kernel). */
+int adt_gram_quad_offsets(int n_meas, int s_rows, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
+                          const int32_t* d_n, uint32_t* d_quad, void* stream);
 int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_off, const double* d_x, int n_meas,
-                         int n_cells, int k_local, const int32_t* d_a, const int32_t* d_b, const int32_t* d_m,
-                         const int32_t* d_n, const int32_t* d_cell_dofs, const double* d_stiff, const double* d_mass,
+                         int n_cells, int k_local, const uint32_t* d_quad, const int32_t* d_cell_dofs,
+                         const double* d_stiff, const double* d_mass,
                          const double* h_ws, const double* h_wm, const int32_t* d_col_ptr,
                          const int32_t* d_col_cells, int n_cols, double* d_out, void* stream);
 

@@ -31,6 +31,19 @@ def random_problem(k, W=5, S=9, N=60, C=40, D=17, seed=0):
     return phi, a, b, m, n, cells, stiff, mass, ws, wm
 
 
+def semidefinite_templates(C, k, seed):
+    """Element-like templates: stiffness symmetric positive semidefinite with the constants in its
+    null space (as every FEM stiffness), mass symmetric positive definite."""
+
+    gen = torch.Generator().manual_seed(seed)
+    a = torch.randn(C, k, k, generator=gen, dtype=torch.float64)
+    centre = torch.eye(k, dtype=torch.float64) - 1.0 / k
+    stiff = centre @ (a @ a.transpose(1, 2)) @ centre / k
+    bm = torch.randn(C, k, k, generator=gen, dtype=torch.float64)
+    mass = bm @ bm.transpose(1, 2) / k + 0.1 * torch.eye(k, dtype=torch.float64)
+    return 0.5 * (stiff + stiff.transpose(1, 2)), 0.5 * (mass + mass.transpose(1, 2))
+
+
 def einsum_reference(phi, a, b, m, n, cells, stiff, mass, ws, wm):
     padded = torch.cat((phi, torch.zeros_like(phi[:, :1])), dim=1)
     cur = (padded[:, a] - padded[:, b])[..., cells]
@@ -404,14 +417,20 @@ def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W, form,
     assert relmax(first.cpu(), expected) < 1e-12
 
 
-@pytest.mark.parametrize("k, S, rp", [(3, 9, 16), (4, 40, 64), (10, 128, 128), (3, 96, 96)])
-def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp):
+@pytest.mark.parametrize("k, S, rp, W", [(3, 9, 16, 7), (4, 40, 64, 7), (10, 128, 128, 7), (3, 96, 96, 7),
+                                         (3, 96, 96, 19), (4, 33, 48, 40), (6, 20, 32, 64)])
+@pytest.mark.parametrize("with_mass", [True, False])
+def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp, W, with_mass):
     """Gram-form Jacobian rows from interleaved (N, Rp) blocks equal the per-output formula
-    (sources 9..128, local sizes 3..10, padding columns ignored); bitwise repeatable.  Element
-    templates are symmetric (as every stiffness and mass template is), which the Gram form uses."""
+    (sources 9..128, local sizes 3..10, 7..64 wavenumbers: one LDS chunk or several, padding
+    columns ignored); bitwise repeatable.  Element templates are positive semidefinite, as every
+    stiffness and mass template is, which the Gram form uses; without the mass term the stiffness
+    alone is singular (its pivots that vanish to rounding are dropped)."""
 
-    phi, a, b, m, n, cells, stiff, mass, ws, wm = random_problem(k, W=7, S=S, N=90, C=53, D=41, seed=k + S)
-    stiff, mass = 0.5 * (stiff + stiff.transpose(1, 2)), 0.5 * (mass + mass.transpose(1, 2))
+    phi, a, b, m, n, cells, _, _, ws, wm = random_problem(k, W=W, S=S, N=90, C=53, D=41, seed=k + S)
+    stiff, mass = semidefinite_templates(cells.shape[0], k, seed=k + 3 * S)
+    if not with_mass:
+        wm = torch.zeros_like(wm)
     problem = (phi, a, b, m, n, cells, stiff, mass, ws, wm)
     expected = einsum_reference(*problem)
     W, _, N = phi.shape
@@ -423,7 +442,8 @@ def test_gram_sensitivity_matches_the_einsum_reference(k, S, rp):
     def run():
         out = torch.empty((a.numel(), cells.shape[0]), dtype=torch.float64, device=dev())
         native.sensitivity_gram(g(blocks), np.arange(W) * N * rp, rp, S, *(g(x, torch.int32) for x in (a, b, m, n)),
-                                g(cells, torch.int32), g(stiff), g(mass), ws.numpy(), wm.numpy(), out)
+                                g(cells, torch.int32), g(stiff), g(mass) if with_mass else None, ws.numpy(),
+                                wm.numpy() if with_mass else None, out)
         return out
 
     out = run()
@@ -451,8 +471,8 @@ def test_gram_sensitivity_with_fused_column_aggregation(k, S):
     Gram blocks) equal the per-cell einsum reference summed over each column's cells; dropped cells
     and empty columns give nothing; bitwise repeatable."""
 
-    phi, a, b, m, n, cells, stiff, mass, ws, wm = random_problem(k, W=5, S=S, N=80, C=61, D=37, seed=k)
-    stiff, mass = 0.5 * (stiff + stiff.transpose(1, 2)), 0.5 * (mass + mass.transpose(1, 2))
+    phi, a, b, m, n, cells, _, _, ws, wm = random_problem(k, W=5, S=S, N=80, C=61, D=37, seed=k)
+    stiff, mass = semidefinite_templates(cells.shape[0], k, seed=k + 5)
     expected_cells = einsum_reference(phi, a, b, m, n, cells, stiff, mass, ws, wm).numpy()
     col_ptr, col_cells, matrix = column_map(cells.shape[0], 9, seed=k + 1)
     expected = expected_cells @ matrix
