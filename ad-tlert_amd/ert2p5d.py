@@ -339,11 +339,6 @@ class _Device:
         self.nnz = nnz
         inverse_map = np.concatenate((pattern.volume_inverse, pattern.boundary_inverse))
         sptr, sent = slot_map(inverse_map, nnz)
-        self.slot_ptr = torch.as_tensor(sptr, **i32)
-        self.slot_entry = torch.as_tensor(sent, **i32)
-        vptr, vent = slot_map(pattern.volume_inverse, nnz)
-        self.vslot_ptr = torch.as_tensor(vptr, **i32)
-        self.vslot_entry = torch.as_tensor(vent, **i32)
         self.stiff = torch.as_tensor(d.stiffness, **f64).contiguous()
         self.mass = torch.as_tensor(d.mass, **f64).contiguous()
         self.bmass = torch.as_tensor(d.boundary_mass, **f64).contiguous()
@@ -355,6 +350,12 @@ class _Device:
         parents = d.parent_cell_ids.numpy()
         self.cell_col = torch.as_tensor(parents, **i32)
         self.bnd_col = torch.as_tensor(parents[d.boundary_cells.numpy()], **i32)
+        # slot-ordered scatter map with the template values of every entry (volume entries first)
+        k_local, k_bnd = self.stiff.shape[1], self.bmass.shape[1]
+        self.amap = native.AssemblyMap(sptr, sent, pattern.volume_inverse.size, k_local * k_local, parents,
+                                       kb=k_bnd * k_bnd, bnd_col=parents[d.boundary_cells.numpy()], device=dev)
+        self.estiff = self.amap.entries(self.stiff, self.bmass)
+        self.emass = self.amap.entries(self.mass)
         self.cell_dofs = torch.as_tensor(cells, **i32).contiguous()
         # the Jacobian kernel walks cells in mesh order and reads fields in mesh node order
         self.cell_dofs_mesh = torch.as_tensor(d.cell_dofs.numpy(), **i32).contiguous()
@@ -529,12 +530,8 @@ class ERTForward2p5D:
 
         sigmas = sigmas.to(device=dev.device, dtype=torch.float64).contiguous()
         values = torch.empty((sigmas.shape[0] * dev.W, dev.nnz), dtype=torch.float64, device=dev.device)
-        if robin:
-            native.assemble(sigmas, dev.cell_col, dev.stiff, dev.mass, dev.k2, dev.slot_ptr, dev.slot_entry, values,
-                            n_waves=dev.W, bnd_col=dev.bnd_col, bmass=dev.bmass, balpha=dev.balpha)
-        else:
-            native.assemble(sigmas, dev.cell_col, dev.stiff, dev.mass, dev.k2, dev.vslot_ptr, dev.vslot_entry, values,
-                            n_waves=dev.W)
+        native.assemble(dev.amap, sigmas, dev.estiff, values, n_waves=dev.W, emass=dev.emass, k2=dev.k2,
+                        balpha=dev.balpha, with_boundary=robin)
         return values
 
     def _spmm(self, dev: _Device, values: Tensor, x: Tensor, y: Tensor, n_sys: int, mode: int = 0) -> None:

@@ -231,13 +231,16 @@ class ERTForward3D:
         s.indices = torch.as_tensor(indices, **i32)
         s.host_pattern = (indptr, indices)
         sptr, sent = slot_map(groups, self.nnz)
-        s.slot_ptr, s.slot_entry = torch.as_tensor(sptr, **i32), torch.as_tensor(sent, **i32)
         s.fixed = torch.as_tensor(fixed_slots.astype(np.int32), **i32) if fixed_slots.size else None
         k = conn.shape[1]
         C = conn.shape[0]
         s.templates = torch.as_tensor(flat.reshape(C, k, k), **f64).contiguous()
         s.volume_templates = torch.as_tensor(flat_v.reshape(C, k, k), **f64).contiguous()
         s.cell_col = torch.arange(C, **i32)
+        # slot-ordered scatter map with both template sets laid out in its order
+        s.amap = native.AssemblyMap(sptr, sent, C * k * k, k * k, np.arange(C), device=dev)
+        s.etemplates = s.amap.entries(s.templates)
+        s.evolume_templates = s.amap.entries(s.volume_templates)
         s.cell_dofs = torch.as_tensor(conn, **i32).contiguous()
         s.cell_dofs_mesh = torch.as_tensor(self._cell_connectivity, **i32).contiguous()
         s.N, s.nnz = self.N, self.nnz
@@ -303,8 +306,8 @@ class ERTForward3D:
 
         sigmas = sigmas.to(device=s.device, dtype=torch.float64).contiguous()
         values = torch.empty((sigmas.shape[0], self.nnz), dtype=torch.float64, device=s.device)
-        native.assemble(sigmas, s.cell_col, s.templates if robin else s.volume_templates, None, None, s.slot_ptr,
-                        s.slot_entry, values, n_waves=1, fixed=s.fixed if fixed else None)
+        native.assemble(s.amap, sigmas, s.etemplates if robin else s.evolume_templates, values, n_waves=1,
+                        fixed=s.fixed if fixed else None)
         return values
 
     def _spmm(self, s, values: Tensor, x: Tensor, y: Tensor, mode: int = 0) -> None:

@@ -28,8 +28,7 @@ _SIGNATURES = {
     "adt_last_error": (ctypes.c_char_p, []),
     "adt_abi_version": (c_int, []),
     "adt_device_info": (c_int, [c_int, P, P, c_int]),
-    "adt_assemble": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_i64, P, P, P, P, P, P, P, P, P,
-                             P, c_int, P, P]),
+    "adt_assemble": (c_int, [c_int, c_int, c_int, P, c_i64, P, P, P, P, P, P, P, P, c_int, c_int, P, c_int, P, P]),
     "adt_spmm": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P]),
     "adt_pcg_create": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P, c_int]),
     "adt_pcg_tiling": (c_int, [P, P, P, P]),
@@ -139,20 +138,60 @@ def device_info(index: int = 0) -> tuple[int, str]:
 # ----------------------------------------------------------------------------------------------
 
 
-def assemble(sigma, cell_col, stiff, mass, k2, slot_ptr, slot_entry, values, *, n_waves, bnd_col=None, bmass=None,
-             balpha=None, fixed=None):
+class AssemblyMap:
+    """Slot-ordered scatter map of one mesh for ``adt_assemble``.
+
+    ``slot_ptr`` / ``slot_entry`` give each CSR slot's ordered local entries: flat indices into the
+    ``(C, K, K)`` cell templates (``< n_vol``) or, past them, into the ``(B, Kb, Kb)`` boundary
+    templates.  Within a slot the volume entries come first, then the boundary ones (each in list
+    order); ``col`` holds each entry's conductivity column and ``bnd`` the boundary element of a
+    boundary entry.  :meth:`entries` lays template values out in the same order (once per mesh)."""
+
+    def __init__(self, slot_ptr, slot_entry, n_vol: int, kk: int, cell_col, *, kb: int = 1, bnd_col=None, device=None):
+        device = device or torch.device("cuda")
+        ptr = np.asarray(slot_ptr, dtype=np.int64)
+        ent = np.asarray(slot_entry, dtype=np.int64)
+        nnz = ptr.size - 1
+        slot_of = np.repeat(np.arange(nnz), np.diff(ptr))
+        is_bnd = ent >= n_vol
+        order = np.lexsort((is_bnd, slot_of))  # by slot, volume entries first, list order kept
+        ent, is_bnd = ent[order], is_bnd[order]
+        self.nnz, self.n_entries = nnz, int(ent.size)
+        self.ptr = torch.as_tensor(ptr.astype(np.int32), device=device)
+        self.vol = torch.as_tensor(np.bincount(slot_of[~is_bnd], minlength=nnz).astype(np.int32), device=device)
+        self._is_bnd = torch.as_tensor(is_bnd, device=device)
+        self._vol_index = torch.as_tensor(ent[~is_bnd], device=device)
+        self._bnd_index = torch.as_tensor(ent[is_bnd] - n_vol, device=device)
+        cell_col = torch.as_tensor(cell_col, device=device).long()
+        col = torch.empty(self.n_entries, dtype=torch.int32, device=device)
+        col[~self._is_bnd] = cell_col[self._vol_index // kk].int()
+        bnd = torch.full((self.n_entries,), -1, dtype=torch.int32, device=device)
+        if bool(is_bnd.any()):
+            b = self._bnd_index // kb
+            col[self._is_bnd] = torch.as_tensor(bnd_col, device=device).long()[b].int()
+            bnd[self._is_bnd] = b.int()
+        self.col, self.bnd = col, bnd
+
+    def entries(self, templates, boundary=None) -> torch.Tensor:
+        """Slot-ordered float64 values of ``templates`` (volume entries) and ``boundary`` (boundary
+        entries; zero when None)."""
+
+        out = torch.zeros(self.n_entries, dtype=torch.float64, device=self.ptr.device)
+        out[~self._is_bnd] = templates.reshape(-1).double()[self._vol_index]
+        if boundary is not None and self._bnd_index.numel():
+            out[self._is_bnd] = boundary.reshape(-1).double()[self._bnd_index]
+        return out
+
+
+def assemble(amap: AssemblyMap, sigma, estiff, values, *, n_waves, emass=None, k2=None, balpha=None,
+             with_boundary=False, fixed=None):
     """values[(t*W + w), :] for conductivity rows sigma[t] (see ``adt_assemble``)."""
 
-    lib = library()
-    n_models = int(sigma.shape[0])
-    n_cells, k = int(stiff.shape[0]), int(stiff.shape[1])
-    n_bnd = 0 if bmass is None else int(bmass.shape[0])
-    k_bnd = 0 if bmass is None else int(bmass.shape[1])
-    nnz = int(slot_ptr.shape[0]) - 1
-    check(lib.adt_assemble(n_models, n_waves, nnz, n_cells, k, n_bnd, k_bnd, ptr(sigma), int(sigma.stride(0)),
-                           ptr(cell_col), ptr(bnd_col), ptr(stiff), ptr(mass), ptr(k2), ptr(bmass), ptr(balpha),
-                           ptr(slot_ptr), ptr(slot_entry), ptr(fixed), 0 if fixed is None else int(fixed.numel()),
-                           ptr(values), stream()))
+    n_bnd = 0 if balpha is None else int(balpha.shape[-1])
+    check(library().adt_assemble(int(sigma.shape[0]), int(n_waves), amap.nnz, ptr(sigma), int(sigma.stride(0)),
+                                 ptr(amap.ptr), ptr(amap.vol), ptr(amap.col), ptr(estiff), ptr(emass), ptr(k2),
+                                 ptr(amap.bnd), ptr(balpha), n_bnd, int(bool(with_boundary)), ptr(fixed),
+                                 0 if fixed is None else int(fixed.numel()), ptr(values), stream()))
 
 
 def spmm(layout, indptr, indices, values, x, y, mode=0):

@@ -49,21 +49,25 @@ int adt_device_info(int device, int* h_cus, char* h_name, int name_len);
  *
  * Replaces the gather-levels GroupSum of ERTForward2p5D._assemble
  * (forward/ert2p5d.py:275-314, forward/kernels.py:30-85) and ERTForward3D._assemble_batch
- * (forward/ert3d.py:436-461).  For system s = model*n_waves + w and CSR slot j:
+ * (forward/ert3d.py:436-461).  The scatter map comes slot-ordered: slot j owns the entries
+ * [slot_ptr[j], slot_ptr[j+1]), its slot_vol[j] volume entries first, then its boundary
+ * entries; entry i carries its conductivity column ecol[i] and template value estiff[i] (and
+ * emass[i]); a boundary entry also its boundary element ebnd[i].  (Built once per mesh from the
+ * per-slot lists of flat template indices.)  For system s = model*n_waves + w:
  *
- *   values[s, j] = sum_{e in slot_entry[slot_ptr[j] : slot_ptr[j+1]]} coef(s, e)
- *   coef(s, e) = sigma[model, cell_col[c]] * (stiff[e] + k2[w] * mass[e])      e <  C*K*K, c = e/(K*K)
- *   coef(s, e) = sigma[model, bnd_col[b]] * balpha[w, b] * bmass[e - C*K*K]       e >= C*K*K, b = .../(Kb*Kb)
+ *   A = sum_{volume i} sigma[model, ecol[i]] * estiff[i],   B = sum_{volume i} sigma[model, ecol[i]] * emass[i]
+ *   values[s, j] = A + k2[w] * B  (+ sum_{boundary i} sigma[model, ecol[i]] * balpha[w, ebnd[i]] * estiff[i]
+ *                                   when with_boundary)
  *
- * summed in the fixed order of slot_entry, so results are bitwise repeatable.  Slots
- * listed in fixed_slots are then set to 1 (Dirichlet rows).  mass/k2 may be NULL
- * (3D: no wavenumber term); n_bnd may be 0.
+ * each sum in the fixed order of the slot's list, so results are bitwise repeatable; the entries
+ * are read once per model, not once per wavenumber.  sigma rows are sigma_stride apart.  Slots
+ * listed in fixed_slots are then set to 1 (Dirichlet rows).  emass/k2 may be NULL (3D: no
+ * wavenumber term); n_bnd may be 0.
  * ---------------------------------------------------------------------------------------- */
-int adt_assemble(int n_models, int n_waves, int nnz, int n_cells, int k_local, int n_bnd, int k_bnd,
-                 const double* d_sigma, int64_t sigma_stride, const int32_t* d_cell_col,
-                 const int32_t* d_bnd_col, const double* d_stiff, const double* d_mass, const double* d_k2,
-                 const double* d_bmass, const double* d_balpha, const int32_t* d_slot_ptr,
-                 const int32_t* d_slot_entry, const int32_t* d_fixed_slots, int n_fixed,
+int adt_assemble(int n_models, int n_waves, int nnz, const double* d_sigma, int64_t sigma_stride,
+                 const int32_t* d_slot_ptr, const int32_t* d_slot_vol, const int32_t* d_ecol,
+                 const double* d_estiff, const double* d_emass, const double* d_k2, const int32_t* d_ebnd,
+                 const double* d_balpha, int n_bnd, int with_boundary, const int32_t* d_fixed_slots, int n_fixed,
                  double* d_values, void* stream);
 
 /* ------------------------------------------------------------------------------------------
