@@ -192,12 +192,13 @@ class Timelapse:
         def run():
             return op._normal_jacobian(fields[0], self.D, columns, return_device=True)
 
-        # Gram form: per cell the upper triangle of G_c = Phi^T (T Phi) over W*K rows of E sources
-        # (FP64 matrix cores), Psi = T Phi, then 4 lookups and 3 adds per output
+        # Gram form: per cell the upper triangle of G_c = Y^T Y over W*K rows of E sources (FP64
+        # matrix cores), Y = R Phi with T = R^T R, then 4 lookups and 3 adds per output (a 16-byte
+        # table entry per row)
         K, S, rp = 3, self.E, self.op._dev.rp
         bytes_ = 8 * self.D * self.C + 8 * self.W * self.N * rp + 8 * 2 * self.C * K * K + 4 * self.C * K + 16 * self.D
-        flops = self.C * (self.W * K * S * (S + 1) + 2 * self.W * K * K * S) + 3 * self.D * self.C
-        return run, bytes_, flops, "gram_kernel<6> (Gram-form sensitivity)", {"bound": "fp64"}
+        flops = self.C * (self.W * K * S * (S + 1) + self.W * K * (K + 1) * S) + 3 * self.D * self.C
+        return run, bytes_, flops, "gram_kernel<6, 4, 1> (Gram-form sensitivity)", {"bound": "fp64"}
 
     def cpu_and_oracle(self, jac_device):
         """CPU baseline (oracle on the host cores) and the GPU's error against it on the same
