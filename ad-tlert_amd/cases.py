@@ -114,13 +114,17 @@ class Case3D:
     conductivity: np.ndarray
 
 
-def quad_strip_case(nx=30, nz=12, n_electrodes=16, anomaly=True) -> Case2D:
-    """Flat quadrilateral strip of ``tests/conftest.py:27-51`` (dx = 2 m, layers 2.0 -> 0.5 m)."""
+def quad_strip_case(nx=30, nz=12, n_electrodes=16, anomaly=True, slope=0.0) -> Case2D:
+    """Terrain-following quadrilateral strip of ``tests/conftest.py:27-51`` (dx = 2 m, layers
+    2.0 -> 0.5 m; ``workflows/terrain.py:748-797``): the surface rises by ``slope`` per metre plus
+    ``0.05 sin(column)`` where the slope is not zero; ``slope = 0`` is flat."""
 
     thickness = np.linspace(2.0, 0.5, nz)[::-1]
     offsets = np.concatenate(([0.0], -np.cumsum(thickness)))
     xs = np.arange(nx + 1, dtype=float) * 2.0
-    nodes = np.asarray([[x, off] for off in offsets for x in xs])
+    slopes = np.full(nx, float(slope)) + (0.0 if slope == 0 else 0.05 * np.sin(np.arange(nx)))
+    top = np.concatenate(([0.0], np.cumsum(slopes * 2.0)))
+    nodes = np.asarray([[x, z + off] for off in offsets for x, z in zip(xs, top)])
     width = nx + 1
     cells = np.asarray(
         [[l * width + c, l * width + c + 1, (l + 1) * width + c + 1, (l + 1) * width + c]
@@ -131,7 +135,7 @@ def quad_strip_case(nx=30, nz=12, n_electrodes=16, anomaly=True) -> Case2D:
     rho = rho[::-1, :].reshape(-1)
     count = min(int(n_electrodes), width)
     ex = np.linspace(xs.min(), xs.max(), count)
-    return Case2D(nodes, cells, np.column_stack((ex, np.zeros_like(ex))), wenner_alpha(count), 1.0 / rho,
+    return Case2D(nodes, cells, np.column_stack((ex, np.interp(ex, xs, top))), wenner_alpha(count), 1.0 / rho,
                   np.arange(width))
 
 

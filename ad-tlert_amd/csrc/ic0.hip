@@ -268,7 +268,11 @@ __global__ __launch_bounds__(kThreads) void ic0_direction(Ic0Vecs V, int first) 
     const int64_t e = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (e >= V.block || !col_active(V, s, e)) return;
     const int64_t i = static_cast<int64_t>(s) * V.block + e;
-    const double beta = first ? 0.0 : adt::gview(colv(V, s, static_cast<int>(e % V.rp)))[kBeta];
+    if (first) {  // p = z: the workspace's earlier contents (possibly not finite) are never read
+        adt::gview(V.p)[i] = adt::gview(V.z)[i];
+        return;
+    }
+    const double beta = adt::gview(colv(V, s, static_cast<int>(e % V.rp)))[kBeta];
     adt::gview(V.p)[i] = fma(beta, adt::gview(V.p)[i], adt::gview(V.z)[i]);
 }
 

@@ -21,7 +21,7 @@ import numpy as np
 import torch
 from scipy.special import k0 as bessel_k0
 
-from . import fem, formats, native
+from . import fem, formats, native, terrain
 from .ordering import dof_order, inverse
 from .dtypes import FLOAT_DTYPE
 from .field_cache import FieldCache, digest16, newest_batches_first
@@ -163,6 +163,62 @@ def build_native_discretization(mesh: Mesh, survey: Survey, wavenumbers: np.ndar
         stiffness=torch.as_tensor(stiffness), mass=torch.as_tensor(mass), boundary_mass=torch.as_tensor(bmass),
         boundary_geometries=torch.as_tensor(alpha), electrode_matrix=torch.as_tensor(electrodes),
         pattern=sparse_pattern(cells, bdofs, n), spd=True)
+
+
+def terrain_discretization(arrays: dict) -> Discretization:
+    """A :class:`Discretization` from the arrays of :func:`terrain.build` (auxiliary: not flagged SPD)."""
+
+    cells = np.asarray(arrays["cell_dofs"], dtype=np.int64)
+    bdofs = np.asarray(arrays["boundary_dofs"], dtype=np.int64)
+    n = int(arrays["dof_nodes"].shape[0])
+    return Discretization(
+        name=arrays["name"], geometry=arrays["geometry"], parent_cell_ids=torch.as_tensor(arrays["parents"]),
+        dof_nodes=torch.as_tensor(arrays["dof_nodes"]), cell_dofs=torch.as_tensor(cells),
+        boundary_dofs=torch.as_tensor(bdofs), stiffness=torch.as_tensor(arrays["stiffness"]),
+        mass=torch.as_tensor(arrays["mass"]), boundary_mass=torch.as_tensor(arrays["boundary_mass"]),
+        boundary_geometries=torch.as_tensor(arrays["alpha"]), electrode_matrix=torch.as_tensor(arrays["electrodes"]),
+        pattern=sparse_pattern(cells, bdofs, n), spd=False)
+
+
+def _solve_unit_potentials(op, d: Discretization) -> np.ndarray:
+    """``(W, E, dofs)`` solutions of ``A_w(1) U = E^T`` on discretization ``d`` (one system per
+    wavenumber, the electrode rows as shared right-hand sides), assembled and solved on the device."""
+
+    dev = native.device()
+    f64 = dict(dtype=torch.float64, device=dev)
+    pattern = d.pattern
+    n, nnz = d.dof_count, pattern.nnz
+    W, E = int(op.wavenumbers.shape[0]), op.survey.electrode_count
+    stiff = torch.as_tensor(d.stiffness, **f64)
+    mass = torch.as_tensor(d.mass, **f64)
+    bmass = torch.as_tensor(d.boundary_mass, **f64)
+    parents = d.parent_cell_ids.numpy()
+    sptr, sent = slot_map(np.concatenate((pattern.volume_inverse, pattern.boundary_inverse)), nnz)
+    k_local, k_bnd = stiff.shape[1], bmass.shape[1]
+    amap = native.AssemblyMap(sptr, sent, pattern.volume_inverse.size, k_local * k_local, parents, kb=k_bnd * k_bnd,
+                              bnd_col=parents[d.boundary_cells.numpy()], device=dev)
+    k = op.wavenumbers.double().numpy()
+    values = torch.empty((W, nnz), **f64)
+    native.assemble(amap, torch.ones((1, op.mesh.cell_count), **f64), amap.entries(stiff, bmass), values, n_waves=W,
+                    emass=amap.entries(mass), k2=torch.as_tensor(k * k, **f64),
+                    balpha=torch.as_tensor(d.boundary_geometries, **f64).contiguous(), with_boundary=True)
+    lanes = op._solver_lanes(E)
+    rp = -(-E // lanes) * lanes
+    b = torch.zeros((n, rp), **f64)
+    b[:, :E] = torch.as_tensor(d.electrode_matrix.double().numpy().T, **f64)
+    x = torch.zeros((W, n, rp), **f64)
+    zeros = np.zeros(W, np.int64)
+    plan = op._solver.plan((d.name, "unit_potentials"), np.full(W, n, dtype=np.int32), E,
+                           pattern_offsets=(zeros, zeros, np.arange(W, dtype=np.int64) * nnz), b_off=zeros,
+                           host_pattern=(pattern.indptr, pattern.indices))
+    if plan.rp != rp:
+        raise RuntimeError(f"interleaved width {rp} does not match the plan's {plan.rp}")
+    indptr = torch.as_tensor(pattern.indptr, dtype=torch.int32, device=dev)
+    indices = torch.as_tensor(pattern.indices, dtype=torch.int32, device=dev)
+    op._solver.solve(plan, indptr, indices, values.contiguous(), b.reshape(-1), x.reshape(-1))
+    out = x[:, :, :E].permute(0, 2, 1).contiguous().cpu().numpy()
+    check_finite(torch.as_tensor(out), "unit potentials")
+    return out
 
 
 def electrode_stencil(matrix: np.ndarray):
@@ -384,8 +440,8 @@ class _Device:
         unit_values = op._assemble_device(self, ones)
         self.rhs = torch.empty_like(self.primary)
         op._spmm(self, unit_values, self.primary, self.rhs, self.W)
-        # source resistivities: cells around each source node (padded table), else the source cell
-        cells = d.cell_dofs.numpy()
+        # source resistivities: mesh cells around each source node (padded table), else the source cell
+        cells = np.asarray(op.mesh.cells)
         n_cells = cells.shape[0]
         lists = [np.flatnonzero(np.any(cells == node, axis=1)) if node >= 0 else np.zeros(0, np.int64)
                  for node in op.source_node_ids.tolist()]
@@ -406,12 +462,13 @@ class ERTForward2p5D:
     def __init__(self, mesh, survey, wavenumbers, weights, discretization, source_cell_ids, source_node_ids,
                  numerical_h2_refined=True, numerical_p2_refined=True, topographic_geometric_factor_mode="analytic",
                  terrain_cache_dir=None, normal_field_cache_max_entries=8, series_batch_steps=8,
-                 solver_tolerance=1e-10, sensitivity_precision="float64", preconditioner="amg"):
+                 solver_tolerance=1e-10, sensitivity_precision="float64", preconditioner="amg",
+                 primary_potential_discretization=None, geometric_discretization=None):
         self.mesh, self.survey = mesh, survey
         self.wavenumbers, self.weights = wavenumbers, weights
         self.discretization = discretization
-        self.primary_potential_discretization = None
-        self.geometric_discretization = None
+        self.primary_potential_discretization = primary_potential_discretization
+        self.geometric_discretization = geometric_discretization
         self.source_cell_ids, self.source_node_ids = source_cell_ids, source_node_ids
         self.numerical_h2_refined, self.numerical_p2_refined = numerical_h2_refined, numerical_p2_refined
         self.topographic_geometric_factor_mode = topographic_geometric_factor_mode
@@ -443,14 +500,24 @@ class ERTForward2p5D:
             raise ValueError("normal_field_cache_max_entries must be non-negative")
         if int(series_batch_steps) < 1:
             raise ValueError("series_batch_steps must be >= 1")
-        if not mesh.is_flat_surface:
-            raise NotImplementedError("terrain discretizations (auxiliary quad/P2 primary fields) are not part of "
-                                      "the MI355X forward path; use a flat-surface mesh")
         if wavenumbers is None:
             k, q = build_inverse_cosine_weights(*survey_wavenumber_bounds(survey))
         else:
             k, q = torch.as_tensor(wavenumbers, dtype=FLOAT_DTYPE), torch.as_tensor(weights, dtype=FLOAT_DTYPE)
-        disc = build_native_discretization(mesh, survey, k.double().numpy(), quadrature_order)
+        potential = geometric = None
+        if mesh.is_flat_surface:
+            disc = build_native_discretization(mesh, survey, k.double().numpy(), quadrature_order)
+        else:
+            # terrain (``ert2p5d.py:170-190``): auxiliary field discretization, serendipity/P2
+            # primary-potential discretization, and the geometric one for numerical factors
+            def build(name, **options):
+                return terrain_discretization(terrain.build(name, mesh, survey.electrode_positions, k.double().numpy(),
+                                                            quadrature_order=quadrature_order, **options))
+
+            disc = build("primary", refine=numerical_h2_refined, quadratic=False)
+            potential = build("primary_potential", refine=numerical_h2_refined, quadratic=True)
+            if gf_mode == "numerical":
+                geometric = build("geometric", refine=numerical_h2_refined, quadratic=numerical_p2_refined)
         positions = np.asarray(survey.electrode_positions, dtype=float)
         nodes = np.asarray(mesh.nodes, dtype=float)
         dist = np.linalg.norm(nodes[None, :, :] - positions[:, None, :], axis=2)
@@ -459,21 +526,23 @@ class ERTForward2p5D:
         return cls(mesh, survey, k, q, disc, mesh.locate_points(survey.electrode_positions)[0].long(),
                    torch.as_tensor(source_nodes, dtype=torch.long), numerical_h2_refined, numerical_p2_refined, gf_mode,
                    terrain_cache_dir, normal_field_cache_max_entries, series_batch_steps, solver_tolerance,
-                   sensitivity_precision, preconditioner)
+                   sensitivity_precision, preconditioner, potential, geometric)
 
     # -- properties ------------------------------------------------------------------------------
 
     @property
     def use_numerical_primary(self) -> bool:
-        return False
+        return self.primary_potential_discretization is not None
 
     @property
     def use_numerical_geometric_factors(self) -> bool:
-        return False
+        return self.geometric_discretization is not None
 
     @property
     def dtype(self) -> torch.dtype:
-        return FLOAT_DTYPE
+        """Field precision: float64 on terrain, ``FLOAT_DTYPE`` otherwise (the device works in FP64)."""
+
+        return torch.float64 if self.use_numerical_primary else FLOAT_DTYPE
 
     @property
     def _field_cache(self) -> FieldCache:
@@ -490,17 +559,47 @@ class ERTForward2p5D:
 
     # -- host precomputation ---------------------------------------------------------------------
 
+    def _cache_arrays(self, d: Discretization):
+        return (self.mesh.nodes, self.mesh.cells, self.mesh.surface_node_ids, self.survey.electrode_positions,
+                self.survey.measurements, self.wavenumbers, self.weights, d.parent_cell_ids, d.dof_nodes, d.cell_dofs,
+                d.boundary_dofs, d.electrode_matrix, d.pattern.indptr, d.pattern.indices, d.stiffness, d.mass,
+                d.boundary_mass, d.boundary_geometries)
+
     def _unit_primary(self) -> np.ndarray:
-        """``(W, E, N)`` unit primary field, read from the operator's disk cache when
+        """``(W, E, N)`` unit primary field of the field discretization: the analytic half-space field
+        on flat surfaces, else the unit potentials of the primary-potential discretization sampled at
+        the field DOFs (``ert2p5d.py:404-436``).  Read from the operator's disk cache when
         ``terrain_cache_dir`` (or ``$ADTLERT_TERRAIN_CACHE_DIR``) names one (``ert2p5d.py:349-402``)."""
 
         d = self.discretization
         shape = (int(self.wavenumbers.shape[0]), self.survey.electrode_count, d.dof_count)
-        arrays = (self.mesh.nodes, self.mesh.cells, self.survey.electrode_positions, self.survey.measurements,
-                  self.wavenumbers, self.weights, self.source_node_ids, d.dof_nodes, d.cell_dofs)
         options = (self.numerical_h2_refined, self.numerical_p2_refined, self.topographic_geometric_factor_mode)
         cache = formats.UnitPotentialCache(self.terrain_cache_dir)
-        return cache.get("native_analytic_primary", "float64", options, arrays, shape, self._analytic_primary)
+        if not self.use_numerical_primary:
+            arrays = (self.mesh.nodes, self.mesh.cells, self.survey.electrode_positions, self.survey.measurements,
+                      self.wavenumbers, self.weights, self.source_node_ids, d.dof_nodes, d.cell_dofs)
+            return cache.get("native_analytic_primary", "float64", options, arrays, shape, self._analytic_primary)
+        potential = self.primary_potential_discretization
+
+        def project():
+            selection = terrain.exact_node_indices(potential.dof_nodes.numpy(), d.dof_nodes.numpy())
+            return self._unit_potentials(potential)[..., selection]
+
+        return cache.get("auxiliary_sub_potentials", "float64", options, self._cache_arrays(d), shape, project)
+
+    def _unit_potentials(self, d: Discretization) -> np.ndarray:
+        """``(W, E, dofs)`` unit-conductivity potentials of every electrode on the auxiliary
+        discretization ``d``: ``A_w(1) U_w = electrode rows`` on the device, one system per
+        wavenumber sharing the right-hand sides (``ert2p5d.py:391-402``); disk-cached like the primary."""
+
+        shape = (int(self.wavenumbers.shape[0]), self.survey.electrode_count, d.dof_count)
+        options = (self.numerical_h2_refined, self.numerical_p2_refined, self.topographic_geometric_factor_mode)
+        cache = formats.UnitPotentialCache(self.terrain_cache_dir)
+        key = (d.name, "unit_potentials")
+        if key not in self._cache:
+            self._cache[key] = cache.get(f"{d.name}_sub_potentials", "float64", options, self._cache_arrays(d), shape,
+                                         lambda: _solve_unit_potentials(self, d))
+        return self._cache[key]
 
     def _analytic_primary(self) -> np.ndarray:
         """``(W, E, N)`` unit primary with the electrode-node value (``ert2p5d.py:438-468``)."""
@@ -633,7 +732,26 @@ class ERTForward2p5D:
         return torch.sign(reciprocal), torch.clamp_min(torch.sqrt(torch.abs(normal * reciprocal)), 1e-30)
 
     def _geometric_factors(self) -> Tensor:
-        return self.survey.geometric_factors()
+        if not self.use_numerical_geometric_factors:
+            return self.survey.geometric_factors()
+        if "geometric_factors" not in self._cache:
+            self._cache["geometric_factors"] = self._numerical_geometric_factors()
+        return self._cache["geometric_factors"]
+
+    def _numerical_geometric_factors(self) -> Tensor:
+        """``1 / (U_AM - U_AN - U_BM + U_BN)`` from the unit potentials of the geometric
+        discretization integrated over wavenumbers (``ert2p5d.py:662-671``)."""
+
+        d = self.geometric_discretization
+        integrated = np.tensordot(self.weights.double().numpy(), self._unit_potentials(d), axes=([0], [0]))
+        potentials = integrated @ d.electrode_matrix.double().numpy().T
+        e = potentials.shape[0]
+        padded = np.zeros((e + 1, e + 1))
+        padded[:e, :e] = potentials
+        a, b, m, n = self.survey.measurement_slots().numpy().T
+        sources = padded[a] - padded[b]
+        rows = np.arange(a.size)
+        return torch.as_tensor(1.0 / (sources[rows, m] - sources[rows, n]), dtype=self.dtype)
 
     def _measure(self, conductivity):
         fields = self._fields(conductivity)
@@ -758,7 +876,7 @@ class ERTForward2p5D:
                 offs = np.arange(dev.W, dtype=np.int64) * dev.N * rp
                 poffs = np.arange(dev.W, dtype=np.int64) * dev.N * dev.rp
                 g = self._gradient(lam[i], rp, offs, fields[t].blocks, dev.rp, poffs, dev.E, robin=True)
-                gradients[t] = g.cpu().to(FLOAT_DTYPE)
+                gradients[t] = self._columns()(g).cpu().to(FLOAT_DTYPE)  # discretization cells -> mesh cells
         gradient = torch.stack([gradients[t] for t in range(len(fields))])
         check_finite(gradient, "conductivity gradient")
         return gradient
@@ -896,9 +1014,10 @@ class ERTForward2p5D:
             lam = self._solve_adjoint(fields.values, rhs, (stop - start) * dev.E, ("adjoint-jac", rp))
             offs = np.arange(dev.W, dtype=np.int64) * dev.N * rp
             poffs = np.arange(dev.W, dtype=np.int64) * dev.N * dev.rp
+            cells = self._columns()  # discretization cells -> mesh cells
             for i in range(stop - start):
                 g = self._gradient(lam, rp, offs + i * dev.E, fields.blocks, dev.rp, poffs, dev.E, robin=robin)
-                rows.append(g)
+                rows.append(cells(g))
         return torch.stack(rows).cpu().to(FLOAT_DTYPE)
 
     def solve_with_jacobian(self, conductivity, currents=1.0, *, batch_size=None,

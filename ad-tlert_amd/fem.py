@@ -53,6 +53,79 @@ def edge_mass(lengths: np.ndarray) -> np.ndarray:
     return np.asarray(lengths, float)[:, None, None] * (np.array([[2.0, 1.0], [1.0, 2.0]]) / 6.0)
 
 
+def quadratic_edge_mass(lengths: np.ndarray) -> np.ndarray:
+    """Robin edge mass of quadratic edges with DOFs (end, midpoint, end) (``fem/boundary.py:12-40``)."""
+
+    return np.asarray(lengths, float)[:, None, None] * (np.array([[4.0, 2.0, -1.0], [2.0, 16.0, 2.0],
+                                                                  [-1.0, 2.0, 4.0]]) / 30.0)
+
+
+def _bilinear(points: np.ndarray):
+    xi, eta = points[:, 0], points[:, 1]
+    values = 0.25 * np.stack(((1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)), -1)
+    dxi = 0.25 * np.stack((-(1 - eta), 1 - eta, 1 + eta, -(1 + eta)), axis=-1)
+    deta = 0.25 * np.stack((-(1 - xi), -(1 + xi), 1 + xi, 1 - xi), axis=-1)
+    return values, np.stack((dxi, deta), axis=-1)
+
+
+def serendipity_shape(points: np.ndarray):
+    """Eight-node serendipity values and reference gradients on [-1, 1]^2: corners (-1,-1), (1,-1),
+    (1,1), (-1,1), then the midpoints of the edges between them (``discretization.py:177-220``)."""
+
+    xi, eta = points[:, 0], points[:, 1]
+    values = np.stack((-0.25 * (1 - xi) * (1 - eta) * (1 + xi + eta), -0.25 * (1 + xi) * (1 - eta) * (1 - xi + eta),
+                       -0.25 * (1 + xi) * (1 + eta) * (1 - xi - eta), -0.25 * (1 - xi) * (1 + eta) * (1 + xi - eta),
+                       0.5 * (1 - xi * xi) * (1 - eta), 0.5 * (1 + xi) * (1 - eta * eta),
+                       0.5 * (1 - xi * xi) * (1 + eta), 0.5 * (1 - xi) * (1 - eta * eta)), axis=-1)
+    dxi = np.stack((0.25 * (1 - eta) * (2 * xi + eta), 0.25 * (1 - eta) * (2 * xi - eta),
+                    0.25 * (1 + eta) * (2 * xi + eta), 0.25 * (1 + eta) * (2 * xi - eta), -xi * (1 - eta),
+                    0.5 * (1 - eta * eta), -xi * (1 + eta), -0.5 * (1 - eta * eta)), axis=-1)
+    deta = np.stack((0.25 * (1 - xi) * (xi + 2 * eta), 0.25 * (1 + xi) * (-xi + 2 * eta),
+                     0.25 * (1 + xi) * (xi + 2 * eta), 0.25 * (1 - xi) * (-xi + 2 * eta), -0.5 * (1 - xi * xi),
+                     -(1 + xi) * eta, 0.5 * (1 - xi * xi), -(1 - xi) * eta), axis=-1)
+    return values, np.stack((dxi, deta), axis=-1)
+
+
+def serendipity_quad(nodes: np.ndarray, cells: np.ndarray):
+    """Serendipity stiffness and mass on bilinear-geometry quads (corner nodes ``cells[:, :4]``),
+    3 x 3 Gauss rule in FP64 (``discretization.py:223-267``)."""
+
+    g, gw = np.polynomial.legendre.leggauss(3)
+    pts = np.array([[x, y] for y in g for x in g])
+    wts = np.array([wx * wy for wy in gw for wx in gw])
+    _, geo = _bilinear(pts)
+    values, grads = serendipity_shape(pts)
+    jac = np.einsum("cid,qia->cqda", nodes[cells[:, :4]], geo)
+    phys = np.einsum("qia,cqab->cqib", grads, np.linalg.inv(jac))
+    wdet = np.abs(np.linalg.det(jac)) * wts
+    return np.einsum("cqid,cqjd,cq->cij", phys, phys, wdet), np.einsum("qi,qj,cq->cij", values, values, wdet)
+
+
+# degree-4 six-point rule on the unit triangle (Strang and Fix), weights summing to 1/2
+_DEG4_POINTS = np.array([[0.445948490915965, 0.445948490915965], [0.445948490915965, 0.108103018168070],
+                         [0.108103018168070, 0.445948490915965], [0.091576213509771, 0.091576213509771],
+                         [0.091576213509771, 0.816847572980459], [0.816847572980459, 0.091576213509771]])
+_DEG4_WEIGHTS = np.array([0.1116907948390055] * 3 + [0.0549758718276610] * 3)
+
+
+def p2_triangle(nodes: np.ndarray, cells: np.ndarray):
+    """P2 stiffness and mass (vertices, then midpoints 01, 12, 20) with the degree-4 rule
+    (``fem/triangle.py:136-257``)."""
+
+    cn = nodes[cells[:, :3]]
+    jac = np.stack((cn[:, 1] - cn[:, 0], cn[:, 2] - cn[:, 0]), axis=-1)
+    l2, l3 = _DEG4_POINTS[:, 0, None], _DEG4_POINTS[:, 1, None]
+    l1 = 1.0 - l2 - l3
+    g1, g2, g3 = _REF_GRAD
+    ref = np.stack(((4 * l1 - 1) * g1, (4 * l2 - 1) * g2, (4 * l3 - 1) * g3, 4 * (l1 * g2 + l2 * g1),
+                    4 * (l2 * g3 + l3 * g2), 4 * (l3 * g1 + l1 * g3)), axis=-2)
+    grads = np.einsum("eij,qnj->eqni", np.linalg.inv(jac).transpose(0, 2, 1), ref)
+    qw = np.abs(np.linalg.det(jac))[:, None] * _DEG4_WEIGHTS[None, :]
+    l1, l2, l3 = l1[:, 0], l2[:, 0], l3[:, 0]
+    shape = np.stack((l1 * (2 * l1 - 1), l2 * (2 * l2 - 1), l3 * (2 * l3 - 1), 4 * l1 * l2, 4 * l2 * l3, 4 * l3 * l1), -1)
+    return np.einsum("eqid,eqjd,eq->eij", grads, grads, qw), np.einsum("qi,qj,eq->eij", shape, shape, qw)
+
+
 def robin_2p5d(centers, normals, surface_mask, surface_level, source_center, wavenumber) -> np.ndarray:
     """Unit-conductivity 2.5D mixed-boundary coefficient ``k (r.n/r) K1(kr)/K0(kr)`` with image."""
 

@@ -501,14 +501,19 @@ def ic0_patterns(indptr: np.ndarray, indices: np.ndarray):
 
 
 def ic0_pcg(n_sys, n, r, rp, indptr, indices, patterns, values, b, n_b_blocks, x, tol, max_iter, *, factor_sweeps=3,
-            solve_sweeps=2, check_every=8):
+            solve_sweeps=2, check_every=8, work=None):
     """IC(0)-preconditioned CG of ``n_sys`` systems sharing one pattern (``adt_ic0_pcg``);
-    ``patterns`` are the device tensors of :func:`ic0_patterns`.  Returns ``(rc, iters, relres)``."""
+    ``patterns`` are the device tensors of :func:`ic0_patterns`; ``work`` (optional) is the caller's
+    workspace of ``adt_ic0_work_doubles`` doubles.  Returns ``(rc, iters, relres)``."""
 
     lib = library()
     nnz = int(indices.numel())
     lnnz = int(patterns["lidx"].numel())
-    work = torch.empty(int(lib.adt_ic0_work_doubles(n_sys, n, nnz, lnnz, rp)), dtype=torch.float64, device=b.device)
+    size = int(lib.adt_ic0_work_doubles(n_sys, n, nnz, lnnz, rp))
+    if work is None:
+        work = torch.empty(size, dtype=torch.float64, device=b.device)
+    elif work.numel() < size or work.dtype != torch.float64:
+        raise ValueError(f"ic0_pcg needs a float64 workspace of {size} doubles")
     iters = np.zeros((n_sys, r), dtype=np.int32)
     relres = np.zeros((n_sys, r), dtype=np.float64)
     rc = lib.adt_ic0_pcg(int(n_sys), int(n), nnz, int(r), int(rp), ptr(indptr), ptr(indices), ptr(patterns["lptr"]),

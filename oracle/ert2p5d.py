@@ -47,12 +47,22 @@ def csr_pattern(cell_dofs, boundary_dofs, n):
 
 
 class Forward2p5D:
-    """Flat-surface 2.5D operator on a triangle (P1) or quadrilateral (Q1) mesh."""
+    """2.5D operator on a triangle (P1) or quadrilateral (Q1) mesh.
 
-    def __init__(self, nodes, cells, positions, quads, surface_node_ids=None, wavenumbers=None, weights=None):
+    Flat surfaces use the mesh as the discretization and the analytic primary field.  On terrain
+    (``ert2p5d.py:170-190, 404-436, 662-671``) ``terrain`` supplies the auxiliary discretizations
+    as arrays -- ``field`` (the secondary-field space), ``potential`` (the space whose
+    unit-conductivity potentials, sampled at the field DOFs, are the primary field) and optionally
+    ``geometric`` (numerical geometric factors) -- each a dict with ``cell_dofs``, ``dof_nodes``,
+    ``boundary_dofs``, ``parents``, ``stiffness``, ``mass``, ``boundary_mass``, ``alpha`` (W, B),
+    ``electrodes`` (E, dofs) and ``boundary_cells``; everything from there on (assembly, the unit
+    potential and field solves, measurement, derivatives) is computed here."""
+
+    def __init__(self, nodes, cells, positions, quads, surface_node_ids=None, wavenumbers=None, weights=None,
+                 terrain=None):
         self.mesh = meshlib.mesh2d(nodes, cells, surface_node_ids)
-        if not self.mesh["flat"]:
-            raise NotImplementedError("the oracle restates the flat-surface (native) path only")
+        if not self.mesh["flat"] and terrain is None:
+            raise NotImplementedError("terrain meshes need their auxiliary discretizations (terrain=...)")
         self.positions, self.quads = surveylib.validate(positions, quads)
         self.E, self.D = self.positions.shape[0], self.quads.shape[0]
         if wavenumbers is None:
@@ -61,36 +71,80 @@ class Forward2p5D:
         self.W = self.k.size
         m = self.mesh
         nodes, cells = m["nodes"], m["cells"]
-        self.N, self.C = nodes.shape[0], cells.shape[0]
-        if cells.shape[1] == 3:
-            self.stiffness, self.mass = fem.p1_triangle(nodes, cells)
-        else:
-            self.stiffness, self.mass = fem.q1_quad(nodes, cells)
-        self.bmass = fem.edge_mass(m["boundary_edge_lengths"])
-        center = self.positions.mean(axis=0)
-        self.alpha = np.stack([fem.robin_2p5d(m, center, kw) for kw in self.k])  # (W, B)
-        self.bdofs, self.bcells = m["boundary_edges"], m["boundary_edge_cells"]
-        self.indptr, self.indices, self.vinv, self.binv = csr_pattern(cells, self.bdofs, self.N)
-        self.nnz = self.indices.size
+        self.C = cells.shape[0]
         cell_ids, bary = meshlib.locate2d(m, self.positions)
-        self.electrodes = np.zeros((self.E, self.N))
-        np.add.at(self.electrodes, (np.repeat(np.arange(self.E), cells.shape[1]), cells[cell_ids].reshape(-1)),
-                  bary.reshape(-1))
         d = np.linalg.norm(nodes[None] - self.positions[:, None], axis=2)
         nearest = np.argmin(d, axis=1)
         self.source_nodes = np.where(d[np.arange(self.E), nearest] <= 1e-2, nearest, -1)  # ert2p5d.py:192-198
         self.source_cells = cell_ids
         self.slots = surveylib.measurement_slots(self.quads, self.E)
-        self.primary = self._analytic_primary()
+        self.terrain = terrain
+        if terrain is None:
+            if cells.shape[1] == 3:
+                stiffness, mass = fem.p1_triangle(nodes, cells)
+            else:
+                stiffness, mass = fem.q1_quad(nodes, cells)
+            center = self.positions.mean(axis=0)
+            electrodes = np.zeros((self.E, nodes.shape[0]))
+            np.add.at(electrodes, (np.repeat(np.arange(self.E), cells.shape[1]), cells[cell_ids].reshape(-1)),
+                      bary.reshape(-1))
+            field = dict(cell_dofs=cells, dof_nodes=nodes, boundary_dofs=m["boundary_edges"], parents=np.arange(self.C),
+                         stiffness=stiffness, mass=mass, boundary_mass=fem.edge_mass(m["boundary_edge_lengths"]),
+                         alpha=np.stack([fem.robin_2p5d(m, center, kw) for kw in self.k]), electrodes=electrodes,
+                         boundary_cells=m["boundary_edge_cells"])
+        else:
+            field = terrain["field"]
+        self._use(field)
+        self.primary = self._analytic_primary() if terrain is None else self._numerical_primary()
         self.b = np.stack([self.matrix(np.ones(self.C), w) @ self.primary[w].T for w in range(self.W)]).transpose(0, 2, 1)
         self._cache = {}
+
+    def _use(self, d):
+        """Make ``d`` the discretization of the operator's assembly, solves and derivatives."""
+
+        self.dcells = np.asarray(d["cell_dofs"], dtype=np.int64)
+        self.parents = np.asarray(d["parents"], dtype=np.int64)
+        self.N = np.asarray(d["dof_nodes"]).shape[0]
+        self.stiffness, self.mass, self.bmass = (np.asarray(d[k], float) for k in ("stiffness", "mass", "boundary_mass"))
+        self.alpha = np.asarray(d["alpha"], float)
+        self.bdofs = np.asarray(d["boundary_dofs"], dtype=np.int64)
+        self.bcells = np.asarray(d["boundary_cells"], dtype=np.int64)
+        self.electrodes = np.asarray(d["electrodes"], float)
+        self.indptr, self.indices, self.vinv, self.binv = csr_pattern(self.dcells, self.bdofs, self.N)
+        self.nnz = self.indices.size
+
+    def unit_potentials(self, d):
+        """``(W, E, dofs)`` solutions of ``A_w(1) U = electrode rows`` on discretization ``d``
+        (``ert2p5d.py:391-402``), by sparse LU."""
+
+        saved = {k: getattr(self, k) for k in ("dcells", "parents", "N", "stiffness", "mass", "bmass", "alpha",
+                                               "bdofs", "bcells", "electrodes", "indptr", "indices", "vinv", "binv",
+                                               "nnz")}
+        self._use(d)
+        try:
+            rhs = self.electrodes.T
+            return np.stack([spla.splu(self.matrix(np.ones(self.C), w).tocsc()).solve(rhs).T for w in range(self.W)])
+        finally:
+            for k, v in saved.items():
+                setattr(self, k, v)
+
+    def _numerical_primary(self):
+        """Unit potentials of the potential discretization at the field DOF nodes (``ert2p5d.py:404-420``)."""
+
+        potential, field = self.terrain["potential"], self.terrain["field"]
+        pn, fn = np.asarray(potential["dof_nodes"], float), np.asarray(field["dof_nodes"], float)
+        dist = np.linalg.norm(fn[:, None, :] - pn[None, :, :], axis=2)
+        selection = np.argmin(dist, axis=1)
+        if np.any(dist[np.arange(fn.shape[0]), selection] > 1e-6):
+            raise ValueError("field DOF node not found among the primary-potential nodes")
+        return self.unit_potentials(potential)[..., selection]
 
     # -- operator -------------------------------------------------------------------------
 
     def values(self, sigma, w, robin=True):
         """CSR values of ``A_w(sigma)`` (``ert2p5d.py:252-314``)."""
 
-        sigma = np.asarray(sigma, float)
+        sigma = np.asarray(sigma, float)[self.parents]  # per discretization cell
         vol = sigma[:, None, None] * (self.stiffness + self.k[w] ** 2 * self.mass)
         out = np.bincount(self.vinv, weights=vol.reshape(-1), minlength=self.nnz)
         if robin:
@@ -175,7 +229,15 @@ class Forward2p5D:
         return np.sign(normal) * np.sqrt(np.abs(normal * reciprocal))  # ert2p5d.py:641-645
 
     def geometric_factors(self):
-        return surveylib.geometric_factors(self.positions, self.quads)
+        if self.terrain is None or self.terrain.get("geometric") is None:
+            return surveylib.geometric_factors(self.positions, self.quads)
+        g = self.terrain["geometric"]  # numerical factors (ert2p5d.py:662-671)
+        integrated = np.tensordot(self.q, self.unit_potentials(g), axes=([0], [0]))
+        p = surveylib.with_remote_slot(integrated @ np.asarray(g["electrodes"], float).T)
+        a, b, m, n = self.slots.T
+        sources = p[a] - p[b]
+        rows = np.arange(self.D)
+        return 1.0 / (sources[rows, m] - sources[rows, n])
 
     def apparent_resistivity(self, sigma, currents=1.0):
         return self.geometric_factors() * self.resistance(sigma) / currents
@@ -210,7 +272,7 @@ class Forward2p5D:
 
         phi = np.concatenate((self.fields(sigma)[0], np.zeros((self.W, 1, self.N))), axis=1)
         T = self.q[:, None, None, None] * self.templates()
-        cells = self.mesh["cells"]
+        cells = self.dcells
         slots = self.slots if rows is None else self.slots[np.asarray(rows)]
         a, b, m, n = slots.T
         rows = []
@@ -218,8 +280,18 @@ class Forward2p5D:
             sl = slice(s, s + chunk)
             cur = (phi[:, a[sl]] - phi[:, b[sl]])[..., cells]
             rec = (phi[:, m[sl]] - phi[:, n[sl]])[..., cells]
-            rows.append(-np.einsum("wdci,wcij,wdcj->dc", rec, T, cur))
+            rows.append(self._to_cells(-np.einsum("wdci,wcij,wdcj->dc", rec, T, cur)))
         return self.aggregate(np.concatenate(rows), ids, count)
+
+    def _to_cells(self, per_dcell):
+        """Sum discretization-cell columns into their mesh cells (the parents)."""
+
+        if self.dcells.shape[0] == self.C and np.array_equal(self.parents, np.arange(self.C)):
+            return per_dcell
+        out = np.zeros(per_dcell.shape[:-1] + (self.C,))
+        for c in range(self.C):
+            out[..., c] = per_dcell[..., self.parents == c].sum(axis=-1)
+        return out
 
     def pair_cotangent(self, g_normal, g_reciprocal=None):
         """Transpose of :meth:`normal_reciprocal` onto ``(E, E)`` (``ert2p5d.py:591-629``)."""
@@ -238,13 +310,13 @@ class Forward2p5D:
     def cell_gradient(self, phi, lam, robin=True):
         """``-sum_ws lam^T dA_w/dsigma_c phi`` (``ert2p5d.py:765-825``)."""
 
-        cells = self.mesh["cells"]
+        cells = self.dcells
         g = -np.einsum("wsci,wcij,wscj->c", lam[..., cells], self.templates(), phi[..., cells])
         if robin:
             bt = self.alpha[:, :, None, None] * self.bmass[None]
             per_edge = -np.einsum("wsbi,wbij,wsbj->b", lam[..., self.bdofs], bt, phi[..., self.bdofs])
-            g = g + np.bincount(self.bcells, weights=per_edge, minlength=self.C)
-        return g
+            g = g + np.bincount(self.bcells, weights=per_edge, minlength=cells.shape[0])
+        return np.bincount(self.parents, weights=g, minlength=self.C)
 
     def _combine_scale(self, normal, reciprocal):
         return np.sign(reciprocal), np.maximum(np.sqrt(np.abs(normal * reciprocal)), 1e-30)  # ert2p5d.py:647-655

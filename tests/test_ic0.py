@@ -184,6 +184,23 @@ def test_device_ic0_converges_like_the_oracle_and_beats_jacobi(systems):
 
 
 @pytest.mark.gpu
+def test_device_ic0_ignores_the_workspace_contents(systems):
+    """The first search direction is the preconditioned residual itself: a workspace left full
+    of NaN by an earlier call does not leak into it (once it did, through 0 * NaN)."""
+
+    mats, rhs = systems
+    indptr, indices, pat, values, b, x = device_batch(mats, rhs, 5, 8)
+    n = mats[0].shape[0]
+    size = int(native.library().adt_ic0_work_doubles(len(mats), n, int(indices.numel()), int(pat["lidx"].numel()), 8))
+    work = torch.full((size,), float("nan"), dtype=torch.float64, device=values.device)
+    rc, iters, relres = native.ic0_pcg(len(mats), n, 5, 8, indptr, indices, pat, values, b, 1, x, 1e-10, 5000,
+                                       work=work)
+    clean = run(mats, rhs, 5, 8, 1e-10, 5000)
+    assert rc == native.ADT_OK and float(relres.max()) <= 1e-10
+    assert torch.equal(x.cpu(), torch.as_tensor(clean[3].reshape(-1))) and np.array_equal(iters, clean[1])
+
+
+@pytest.mark.gpu
 def test_device_ic0_own_right_hand_sides_and_warm_start(systems):
     mats, _ = systems
     rng = np.random.default_rng(11)
