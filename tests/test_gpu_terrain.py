@@ -119,3 +119,77 @@ def test_terrain_unit_potentials_come_back_from_the_disk_cache(tmp_path):
     assert np.array_equal(again, rho) and sorted(p.name for p in tmp_path.iterdir()) == files
     first.close()
     second.close()
+
+
+R = adt.REMOTE_ELECTRODE
+
+
+def with_poles(quads, count):
+    """Dipole-dipole rows plus pole-dipole, dipole-pole and pole-pole rows (``test_remote_electrodes_gpu.py:20-31``)."""
+
+    poles = [(a, R, a + 1 + s, a + 2 + s) for a in range(count - 3) for s in (0, 1) if a + 2 + s < count]
+    poles += [(a, a + 1, a + 2 + s, R) for a in range(count - 2) for s in (0, 1) if a + 2 + s < count]
+    poles += [(a, R, a + 1 + s, R) for a in range(count - 1) for s in (0, 1) if a + 1 + s < count]
+    return np.vstack((np.asarray(quads), np.asarray(poles)))
+
+
+def test_terrain_spaces_have_robin_far_field_boundaries_and_sane_pole_pole_readings():
+    """``test_remote_electrodes_gpu.py:140-175``: the sides and bottom of both auxiliary spaces
+    carry the 2.5D mixed condition and the ground surface does not; homogeneous pole-pole
+    resistances on the sloped strip stay within half of the flat strip's (natural outer
+    boundaries would make them orders of magnitude larger).  The terrain resistances also match
+    the oracle, remote rows included."""
+
+    def pole_pole(slope):
+        case = adt.cases.quad_strip_case(anomaly=False, slope=slope)
+        quads = with_poles(case.measurements, case.electrodes.shape[0])
+        mesh = adt.Mesh.from_arrays(case.nodes, case.cells, surface_node_ids=case.surface_node_ids)
+        op = adt.ERTForward2p5D.from_mesh_survey(mesh, adt.Survey.from_arrays(case.electrodes, quads))
+        sigma = np.full(mesh.cell_count, 0.01)
+        resistance = op.resistance(torch.as_tensor(sigma)).numpy()
+        return case, quads, op, sigma, resistance
+
+    _, quads, flat_op, _, flat = pole_pole(0.0)
+    case, _, op, sigma, terrain = pole_pole(0.03)
+    for d in (op.discretization, op.primary_potential_discretization):
+        surface = np.asarray(d.geometry.surface_edge_mask, dtype=bool)
+        robin = np.asarray(d.boundary_geometries)
+        assert surface.any() and (~surface).any()
+        assert np.all(robin[:, surface] == 0) and np.all(np.abs(robin[:, ~surface]).max(axis=0) > 0)
+    rows = (quads[:, 1] == R) & (quads[:, 3] == R)
+    assert rows.sum() > 0 and np.all(np.abs(terrain[rows] - flat[rows]) < 0.5 * flat[rows])
+    ref = oracle2p5d.Forward2p5D(case.nodes, case.cells, case.electrodes, quads, surface_node_ids=case.surface_node_ids,
+                                 wavenumbers=op.wavenumbers.double().numpy(), weights=op.weights.double().numpy(),
+                                 terrain=dict(field=arrays(op.discretization),
+                                              potential=arrays(op.primary_potential_discretization),
+                                              geometric=arrays(op.geometric_discretization)))
+    err = relmax(terrain, ref.resistance(sigma))
+    print(f"[terrain poles] resistance vs oracle {err:.2e}")
+    assert err < 1e-6
+    flat_op.close()
+    op.close()
+
+
+def test_swapped_receivers_on_terrain_give_the_same_data_and_derivatives(strip):
+    """``test_polarity_gpu.py:53-90`` (terrain case): reversed receivers flip the sign of the
+    numerical geometric factor and of the resistance, and leave the apparent resistivities and
+    the gradient of their logarithm unchanged."""
+
+    case, mesh, op, _ = strip
+    quads = case.measurements.copy()
+    rows = np.arange(0, len(quads), 2)
+    quads[rows, 2], quads[rows, 3] = case.measurements[rows, 3], case.measurements[rows, 2]
+    flipped = adt.ERTForward2p5D.from_mesh_survey(mesh, adt.Survey.from_arrays(case.electrodes, quads),
+                                                  topographic_geometric_factor_mode="numerical")
+    sigma = torch.as_tensor(case.conductivity)
+    assert bool(torch.all(flipped._geometric_factors()[rows] < 0))
+    rho = flipped.apparent_resistivity_values(sigma)
+    assert bool(torch.all(rho > 0)) and relmax(rho, op.apparent_resistivity_values(sigma)) < 1e-10
+    assert relmax(flipped.resistance(sigma)[rows], -op.resistance(sigma)[rows]) < 1e-10
+    grads = []
+    for forward in (op, flipped):
+        model = sigma.clone().requires_grad_()
+        torch.log(adt.apparent_resistivity_autograd(model, forward)).sum().backward()
+        grads.append(model.grad)
+    assert relmax(grads[1], grads[0]) < 1e-8
+    flipped.close()
