@@ -1,16 +1,21 @@
 """2.5D multi-wavenumber ERT forward operator on one MI355X.
 
-Drop-in for ``adtlert.forward.ERTForward2p5D`` (``forward/ert2p5d.py:108-1121``) on flat
-surfaces.  Every cosine-transform wavenumber ``w`` and source electrode ``s`` solves
+Drop-in for ``adtlert.forward.ERTForward2p5D`` (``forward/ert2p5d.py:108-1121``).  Every
+cosine-transform wavenumber ``w`` and source electrode ``s`` solves
 
     A_w(sigma) u_ws = A_w(1) u_p,ws
 
 with the reference's secondary-field split ``u = u_s + rho_src u_p`` used as the initial guess
-``x0 = rho_src u_p`` of FP64 Jacobi-PCG (mathematically the same system the reference
+``x0 = rho_src u_p`` of an FP64 CG-family solver (mathematically the same system the reference
 factorizes with cuDSS).  Assembly, solves, electrode potentials, the fused sensitivity and the
 adjoint gradient run in the native gfx950 library; results return to the host in
-``FLOAT_DTYPE`` like the reference.  The terrain (auxiliary quad/P2) discretizations are not
-part of this path and raise ``NotImplementedError``.
+``FLOAT_DTYPE`` like the reference.
+
+On a flat surface the fields live on the input mesh with the analytic primary field.  On
+terrain (``ert2p5d.py:170-190, 349-436, 662-671``) they live on an auxiliary space built by
+``terrain.build`` (H2-refined quad strips or refined P1 triangles), the primary field is the
+unit-conductivity potential of a serendipity/P2 space solved on the device, geometric factors
+may be numerical, and cell quantities are summed back to the input cells.
 """
 
 from __future__ import annotations
