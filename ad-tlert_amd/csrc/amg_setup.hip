@@ -109,11 +109,11 @@ double lambda_max(const HostLevel& L, const std::vector<double>& bv, const std::
 }
 
 // Dampings, the filtered diagonal map, the prolongator pattern with its gather map, and R = P^T.
-void prolongator(HostLevel& L, const std::vector<char>& strong) {
+void prolongator(HostLevel& L, const std::vector<char>& strong, double smoother_scale) {
     const int n = L.n, nc = L.nc;
     std::vector<double> d(n);
     for (int i = 0; i < n; ++i) d[i] = L.rep[L.diag[i]];
-    L.omega = 4.0 / (3.0 * lambda_max(L, L.rep, d));
+    L.omega = smoother_scale * 4.0 / (3.0 * lambda_max(L, L.rep, d));
     // filtered matrix: strong off-diagonals, weak ones lumped into the diagonal
     L.dmap_ptr.assign(n + 1, 0);
     L.dmap.clear();
@@ -303,7 +303,7 @@ std::vector<HostLevel> build_hierarchy(int n, const int32_t* ptr, const int32_t*
         if (nc == 0 || nc > L.n * 9 / 10) break;  // coarsening stalled: this level is the coarsest
         L.agg = std::move(agg);
         L.nc = nc;
-        prolongator(L, strong);
+        prolongator(L, strong, opt.smoother_scale);
         HostLevel next;
         galerkin(L, next);
         bool ok = true;

@@ -26,7 +26,7 @@ struct HostLevel {
     int nc = 0;  // rows of A_{l+1} (aggregates); 0 on the coarsest level
     std::vector<int> ptr, idx, diag;  // pattern of A_l and the slot of each diagonal entry
     std::vector<double> rep;          // representative values of A_l
-    double omega = 2.0 / 3.0;         // smoother damping 4 / (3 lambda_max(D^-1 A))
+    double omega = 2.0 / 3.0;         // smoother damping s * 4 / (3 lambda_max(D^-1 A)), s = smoother_scale
     double omega_p = 2.0 / 3.0;       // prolongation damping 4 / (3 lambda_max(D_F^-1 A_F))
     std::vector<int> agg;             // aggregate of each row, -1: left out (no strong connection)
     std::vector<int> dmap_ptr, dmap;  // per row: slots summed into D_F (diagonal + weak entries)
@@ -46,6 +46,11 @@ struct Options {
     double theta = 0.08;    // strength threshold |a_ij| >= theta sqrt(a_ii a_jj)
     int coarse_max = 256;   // stop coarsening at this many rows (dense inverse on the coarsest level)
     int max_levels = 12;
+    // Jacobi smoother damping in units of 4 / (3 lambda_max(D^-1 A)), the choice that minimizes the
+    // largest amplification of the upper half of the spectrum for a stand-alone smoother.  Inside
+    // a Krylov-accelerated cycle a larger damping is better; the cycle stays a symmetric positive
+    // definite preconditioner while omega lambda_max < 2, i.e. smoother_scale < 1.5.
+    double smoother_scale = 1.0;
 };
 
 // Builds the levels from a CSR pattern and representative values (rows = n).  The last

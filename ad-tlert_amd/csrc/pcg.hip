@@ -1720,16 +1720,18 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
 
 extern "C" int adt_pcg_attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indices,
                                   const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
-                                  int max_levels, int stall_limit, int* h_levels) {
+                                  int max_levels, int stall_limit, double smoother_scale, int* h_levels) {
     ADT_REQUIRE(p && h_indptr && h_indices && h_values && h_val_off, "adt_pcg_attach_amg: null pointer");
     ADT_REQUIRE(theta >= 0.0 && theta < 1.0, "adt_pcg_attach_amg: theta must be in [0, 1)");
     ADT_REQUIRE(coarse_max >= 1 && coarse_max <= 4096, "adt_pcg_attach_amg: coarse_max must be in [1, 4096]");
     ADT_REQUIRE(max_levels >= 2 && max_levels <= 32, "adt_pcg_attach_amg: max_levels must be in [2, 32]");
     ADT_REQUIRE(stall_limit >= 0, "adt_pcg_attach_amg: stall_limit must be >= 0");
+    ADT_REQUIRE(smoother_scale > 0.0 && smoother_scale < 1.5, "adt_pcg_attach_amg: smoother_scale must be in (0, 1.5)");
     adt::amg::Options opt;
     opt.theta = theta;
     opt.coarse_max = coarse_max;
     opt.max_levels = max_levels;
+    opt.smoother_scale = smoother_scale;
     const int rc = attach_amg(p, h_indptr, h_indices, h_values, h_val_off, opt);
     if (rc) {
         p->free_amg();

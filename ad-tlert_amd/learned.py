@@ -198,7 +198,7 @@ class TrainingSystem:
 
 
 def prepare_system(indptr, indices, values: torch.Tensor, errors: torch.Tensor, *, theta=0.08,
-                   device=None) -> TrainingSystem:
+                   smoother_scale=native.SMOOTHER_SCALE, device=None) -> TrainingSystem:
     """Aggregation (as the multigrid setup builds it), the prolongator terms and dampings."""
 
     device = values.device if device is None else device
@@ -224,10 +224,11 @@ def prepare_system(indptr, indices, values: torch.Tensor, errors: torch.Tensor, 
     slot = np.concatenate((self_terms, strong_terms))
     coef = np.concatenate((np.zeros(self_terms.size), vals[strong_terms] / df[rows[strong_terms]]))
     is_self = np.concatenate((np.ones(self_terms.size), np.zeros(strong_terms.size)))
-    # damping constants of the multigrid setup: 4 / (3 lambda_max) of D^-1 A and of D_F^-1 A_F
+    # damping constants of the multigrid setup: smoother_scale * 4 / (3 lambda_max) of D^-1 A and
+    # 4 / (3 lambda_max) of D_F^-1 A_F
     fv = np.where(strong, vals, 0.0)
     fv[isdiag] = df[rows[isdiag]]
-    omega_s = 4.0 / (3.0 * _power_lambda(indptr, indices, vals, diag))
+    omega_s = smoother_scale * 4.0 / (3.0 * _power_lambda(indptr, indices, vals, diag))
     omega_p = 4.0 / (3.0 * _power_lambda(indptr, indices, fv, df))
     t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=device)  # noqa: E731
     graph = GnnGraph.from_pattern(indptr, indices, agg, device)

@@ -20,6 +20,11 @@ LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().paren
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
 PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_AMG = 0, 1, 2
 LDS_BYTES = int(os.environ.get("ADT_SPMV_LDS_BYTES", 49152))  # 0 disables the LDS-staged product
+# Jacobi smoother damping of the multigrid cycle in units of 4 / (3 lambda_max(D^-1 A)).  Measured
+# with flexible CG to 1e-10 (one MI355X): 1.3 takes the 2.5D time-lapse batch from 24.3 to 21.8
+# mean iterations, the 3D P1 box from 33.9 to 31.9 and the 3D P2 terrain mesh from 128 to 113;
+# 1.4 is slower on the 2.5D batch (23.3) and the SPD limit is 1.5.
+SMOOTHER_SCALE = 1.3
 
 c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 P = c_void_p
@@ -37,7 +42,7 @@ _SIGNATURES = {
     "adt_pcg_destroy": (c_int, [P]),
     "adt_pcg_profile": (c_int, [P, c_int]),
     "adt_pcg_profile_read": (c_int, [P, P]),
-    "adt_pcg_attach_amg": (c_int, [P, P, P, P, P, c_double, c_int, c_int, c_int, P]),
+    "adt_pcg_attach_amg": (c_int, [P, P, P, P, P, c_double, c_int, c_int, c_int, c_double, P]),
     "adt_pcg_amg_level": (c_int, [P, c_int, c_int, P, P, P]),
     "adt_pcg_stats": (c_int, [P, P, P, P]),
     "adt_amg_hierarchy_info": (c_int, [c_int, P, P, P, c_double, c_int, c_int, P, P, P]),
@@ -234,9 +239,11 @@ class PcgPlan:
         check(self._lib.adt_pcg_tiling(handle, ctypes.byref(rb), ctypes.byref(tiles), ctypes.byref(lds_rows)))
         self.rows_per_tile, self.tiles, self.lds_rows = rb.value, tiles.value, lds_rows.value
 
-    def attach_amg(self, values, *, theta=0.08, coarse_max=256, max_levels=12, stall_limit=64) -> int:
+    def attach_amg(self, values, *, theta=0.08, coarse_max=256, max_levels=12, stall_limit=64,
+                   smoother_scale=SMOOTHER_SCALE) -> int:
         """Build the multigrid hierarchy from the first system of each CSR pattern in the device
-        ``values`` (read back to the host once); returns the number of levels."""
+        ``values`` (read back to the host once); returns the number of levels.  ``smoother_scale``
+        multiplies every level's Jacobi smoother damping 4 / (3 lambda_max)."""
 
         if self.host_pattern is None:
             raise ValueError("the multigrid preconditioner needs the host CSR pattern")
@@ -258,7 +265,8 @@ class PcgPlan:
         levels = c_int(0)
         check(self._lib.adt_pcg_attach_amg(self._handle, host(self.host_pattern[0]), host(self.host_pattern[1]),
                                            host(h_values), host(h_val_off), float(theta), int(coarse_max),
-                                           int(max_levels), int(stall_limit), ctypes.byref(levels)))
+                                           int(max_levels), int(stall_limit), float(smoother_scale),
+                                           ctypes.byref(levels)))
         self.amg_levels = levels.value
         return self.amg_levels
 

@@ -81,6 +81,7 @@ class BatchedSolver:
     amg_theta: float = 0.08
     amg_coarse_max: int = 128      # coarsest operators up to 128 rows are inverted in LDS
     amg_stall_limit: int = 64
+    amg_smoother_scale: float = native.SMOOTHER_SCALE  # Jacobi smoother damping / (4 / (3 lambda_max))
     damping_network: object = None  # learned.DampingNetwork for preconditioner="learned"
     ic0_factor_sweeps: int = 3     # fixed-point sweeps of the IC(0) factor
     ic0_solve_sweeps: int = 2      # Jacobi sweeps per triangular solve of the IC(0) preconditioner
@@ -163,7 +164,7 @@ class BatchedSolver:
                 if self.preconditioner in ("amg", "learned"):
                     try:
                         plan.attach_amg(values_d, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
-                                        stall_limit=0)
+                                        stall_limit=0, smoother_scale=self.amg_smoother_scale)
                         precond = native.PRECOND_AMG
                     except ValueError:  # no hierarchy for this pattern: time Jacobi PCG instead
                         pass
@@ -202,7 +203,7 @@ class BatchedSolver:
         if plan.amg_levels == 0 and not getattr(plan, "amg_failed", False):
             try:
                 plan.attach_amg(values, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
-                                stall_limit=self.amg_stall_limit)
+                                stall_limit=self.amg_stall_limit, smoother_scale=self.amg_smoother_scale)
             except ValueError:
                 # no usable hierarchy (tiny or unstructured matrices): plain Jacobi PCG for this plan
                 plan.amg_failed = True

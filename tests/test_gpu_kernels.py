@@ -150,7 +150,7 @@ def random_spd_batch(rng, side=20, S=2):
 
 
 def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2, omega=None, rows_per_tile=32,
-                coarse_max=16):
+                coarse_max=16, smoother_scale=native.SMOOTHER_SCALE):
     rng = np.random.default_rng(seed)
     indptr, indices, values, dense = random_spd_batch(rng, side)
     S, n = values.shape[0], side * side
@@ -161,7 +161,8 @@ def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2
     values_d = torch.as_tensor(values, device=dev())
     data = None
     if precond != native.PRECOND_JACOBI:
-        assert plan.attach_amg(values_d, coarse_max=coarse_max, stall_limit=stall_limit) >= (3 if coarse_max < 64 else 2)
+        assert plan.attach_amg(values_d, coarse_max=coarse_max, stall_limit=stall_limit,
+                               smoother_scale=smoother_scale) >= (3 if coarse_max < 64 else 2)
         assert plan.amg_level(1)["rows"] < n
     if precond == native.PRECOND_LEARNED:
         data = torch.full_like(values_d, omega)
@@ -181,6 +182,29 @@ def solve_batch(precond, staged, stall_limit=64, side=20, lanes=16, R=13, seed=2
     stats = plan.stats()
     plan.close()
     return iters, stats
+
+
+def test_smoother_damping_scale_is_bounded_and_every_scale_solves():
+    """The cycle is a symmetric positive definite preconditioner only while the Jacobi damping
+    times lambda_max stays below 2: the native setup rejects scales outside (0, 1.5).  Inside the
+    range the flexible CG solves to the tolerance (against dense solves) at the stand-alone
+    smoother's damping (1.0) and at the library default."""
+
+    rng = np.random.default_rng(3)
+    indptr, indices, values, _ = random_spd_batch(rng, 12)
+    S, n = values.shape[0], 144
+    plan = native.PcgPlan(np.full(S, n), 8, 8, 32, np.zeros(S), np.zeros(S), np.arange(S) * values.shape[1],
+                          host_pattern=(indptr, indices), lds_bytes=8192)
+    for bad in (0.0, 1.5, -1.0):
+        with pytest.raises(ValueError, match="smoother_scale"):
+            plan.attach_amg(torch.as_tensor(values, device=dev()), coarse_max=16, smoother_scale=bad)
+    plan.close()
+    counts = {}
+    for scale in (1.0, native.SMOOTHER_SCALE):
+        iters, stats = solve_batch(native.PRECOND_AMG, True, smoother_scale=scale)
+        assert stats["fallback_rhs"] == 0
+        counts[scale] = float(iters[iters > 0].mean())
+    print(f"mean FCG iterations by smoother scale: {counts}")
 
 
 def test_iteration_cap_under_the_cycle_runs_exactly_that_many_cycle_iterations():
