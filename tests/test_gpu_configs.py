@@ -323,5 +323,5 @@ def test_c5_terrain_p2_learned_matches_the_oracle_fixture_and_properties(c5):
     case, forward, data = c5
     assert forward.element_order == 2 and not forward.mesh.is_flat_surface
     jac = golden_vectors("C5", case, forward, data)
-    assert forward._solver.last.preconditioner in ("learned",)
+    assert forward._solver.last.preconditioner == "learned" and forward._solver.last.fallback_rhs == 0
     properties("C5", case, forward, jac)

@@ -207,6 +207,83 @@ def test_smoother_damping_scale_is_bounded_and_every_scale_solves():
     print(f"mean FCG iterations by smoother scale: {counts}")
 
 
+def test_prolongation_damping_is_four_thirds_over_the_largest_eigenvalue():
+    """The finest prolongator is smoothed with omega_p = 4 / (3 lambda_max(D_F^-1 A_F)), A_F the
+    filtered matrix (strong off-diagonals, weak ones lumped into the diagonal D_F).  The setup's
+    Lanczos estimate of lambda_max matches a dense symmetric eigensolver."""
+
+    rng = np.random.default_rng(8)
+    indptr, indices, values, dense = random_spd_batch(rng, 14, S=1)
+    n, theta = 196, 0.08
+    plan = native.PcgPlan(np.array([n]), 8, 8, 32, np.zeros(1), np.zeros(1), np.zeros(1),
+                          host_pattern=(indptr, indices), lds_bytes=8192)
+    plan.attach_amg(torch.as_tensor(values, device=dev()), coarse_max=16, theta=theta)
+    omega_p = plan.amg_level(0)["omega"]
+    plan.close()
+    A = dense[0]
+    d = np.diag(A).copy()
+    off = A - np.diag(d)
+    strong = (off != 0) & (np.abs(off) >= theta * np.sqrt(np.outer(d, d)))
+    df = d + np.where(strong, 0.0, off).sum(axis=1)
+    df = np.where(df > 0, df, d)
+    AF = np.where(strong, off, 0.0) + np.diag(df)
+    s = 1.0 / np.sqrt(df)
+    lam = np.linalg.eigvalsh(s[:, None] * AF * s[None, :]).max()
+    print(f"omega_p {omega_p:.12f} against 4/(3 lambda_max) {4 / (3 * lam):.12f}")
+    assert abs(omega_p - 4.0 / (3.0 * lam)) <= 1e-6 * omega_p
+
+
+def stencil_27_pattern(side):
+    """CSR pattern of the 27-point stencil on a side^3 grid (rows of up to 27 entries)."""
+
+    n = side ** 3
+    g = np.arange(n).reshape(side, side, side)
+    rows, cols = [], []
+    for d in np.ndindex(3, 3, 3):
+        di, dj, dk = (np.asarray(d) - 1).tolist()
+        i, j, k = np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij")
+        ok = (i + di >= 0) & (i + di < side) & (j + dj >= 0) & (j + dj < side) & (k + dk >= 0) & (k + dk < side)
+        rows.append(g[i[ok], j[ok], k[ok]])
+        cols.append(g[i[ok] + di, j[ok] + dj, k[ok] + dk])
+    keys = np.unique(np.concatenate(rows).astype(np.int64) * n + np.concatenate(cols))
+    r = keys // n
+    indptr = np.concatenate(([0], np.cumsum(np.bincount(r, minlength=n)))).astype(np.int32)
+    return indptr, (keys % n).astype(np.int32)
+
+
+def test_autotuner_times_the_wide_lds_budget_where_the_default_cuts_the_tile():
+    """Dense rows (a 27-point stencil): at the default LDS budget the staged tiles of the
+    narrow lane groups are cut short, so the autotuner also times the wide budget for them, keeps
+    the fastest (lanes, budget) pair and builds the solve's plan with it; the solve then meets
+    the tolerance."""
+
+    side, R = 16, 64
+    indptr, indices = stencil_27_pattern(side)
+    n = side ** 3
+    rows = indptr.size - 1
+    counts = np.diff(indptr)
+    diag_slot = np.array([indptr[i] + np.searchsorted(indices[indptr[i]:indptr[i + 1]], i) for i in range(rows)])
+    values = np.full(indices.size, -1.0)
+    values[diag_slot] = counts - 1 + 0.05
+    solver = adt.solver.BatchedSolver(preconditioner="amg", rows_per_tile=128)
+    plan = solver.plan("stencil27", np.array([n]), R, pattern_offsets=(np.zeros(1), np.zeros(1), np.zeros(1)),
+                       host_pattern=(indptr, indices))
+    (times,) = solver.tuning.values()
+    print({f"{k[0]} lanes, {k[1] // 1024} KB": round(v * 1e3, 3) for k, v in times.items()})
+    assert (16, native.LDS_BYTES_WIDE) in times and (16, native.LDS_BYTES) in times
+    best = min(times, key=times.get)
+    assert (plan.lanes, plan.lds_bytes) == best
+    rng = np.random.default_rng(4)
+    b = torch.zeros((1, n, plan.rp), dtype=torch.float64, device=dev())
+    b[0, :, :R] = torch.as_tensor(rng.standard_normal((n, R)), device=dev())
+    x = torch.zeros_like(b)
+    values_d = torch.as_tensor(values[None], device=dev())
+    stats = solver.solve(plan, torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()), values_d,
+                         b.reshape(-1), x.reshape(-1))
+    assert stats.preconditioner == "amg" and stats.relative_residual.max() <= 1e-10
+    solver.close()
+
+
 def test_iteration_cap_under_the_cycle_runs_exactly_that_many_cycle_iterations():
     """The lane-width autotuner times a fixed number of preconditioned iterations by solving at
     an unreachable tolerance.  A right-hand side that merely reaches max_iter under the cycle
