@@ -348,7 +348,7 @@ std::vector<HostLevel> build_hierarchy(int n, const int32_t* ptr, const int32_t*
         if (nc == 0 || nc > L.n * 9 / 10) break;  // coarsening stalled: this level is the coarsest
         L.agg = std::move(agg);
         L.nc = nc;
-        prolongator(L, strong, opt.smoother_scale);
+        prolongator(L, strong, levels.size() == 1 ? opt.smoother_scale : 1.0);
         HostLevel next;
         galerkin(L, next);
         bool ok = true;

@@ -1403,7 +1403,10 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
     if (hipGetDevice(&device) != hipSuccess ||
         hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
         lds_limit = 0;
-    lds_limit = std::max(0, std::min(lds_limit - 8192, 65536));  // the block reductions keep static LDS
+    // the block reductions keep static LDS.  Up to 96 KB per workgroup (one per CU): the coarse
+    // operators of ten-node tetrahedra have ~70 entries per row, and their 16-row tiles stage ~450
+    // rows; at a 64 KB cap they fell back to the gather kernels (3D P2 terrain cycle 4.05 -> 1.86 ms)
+    lds_limit = std::max(0, std::min(lds_limit - 8192, 98304));
     // tile height per level: the finest level's tiles; coarse levels start at rbc rows and halve
     // (down to one pass of the workgroup) until their staged smoothing tiles fit the LDS budget
     for (int l = 0; l < depth; ++l) {

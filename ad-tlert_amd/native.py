@@ -23,12 +23,13 @@ LDS_BYTES = int(os.environ.get("ADT_SPMV_LDS_BYTES", 49152))  # 0 disables the L
 # Wider budget the solver's autotuner also times when the default one cuts the tile height (two
 # workgroups per CU instead of three): taller tiles for dense rows (ten-node tetrahedra)
 LDS_BYTES_WIDE = 81920
-# Jacobi smoother damping of the multigrid cycle in units of 4 / (3 lambda_max(D^-1 A)), lambda_max
-# from 40 Lanczos steps on the plan's representative matrix.  Measured with flexible CG to 1e-10
-# (one MI355X, mean iterations): 2.5D time-lapse batch 24.3 at 1.0, 22.4 at 1.2, 21.8 at 1.3,
-# 22.4 at 1.4; 3D P1 box 33.9, 32.6, 32.1, 32.3; 3D P2 terrain 128, 117, 114, and at 1.4 the cycle
-# stalled on two of twelve models (omega lambda near 2 on them: the limit of a positive definite
-# cycle), whose right-hand sides then finished in Jacobi PCG.
+# Jacobi smoother damping of the finest level of the multigrid cycle in units of
+# 4 / (3 lambda_max(D^-1 A)), lambda_max from 40 Lanczos steps on the plan's representative matrix.
+# Measured with flexible CG to 1e-10 (one MI355X, mean iterations, scale 1.0 / 1.3 / 1.45): 2.5D
+# time-lapse batch 24.3 / 22.1 / 26.8; 3D P2 terrain 128 / 114 / 108; 3D P1 box 33.9 / 34.4 / 37.0.
+# Scaling the coarse levels too took the box to 32.1, but their Galerkin operators vary from
+# system to system (lambda_max 35% above the representative's on the terrain mesh's first coarse
+# level for one of twelve models), which made that model's cycle indefinite.
 SMOOTHER_SCALE = 1.3
 
 c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -249,7 +250,7 @@ class PcgPlan:
                    smoother_scale=SMOOTHER_SCALE) -> int:
         """Build the multigrid hierarchy from the first system of each CSR pattern in the device
         ``values`` (read back to the host once); returns the number of levels.  ``smoother_scale``
-        multiplies every level's Jacobi smoother damping 4 / (3 lambda_max)."""
+        multiplies the finest level's Jacobi smoother damping 4 / (3 lambda_max)."""
 
         if self.host_pattern is None:
             raise ValueError("the multigrid preconditioner needs the host CSR pattern")

@@ -141,8 +141,9 @@ int adt_pcg_destroy(adt_pcg_plan* plan);
  * level has at most coarse_max rows (dense inverse) or max_levels is reached.  Every solve
  * recomputes the numeric prolongators and Galerkin operators of every system on the device.
  * stall_limit: iterations without a 10% residual decrease before Jacobi PCG takes over (0: never).
- * smoother_scale: damping of every level's Jacobi smoother in units of 4 / (3 lambda_max(D^-1 A)),
- * in (0, 1.5) so that the cycle stays symmetric positive definite (1.3 is the library's default).
+ * smoother_scale: damping of the finest level's Jacobi smoother in units of 4 / (3 lambda_max(D^-1 A))
+ * of the representative matrix, in (0, 1.5) so that the cycle stays symmetric positive definite
+ * (1.3 is the library's default; coarse levels use 4 / (3 lambda_max)).
  * h_levels receives the number of levels. */
 int adt_pcg_attach_amg(adt_pcg_plan* plan, const int32_t* h_indptr, const int32_t* h_indices,
                        const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
