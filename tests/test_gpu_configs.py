@@ -325,3 +325,23 @@ def test_c5_terrain_p2_learned_matches_the_oracle_fixture_and_properties(c5):
     jac = golden_vectors("C5", case, forward, data)
     assert forward._solver.last.preconditioner == "learned" and forward._solver.last.fallback_rhs == 0
     properties("C5", case, forward, jac)
+
+
+def test_c5_cycle_stays_positive_definite_over_a_model_series(c5):
+    """Regression: with the smoother damping scaled on every level, the first coarse Galerkin
+    operator of model 11 of this series had lambda_max(D^-1 A) 35% above the representative's
+    (model 0), the scaled coarse smoother went past omega lambda = 2 and the cycle's z.r turned
+    non-positive: all 64 right-hand sides of that model finished in Jacobi PCG.  Only the
+    finest level is scaled now; every model solves in the cycle."""
+
+    case, forward, _ = c5
+    rng = np.random.default_rng(44)
+    centers = case.nodes[case.cells].mean(axis=1)
+    g1 = np.log10(adt.cases.grf_3d(centers, rng, mean=0.0, std=1.0))
+    g2 = np.log10(adt.cases.grf_3d(centers, rng, mean=0.0, std=1.0))
+    for m in (0, 6, 7, 11):
+        sigma = 10.0 ** (np.log10(case.conductivity) + 0.2 * (np.sin(0.7 * m) * g1 + np.cos(0.3 * m) * g2))
+        forward.apparent_resistivity_values(torch.as_tensor(sigma))
+        st = forward._solver.last
+        print(f"[C5 model {m}] mean iterations {st.iterations.mean():.1f}, fallback {st.fallback_rhs}", flush=True)
+        assert st.fallback_rhs == 0 and st.relative_residual.max() <= 1e-10
