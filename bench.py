@@ -417,7 +417,7 @@ class TerrainP2(Box3D):
     data = "synthetic (seeded graded terrain box, random ABMN survey, log-normal GRF conductivity models)"
     order = 2
     fixture = "terrain_p2_150k"
-    sensitivity_kernel = "cell_block_kernel (cell-block sensitivity, P2)"
+    sensitivity_kernel = "cell_block_kernel<4, 256, 0> (cell-block sensitivity, P2)"
 
     @staticmethod
     def make_case(adt):
