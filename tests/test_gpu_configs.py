@@ -185,11 +185,14 @@ def test_c3_central_differences(c3):
     """Central differences with h = 1e-6 along three relative directions ``sigma * N(0, 1)``.
 
     The resistance vector's difference quotient is compared with jvp in the max norm, as the
-    reference suite does (``tests/test_forward_gpu.py:114-122``).  It divides the solver's
-    error by h: an operator solving to 1e-13 checks the derivative at the 1e-5 bar; at the
-    default tolerance (1e-10 relative residual, ~1e-11 of relative noise in each resistance)
-    the quotient carries ~1e-5 of noise, so the bar there is 1e-4.  The misfit's directional
-    derivative (a scalar with cancellation between measurements) is printed for both."""
+    reference suite does (``tests/test_forward_gpu.py:114-122``).  The quotient divides the
+    solver's error by h, so it is formed with an operator solving to 1e-13 (~1e-7 of noise).
+    Both that operator's jvp and the jvp of the operator at the shipped tolerance (1e-10) are
+    checked against it at the 1e-5 bar.  A quotient formed from solves at 1e-10 carries the
+    resistances' error over h: a relative residual of 1e-10 leaves relative resistance errors of
+    1e-11 to a few 1e-10 depending on how the iterates approach the solution, i.e. 1e-5 to a
+    few 1e-4 in the quotient; it is printed, not asserted.  So is the misfit's directional
+    derivative (a scalar with cancellation between measurements)."""
 
     case, ref, forward, observed, directions = c3
     sigma = torch.as_tensor(case.conductivity)
@@ -200,16 +203,22 @@ def test_c3_central_differences(c3):
         rhoa = op.apparent_resistivity_values(s).numpy()
         return 0.5 * float(((np.log(rhoa) - observed) ** 2).sum())
 
-    for op, name, bar in ((tight, "tol 1e-13", DERIV_BAR), (forward, "tol 1e-10", 1e-4)):
-        _, grad, _ = misfit_and_gradient(op, case.conductivity, observed)
-        for i, d in enumerate(directions):
-            step = torch.as_tensor(h * d)
-            numeric = ((op.resistance(sigma + step) - op.resistance(sigma - step)) / (2 * h)).numpy()
+    grads = {name: misfit_and_gradient(op, case.conductivity, observed)[1]
+             for op, name in ((tight, "tol 1e-13"), (forward, "tol 1e-10"))}
+    for i, d in enumerate(directions):
+        step = torch.as_tensor(h * d)
+        numeric = ((tight.resistance(sigma + step) - tight.resistance(sigma - step)) / (2 * h)).numpy()
+        for op, name in ((tight, "tol 1e-13"), (forward, "tol 1e-10")):
             jvp = op.jvp(sigma, torch.as_tensor(d)).numpy()
-            err = report(f"C3 FD of the resistances vs jvp, {name}, direction {i}", relmax(jvp, numeric), bar)
-            assert err < bar
+            err = report(f"C3 FD (solves at 1e-13) of the resistances vs jvp at {name}, direction {i}",
+                         relmax(jvp, numeric), DERIV_BAR)
+            assert err < DERIV_BAR
+        loose = ((forward.resistance(sigma + step) - forward.resistance(sigma - step)) / (2 * h)).numpy()
+        print(f"[C3 FD from solves at tol 1e-10 vs jvp, direction {i}] "
+              f"{relmax(forward.jvp(sigma, torch.as_tensor(d)).numpy(), loose):.2e}", flush=True)
+        for op, name in ((tight, "tol 1e-13"), (forward, "tol 1e-10")):
             fd = (misfit(op, sigma + step) - misfit(op, sigma - step)) / (2 * h)
-            analytic = float(grad @ d)
+            analytic = float(grads[name] @ d)
             print(f"[C3 misfit FD {name} direction {i}] numeric {fd:.9e} analytic {analytic:.9e} "
                   f"rel {abs(fd - analytic) / abs(analytic):.2e}", flush=True)
     tight.close()
