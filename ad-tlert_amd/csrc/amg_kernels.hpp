@@ -25,7 +25,8 @@ namespace krylov {
 // ------------------------------------------------------------------------------------------
 
 // Thread per row i of system blockIdx.y: inverse diagonal (for the smoother), then the
-// prolongator row from the filtered matrix (strong entries, weak ones lumped into D_F):
+// prolongator row: a given prolongator's values (pfix), or from the filtered matrix (strong
+// entries, weak ones lumped into D_F):
 //   P[i, J] = [J == agg(i)] (1 - omega_ii) - D_F,ii^-1 sum_{strong k in row i, agg(col k) = J} omega_k A_k
 // with omega_k the level's constant damping or, when given, the learned per-slot damping.
 __global__ __launch_bounds__(256) void amg_pvals(const LevelArgs* __restrict__ la) {
@@ -37,6 +38,13 @@ __global__ __launch_bounds__(256) void amg_pvals(const LevelArgs* __restrict__ l
     const int dslot = adt::gview(a.diag)[S.diag + i];
     if (a.dinv_out) adt::gview(a.dinv_out)[S.dinv + i] = 1.0 / V[dslot];
     if (S.nc == 0) return;
+    if (a.pfix) {  // a given prolongator: the same values for every system of the pattern
+        const auto pptr = adt::gview(a.pptr + S.pptr);
+        const auto fix = adt::gview(a.pfix + S.pidx);
+        const auto P = adt::gview(a.pval + S.pval);
+        for (int t = pptr[i]; t < pptr[i + 1]; ++t) P[t] = fix[t];
+        return;
+    }
     const auto dmap_ptr = adt::gview(a.dmap_ptr + S.dmap_ptr);
     const auto dmap = adt::gview(a.dmap + S.dmap);
     double df = 0.0;

@@ -153,9 +153,27 @@ double lambda_max(const HostLevel& L, const std::vector<double>& bv, const std::
     return lam > 0.0 && std::isfinite(lam) ? lam : 2.0;
 }
 
+// R = P^T: per coarse row, its (fine row, P slot) entries in fine-row order.
+void restriction(HostLevel& L) {
+    const int n = L.n, nc = L.nc;
+    L.rptr.assign(nc + 1, 0);
+    for (int J : L.pidx) L.rptr[J + 1]++;
+    for (int J = 0; J < nc; ++J) L.rptr[J + 1] += L.rptr[J];
+    L.rrow.assign(L.pidx.size(), 0);
+    L.rslot.assign(L.pidx.size(), 0);
+    std::vector<int> fill(L.rptr.begin(), L.rptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int s = L.pptr[i]; s < L.pptr[i + 1]; ++s) {
+            const int J = L.pidx[s];
+            L.rrow[fill[J]] = i;
+            L.rslot[fill[J]] = s;
+            fill[J]++;
+        }
+}
+
 // Dampings, the filtered diagonal map, the prolongator pattern with its gather map, and R = P^T.
 void prolongator(HostLevel& L, const std::vector<char>& strong, double smoother_scale) {
-    const int n = L.n, nc = L.nc;
+    const int n = L.n;
     std::vector<double> d(n);
     for (int i = 0; i < n; ++i) d[i] = L.rep[L.diag[i]];
     L.omega = smoother_scale * 4.0 / (3.0 * lambda_max(L, L.rep, d));
@@ -198,19 +216,29 @@ void prolongator(HostLevel& L, const std::vector<char>& strong, double smoother_
         }
         L.pptr[i + 1] = static_cast<int>(L.pidx.size());
     }
-    L.rptr.assign(nc + 1, 0);
-    for (int J : L.pidx) L.rptr[J + 1]++;
-    for (int J = 0; J < nc; ++J) L.rptr[J + 1] += L.rptr[J];
-    L.rrow.assign(L.pidx.size(), 0);
-    L.rslot.assign(L.pidx.size(), 0);
-    std::vector<int> fill(L.rptr.begin(), L.rptr.end() - 1);
-    for (int i = 0; i < n; ++i)
-        for (int s = L.pptr[i]; s < L.pptr[i + 1]; ++s) {
-            const int J = L.pidx[s];
-            L.rrow[fill[J]] = i;
-            L.rslot[fill[J]] = s;
-            fill[J]++;
-        }
+    restriction(L);
+}
+
+// A given prolongator (FixedProlongator) on this level: no aggregation, no filtered diagonal.
+void fixed_prolongator(HostLevel& L, const FixedProlongator& fp, double smoother_scale) {
+    const int n = L.n;
+    std::vector<double> d(n);
+    for (int i = 0; i < n; ++i) d[i] = L.rep[L.diag[i]];
+    L.omega = smoother_scale * 4.0 / (3.0 * lambda_max(L, L.rep, d));
+    L.omega_p = 0.0;
+    L.nc = fp.nc;
+    L.agg.assign(n, -1);
+    L.dmap_ptr.resize(n + 1);
+    L.dmap.resize(n);
+    for (int i = 0; i <= n; ++i) L.dmap_ptr[i] = i;
+    for (int i = 0; i < n; ++i) L.dmap[i] = L.diag[i];
+    L.pptr.assign(fp.ptr, fp.ptr + n + 1);
+    L.pidx.assign(fp.idx, fp.idx + fp.ptr[n]);
+    L.pfix.assign(fp.val, fp.val + fp.ptr[n]);
+    L.pself.assign(L.pidx.size(), 0);
+    L.pmap_ptr.assign(L.pidx.size() + 1, 0);
+    L.pmap.clear();
+    restriction(L);
 }
 
 // Sparse product pattern with pair lists: for every output row, the (x, y) contributions
@@ -299,7 +327,7 @@ void galerkin(HostLevel& L, HostLevel& next) {
             if (next.idx[k] == J) next.diag[J] = k;
     // representative numeric values (same formulas and orders as the device kernels)
     std::vector<double> pv(L.pidx.size()), tv(L.tidx.size());
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n && L.pfix.empty(); ++i) {
         double df = 0.0;
         for (int m = L.dmap_ptr[i]; m < L.dmap_ptr[i + 1]; ++m) df += L.rep[L.dmap[m]];
         const double dinv = 1.0 / (df > 0.0 ? df : L.rep[L.diag[i]]);
@@ -309,6 +337,7 @@ void galerkin(HostLevel& L, HostLevel& next) {
             pv[s] = (L.pself[s] ? 1.0 - L.omega_p : 0.0) - dinv * acc;
         }
     }
+    if (!L.pfix.empty()) pv = L.pfix;
     for (size_t t = 0; t < tv.size(); ++t) {
         double acc = 0.0;
         for (int m = L.tpair_ptr[t]; m < L.tpair_ptr[t + 1]; ++m) acc += L.rep[L.tpair_a[m]] * pv[L.tpair_p[m]];
@@ -342,13 +371,17 @@ std::vector<HostLevel> build_hierarchy(int n, const int32_t* ptr, const int32_t*
         if (L0.diag[i] < 0 || !(L0.rep[L0.diag[i]] > 0.0)) return {};
     while (static_cast<int>(levels.size()) < opt.max_levels && levels.back().n > opt.coarse_max) {
         HostLevel& L = levels.back();
-        const std::vector<char> strong = strength(L, opt.theta);
-        std::vector<int> agg;
-        const int nc = aggregate(L, strong, agg);
-        if (nc == 0 || nc > L.n * 9 / 10) break;  // coarsening stalled: this level is the coarsest
-        L.agg = std::move(agg);
-        L.nc = nc;
-        prolongator(L, strong, levels.size() == 1 ? opt.smoother_scale : 1.0);
+        if (levels.size() == 1 && opt.first) {
+            fixed_prolongator(L, *opt.first, opt.smoother_scale);
+        } else {
+            const std::vector<char> strong = strength(L, opt.theta);
+            std::vector<int> agg;
+            const int nc = aggregate(L, strong, agg);
+            if (nc == 0 || nc > L.n * 9 / 10) break;  // coarsening stalled: this level is the coarsest
+            L.agg = std::move(agg);
+            L.nc = nc;
+            prolongator(L, strong, levels.size() == 1 ? opt.smoother_scale : 1.0);
+        }
         HostLevel next;
         galerkin(L, next);
         bool ok = true;

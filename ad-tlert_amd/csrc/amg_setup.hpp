@@ -29,6 +29,7 @@ struct HostLevel {
     double omega = 2.0 / 3.0;         // smoother damping s * 4 / (3 lambda_max(D^-1 A)), s = smoother_scale
     double omega_p = 2.0 / 3.0;       // prolongation damping 4 / (3 lambda_max(D_F^-1 A_F))
     std::vector<int> agg;             // aggregate of each row, -1: left out (no strong connection)
+    std::vector<double> pfix;         // a given prolongator's values (empty: smoothed aggregation)
     std::vector<int> dmap_ptr, dmap;  // per row: slots summed into D_F (diagonal + weak entries)
     // prolongator P (n x nc)
     std::vector<int> pptr, pidx;
@@ -40,6 +41,16 @@ struct HostLevel {
     std::vector<int> tptr, tidx, tpair_ptr, tpair_a, tpair_p;
     // A_{l+1} = R T: (P slot, T slot) product pairs per slot of the next level's pattern
     std::vector<int> cpair_ptr, cpair_p, cpair_t;
+};
+
+// A given prolongator for the finest level (CSR over the fine rows, nc coarse columns), e.g.
+// the interpolation of ten-node (P2) tetrahedra from their vertices (P1): vertex rows copy their
+// vertex, edge-midpoint rows average the edge's two ends.  Coarser levels aggregate as usual.
+struct FixedProlongator {
+    int nc = 0;
+    const int32_t* ptr = nullptr;
+    const int32_t* idx = nullptr;
+    const double* val = nullptr;
 };
 
 struct Options {
@@ -56,6 +67,7 @@ struct Options {
     // wavenumbers of a 2.5D pattern); the coarse Galerkin operators vary more (35% on the first
     // coarse level of the terrain mesh), so coarse levels keep the unscaled damping.
     double smoother_scale = 1.0;
+    const FixedProlongator* first = nullptr;  // finest-level prolongator (null: aggregation)
 };
 
 // Builds the levels from a CSR pattern and representative values (rows = n).  The last

@@ -132,6 +132,7 @@ struct LevelArgs {
     double* next_valp;       // next level values (all systems)
     double* dinv_out;        // this level's inverse diagonal (written by the numeric setup)
     const double* omega_slot;  // finest level: per-slot learned damping (NULL: constant omega)
+    const double* pfix;        // a given prolongator's values, indexed like pidx (NULL: aggregation)
     double* dense;           // coarsest: dense inverses
     // vectors
     const double* b;   // right-hand side of this level (finest: the Krylov residual r)

@@ -1436,6 +1436,7 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         AmgLevel& A = p->amg[l];
         std::vector<int> ptr, idx, diag, pptr, pidx, pmap_ptr, pmap, pself, rptr, rrow, rslot, tpp, ta, tq, cpp, cp,
             ct, dmp, dmap;
+        std::vector<double> pfix;  // a given prolongator's values (single-pattern plans only)
         for (int q = 0; q < n_pat; ++q) {
             const HostLevel& h = H[q][l];
             off_ptr[l][q] = static_cast<int64_t>(ptr.size());
@@ -1462,6 +1463,7 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
             append(diag, h.diag);
             append(pptr, h.pptr);
             append(pidx, h.pidx);
+            pfix.insert(pfix.end(), h.pfix.begin(), h.pfix.end());
             append(pself, h.pself);
             append(pmap_ptr, h.pmap_ptr);
             append(pmap, h.pmap);
@@ -1506,6 +1508,11 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
         ADT_UP(dmp, dmap_ptr)
         ADT_UP(dmap, dmap)
 #undef ADT_UP
+        if (!pfix.empty()) {
+            double* dp = nullptr;
+            if ((rc = upload(p, pfix, &dp))) return rc;
+            a.pfix = dp;
+        }
         // LDS staging tables: smoothing on levels >= 1, prolongation on every level but the
         // coarsest; per pattern tile (tiles of th rows, numbered pattern by pattern), kept only
         // when the staged tile fits the LDS budget
@@ -1732,7 +1739,8 @@ int attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indice
 
 extern "C" int adt_pcg_attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, const int32_t* h_indices,
                                   const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
-                                  int max_levels, int stall_limit, double smoother_scale, int* h_levels) {
+                                  int max_levels, int stall_limit, double smoother_scale, const int32_t* h_p_ptr,
+                                  const int32_t* h_p_idx, const double* h_p_val, int p_cols, int* h_levels) {
     ADT_REQUIRE(p && h_indptr && h_indices && h_values && h_val_off, "adt_pcg_attach_amg: null pointer");
     ADT_REQUIRE(theta >= 0.0 && theta < 1.0, "adt_pcg_attach_amg: theta must be in [0, 1)");
     ADT_REQUIRE(coarse_max >= 1 && coarse_max <= 4096, "adt_pcg_attach_amg: coarse_max must be in [1, 4096]");
@@ -1744,6 +1752,24 @@ extern "C" int adt_pcg_attach_amg(adt_pcg_plan* p, const int32_t* h_indptr, cons
     opt.coarse_max = coarse_max;
     opt.max_levels = max_levels;
     opt.smoother_scale = smoother_scale;
+    adt::amg::FixedProlongator fp;
+    if (h_p_ptr) {
+        ADT_REQUIRE(h_p_idx && h_p_val && p_cols >= 1, "adt_pcg_attach_amg: incomplete first-level prolongator");
+        for (int s = 0; s < p->n_sys; ++s)
+            ADT_REQUIRE(p->pattern_of_sys[s] == 0,
+                        "adt_pcg_attach_amg: a first-level prolongator needs systems of one CSR pattern");
+        const int n = static_cast<int>(p->sys_h[0].rows);
+        ADT_REQUIRE(h_p_ptr[0] == 0, "adt_pcg_attach_amg: prolongator row pointers must start at 0");
+        for (int i = 0; i < n; ++i) ADT_REQUIRE(h_p_ptr[i + 1] >= h_p_ptr[i], "adt_pcg_attach_amg: prolongator row pointers decrease");
+        for (int t = 0; t < h_p_ptr[n]; ++t)
+            ADT_REQUIRE(h_p_idx[t] >= 0 && h_p_idx[t] < p_cols && std::isfinite(h_p_val[t]),
+                        "adt_pcg_attach_amg: prolongator entry %d out of range or not finite", t);
+        fp.nc = p_cols;
+        fp.ptr = h_p_ptr;
+        fp.idx = h_p_idx;
+        fp.val = h_p_val;
+        opt.first = &fp;
+    }
     const int rc = attach_amg(p, h_indptr, h_indices, h_values, h_val_off, opt);
     if (rc) {
         p->free_amg();

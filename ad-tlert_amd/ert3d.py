@@ -22,7 +22,7 @@ from .ert2p5d import _Columns, chunked_rows, electrode_stencil, jacobian_rows_pe
 from .field_cache import FieldCache, digest16, newest_batches_first, step_batches
 from .inputs import (cell_vector, check_finite, conductivity_model, conductivity_series, cotangent_series,
                      measurement_vector, option, parameter_targets)
-from .mesh import Mesh3D
+from .mesh import TET_EDGES, Mesh3D
 from .solver import BatchedSolver, choose_lanes
 from .survey import Survey
 
@@ -293,7 +293,33 @@ class ERTForward3D:
         s.abmn_host = (a, b, m, n)
         s.abmn = [torch.as_tensor(x, dtype=torch.long, device=dev) for x in (a, b, m, n)]
         s.abmn_i32 = [torch.as_tensor(x.astype(np.int32), device=dev) for x in (a, b, m, n)]
+        if self.element_order == 2:
+            self._solver.first_prolongator = self._vertex_prolongator(inv)
         return s
+
+    def _vertex_prolongator(self, inv: np.ndarray) -> tuple:
+        """Interpolation of the ten-node DOFs from the vertices in the device numbering (rows: DOFs,
+        columns: vertices ordered by their device row): a vertex copies itself, an edge midpoint
+        takes half of each end.  The multigrid cycle's finest prolongator on P2 operators: point
+        smoothing reduces the P2-specific error components poorly, and this interpolation hands
+        them to a P1-like Galerkin operator (14 entries per row on the terrain mesh instead of ~70
+        on the first aggregation level), which aggregates on."""
+
+        nv = int(self.mesh.node_count)
+        conn = self._cell_connectivity
+        mid = conn[:, 4:].reshape(-1)
+        ends = np.asarray(TET_EDGES)
+        a = conn[:, ends[:, 0]].reshape(-1)
+        b = conn[:, ends[:, 1]].reshape(-1)
+        mid, first = np.unique(mid, return_index=True)
+        col = np.empty(nv, dtype=np.int64)
+        col[np.argsort(inv[:nv], kind="stable")] = np.arange(nv)
+        rows = np.concatenate((inv[:nv], inv[mid], inv[mid]))
+        cols = np.concatenate((col, col[a[first]], col[b[first]]))
+        vals = np.concatenate((np.ones(nv), np.full(2 * mid.size, 0.5)))
+        order = np.lexsort((cols, rows))
+        indptr = np.concatenate(([0], np.cumsum(np.bincount(rows, minlength=self.N)))).astype(np.int32)
+        return indptr, cols[order].astype(np.int32), vals[order], nv
 
     @staticmethod
     def _interleave(rows: np.ndarray, rp: int, dev) -> Tensor:

@@ -48,7 +48,7 @@ _SIGNATURES = {
     "adt_pcg_destroy": (c_int, [P]),
     "adt_pcg_profile": (c_int, [P, c_int]),
     "adt_pcg_profile_read": (c_int, [P, P]),
-    "adt_pcg_attach_amg": (c_int, [P, P, P, P, P, c_double, c_int, c_int, c_int, c_double, P]),
+    "adt_pcg_attach_amg": (c_int, [P, P, P, P, P, c_double, c_int, c_int, c_int, c_double, P, P, P, c_int, P]),
     "adt_pcg_amg_level": (c_int, [P, c_int, c_int, P, P, P]),
     "adt_pcg_stats": (c_int, [P, P, P, P]),
     "adt_amg_hierarchy_info": (c_int, [c_int, P, P, P, c_double, c_int, c_int, P, P, P]),
@@ -247,10 +247,12 @@ class PcgPlan:
         self.rows_per_tile, self.tiles, self.lds_rows = rb.value, tiles.value, lds_rows.value
 
     def attach_amg(self, values, *, theta=0.08, coarse_max=256, max_levels=12, stall_limit=64,
-                   smoother_scale=SMOOTHER_SCALE) -> int:
+                   smoother_scale=SMOOTHER_SCALE, prolongator=None) -> int:
         """Build the multigrid hierarchy from the first system of each CSR pattern in the device
         ``values`` (read back to the host once); returns the number of levels.  ``smoother_scale``
-        multiplies the finest level's Jacobi smoother damping 4 / (3 lambda_max)."""
+        multiplies the finest level's Jacobi smoother damping 4 / (3 lambda_max).
+        ``prolongator`` (optional): ``(indptr, indices, values, ncols)`` of a given finest-level
+        prolongator (single-pattern plans); coarser levels aggregate."""
 
         if self.host_pattern is None:
             raise ValueError("the multigrid preconditioner needs the host CSR pattern")
@@ -270,9 +272,16 @@ class PcgPlan:
         h_values = np.ascontiguousarray(np.concatenate(segments), dtype=np.float64)
         self.amg_representative = (h_values, h_val_off)
         levels = c_int(0)
+        fixed = (None, None, None, 0)
+        if prolongator is not None:
+            pp, pi, pv, ncols = prolongator
+            fixed = (np.ascontiguousarray(pp, dtype=np.int32), np.ascontiguousarray(pi, dtype=np.int32),
+                     np.ascontiguousarray(pv, dtype=np.float64), int(ncols))
+        self._fixed_prolongator = fixed  # the arrays stay alive during the call
         check(self._lib.adt_pcg_attach_amg(self._handle, host(self.host_pattern[0]), host(self.host_pattern[1]),
                                            host(h_values), host(h_val_off), float(theta), int(coarse_max),
                                            int(max_levels), int(stall_limit), float(smoother_scale),
+                                           *(c_void_p(0) if a is None else host(a) for a in fixed[:3]), fixed[3],
                                            ctypes.byref(levels)))
         self.amg_levels = levels.value
         return self.amg_levels

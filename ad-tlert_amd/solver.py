@@ -83,6 +83,9 @@ class BatchedSolver:
     amg_stall_limit: int = 64
     amg_smoother_scale: float = native.SMOOTHER_SCALE  # Jacobi smoother damping / (4 / (3 lambda_max))
     damping_network: object = None  # learned.DampingNetwork for preconditioner="learned"
+    # a given finest-level prolongator (indptr, indices, values, ncols) for plans of its row count
+    # whose systems share one CSR pattern, e.g. P2 -> P1 interpolation; coarser levels aggregate
+    first_prolongator: tuple | None = None
     ic0_factor_sweeps: int = 3     # fixed-point sweeps of the IC(0) factor
     ic0_solve_sweeps: int = 2      # Jacobi sweeps per triangular solve of the IC(0) preconditioner
     autotune: bool = True          # time the candidate lane widths before the first plan of a shape
@@ -183,7 +186,8 @@ class BatchedSolver:
         if self.preconditioner in ("amg", "learned"):
             try:
                 plan.attach_amg(values_d, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
-                                stall_limit=0, smoother_scale=self.amg_smoother_scale)
+                                stall_limit=0, smoother_scale=self.amg_smoother_scale,
+                                prolongator=self._fixed_prolongator(plan))
                 precond = native.PRECOND_AMG
             except ValueError:  # no hierarchy for this pattern: time Jacobi PCG instead
                 pass
@@ -205,6 +209,14 @@ class BatchedSolver:
         run(8)  # graphs and hierarchy values
         return (run(24) - run(8)) / 16
 
+    def _fixed_prolongator(self, plan: native.PcgPlan):
+        """``first_prolongator`` when it fits ``plan`` (one CSR pattern of its row count), else None."""
+
+        fp = self.first_prolongator
+        if fp is None or len(set(zip(plan.ptr_off.tolist(), plan.idx_off.tolist(), plan.rows.tolist()))) != 1:
+            return None
+        return fp if int(plan.rows[0]) == len(fp[0]) - 1 else None
+
     def _precond(self, plan: native.PcgPlan, values, preconditioner):
         """(native preconditioner id, name used, per-slot damping array or None)."""
 
@@ -215,10 +227,14 @@ class BatchedSolver:
             return None, "ic0", None
         if name not in ("amg", "learned"):
             raise ValueError(f"preconditioner must be one of {PRECONDITIONERS}, got {name!r}")
+        fixed = self._fixed_prolongator(plan)
+        if name == "learned" and fixed is not None:
+            name = "amg"  # the network damps an aggregation prolongator; this plan's finest one is given
         if plan.amg_levels == 0 and not getattr(plan, "amg_failed", False):
             try:
                 plan.attach_amg(values, theta=self.amg_theta, coarse_max=self.amg_coarse_max,
-                                stall_limit=self.amg_stall_limit, smoother_scale=self.amg_smoother_scale)
+                                stall_limit=self.amg_stall_limit, smoother_scale=self.amg_smoother_scale,
+                                prolongator=fixed)
             except ValueError:
                 # no usable hierarchy (tiny or unstructured matrices): plain Jacobi PCG for this plan
                 plan.amg_failed = True

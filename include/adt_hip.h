@@ -144,10 +144,15 @@ int adt_pcg_destroy(adt_pcg_plan* plan);
  * smoother_scale: damping of the finest level's Jacobi smoother in units of 4 / (3 lambda_max(D^-1 A))
  * of the representative matrix, in (0, 1.5) so that the cycle stays symmetric positive definite
  * (1.3 is the library's default; coarse levels use 4 / (3 lambda_max)).
+ * h_p_ptr/h_p_idx/h_p_val, p_cols (optional, null: aggregation on every level): a given
+ * prolongator for the finest level, CSR over the fine rows with p_cols coarse columns (e.g. the
+ * interpolation of ten-node tetrahedra from their vertices); coarser levels aggregate.  Only
+ * for plans whose systems share one CSR pattern.
  * h_levels receives the number of levels. */
 int adt_pcg_attach_amg(adt_pcg_plan* plan, const int32_t* h_indptr, const int32_t* h_indices,
                        const double* h_values, const int64_t* h_val_off, double theta, int coarse_max,
-                       int max_levels, int stall_limit, double smoother_scale, int* h_levels);
+                       int max_levels, int stall_limit, double smoother_scale, const int32_t* h_p_ptr,
+                       const int32_t* h_p_idx, const double* h_p_val, int p_cols, int* h_levels);
 /* Rows, matrix slots (-1 on the finest level: the caller's pattern) and the constant prolongation
  * damping of one level (the value a learned per-slot damping array replaces). */
 int adt_pcg_amg_level(const adt_pcg_plan* plan, int sys, int level, int* h_rows, int64_t* h_nnz,

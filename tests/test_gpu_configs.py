@@ -329,10 +329,18 @@ def test_c4_box_matches_the_oracle_fixture_and_properties(c4):
 
 
 def test_c5_terrain_p2_learned_matches_the_oracle_fixture_and_properties(c5):
+    """C5 built with preconditioner="learned".  On ten-node tetrahedra the cycle's finest
+    prolongator is the interpolation from the vertices (no aggregation prolongator for the
+    network to damp), so the solves run the constant cycle and say so; its first coarse level
+    is the vertex operator."""
+
     case, forward, data = c5
     assert forward.element_order == 2 and not forward.mesh.is_flat_surface
     jac = golden_vectors("C5", case, forward, data)
-    assert forward._solver.last.preconditioner == "learned" and forward._solver.last.fallback_rhs == 0
+    last = forward._solver.last
+    assert forward._solver.preconditioner == "learned" and last.preconditioner == "amg" and last.fallback_rhs == 0
+    plan = forward._solver.last_plan
+    assert plan.amg_level(1)["rows"] == forward.mesh.node_count
     properties("C5", case, forward, jac)
 
 

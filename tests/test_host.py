@@ -408,3 +408,33 @@ def test_ctypes_signatures_match_the_header_argument_counts():
         assert len(params) == len(argtypes), f"{name}: header {len(params)} arguments, binding {len(argtypes)}"
         checked += 1
     assert checked >= 25
+
+
+def test_vertex_prolongator_of_ten_node_tets_interpolates_linear_functions():
+    """The finest prolongator of P2 operators (``ERTForward3D._vertex_prolongator``) in a
+    shuffled device numbering: rows sum to one, vertex rows are unit rows, and interpolating the
+    vertex values of any linear function reproduces it at every ten-node DOF (edge midpoints are
+    the averages of their ends); columns are the vertices in device-row order."""
+
+    import types
+
+    case = adt.cases.box_case()
+    mesh = adt.Mesh3D.from_arrays(case.nodes, case.cells)
+    points, conn, _ = (np.asarray(a) for a in mesh.build_quadratic_topology())
+    n = points.shape[0]
+    inv = np.random.default_rng(2).permutation(n)  # mesh DOF -> device row
+    stub = types.SimpleNamespace(mesh=mesh, _cell_connectivity=conn, N=n)
+    indptr, indices, values, nc = adt.ERTForward3D._vertex_prolongator(stub, inv)
+    nv = mesh.node_count
+    assert nc == nv and indptr.size == n + 1 and indptr[-1] == indices.size == values.size
+    import scipy.sparse as sp
+
+    P = sp.csr_matrix((values, indices, indptr), shape=(n, nv)).toarray()
+    np.testing.assert_allclose(P.sum(axis=1), 1.0)
+    order = np.argsort(inv[:nv], kind="stable")  # column j = the j-th vertex in device-row order
+    for f in (points[:, 0], points[:, 1], points[:, 2], 0.3 + points @ np.array([0.7, -0.4, 1.1])):
+        coarse = f[order]
+        fine = np.empty(n)
+        fine[inv] = f
+        np.testing.assert_allclose(P @ coarse, fine, atol=1e-12)
+    assert np.all((P[inv[:nv]] == 1.0).sum(axis=1) == 1)
