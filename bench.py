@@ -14,7 +14,7 @@ Workloads (``--workload``; by default all three, the time-lapse batch as the top
   right-hand sides of ``--batch-steps`` conductivity models.
 * ``terrain_p2``: the same kind of box with a relief of 6 seeded sinusoids (up to 5 m) and
   ten-node (P2) tetrahedra, 138 720 cells / 195 201 DOFs, 64 electrodes, numerical geometric
-  factors; timed like ``box3d``.
+  factors; timed like ``box3d``; the cycle starts with the interpolation from the vertices.
 
 Every step solves to a relative residual of 1e-10 and reduces the fields to apparent
 resistivities.  The main solver (``--preconditioner``, default ``learned``: flexible CG
@@ -408,9 +408,11 @@ class Box3D:
 
 
 class TerrainP2(Box3D):
-    """3D P2 terrain workload (ten-node tetrahedra, sinusoid relief, numerical geometric factors):
-    the mesh the learned preconditioner is judged on.  Oracle errors against the committed
-    fixture of the seeded model; no CPU baseline (the 3D box carries one)."""
+    """3D P2 terrain workload (ten-node tetrahedra, sinusoid relief, numerical geometric factors).
+    The cycle's finest prolongator is the interpolation from the vertices, so the learned
+    damping (which acts on aggregation prolongators) has nothing to damp and the "learned" leg
+    runs the constant cycle.  Oracle errors against the committed fixture of the seeded model;
+    no CPU baseline (the 3D box carries one)."""
 
     name = ("3D P2 terrain: 138 720 ten-node tetrahedra / 195 201 DOFs, sinusoid relief up to 5 m, 64 electrodes, "
             "2000 random quadrupoles, numerical geometric factors")
