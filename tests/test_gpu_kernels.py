@@ -233,6 +233,66 @@ def test_prolongation_damping_is_four_thirds_over_the_largest_eigenvalue():
     assert abs(omega_p - 4.0 / (3.0 * lam)) <= 1e-6 * omega_p
 
 
+def test_given_finest_prolongator_is_validated_and_solves():
+    """A given finest-level prolongator (here: each fine row of a 2D grid interpolated from the
+    even-even grid points around it, bilinearly) replaces the aggregation on level 0; coarser
+    levels aggregate the Galerkin operator.  The cycle solves to the tolerance against dense
+    solves; prolongators with entries out of range, non-finite values or plans of several CSR
+    patterns are rejected."""
+
+    rng = np.random.default_rng(12)
+    side = 20
+    indptr, indices, values, dense = random_spd_batch(rng, side, S=2)
+    n = side * side
+    cs = side // 2
+    rows, cols, vals = [], [], []
+    for i in range(side):
+        for j in range(side):
+            wi = [(i // 2, 1.0)] if i % 2 == 0 else [(i // 2, 0.5)] + ([(i // 2 + 1, 0.5)] if i // 2 + 1 < cs else [])
+            wj = [(j // 2, 1.0)] if j % 2 == 0 else [(j // 2, 0.5)] + ([(j // 2 + 1, 0.5)] if j // 2 + 1 < cs else [])
+            for a, x in wi:
+                for b, y in wj:
+                    rows.append(i * side + j)
+                    cols.append(a * cs + b)
+                    vals.append(x * y)
+    rows, cols, vals = map(np.asarray, (rows, cols, vals))
+    order = np.lexsort((cols, rows))
+    p_ptr = np.concatenate(([0], np.cumsum(np.bincount(rows, minlength=n)))).astype(np.int32)
+    p_idx, p_val = cols[order].astype(np.int32), vals[order]
+    R, S = 13, 2
+    values_d = torch.as_tensor(values, device=dev())
+
+    plan = native.PcgPlan(np.full(S, n), R, 16, 32, np.zeros(S), np.zeros(S), np.arange(S) * values.shape[1],
+                          host_pattern=(indptr, indices), lds_bytes=8192)
+    bad_idx = p_idx.copy()
+    bad_idx[3] = cs * cs
+    bad_val = p_val.copy()
+    bad_val[5] = np.nan
+    for bad in ((p_ptr, bad_idx, p_val, cs * cs), (p_ptr, p_idx, bad_val, cs * cs)):
+        with pytest.raises(ValueError, match="prolongator"):
+            plan.attach_amg(values_d, coarse_max=16, prolongator=bad)
+    levels = plan.attach_amg(values_d, coarse_max=16, prolongator=(p_ptr, p_idx, p_val, cs * cs))
+    assert levels >= 2 and plan.amg_level(1)["rows"] == cs * cs
+    b = rng.standard_normal((S, n, plan.rp))
+    b[:, :, R:] = 0.0
+    x = torch.zeros((S, n, plan.rp), dtype=torch.float64, device=dev())
+    rc, iters, relres = plan.solve(torch.as_tensor(indptr, device=dev()), torch.as_tensor(indices, device=dev()),
+                                   values_d, torch.as_tensor(b, device=dev()), x, 1e-10, 2000,
+                                   precond=native.PRECOND_AMG)
+    assert rc == 0 and relres.max() <= 1e-10 and plan.stats()["fallback_rhs"] == 0
+    assert relmax(x.cpu().numpy()[:, :, :R], np.linalg.solve(dense, b[:, :, :R])) < 1e-7
+    print(f"mean FCG iterations with the bilinear first level: {iters[:, :R].mean():.1f}")
+    plan.close()
+    pb, ib, vb, _ = random_spd_batch(rng, 13, S=1)
+    two = native.PcgPlan(np.array([n, 169]), R, 16, 32, np.array([0, indptr.size]), np.array([0, indices.size]),
+                         np.array([0, values.shape[1]]), host_pattern=(np.concatenate([indptr, pb]),
+                                                                       np.concatenate([indices, ib])), lds_bytes=8192)
+    with pytest.raises(ValueError, match="one CSR pattern"):
+        two.attach_amg(torch.as_tensor(np.concatenate([values[0], vb[0]]), device=dev()), coarse_max=16,
+                       prolongator=(p_ptr, p_idx, p_val, cs * cs))
+    two.close()
+
+
 def stencil_27_pattern(side):
     """CSR pattern of the 27-point stencil on a side^3 grid (rows of up to 27 entries)."""
 
@@ -471,22 +531,26 @@ def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
 @pytest.mark.parametrize("k, Sr, Sc, W", [(4, 64, 64, 1), (4, 64, 64, 3), (10, 17, 40, 3), (4, 128, 33, 3),
                                           (3, 5, 96, 3), (10, 128, 96, 8)],
                          ids=["p1-3d", "p1-3w", "p2-3w", "wide", "narrow", "large"])
-@pytest.mark.parametrize("form", ["gram", "default", "block8"])
-def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W, form, monkeypatch):
+@pytest.mark.parametrize("form", ["gram", "default", "block", "block8"])
+@pytest.mark.parametrize("C", [47, 48])
+def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W, form, C, monkeypatch):
     """Jacobian rows from separate receiver and current fields (as in 3D) equal the per-output
     formula with non-symmetric templates, remote indices reading zero and padding columns ignored;
     bitwise repeatable.  Both forms: the per-cell Gram block (matrix cores, four lookups per
-    output; forced by ADT_CROSS_FORM=gram) and the default cell-block kernel (32 cells and 1024
-    threads where its LDS fits, else 16/8/4/2 cells; ADT_CELL_BLOCK=8 caps it at the narrower
-    widths; a tail block past the last cell); the last case is too large for the cell block and
-    takes the Gram form either way."""
+    output; forced by ADT_CROSS_FORM=gram) and the cell-block kernel (forced by
+    ADT_CROSS_FORM=block: 32 cells and 1024 threads where its LDS fits, else 16/8/4/2 cells;
+    ADT_CELL_BLOCK=8 caps it at the narrower widths; a tail block past the last cell).  The
+    default takes the 32-cell block where it fits and the Gram block otherwise; the last case is
+    too large for any cell block and takes the Gram form every time.  Odd and even cell counts:
+    the 32-cell block stores two cells' outputs as one 16-byte store where the row length is
+    even."""
 
-    monkeypatch.setenv("ADT_CROSS_FORM", form)
+    monkeypatch.setenv("ADT_CROSS_FORM", "block" if form.startswith("block") else form)
     if form == "block8":
         monkeypatch.setenv("ADT_CELL_BLOCK", "8")
 
     gen = torch.Generator().manual_seed(100 * k + Sr + Sc + W)
-    N, C, D = 80, 47, 39
+    N, D = 80, 39
     phr = torch.randn(W, Sr, N, generator=gen, dtype=torch.float64)
     phc = torch.randn(W, Sc, N, generator=gen, dtype=torch.float64)
     stiff = torch.randn(C, k, k, generator=gen, dtype=torch.float64)
