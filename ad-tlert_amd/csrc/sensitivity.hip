@@ -1225,14 +1225,20 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
         A.wm[w] = d_mass ? h_wm[w] : 0.0;
     }
     hipStream_t st = adt::as_stream(stream);
-    // Form: the cell block where its staging fits 64 KB (CB cells share the staging and write
-    // CB-wide row segments; 3D box: 12 ms per launch against 19 ms for the Gram block, whose
-    // per-cell staging and 8-byte row writes outweigh its four LDS reads per output); the Gram
-    // block otherwise.  ADT_CROSS_FORM=gram forces the Gram block.
+    // Form: the 32-cell block where its staging fits one CU (3D P1, one field set: two cells per
+    // thread, 16-byte LDS reads and 256-byte row segments; 3D box 6.3 ms per launch, where the
+    // Gram block took 19 ms: its per-cell staging and 8-byte row writes outweigh its four LDS
+    // reads per output); the Gram block otherwise.  Narrower cell blocks read 4 W K values per
+    // output, one cell per thread, and write 8- to 128-byte row segments: on ten-node tetrahedra
+    // (4 cells per block, 40 reads per output) they took 3.99 ms against 2.5 ms for the Gram
+    // block.  ADT_CROSS_FORM=gram forces the Gram block, ADT_CROSS_FORM=block the widest cell
+    // block that fits (ADT_CELL_BLOCK=8 caps it below 32 cells).
     const char* form = std::getenv("ADT_CROSS_FORM");
     const bool force_gram = (form && std::strcmp(form, "gram") == 0) || d_col_ptr;  // columns of many cells
-    if (!force_gram) {
-        switch (cell_block_width(A)) {
+    const bool force_block = form && std::strcmp(form, "block") == 0;
+    const int width = force_gram ? 0 : cell_block_width(A);
+    if (width == 32 || (force_block && width > 0)) {
+        switch (width) {
             case 32: return launch_cell_block<32, 1024>(A, st);
             case 16: return launch_cell_block<16, 256>(A, st);
             case 8: return launch_cell_block<8, 256>(A, st);

@@ -351,16 +351,24 @@ class Box3D:
         def run():
             return op.jacobian(sigma, return_device=True, batch_size=self.D)
 
-        # cell-block form: per cell Psi = T Phi_C (K x nc), then per output a K-term sum of
-        # (receiver difference) x (Psi difference), four LDS reads per term; the 16 GB output
-        # writes bound it (a variant with the stores alone takes ~80% of its time)
         K, nc, nr = (4 if self.order == 1 else 10), op._dev.nc, op._dev.nr
         bytes_ = 8 * self.D * self.C + 8 * self.N * (nc + nr) + 8 * self.C * K * K + 4 * self.C * K + 16 * self.D
-        flops = self.C * 2 * K * K * nc + 4 * K * self.D * self.C
-        lds_bytes = 4 * K * 8 * self.D * self.C
+        if self.gram_form:
+            # Gram form: per cell Psi = T Phi_C (K x nc) and G_c = Phi_R^T Psi ((nr+1) x (nc+1), on
+            # the matrix cores), then per output four LDS reads and three adds; the (D, C) block is
+            # written one 8-byte column entry per row
+            flops = self.C * (2 * K * K * nc + 2 * K * nr * nc) + 3 * self.D * self.C
+            lds_bytes = 4 * 8 * self.D * self.C
+        else:
+            # cell-block form: per cell Psi = T Phi_C (K x nc), then per output a K-term sum of
+            # (receiver difference) x (Psi difference), four LDS reads per term; the 16 GB output
+            # writes bound it (a variant with the stores alone takes ~80% of its time)
+            flops = self.C * 2 * K * K * nc + 4 * K * self.D * self.C
+            lds_bytes = 4 * K * 8 * self.D * self.C
         return run, bytes_, flops, self.sensitivity_kernel, {"bound": "hbm", "lds_bytes_per_launch": lds_bytes}
 
     sensitivity_kernel = "cell_block_kernel<32, 1024, 4> (cell-block sensitivity)"
+    gram_form = False
 
     def cpu_and_oracle(self, jac_device):
         """CPU baseline: the oracle's matrix solved for ``--cpu-rhs-3d`` of the 64 source
@@ -419,7 +427,8 @@ class TerrainP2(Box3D):
     data = "synthetic (seeded graded terrain box, random ABMN survey, log-normal GRF conductivity models)"
     order = 2
     fixture = "terrain_p2_150k"
-    sensitivity_kernel = "cell_block_kernel<4, 256, 0> (cell-block sensitivity, P2)"
+    sensitivity_kernel = "gram_cross_kernel<4> (Gram-form cross sensitivity, P2)"
+    gram_form = True
 
     @staticmethod
     def make_case(adt):
