@@ -619,7 +619,7 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
 // over the cell's K DOFs.  A block of NT threads takes CB consecutive cells and stages, per cell,
 // the receiver rows phi_R (W K x Sr) and Psi (W K x Sc) in LDS, electrode-major: the R = W K rows
 // of CB cells of one electrode are R x CB consecutive doubles (cell fastest); a zero electrode
-// slot stands for the remote electrode.  Staging is two rounds of independent loads: the cells'
+// slot stands for the remote electrode.  Staging is two passes of independent loads: the cells'
 // nodes and element templates, then every (cell, w, i) row of both fields (wave-uniform rows,
 // kCellBlockRows rows per wave in flight); Psi then replaces phi_C in place, one thread per
 // (electrode, cell, w) column.  Output rows come out as CB consecutive doubles per quadrupole.
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(NT) void cell_block_kernel(GramCrossArgs A) {
     const int c0 = adt::xcd_tile(static_cast<int>(blockIdx.x), nblk) * CB;
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // round 1: nodes and templates of the block's cells (cells past C repeat cell C - 1 and are
+    // staging pass 1: nodes and templates of the block's cells (cells past C repeat cell C - 1 and are
     // never stored)
     {
         const int64_t first = static_cast<int64_t>(c0) * K, last = static_cast<int64_t>(A.C) * K - 1;
@@ -667,7 +667,7 @@ __global__ __launch_bounds__(NT) void cell_block_kernel(GramCrossArgs A) {
         }
     }
     __syncthreads();
-    // round 2: row (cb, w, i) of phi_R (rows 0 .. CB R - 1) and of phi_C (rows CB R .. 2 CB R - 1),
+    // staging pass 2: row (cb, w, i) of phi_R (rows 0 .. CB R - 1) and of phi_C (rows CB R .. 2 CB R - 1),
     // lanes along the electrodes (S <= 128)
     const int n_rows = 2 * CB * R;
     for (int q0 = wv; q0 < n_rows; q0 += NW * kCellBlockRows) {
