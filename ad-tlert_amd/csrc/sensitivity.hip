@@ -1225,19 +1225,21 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
         A.wm[w] = d_mass ? h_wm[w] : 0.0;
     }
     hipStream_t st = adt::as_stream(stream);
-    // Form: the 32-cell block where its staging fits one CU (3D P1, one field set: two cells per
-    // thread, 16-byte LDS reads and 256-byte row segments; 3D box 6.3 ms per launch, where the
-    // Gram block took 19 ms: its per-cell staging and 8-byte row writes outweigh its four LDS
-    // reads per output); the Gram block otherwise.  Narrower cell blocks read 4 W K values per
-    // output, one cell per thread, and write 8- to 128-byte row segments: on ten-node tetrahedra
-    // (4 cells per block, 40 reads per output) they took 3.99 ms against 2.5 ms for the Gram
-    // block.  ADT_CROSS_FORM=gram forces the Gram block, ADT_CROSS_FORM=block the widest cell
-    // block that fits (ADT_CELL_BLOCK=8 caps it below 32 cells).
+    // Form (kernel times per call, 2000 quadrupoles, one field set unless noted): the cell block
+    // reads 4 W K staged values per output, the Gram block 4 (plus the per-cell G block of
+    // (S_r + 1) (S_c + 1) doubles).  The cell block is taken where W K <= 4 (linear tetrahedra:
+    // 1.15M cells, 64 / 96 / 128 electrodes 6.3 / 12.6 / 15.3 ms against 21.6 / 31.1 / 58.5 ms for
+    // the Gram block) or where 32 cells fit one CU; otherwise the Gram block (ten-node
+    // tetrahedra, 138 720 cells, 32 / 64 electrodes: 2.0 / 2.6 ms against 3.0 / 4.2 ms; at 96
+    // electrodes, where the G block leaves two workgroups per CU, 4.6 against 4.0 ms; two field
+    // sets of linear tetrahedra 6.9 against 6.0 ms, three 7.5 against 30 ms).
+    // ADT_CROSS_FORM=gram forces the Gram block, ADT_CROSS_FORM=block the widest cell block
+    // that fits (ADT_CELL_BLOCK=8 caps it below 32 cells).
     const char* form = std::getenv("ADT_CROSS_FORM");
     const bool force_gram = (form && std::strcmp(form, "gram") == 0) || d_col_ptr;  // columns of many cells
     const bool force_block = form && std::strcmp(form, "block") == 0;
     const int width = force_gram ? 0 : cell_block_width(A);
-    if (width == 32 || (force_block && width > 0)) {
+    if (width == 32 || (width > 0 && (force_block || n_waves * k_local <= 4))) {
         switch (width) {
             case 32: return launch_cell_block<32, 1024>(A, st);
             case 16: return launch_cell_block<16, 256>(A, st);

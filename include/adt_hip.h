@@ -236,13 +236,14 @@ int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_o
  * phiR_w / phiC_w are the (N, rp_r) / (N, rp_c) blocks at d_xr + h_off_r[w] / d_xc + h_off_c[w];
  * their first s_r / s_c (<= 128) columns are receivers / current sources; a, b >= s_c and
  * m, n >= s_r read zero (remote electrodes).  Templates need not be symmetric.  When the
- * staged rows of 32 cells fit one CU's LDS (160 KB; 1024-thread workgroups), G_c is not
- * formed: a workgroup of consecutive cells evaluates
+ * staged rows of 32 cells fit one CU's LDS (160 KB; 1024-thread workgroups), or of 16/8/4/2
+ * cells 64 KB with n_waves * k_local <= 4 (linear tetrahedra), G_c is not formed: a workgroup
+ * of consecutive cells evaluates
  * out[d, c] = -sum_{w,i} (phiR[m] - phiR[n])_{w,i} (Psi[a] - Psi[b])_{w,i}
  * with Psi_w = T_w phiC_w on the cell's DOFs, and writes each output row as one contiguous
- * segment (ADT_CROSS_FORM=block also takes blocks of 16/8/4/2 cells within 64 KB); otherwise
- * (or with ADT_CROSS_FORM=gram) one workgroup per cell forms G_c on the FP64 matrix cores and
- * keeps it in LDS with a zero row and column for the remote electrodes.  No workspace.  FP64, fixed summation order: bitwise repeatable.  out
+ * segment (ADT_CROSS_FORM=block takes this form wherever a block fits); otherwise (or with
+ * ADT_CROSS_FORM=gram) one workgroup per cell forms G_c on the FP64 matrix cores and keeps it
+ * in LDS with a zero row and column for the remote electrodes.  No workspace.  FP64, fixed summation order: bitwise repeatable.  out
  * is (D, C) row-major.  Column aggregation as in adt_sensitivity_gram (d_col_ptr, d_col_cells,
  * n_cols; out (D, n_cols)); it always takes the whole-Gram form, summing its cells' G_c. */
 int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,

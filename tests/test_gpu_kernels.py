@@ -540,8 +540,9 @@ def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W, form,
     output; forced by ADT_CROSS_FORM=gram) and the cell-block kernel (forced by
     ADT_CROSS_FORM=block: 32 cells and 1024 threads where its LDS fits, else 16/8/4/2 cells;
     ADT_CELL_BLOCK=8 caps it at the narrower widths; a tail block past the last cell).  The
-    default takes the 32-cell block where it fits and the Gram block otherwise; the last case is
-    too large for any cell block and takes the Gram form every time.  Odd and even cell counts:
+    default takes the cell block at 32 cells or for one field set of linear tetrahedra, and the
+    Gram block otherwise; the last case is too large for any cell block and takes the Gram form
+    every time.  Odd and even cell counts:
     the 32-cell block stores two cells' outputs as one 16-byte store where the row length is
     even."""
 
