@@ -528,9 +528,9 @@ def test_multigrid_handles_patterns_of_different_sizes_in_one_batch():
     assert results[native.PRECOND_AMG].max() < 0.5 * results[native.PRECOND_JACOBI].max()
 
 
-@pytest.mark.parametrize("k, Sr, Sc, W", [(4, 64, 64, 1), (4, 64, 64, 3), (10, 17, 40, 3), (4, 128, 33, 3),
-                                          (3, 5, 96, 3), (10, 128, 96, 8)],
-                         ids=["p1-3d", "p1-3w", "p2-3w", "wide", "narrow", "large"])
+@pytest.mark.parametrize("k, Sr, Sc, W", [(4, 64, 64, 1), (4, 96, 96, 1), (4, 64, 64, 3), (10, 17, 40, 3),
+                                          (4, 128, 33, 3), (3, 5, 96, 3), (10, 128, 96, 8)],
+                         ids=["p1-3d", "p1-96", "p1-3w", "p2-3w", "wide", "narrow", "large"])
 @pytest.mark.parametrize("form", ["gram", "default", "block", "block8"])
 @pytest.mark.parametrize("C", [47, 48])
 def test_cross_gram_sensitivity_matches_the_einsum_reference(k, Sr, Sc, W, form, C, monkeypatch):
