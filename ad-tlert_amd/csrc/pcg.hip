@@ -161,6 +161,90 @@ __global__ __launch_bounds__(256) void cg_init(const PcgArgs* __restrict__ pa) {
     }
 }
 
+// cg_init for staged plans: x of the tile and its halo rows in LDS (as cg_spmv_lds stages p), the
+// product from LDS in slot order (the same sums as cg_init's gathers), the diagonal from the
+// staged CSR slice (an own row's staged slot is its tile row); same outputs and partials.
+// Padding columns read column 0 of b and store r = 0.
+template <int L>
+__global__ __launch_bounds__(256) void cg_init_lds(const PcgArgs* __restrict__ pa) {
+    extern __shared__ double smem[];
+    const PcgArgs& a = *pa;
+    const int tix = tile_index(a);
+    const Tile t = adt::gview(a.tiles)[tix];
+    constexpr int RPP = kRowsPerPass(L);
+    const Sys S = adt::gview(a.sys)[t.sys];
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    const int col = (t.grp % a.groups) * L + lane;
+    const bool real = col < a.r_real;
+    const bool first_group = (t.grp % a.groups) == 0;
+    const int rp = a.rp;
+    const auto B = adt::gview(a.b + S.b_off + (real ? col : 0));
+    const auto X = adt::gview(a.x + S.vec_off + col);
+    const auto R = adt::gview(a.r + S.vec_off + col);
+    const auto P0 = adt::gview(a.p[0] + S.vec_off + col);
+    const auto P1 = adt::gview(a.p[1] + S.vec_off + col);
+    const auto Q = adt::gview(a.q + S.vec_off + col);
+    const auto D = adt::gview(a.dinv + S.diag_off);
+    const StageSrc src = fine_src(a, S);
+    const LdsTile s = lds_tile(src, t, L, smem, true);
+    const auto value = [&](int j) { return X[static_cast<int64_t>(j) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(src, s, 0, hid);
+    double xv[own_max(L)], bv[own_max(L)];
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int64_t o = static_cast<int64_t>(own_row<L>(t, s.rows, i)) * rp;
+        xv[i] = X[o];
+        bv[i] = B[o];
+    }
+    const CsrRegs cr = load_csr(src, t, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(src, t, s, cr);
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) s.stage[q * L + lane] = xv[i];
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(src, s, value);
+    adt::wait_vector_memory();
+    __syncthreads();
+    double sb = 0.0, sz = 0.0, sr = 0.0;
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int q = rsub + i * RPP;
+        if (q >= s.rows) continue;
+        double d = 0.0;
+        for (int k = s.sptr[q]; k < s.sptr[q + 1]; ++k)
+            if (s.sidx[k] == q) d = s.sval[k];
+        const double di = 1.0 / d;
+        const int row = t.row0 + q;
+        if (first_group && lane == 0) D[row] = di;
+        const int64_t o = static_cast<int64_t>(row) * rp;
+        P0[o] = 0.0;
+        P1[o] = 0.0;
+        Q[o] = 0.0;
+        const double acc = lds_row_product<L>(s, q, lane);
+        const double bi = bv[i];
+        const double ri = bi - acc;
+        R[o] = real ? ri : 0.0;
+        if (real) {
+            sb = fma(bi, bi, sb);
+            sz = fma(ri * ri, di, sz);
+            sr = fma(ri, ri, sr);
+        }
+    }
+    double s1 = sb, s2 = sz;
+    block_sum2<L>(s1, s2);
+    double s3 = block_sum1<L>(sr);
+    if (threadIdx.x < L) {
+        adt::gview(a.part0)[static_cast<int64_t>(tix) * L + threadIdx.x] = s1;
+        adt::gview(a.part1)[static_cast<int64_t>(tix) * L + threadIdx.x] = s2;
+        adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = s3;
+    }
+}
+
 template <int L>
 __global__ __launch_bounds__(kReduceThreads) void cg_init_reduce(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
@@ -644,6 +728,59 @@ __global__ __launch_bounds__(256) void cg_true_residual(const PcgArgs* __restric
     if (threadIdx.x < L) adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
 }
 
+// cg_true_residual for staged plans: x of the tile and its halo in LDS, products in slot order.
+template <int L>
+__global__ __launch_bounds__(256) void cg_true_residual_lds(const PcgArgs* __restrict__ pa) {
+    extern __shared__ double smem[];
+    const PcgArgs& a = *pa;
+    const int tix = tile_index(a);
+    const Tile t = adt::gview(a.tiles)[tix];
+    constexpr int RPP = kRowsPerPass(L);
+    const Sys S = adt::gview(a.sys)[t.sys];
+    const int lane = static_cast<int>(threadIdx.x % L), rsub = static_cast<int>(threadIdx.x / L);
+    const int col = (t.grp % a.groups) * L + lane;
+    const bool real = col < a.r_real;
+    const int rp = a.rp;
+    const auto B = adt::gview(a.b + S.b_off + (real ? col : 0));
+    const auto X = adt::gview(a.x + S.vec_off + col);
+    const StageSrc src = fine_src(a, S);
+    const LdsTile s = lds_tile(src, t, L, smem, true);
+    const auto value = [&](int j) { return X[static_cast<int64_t>(j) * rp]; };
+    int hid[halo_batch(L)];
+    halo_ids<L>(src, s, 0, hid);
+    double xv[own_max(L)], bv[own_max(L)];
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int64_t o = static_cast<int64_t>(own_row<L>(t, s.rows, i)) * rp;
+        xv[i] = X[o];
+        bv[i] = B[o];
+    }
+    const CsrRegs cr = load_csr(src, t, s);
+    double hv[halo_batch(L)];
+    halo_values<L>(hid, hv, value);
+    store_csr(src, t, s, cr);
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) s.stage[q * L + lane] = xv[i];
+    }
+    store_halo<L>(s, 0, hv);
+    stage_halo_rest<L>(src, s, value);
+    adt::wait_vector_memory();
+    __syncthreads();
+    double sr = 0.0;
+#pragma unroll
+    for (int i = 0; i < own_max(L); ++i) {
+        const int q = rsub + i * RPP;
+        if (q < s.rows) {
+            const double ri = bv[i] - lds_row_product<L>(s, q, lane);
+            sr = real ? fma(ri, ri, sr) : sr;
+        }
+    }
+    sr = block_sum1<L>(sr);
+    if (threadIdx.x < L) adt::gview(a.part2)[static_cast<int64_t>(tix) * L + threadIdx.x] = sr;
+}
+
 template <int L>
 __global__ __launch_bounds__(kReduceThreads) void cg_true_reduce(const PcgArgs* __restrict__ pa) {
     const PcgArgs& a = *pa;
@@ -988,7 +1125,10 @@ template <int L>
 int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mode) {
     const PcgArgs* pa = p->args_d;
     ADT_HIP_CHECK(hipMemsetAsync(p->args_h.active_total, 0, sizeof(int), st));
-    hipLaunchKernelGGL(cg_init<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
+    if (p->lds_rows > 0)
+        hipLaunchKernelGGL(cg_init_lds<L>, dim3(p->n_tiles), dim3(256), fine_lds_bytes(p), st, pa);
+    else
+        hipLaunchKernelGGL(cg_init<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     launch_reduce<L>(p, st, cg_init_reduce<L>, 15);
     hipLaunchKernelGGL(cg_zero_solution<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     if (fused_update(p, mode))  // the first cycle's finest pre-smoothing (later ones: fcg_update_smooth)
@@ -1021,7 +1161,10 @@ int solve_impl(adt_pcg_plan* p, hipStream_t st, int max_iter, int chunk, int mod
         ADT_HIP_CHECK(hipEventSynchronize(p->ev[last]));
         if (p->profile && (rc = accumulate_profile(p, last, chunk))) return rc;
     }
-    hipLaunchKernelGGL(cg_true_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
+    if (p->lds_rows > 0)
+        hipLaunchKernelGGL(cg_true_residual_lds<L>, dim3(p->n_tiles), dim3(256), fine_lds_bytes(p), st, pa);
+    else
+        hipLaunchKernelGGL(cg_true_residual<L>, dim3(p->n_tiles), dim3(256), 0, st, pa);
     launch_reduce<L>(p, st, cg_true_reduce<L>, 12);
     ADT_HIP_CHECK(hipGetLastError());
     return ADT_OK;
