@@ -1227,11 +1227,11 @@ extern "C" int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int r
     // Form (kernel times per call, 2000 quadrupoles, one field set unless noted): the cell block
     // reads 4 W K staged values per output, the Gram block 4 (plus the per-cell G block of
     // (S_r + 1) (S_c + 1) doubles).  The cell block is taken where W K <= 4 (linear tetrahedra:
-    // 1.15M cells, 64 / 96 / 128 electrodes 6.3 / 12.6 / 15.3 ms against 21.6 / 31.1 / 58.5 ms for
+    // 1.15M cells, 64 / 96 / 128 electrodes 6.3 / 12.8 / 15.3 ms against 21.6 / 29.1 / 58.5 ms for
     // the Gram block) or where 32 cells fit one CU; otherwise the Gram block (ten-node
-    // tetrahedra, 138 720 cells, 32 / 64 electrodes: 2.0 / 2.6 ms against 3.0 / 4.2 ms; at 96
-    // electrodes, where the G block leaves two workgroups per CU, 4.6 against 4.0 ms; two field
-    // sets of linear tetrahedra 6.9 against 6.0 ms, three 7.5 against 30 ms).
+    // tetrahedra, 138 720 cells, 32 / 64 / 96 electrodes: 2.3 / 2.2 / 4.0 ms against
+    // 3.0 / 4.1 / 4.0 ms; two field sets of linear tetrahedra 6.7 against 5.9 ms, three 7.5
+    // against 30 ms).
     // ADT_CROSS_FORM=gram forces the Gram block, ADT_CROSS_FORM=block the widest cell block
     // that fits (ADT_CELL_BLOCK=8 caps it below 32 cells).
     const char* form = std::getenv("ADT_CROSS_FORM");
