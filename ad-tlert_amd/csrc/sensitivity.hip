@@ -236,7 +236,8 @@ __device__ void template_root(double* R, int K) {
 // LDS each) overlap their load, product and store phases.  Blocks of 2 or 4 adjacent columns
 // (16- or 32-byte row segments per store) were measured slower on the 20k-node time-lapse mesh
 // (1.36 and 1.38 ms against 1.17 ms): their triangles take the CU's whole LDS, leaving one
-// workgroup per CU whose phases run one after another.
+// workgroup per CU whose phases run one after another.  Two columns per 512-thread workgroup at
+// two workgroups per CU (as many waves per CU) measured 1.20 against 1.14 ms.
 constexpr int kGramColumns = 1;
 
 // Y in LDS is stored in 4-row x 16-source bricks of 64 doubles, brick (r / 4, e / 16) at
@@ -316,16 +317,19 @@ __global__ __launch_bounds__(256 * CB) void gram_kernel(GramArgs A) {
         const double* Mc = A.mass ? A.mass + static_cast<int64_t>(c) * K * K : nullptr;
         const int32_t* dof = A.dofs + static_cast<int64_t>(c) * K;
         __syncthreads();  // the previous step's factors and rows are consumed
-        if (active && gtid < A.W) {
-            double* R = rl + gtid * K * K;
-            const double ws = A.ws[gtid], wm = A.wm[gtid];
-            for (int e = 0; e < K * K; ++e) R[e] = Mc ? fma(wm, Mc[e], ws * Kc[e]) : ws * Kc[e];
-            template_root(R, K);
-        }
+        // T_c,w for every wavenumber, one thread per entry (the loads of all entries in flight
+        // together), then one thread per wavenumber factors its T in place
+        if (active)
+            for (int t = gtid; t < A.W * K * K; t += 256) {
+                const int w = t / (K * K), e = t - w * (K * K);
+                rl[t] = Mc ? fma(A.wm[w], Mc[e], A.ws[w] * Kc[e]) : A.ws[w] * Kc[e];
+            }
         int64_t node[KM];  // wave-uniform row offsets of the cell's nodes
 #pragma unroll
         for (int k = 0; k < KM; ++k)
             node[k] = (active && k < K) ? static_cast<int64_t>(__builtin_amdgcn_readfirstlane(dof[k])) * A.rp : 0;
+        __syncthreads();
+        if (active && gtid < A.W) template_root(rl + gtid * K * K, K);
         __syncthreads();
         for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
             const int nw = min(A.wchunk, A.W - w0);
