@@ -232,6 +232,48 @@ __device__ void template_root(double* R, int K) {
     }
 }
 
+// template_root on a K x K matrix with K <= KM held in registers (the same operations in the same
+// order; the LDS version's chain of dependent LDS accesses is the longest step of the factor
+// phase for small K).
+template <int KM>
+__device__ void template_root_regs(double* R, int K) {
+    double t[KM][KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+#pragma unroll
+        for (int j = 0; j < KM; ++j) t[i][j] = (i < K && j < K) ? R[i * K + j] : 0.0;
+    double tmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+        if (i < K) tmax = fmax(tmax, t[i][i]);
+    const double tiny = 1e-13 * tmax;
+#pragma unroll
+    for (int i = 0; i < KM; ++i) {
+        if (i >= K) break;
+        const double p = t[i][i];
+        if (p > tiny) {
+            const double sq = sqrt(p), inv = 1.0 / sq;
+            t[i][i] = sq;
+#pragma unroll
+            for (int j = i + 1; j < KM; ++j) t[i][j] *= inv;
+#pragma unroll
+            for (int j = i + 1; j < KM; ++j)
+#pragma unroll
+                for (int k = j; k < KM; ++k) t[j][k] = fma(-t[i][j], t[i][k], t[j][k]);
+        } else {
+#pragma unroll
+            for (int j = i; j < KM; ++j) t[i][j] = 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < i; ++j) t[i][j] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (i < K && j < K) R[i * K + j] = t[i][j];
+}
+
 // Output columns per workgroup of gram_kernel.  One: four independent workgroups per CU (40 KB of
 // LDS each) overlap their load, product and store phases.  Blocks of 2 or 4 adjacent columns
 // (16- or 32-byte row segments per store) were measured slower on the 20k-node time-lapse mesh
@@ -329,7 +371,12 @@ __global__ __launch_bounds__(256 * CB) void gram_kernel(GramArgs A) {
         for (int k = 0; k < KM; ++k)
             node[k] = (active && k < K) ? static_cast<int64_t>(__builtin_amdgcn_readfirstlane(dof[k])) * A.rp : 0;
         __syncthreads();
-        if (active && gtid < A.W) template_root(rl + gtid * K * K, K);
+        if (active && gtid < A.W) {
+            if constexpr (KM <= 4)
+                template_root_regs<KM>(rl + gtid * K * K, K);
+            else
+                template_root(rl + gtid * K * K, K);
+        }
         __syncthreads();
         for (int w0 = 0; w0 < A.W; w0 += A.wchunk) {
             const int nw = min(A.wchunk, A.W - w0);
