@@ -584,7 +584,8 @@ __global__ __launch_bounds__(256) void gram_cross_kernel(GramCrossArgs A) {
                 const int r = r0 + 4 * u;
                 const int rc = min(r, rows - 1);  // rows past the chunk: index math only
                 const int wl = rc / K, i = rc - wl * K;
-                const int64_t node = dof[i];
+                // wave-uniform row: the node index goes to a scalar register
+                const int64_t node = __builtin_amdgcn_readfirstlane(adt::gview(dof)[i]);
                 const double* rowr = A.xr + A.off_r[w0 + wl] + node * A.rp_r;
                 const double* rowc = A.xc + A.off_c[w0 + wl] + node * A.rp_c;
 #pragma unroll
