@@ -243,7 +243,7 @@ int adt_sensitivity_gram(int n_waves, int rp, int s_rows, const int64_t* h_vec_o
  * with Psi_w = T_w phiC_w on the cell's DOFs, and writes each output row as one contiguous
  * segment (ADT_CROSS_FORM=block takes this form wherever a block fits); otherwise (or with
  * ADT_CROSS_FORM=gram) one workgroup per cell forms G_c on the FP64 matrix cores and keeps it
- * in LDS with a zero row and column for the remote electrodes.  No workspace.  FP64, fixed summation order: bitwise repeatable.  out
+ * in LDS (remote electrodes read zero).  No workspace.  FP64, fixed summation order: bitwise repeatable.  out
  * is (D, C) row-major.  Column aggregation as in adt_sensitivity_gram (d_col_ptr, d_col_cells,
  * n_cols; out (D, n_cols)); it always takes the whole-Gram form, summing its cells' G_c. */
 int adt_sensitivity_gram_cross(int n_waves, const double* d_xr, int rp_r, int s_r, const int64_t* h_off_r,
